@@ -1,0 +1,572 @@
+// extern "C" boundary over the engine (declarations in include/frank_amd.h).
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/frank_amd.h"
+#include "engine.hpp"
+
+using namespace frk;
+
+struct frk_ctx_s {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<Context> ctx;
+};
+
+struct frk_blr_s {
+  frk_ctx_s* owner = nullptr;
+  int precision = FRK_PREC_F64;
+  std::unique_ptr<BlrMatrix<double>> a64;
+  std::unique_ptr<BlrMatrix<float>> a32;
+  bool has_kernel = false;
+  KernelSpec ks{};
+  DevBuf pts;
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+template <class F>
+int guard(F&& f) {
+  try {
+    return f();
+  } catch (const std::invalid_argument& e) {
+    return fail(FRK_ERR_ARG, e.what());
+  } catch (const std::runtime_error& e) {
+    const std::string m = e.what();
+    if (m.find("capacity") != std::string::npos) return fail(FRK_ERR_CAPACITY, m);
+    if (m.find("zero pivot") != std::string::npos) return fail(1, m);
+    if (m.find("HIP error") != std::string::npos) return fail(FRK_ERR_HIP, m);
+    if (m.find("undefined") != std::string::npos) return fail(FRK_ERR_UNSUPPORTED, m);
+    return fail(FRK_ERR_ARG, m);
+  } catch (const std::exception& e) {
+    return fail(FRK_ERR_ARG, e.what());
+  }
+}
+
+template <class T>
+BlrMatrix<T>& mat(frk_blr a);
+template <>
+BlrMatrix<double>& mat<double>(frk_blr a) {
+  return *a->a64;
+}
+template <>
+BlrMatrix<float>& mat<float>(frk_blr a) {
+  return *a->a32;
+}
+
+void check_tile(frk_blr a, int i, int j) {
+  int nr, nc;
+  if (a->a64) {
+    nr = a->a64->nb_r();
+    nc = a->a64->nb_c();
+  } else {
+    nr = a->a32->nb_r();
+    nc = a->a32->nb_c();
+  }
+  if (i < 0 || j < 0 || i >= nr || j >= nc) throw std::invalid_argument("tile index out of range");
+}
+
+template <class T>
+void to_dev(T* dst, const double* src, size_t n, hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    FRK_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s));
+    FRK_HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    std::vector<float> tmp(src, src + n);
+    FRK_HIP_CHECK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
+    FRK_HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+template <class T>
+void from_dev(double* dst, const T* src, size_t n, hipStream_t s) {
+  if constexpr (sizeof(T) == 8) {
+    FRK_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    FRK_HIP_CHECK(hipStreamSynchronize(s));
+  } else {
+    std::vector<float> tmp(n);
+    FRK_HIP_CHECK(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, s));
+    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < n; ++k) dst[k] = tmp[k];
+  }
+}
+
+template <class T>
+int get_dense_t(frk_blr a, int i, int j, double* host) {
+  auto& A = mat<T>(a);
+  const int t = A.idx(i, j);
+  if (A.kind[t] != kDense) throw std::invalid_argument("tile is not dense");
+  from_dev<T>(host, A.D[t], (size_t)A.rows(i) * A.cols(j), a->owner->stream);
+  return FRK_OK;
+}
+
+template <class T>
+int set_dense_t(frk_blr a, int i, int j, const double* host) {
+  auto& A = mat<T>(a);
+  const int t = A.idx(i, j);
+  if (A.kind[t] != kDense) throw std::invalid_argument("tile is not dense");
+  to_dev<T>(A.D[t], host, (size_t)A.rows(i) * A.cols(j), a->owner->stream);
+  return FRK_OK;
+}
+
+template <class T>
+int get_lowrank_t(frk_blr a, int i, int j, double* U, double* S, double* V) {
+  auto& A = mat<T>(a);
+  const hipStream_t s = a->owner->stream;
+  A.pull_ranks(s);
+  const int t = A.idx(i, j);
+  if (A.kind[t] != kLowRank) throw std::invalid_argument("tile is not low-rank");
+  const int r = A.rank[t];
+  const int64_t m = A.rows(i), n = A.cols(j);
+  std::vector<double> u((size_t)m * A.kcap), sv((size_t)A.kcap * A.kcap), v((size_t)A.kcap * n);
+  from_dev<T>(u.data(), A.U[t], u.size(), s);
+  from_dev<T>(sv.data(), A.S[t], sv.size(), s);
+  from_dev<T>(v.data(), A.V[t], v.size(), s);
+  for (int64_t row = 0; row < m; ++row)
+    for (int c = 0; c < r; ++c) U[row * r + c] = u[row + (size_t)c * m];  // column-major slot
+  for (int p = 0; p < r; ++p)
+    for (int q = 0; q < r; ++q) S[p * r + q] = sv[(size_t)p * A.kcap + q];
+  for (int p = 0; p < r; ++p) std::memcpy(V + (size_t)p * n, v.data() + (size_t)p * n, sizeof(double) * n);
+  return FRK_OK;
+}
+
+template <class T>
+int set_lowrank_t(frk_blr a, int i, int j, int r, const double* U, const double* S, const double* V) {
+  auto& A = mat<T>(a);
+  const hipStream_t s = a->owner->stream;
+  const int t = A.idx(i, j);
+  if (A.kind[t] != kLowRank) throw std::invalid_argument("tile is not low-rank");
+  if (r < 0 || r > A.kcap) throw std::runtime_error("rank exceeds the low-rank capacity");
+  const int64_t m = A.rows(i), n = A.cols(j);
+  std::vector<double> u((size_t)m * A.kcap, 0.0), sv((size_t)A.kcap * A.kcap, 0.0), v((size_t)A.kcap * n, 0.0);
+  for (int64_t row = 0; row < m; ++row)
+    for (int c = 0; c < r; ++c) u[row + (size_t)c * m] = U[row * r + c];
+  for (int p = 0; p < r; ++p)
+    for (int q = 0; q < r; ++q) sv[(size_t)p * A.kcap + q] = S[p * r + q];
+  for (int p = 0; p < r; ++p) std::memcpy(v.data() + (size_t)p * n, V + (size_t)p * n, sizeof(double) * n);
+  to_dev<T>(A.U[t], u.data(), u.size(), s);
+  to_dev<T>(A.S[t], sv.data(), sv.size(), s);
+  to_dev<T>(A.V[t], v.data(), v.size(), s);
+  A.pull_ranks(s);
+  A.rank[t] = r;
+  A.push_ranks(s);
+  return FRK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* frk_last_error(void) { return g_err.c_str(); }
+
+int frk_version(void) { return 1; }
+
+int frk_ctx_create(frk_ctx* out) {
+  return guard([&] {
+    if (!out) throw std::invalid_argument("out is null");
+    auto c = std::make_unique<frk_ctx_s>();
+    FRK_HIP_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->ctx = std::make_unique<Context>(c->stream);
+    *out = c.release();
+    return FRK_OK;
+  });
+}
+
+int frk_ctx_destroy(frk_ctx ctx) {
+  return guard([&] {
+    if (!ctx) return FRK_OK;
+    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ctx->ctx.reset();
+    FRK_HIP_CHECK(hipStreamDestroy(ctx->stream));
+    delete ctx;
+    return FRK_OK;
+  });
+}
+
+int frk_ctx_synchronize(frk_ctx ctx) {
+  return guard([&] {
+    if (!ctx) throw std::invalid_argument("ctx is null");
+    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return FRK_OK;
+  });
+}
+
+int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
+                          double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
+                          int precision, frk_blr* out) {
+  return guard([&] {
+    if (!ctx || !out || !pts) throw std::invalid_argument("null argument");
+    if (dim < 1 || dim > 3) throw std::invalid_argument("dim must be 1, 2 or 3");
+    if (nb < 1 || n < nb) throw std::invalid_argument("bad block count");
+    if ((n + nb - 1) / nb > 512) throw std::invalid_argument("tile size above 512");
+    if (eps == 0.0 && rank <= 0) throw std::invalid_argument("need eps != 0 or rank > 0");
+    if (kcap <= 0) kcap = (eps == 0.0) ? rank : 64;
+    if (eps == 0.0 && rank > kcap) throw std::invalid_argument("rank exceeds kcap");
+    KernelDesc kd{kernel_kind, dim, n, pts, ell, shift};
+    auto h = std::make_unique<frk_blr_s>();
+    h->owner = ctx;
+    h->precision = precision;
+    if (precision == FRK_PREC_F64)
+      h->a64 = build_blr_kernel<double>(kd, nb, eps, rank, admis, admis_type, kcap, *ctx->ctx);
+    else if (precision == FRK_PREC_F32)
+      h->a32 = build_blr_kernel<float>(kd, nb, eps, rank, admis, admis_type, kcap, *ctx->ctx);
+    else
+      throw std::invalid_argument("unknown precision");
+    h->ks = upload_kernel(kd, h->pts, ctx->stream);
+    h->has_kernel = true;
+    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = h.release();
+    return FRK_OK;
+  });
+}
+
+int frk_blr_create(frk_ctx ctx, int nb_r, int nb_c, const int64_t* roff, const int64_t* coff, const int32_t* kinds,
+                   int kcap, double eps, int rank, int precision, frk_blr* out) {
+  return guard([&] {
+    if (!ctx || !out || !roff || !coff || !kinds) throw std::invalid_argument("null argument");
+    if (nb_r < 1 || nb_c < 1) throw std::invalid_argument("bad block counts");
+    std::vector<int64_t> ro(roff, roff + nb_r + 1), co(coff, coff + nb_c + 1);
+    for (int i = 0; i < nb_r; ++i)
+      if (ro[i + 1] - ro[i] <= 0 || ro[i + 1] - ro[i] > 512) throw std::invalid_argument("bad row block size");
+    for (int j = 0; j < nb_c; ++j)
+      if (co[j + 1] - co[j] <= 0 || co[j + 1] - co[j] > 512) throw std::invalid_argument("bad column block size");
+    std::vector<int32_t> kd(kinds, kinds + (size_t)nb_r * nb_c);
+    if (kcap <= 0) throw std::invalid_argument("kcap must be positive");
+    auto h = std::make_unique<frk_blr_s>();
+    h->owner = ctx;
+    h->precision = precision;
+    if (precision == FRK_PREC_F64)
+      h->a64 = std::make_unique<BlrMatrix<double>>(ro, co, kd, kcap, eps, rank, *ctx->ctx);
+    else if (precision == FRK_PREC_F32)
+      h->a32 = std::make_unique<BlrMatrix<float>>(ro, co, kd, kcap, eps, rank, *ctx->ctx);
+    else
+      throw std::invalid_argument("unknown precision");
+    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = h.release();
+    return FRK_OK;
+  });
+}
+
+int frk_blr_destroy(frk_blr a) {
+  return guard([&] {
+    if (!a) return FRK_OK;
+    FRK_HIP_CHECK(hipStreamSynchronize(a->owner->stream));
+    delete a;
+    return FRK_OK;
+  });
+}
+
+int frk_blr_set_lra(frk_blr a, int lra) {
+  return guard([&] {
+    if (!a) throw std::invalid_argument("null matrix");
+    if (lra == FRK_LRA_NAIVE) throw std::runtime_error("operator+=(LowRank, LowRank) naive addition undefined! (GPU)");
+    if (lra != FRK_LRA_ROUNDED && lra != FRK_LRA_FAST) throw std::invalid_argument("unknown FRANK_LRA value");
+    if (a->a64) a->a64->lra = lra;
+    if (a->a32) a->a32->lra = lra;
+    return FRK_OK;
+  });
+}
+
+int frk_blr_shape(frk_blr a, int* nb_r, int* nb_c) {
+  return guard([&] {
+    if (!a || !nb_r || !nb_c) throw std::invalid_argument("null argument");
+    *nb_r = a->a64 ? a->a64->nb_r() : a->a32->nb_r();
+    *nb_c = a->a64 ? a->a64->nb_c() : a->a32->nb_c();
+    return FRK_OK;
+  });
+}
+
+int frk_blr_tile_info(frk_blr a, int i, int j, int32_t* kind, int32_t* rank, int64_t* rows, int64_t* cols) {
+  return guard([&] {
+    if (!a) throw std::invalid_argument("null matrix");
+    check_tile(a, i, j);
+    auto fill = [&](auto& A) {
+      A.pull_ranks(a->owner->stream);
+      const int t = A.idx(i, j);
+      if (kind) *kind = A.kind[t];
+      if (rank) *rank = A.rank[t];
+      if (rows) *rows = A.rows(i);
+      if (cols) *cols = A.cols(j);
+    };
+    if (a->a64)
+      fill(*a->a64);
+    else
+      fill(*a->a32);
+    return FRK_OK;
+  });
+}
+
+int frk_blr_get_dense(frk_blr a, int i, int j, double* host) {
+  return guard([&] {
+    if (!a || !host) throw std::invalid_argument("null argument");
+    check_tile(a, i, j);
+    return a->a64 ? get_dense_t<double>(a, i, j, host) : get_dense_t<float>(a, i, j, host);
+  });
+}
+
+int frk_blr_set_dense(frk_blr a, int i, int j, const double* host) {
+  return guard([&] {
+    if (!a || !host) throw std::invalid_argument("null argument");
+    check_tile(a, i, j);
+    return a->a64 ? set_dense_t<double>(a, i, j, host) : set_dense_t<float>(a, i, j, host);
+  });
+}
+
+int frk_blr_get_lowrank(frk_blr a, int i, int j, double* U, double* S, double* V) {
+  return guard([&] {
+    if (!a || !U || !S || !V) throw std::invalid_argument("null argument");
+    check_tile(a, i, j);
+    return a->a64 ? get_lowrank_t<double>(a, i, j, U, S, V) : get_lowrank_t<float>(a, i, j, U, S, V);
+  });
+}
+
+int frk_blr_set_lowrank(frk_blr a, int i, int j, int rank, const double* U, const double* S, const double* V) {
+  return guard([&] {
+    if (!a || !U || !S || !V) throw std::invalid_argument("null argument");
+    check_tile(a, i, j);
+    return a->a64 ? set_lowrank_t<double>(a, i, j, rank, U, S, V) : set_lowrank_t<float>(a, i, j, rank, U, S, V);
+  });
+}
+
+int frk_blr_getrf(frk_blr a) {
+  return guard([&] {
+    if (!a) throw std::invalid_argument("null matrix");
+    if (a->a64)
+      blr_getrf<double>(*a->a64, *a->owner->ctx);
+    else
+      blr_getrf<float>(*a->a32, *a->owner->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_blr_solve(frk_blr f, double* b_dev, int nrhs, int trans) {
+  return guard([&] {
+    if (!f || !b_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
+    if (!f->a64) throw std::runtime_error("frk_blr_solve(FP32 factors, FP64 vector) undefined!");
+    blr_solve<double>(*f->a64, b_dev, nrhs, trans != 0, *f->owner->ctx);
+    FRK_HIP_CHECK(hipStreamSynchronize(f->owner->stream));
+    return FRK_OK;
+  });
+}
+
+int frk_blr_matvec(frk_blr a, const double* x_dev, double* y_dev, int nrhs) {
+  return guard([&] {
+    if (!a || !x_dev || !y_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
+    if (!a->a64) throw std::runtime_error("frk_blr_matvec(FP32 tiles, FP64 vector) undefined!");
+    blr_matvec<double>(*a->a64, x_dev, y_dev, nrhs, *a->owner->ctx);
+    FRK_HIP_CHECK(hipStreamSynchronize(a->owner->stream));
+    return FRK_OK;
+  });
+}
+
+int frk_blr_kernel_residual(frk_blr a, const double* x_dev, const double* b_dev, double* r_dev, int nrhs) {
+  return guard([&] {
+    if (!a || !x_dev || !b_dev || !r_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
+    if (!a->has_kernel) throw std::invalid_argument("matrix was not built from a kernel");
+    const int64_t n = a->a64 ? a->a64->roff.back() : a->a32->roff.back();
+    launch_kernel_residual(a->ks, x_dev, b_dev, r_dev, n, nrhs, a->owner->stream);
+    FRK_HIP_CHECK(hipStreamSynchronize(a->owner->stream));
+    return FRK_OK;
+  });
+}
+
+int frk_dgetrf_nopiv_batched(frk_ctx ctx, double* const* tiles_host, int batch, int n, int64_t ld,
+                             int32_t* info_host) {
+  return guard([&] {
+    if (!ctx || !tiles_host || batch <= 0 || n <= 0 || n > 512 || ld < n) throw std::invalid_argument("bad argument");
+    Context& c = *ctx->ctx;
+    c.sync();
+    std::vector<double*> tiles(tiles_host, tiles_host + batch);
+    const int npan = (n + 63) / 64;
+    DevBuf info(sizeof(int32_t) * batch * npan);
+    FRK_HIP_CHECK(hipMemsetAsync(info.as<int32_t>(), 0, sizeof(int32_t) * batch * npan, ctx->stream));
+    getrf_tiles<double>(tiles, n, ld, info.as<int32_t>(), c);
+    std::vector<int32_t> h((size_t)batch * npan);
+    FRK_HIP_CHECK(hipMemcpyAsync(h.data(), info.as<int32_t>(), sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    c.sync();
+    int status = FRK_OK;
+    for (int b = 0; b < batch; ++b) {
+      int first = 0;
+      for (int p = 0; p < npan && first == 0; ++p)
+        if (h[(size_t)p * batch + b] != 0) first = p * 64 + h[(size_t)p * batch + b];
+      if (info_host) info_host[b] = first;
+      if (first != 0 && status == FRK_OK) status = 1;
+    }
+    if (status != FRK_OK) g_err = "zero pivot in dense getrf";
+    return status;
+  });
+}
+
+int frk_dtrsm(frk_ctx ctx, int side, int lower, int trans, int m, int n, const double* A, int64_t lda, double* B,
+              int64_t ldb) {
+  return guard([&] {
+    if (!ctx || !A || !B || m <= 0 || n <= 0) throw std::invalid_argument("bad argument");
+    Context& c = *ctx->ctx;
+    c.sync();
+    TrsmItem x{};
+    const bool unit = lower != 0;  // trsm.cpp:92-104: Lower => Unit, Upper => NonUnit
+    // op(A) is lower-triangular iff (lower XOR trans)
+    const bool op_lower = (lower != 0) != (trans != 0);
+    x.unit = unit ? 1 : 0;
+    if (side == 0) {  // op(A) X = B, A m x m
+      if (m > 512) throw std::invalid_argument("triangle above 512");
+      x.A = A;
+      x.a_rs = trans ? 1 : lda;
+      x.a_cs = trans ? lda : 1;
+      x.B = B;
+      x.b_rs = ldb;
+      x.b_cs = 1;
+      x.n = m;
+      x.nrhs = n;
+      x.lower = op_lower ? 1 : 0;
+    } else {  // X op(A) = B  <=>  op(A)^T X^T = B^T, A n x n
+      if (n > 512) throw std::invalid_argument("triangle above 512");
+      x.A = A;
+      x.a_rs = trans ? lda : 1;
+      x.a_cs = trans ? 1 : lda;
+      x.B = B;
+      x.b_rs = 1;
+      x.b_cs = ldb;
+      x.n = n;
+      x.nrhs = m;
+      x.lower = op_lower ? 0 : 1;
+    }
+    launch_trsm<double>(c.staging().push(std::vector<TrsmItem>{x}, ctx->stream), 1, x.n, x.nrhs, ctx->stream);
+    c.sync();
+    return FRK_OK;
+  });
+}
+
+int frk_dgemm(frk_ctx ctx, int trans_a, int trans_b, int m, int n, int k, double alpha, const double* A,
+              int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc) {
+  return guard([&] {
+    if (!ctx || !C || m < 0 || n < 0 || k < 0) throw std::invalid_argument("bad argument");
+    Context& c = *ctx->ctx;
+    c.sync();
+    GemmItem g{};
+    g.A = A;
+    g.B = B;
+    g.C = C;
+    g.a_rs = trans_a ? 1 : lda;
+    g.a_cs = trans_a ? lda : 1;
+    g.b_rs = trans_b ? 1 : ldb;
+    g.b_cs = trans_b ? ldb : 1;
+    g.c_rs = ldc;
+    g.c_cs = 1;
+    g.m = m;
+    g.n = n;
+    g.k = k;
+    g.alpha = alpha;
+    g.beta = beta;
+    launch_gemm<double>(c.staging().push(std::vector<GemmItem>{g}, ctx->stream), 1, m, n, ctx->stream);
+    c.sync();
+    return FRK_OK;
+  });
+}
+
+int frk_dlr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const int32_t* n, double eps, int cap,
+                        double* const* Ua, double* const* Sa, double* const* Va, int32_t* ranks_host,
+                        const double* const* Ub, const double* const* Sb, const double* const* Vb,
+                        const int32_t* kb) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || !m || !n || !Ua || !Sa || !Va || !ranks_host || !Ub || !Sb || !Vb || !kb)
+      throw std::invalid_argument("null argument");
+    if (lra == FRK_LRA_NAIVE) throw std::runtime_error("operator+=(LowRank, LowRank) naive addition undefined! (GPU)");
+    Context& c = *ctx->ctx;
+    c.sync();
+    DevBuf dr(sizeof(int32_t) * batch);
+    FRK_HIP_CHECK(hipMemcpyAsync(dr.as<int32_t>(), ranks_host, sizeof(int32_t) * batch, hipMemcpyHostToDevice,
+                                 ctx->stream));
+    std::vector<LrAddDesc<double>> d(batch);
+    size_t need = 4096;
+    int kmax = 0, mmax = 0, nmax = 0;
+    for (int b = 0; b < batch; ++b) {
+      if (m[b] <= 0 || n[b] <= 0 || m[b] > 512 || n[b] > 512) throw std::invalid_argument("tile dims must be in [1, 512]");
+      if (ranks_host[b] <= 0 || ranks_host[b] > cap || kb[b] <= 0) throw std::invalid_argument("bad ranks");
+      kmax = std::max(kmax, ranks_host[b] + kb[b]);
+      mmax = std::max(mmax, m[b]);
+      nmax = std::max(nmax, n[b]);
+    }
+    need += (size_t)batch * sizeof(double) * ((size_t)(mmax + nmax) * kmax + 6 * (size_t)kmax * kmax + 2 * kmax + 512);
+    c.reserve(need);
+    for (int b = 0; b < batch; ++b) {
+      LrAddDesc<double> x{};
+      x.cu = Ua[b];
+      x.cs = Sa[b];
+      x.cv = Va[b];
+      x.d_rank = dr.as<int32_t>() + b;
+      x.kc = ranks_host[b];
+      x.cu_rs = cap;
+      x.cu_cs = 1;
+      x.cv_rs = n[b];
+      x.cv_cs = 1;
+      x.lds = cap;
+      x.m = m[b];
+      x.n = n[b];
+      x.eps = eps;
+      x.cap = cap;
+      x.xu = Ub[b];
+      x.xu_rs = kb[b];
+      x.xu_cs = 1;
+      x.kx1 = kb[b];
+      x.xs = Sb[b];
+      x.xs_ld = kb[b];
+      x.kx2 = kb[b];
+      x.xv = Vb[b];
+      x.xv_rs = n[b];
+      x.xv_cs = 1;
+      d[b] = x;
+    }
+    lr_add_batched<double>(d, lra, c);
+    FRK_HIP_CHECK(hipMemcpyAsync(ranks_host, dr.as<int32_t>(), sizeof(int32_t) * batch, hipMemcpyDeviceToHost,
+                                 ctx->stream));
+    c.sync();
+    for (int b = 0; b < batch; ++b)
+      if (ranks_host[b] < 0) throw std::runtime_error("low-rank capacity exceeded in addition");
+    return FRK_OK;
+  });
+}
+
+int frk_dcompress(frk_ctx ctx, int m, int n, double* A, int64_t lda, double eps, int rank, int cap, double* U,
+                  double* S, double* V, int32_t* rank_out) {
+  return guard([&] {
+    if (!ctx || !A || !U || !S || !V || !rank_out) throw std::invalid_argument("null argument");
+    if (m <= 0 || n <= 0 || m > 512 || n > 512) throw std::invalid_argument("tile dims must be in [1, 512]");
+    if (eps == 0.0 && (rank <= 0 || rank > cap)) throw std::invalid_argument("bad rank");
+    Context& c = *ctx->ctx;
+    c.sync();
+    DevBuf dr(sizeof(int32_t));
+    CompressDesc<double> d{};
+    d.a = A;
+    d.m = m;
+    d.n = n;
+    d.lda = lda;
+    d.eps = eps;
+    d.rank = rank;
+    d.u = U;
+    d.s = S;
+    d.v = V;
+    d.u_rs = cap;
+    d.u_cs = 1;
+    d.v_rs = n;
+    d.v_cs = 1;
+    d.lds = cap;
+    d.d_rank = dr.as<int32_t>();
+    d.cap = cap;
+    compress_batched<double>({d}, c);
+    FRK_HIP_CHECK(hipMemcpy(rank_out, dr.as<int32_t>(), sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (*rank_out < 0) throw std::runtime_error("low-rank capacity exceeded in compression");
+    return FRK_OK;
+  });
+}
+
+}  // extern "C"

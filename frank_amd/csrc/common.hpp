@@ -1,0 +1,140 @@
+// Shared device/host definitions for the gfx950 BLR kernels.
+//
+// Matrices are addressed through strided views: element (i, j) of a view
+// lives at p[i * rs + j * cs].  Row-major FRANK Dense tiles use (rs = ld,
+// cs = 1); the engine's low-rank bases use column-major slots (rs = 1,
+// cs = ld) so every basis vector is one contiguous, coalesced stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace frk {
+
+#define FRK_HIP_CHECK(expr)                                                        \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(_e) + \
+                               " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+    }                                                                              \
+  } while (0)
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Batched GEMM item: C = alpha * A * B + beta * C, A m x k, B k x n.
+// ---------------------------------------------------------------------------
+struct GemmItem {
+  const void* A;
+  const void* B;
+  void* C;
+  int64_t a_rs, a_cs, b_rs, b_cs, c_rs, c_cs;
+  int32_t m, n, k, pad;
+  double alpha, beta;
+};
+
+// ---------------------------------------------------------------------------
+// Batched triangular solve item (left side, op(A) already folded into the
+// strides): solve T X = B in place, T n x n, B n x nrhs.
+// lower: forward substitution; upper: backward substitution.
+// ---------------------------------------------------------------------------
+struct TrsmItem {
+  const void* A;
+  void* B;
+  int64_t a_rs, a_cs, b_rs, b_cs;
+  int32_t n, nrhs;
+  int32_t lower, unit;
+};
+
+// ---------------------------------------------------------------------------
+// Householder QR item.  The m x K input is the column concatenation
+// [A1 * S1 | A2 * S2] formed on load (S == nullptr means identity).  Optional
+// projection: before factorisation every column x is replaced by
+// x - P (P^T x), with P^T x written to Z (K2 x kp row-major per column... see
+// qr kernel).  Outputs: reflectors Y (column-major, m x K, ld = ldy; unit
+// diagonal implicit), tau[K], R (K x K row-major, ld = ldr).
+// ---------------------------------------------------------------------------
+struct QrItem {
+  const void* A1;
+  const void* S1;
+  const void* A2;
+  const void* S2;
+  const void* P;  // projection basis (m x kp), may be nullptr
+  void* Y;
+  void* tau;
+  void* R;
+  void* Z;        // projection coefficients, row-major K x kp (ldz = kp)
+  int64_t a1_rs, a1_cs, a2_rs, a2_cs, p_rs, p_cs;
+  int32_t s1_ld, s2_ld;
+  int32_t m;
+  int32_t k1a, k1;  // A1 has k1a columns, S1 is k1a x k1
+  int32_t k2a, k2;  // A2 has k2a columns, S2 is k2a x k2
+  int32_t kp;
+  int32_t ldy, ldr;
+  int32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// Apply the Householder product Q = H_0 ... H_{nref-1} (or Q^T) to the
+// zero-padded m x ncol matrix [X; 0] (X is kx x ncol), optionally adding
+// Base (m x kb) * Xb (kb x ncol).  Result written through a strided view.
+// ---------------------------------------------------------------------------
+struct ApplyQItem {
+  const void* Y;     // reflector j is column j of the strided view (y_rs, y_cs)
+  const void* tau;
+  const void* X;
+  const void* Base;
+  const void* Xb;
+  void* out;
+  int64_t y_rs, y_cs, x_rs, x_cs, base_rs, base_cs, xb_rs, xb_cs, out_rs, out_cs;
+  const int32_t* ncol_ptr;  // when set, the column count is read on the device
+  int32_t m, nref, kx, ncol, kb;
+  int32_t trans;
+};
+
+// ---------------------------------------------------------------------------
+// SVD + truncation item.  The K x K core M is built in the kernel:
+//   mode 0: M given (row-major, ld = ldm)
+//   mode 1: M = Ru * Rv^T               (rounded addition)
+//   mode 2: fast rounded addition block core
+//           [[Sa + Zu Zv^T, Zu Rv^T], [Ru Zv^T, Ru Rv^T]] with ka = k_a
+// Outputs: rank (fixed when fixed_rank >= 0, else the H2 rule at eps), the
+// r x r diagonal S (ld = lds), left vectors Um (K x r, row-major, ld = ldu),
+// right vectors Vm (K x r, ld = ldv).
+// ---------------------------------------------------------------------------
+struct SvdItem {
+  const void* M;
+  const void* Ru;
+  const void* Rv;
+  const void* Sa;
+  const void* Zu;
+  const void* Zv;
+  void* S;
+  void* Um;
+  void* Vm;
+  int32_t* rank_out;
+  int64_t m_rs, m_cs;  // mode 0 input strides
+  int32_t K, ldr, lds_in, kdim;  // kdim > 0: min-dimension used by the truncation rule
+  int32_t ka, kx, fixed_rank, mode;
+  int32_t lds, ldu, ldv, cap;
+  double eps;
+};
+
+// ---------------------------------------------------------------------------
+// Truncated column-pivoted QR item (construction).  A (m x n, row-major,
+// ld = lda) is overwritten.  Outputs rank, tau, pivot order.
+// ---------------------------------------------------------------------------
+struct Geqp3Item {
+  void* A;
+  void* tau;
+  int32_t* piv;
+  int32_t* rank_out;
+  int32_t m, n, lda, cap;
+  double eps;
+};
+
+}  // namespace frk

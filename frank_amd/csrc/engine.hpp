@@ -1,0 +1,221 @@
+// BLR engine: tile storage resident in HBM and the batched BLR-LU / solve /
+// construction drivers built on the kernels in kernels.hip.
+//
+// Storage (per tile (i, j), rows m = roff[i+1]-roff[i], cols n):
+//   dense    : m x n row-major, ld = n
+//   low-rank : fixed-capacity slots (capacity kcap, ranks in a device table)
+//              U  column-major m x kcap (ld = m)  -> every basis vector contiguous
+//              S  kcap x kcap row-major (ld = kcap)
+//              V  kcap x n row-major (ld = n)     -> every basis vector contiguous
+// Updates rewrite the slots in place, so a factorization never reallocates.
+#pragma once
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace frk {
+
+class DevBuf {
+ public:
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) { ensure(bytes); }
+  ~DevBuf();
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), bytes_(o.bytes_) {
+    o.p_ = nullptr;
+    o.bytes_ = 0;
+  }
+  void ensure(size_t bytes);  // grows; contents are not preserved
+  template <class X>
+  X* as() const {
+    return static_cast<X*>(p_);
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  void* p_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+// Stream-ordered upload of small descriptor arrays through a pinned ring.
+class Staging {
+ public:
+  Staging() = default;
+  ~Staging();
+  template <class Item>
+  const Item* push(const std::vector<Item>& v, hipStream_t s) {
+    return static_cast<const Item*>(push_bytes(v.data(), v.size() * sizeof(Item), s));
+  }
+  const void* push_bytes(const void* src, size_t bytes, hipStream_t s);
+  void reset() { off_ = 0; }
+
+ private:
+  char* h_ = nullptr;
+  char* d_ = nullptr;
+  size_t cap_ = 0, off_ = 0;
+};
+
+// Execution context: stream, staging ring, workspace arena, constants.
+class Context {
+ public:
+  explicit Context(hipStream_t s);
+  ~Context();
+  hipStream_t stream() const { return s_; }
+  Staging& staging() { return stage_; }
+  // Workspace: reserve() after a sync, then bump-allocate with take().
+  void reserve(size_t bytes);
+  template <class X>
+  X* take(size_t n) {
+    return static_cast<X*>(take_bytes(n * sizeof(X)));
+  }
+  void* take_bytes(size_t bytes);
+  void reset_ws() { ws_off_ = 0; }
+  void sync();  // drain the stream, reset staging and workspace
+  // identity matrix (ident_n x ident_n) in both precisions
+  template <class X>
+  const X* identity(int n);
+  int ident_ld() const { return ident_n_; }
+  // rsvd sample matrix Omega = random_uniform(n, p) (mt19937 seed 0)
+  template <class X>
+  const X* omega(int n, int p);
+  int32_t* info() { return info_.as<int32_t>(); }
+
+ private:
+  hipStream_t s_;
+  Staging stage_;
+  DevBuf ws_;
+  size_t ws_off_ = 0;
+  DevBuf ident64_, ident32_;
+  int ident_n_ = 0;
+  DevBuf omega64_, omega32_;
+  int omega_n_ = 0, omega_p_ = 0;
+  DevBuf info_;
+};
+
+enum TileKind : int32_t { kDense = 0, kLowRank = 1, kEmpty = 2 };
+enum Lra : int32_t { kLraNaive = 0, kLraRounded = 1, kLraFast = 2 };
+
+template <typename T>
+class BlrMatrix {
+ public:
+  BlrMatrix(std::vector<int64_t> roff, std::vector<int64_t> coff, std::vector<int32_t> kinds, int kcap,
+            double eps, int fixed_rank, Context& ctx);
+  int nb_r() const { return (int)roff.size() - 1; }
+  int nb_c() const { return (int)coff.size() - 1; }
+  int64_t rows(int i) const { return roff[i + 1] - roff[i]; }
+  int64_t cols(int j) const { return coff[j + 1] - coff[j]; }
+  int idx(int i, int j) const { return i * nb_c() + j; }
+  int32_t* d_rank(int i, int j) const { return d_ranks_.as<int32_t>() + idx(i, j); }
+  void pull_ranks(hipStream_t s);  // synchronous device -> host copy of the rank table
+  void push_ranks(hipStream_t s);
+
+  std::vector<int64_t> roff, coff;
+  std::vector<int32_t> kind;
+  std::vector<int32_t> rank;  // host mirror (valid after pull_ranks)
+  int kcap;
+  double eps;      // fixed accuracy when != 0
+  int fixed_rank;  // used when eps == 0
+  int lra = kLraFast;
+  std::vector<T*> D, U, S, V;  // per tile slots (nullptr when unused)
+
+ private:
+  DevBuf arena_;
+  DevBuf d_ranks_;
+};
+
+// ----------------------------------------------------------------------------
+// Batched building blocks (all asynchronous on ctx.stream()).
+// ----------------------------------------------------------------------------
+template <typename T>
+struct LrAddDesc {
+  // target C = U S V (slots, rank kc known on the host at launch time)
+  T* cu;
+  T* cs;
+  T* cv;
+  int32_t* d_rank;
+  int kc;
+  int64_t cu_rs, cu_cs, cv_rs, cv_cs;  // element (i, t) of U at cu[i*cu_rs + t*cu_cs]; (t, j) of V likewise
+  int lds;
+  int m, n;
+  double eps;  // != 0: fixed accuracy (H2 rule); 0: keep kc
+  int cap;
+  // addend X = Ux Sx Vx, Ux m x kx1 (strided), Sx kx1 x kx2 (nullptr = identity),
+  // Vx kx2 x n (strided)
+  const T* xu;
+  int64_t xu_rs, xu_cs;
+  int kx1;
+  const T* xs;
+  int xs_ld;
+  int kx2;
+  const T* xv;
+  int64_t xv_rs, xv_cs;
+};
+
+// C += X for every descriptor (reference: rounded_addition / fast_rounded_addition).
+template <typename T>
+void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx);
+
+template <typename T>
+struct CompressDesc {
+  T* a;  // dense m x n row-major (overwritten), ld = lda
+  int m, n;
+  int64_t lda;
+  double eps;  // != 0: truncated_geqp3 + RQ; 0: rsvd at fixed rank
+  int rank;
+  T* u;
+  T* s;
+  T* v;
+  int64_t u_rs, u_cs, v_rs, v_cs;  // strided outputs: U (m x r), V (r x n)
+  int lds;
+  int32_t* d_rank;
+  int cap;
+};
+
+// LowRank(Dense, eps) / LowRank(Dense, rank) for every descriptor.  Drains
+// the stream (ranks are needed on the host to size the follow-up launches).
+template <typename T>
+void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx);
+
+// No-pivot LU of square row-major tiles (n <= 512) in place; blocked with
+// 64-wide panels.  info_dev[t] receives 1-based first zero pivot (0 if none).
+template <typename T>
+void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_dev, Context& ctx);
+
+// ----------------------------------------------------------------------------
+// BLR drivers.
+// ----------------------------------------------------------------------------
+struct KernelDesc {
+  int kind;  // 0 laplace, 1 gaussian
+  int dim;
+  int64_t n;
+  const double* host_pts;  // dim x n, structure of arrays
+  double ell, shift;
+};
+
+// Hierarchical(kernel, ..., eps | rank, admis, nb, nb) for a one-level tree.
+template <typename T>
+std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, double eps, int rank,
+                                               double admis, int admis_type, int kcap, Context& ctx);
+
+// getrf(Hierarchical&): right-looking tile LU without pivoting, in place
+// (lower tiles become L, upper tiles U, diagonal tiles packed L\U).
+template <typename T>
+void blr_getrf(BlrMatrix<T>& A, Context& ctx);
+
+// Forward/backward block substitution on device vectors b (N x nrhs row-major).
+// trans = false: L then U (solves A x = b); trans = true: U^T then L^T.
+template <typename T>
+void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx);
+
+// y += A x on an unfactored BLR matrix.
+template <typename T>
+void blr_matvec(BlrMatrix<T>& A, const T* x, T* y, int nrhs, Context& ctx);
+
+// Device kernel spec with uploaded points (owned by the returned buffer).
+KernelSpec upload_kernel(const KernelDesc& kd, DevBuf& pts, hipStream_t s);
+
+}  // namespace frk

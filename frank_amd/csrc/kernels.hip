@@ -1,0 +1,1171 @@
+// gfx950 kernels for the BLR factorization hot path.
+//
+// Design notes (CDNA4, wave64):
+//  * Small dense factorizations (Householder QR, reflector application) keep
+//    the whole m x K operand in VGPRs: wave w owns columns c = q*W + w and
+//    lane l holds rows l + 64*t.  Column reductions are wave shuffles
+//    (xor butterfly, bitwise identical in every lane); the only workgroup
+//    barrier per Householder step publishes the reflector through LDS.
+//  * Jacobi SVD of the K x K core runs in LDS with a round-robin pairing so
+//    K/2 rotations proceed in parallel, one lane group per pair.
+//  * Column-pivoted QR streams the tile through L2 with one thread per column
+//    (row-major tiles => every row sweep is coalesced).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+
+#include "kernels.hpp"
+
+namespace frk {
+
+template <typename T>
+struct Eps;
+template <>
+struct Eps<double> {
+  static constexpr double v = 2.220446049250313080847e-16;
+  static constexpr double half = 1.1102230246251565404e-16;  // dlamch('e')
+};
+template <>
+struct Eps<float> {
+  static constexpr float v = 1.1920928955078125e-07f;
+  static constexpr float half = 5.9604644775390625e-08f;
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T group_sum(T v, int g) {
+  for (int off = g >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// ===========================================================================
+// Strided batched GEMM: 64 x 64 output tile per workgroup, 4 x 4 per thread.
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ items) {
+  const GemmItem it = items[blockIdx.y];
+  const int tiles_n = (it.n + 63) >> 6;
+  const int tiles_m = (it.m + 63) >> 6;
+  if ((int)blockIdx.x >= tiles_m * tiles_n) return;
+  const int row0 = (blockIdx.x / tiles_n) * 64;
+  const int col0 = (blockIdx.x % tiles_n) * 64;
+  __shared__ T As[16][65];
+  __shared__ T Bs[16][65];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+  const T* __restrict__ A = static_cast<const T*>(it.A);
+  const T* __restrict__ B = static_cast<const T*>(it.B);
+  const bool a_kfast = (it.a_cs == 1 && it.a_rs != 1);
+  const bool b_kfast = (it.b_rs == 1 && it.b_cs != 1);
+  for (int k0 = 0; k0 < it.k; k0 += 16) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + e * 256;
+      int i, kk;
+      if (a_kfast) {
+        kk = idx & 15;
+        i = idx >> 4;
+      } else {
+        i = idx & 63;
+        kk = idx >> 6;
+      }
+      const int gi = row0 + i, gk = k0 + kk;
+      As[kk][i] = (gi < it.m && gk < it.k) ? A[(int64_t)gi * it.a_rs + (int64_t)gk * it.a_cs] : T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + e * 256;
+      int j, kk;
+      if (b_kfast) {
+        kk = idx & 15;
+        j = idx >> 4;
+      } else {
+        j = idx & 63;
+        kk = idx >> 6;
+      }
+      const int gj = col0 + j, gk = k0 + kk;
+      Bs[kk][j] = (gj < it.n && gk < it.k) ? B[(int64_t)gk * it.b_rs + (int64_t)gj * it.b_cs] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      T av[4], bv[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) av[a] = As[kk][ty + 16 * a];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tx + 16 * b];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+    }
+    __syncthreads();
+  }
+  T* C = static_cast<T*>(it.C);
+  const T alpha = static_cast<T>(it.alpha), beta = static_cast<T>(it.beta);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = row0 + ty + 16 * a;
+    if (i >= it.m) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = col0 + tx + 16 * b;
+      if (j >= it.n) continue;
+      const int64_t idx = (int64_t)i * it.c_rs + (int64_t)j * it.c_cs;
+      T v = alpha * acc[a][b];
+      if (beta != T(0)) v += beta * C[idx];
+      C[idx] = v;
+    }
+  }
+}
+
+template <typename T>
+void launch_gemm(const GemmItem* d_items, int n_items, int max_m, int max_n, hipStream_t s) {
+  if (n_items <= 0 || max_m <= 0 || max_n <= 0) return;
+  dim3 grid(cdiv(max_m, 64) * cdiv(max_n, 64), n_items);
+  hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(256), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// Batched left triangular solve, in place.  One workgroup per (item, strip of
+// 16 right-hand sides); the strip lives in LDS, rows are solved in blocks of
+// 16: a dot-product update against solved rows, then the 16 x 16 diagonal
+// block by substitution.
+// ===========================================================================
+constexpr int kTrsmMaxN = 512;
+
+template <typename T>
+__global__ __launch_bounds__(256) void trsm_kernel(const TrsmItem* __restrict__ items) {
+  const TrsmItem it = items[blockIdx.y];
+  const int c0 = blockIdx.x * 16;
+  if (c0 >= it.nrhs) return;
+  const int nc = min(16, it.nrhs - c0);
+  const int n = it.n;
+  __shared__ T X[kTrsmMaxN * 17];
+  const T* __restrict__ A = static_cast<const T*>(it.A);
+  T* B = static_cast<T*>(it.B);
+  const int tid = threadIdx.x;
+  const bool rfast = (it.b_rs == 1);
+  for (int e = tid; e < n * 16; e += 256) {
+    int r, c;
+    if (rfast) {
+      r = e % n;
+      c = e / n;
+    } else {
+      r = e >> 4;
+      c = e & 15;
+    }
+    X[r * 17 + c] = (c < nc) ? B[(int64_t)r * it.b_rs + (int64_t)(c0 + c) * it.b_cs] : T(0);
+  }
+  __syncthreads();
+  const int c = tid & 15, g = tid >> 4;
+  const int nblk = (n + 15) >> 4;
+  const int64_t ars = it.a_rs, acs = it.a_cs;
+  for (int bb = 0; bb < nblk; ++bb) {
+    const int blk = it.lower ? bb : nblk - 1 - bb;
+    const int r0 = blk * 16, r1 = min(n, r0 + 16);
+    const int row = r0 + g;
+    if (row < r1 && c < nc) {
+      T s = X[row * 17 + c];
+      if (it.lower) {
+        for (int j = 0; j < r0; ++j) s -= A[row * ars + j * acs] * X[j * 17 + c];
+      } else {
+        for (int j = r1; j < n; ++j) s -= A[row * ars + j * acs] * X[j * 17 + c];
+      }
+      X[row * 17 + c] = s;
+    }
+    __syncthreads();
+    if (g == 0 && c < nc) {
+      if (it.lower) {
+        for (int r = r0; r < r1; ++r) {
+          T s = X[r * 17 + c];
+          for (int j = r0; j < r; ++j) s -= A[r * ars + j * acs] * X[j * 17 + c];
+          if (!it.unit) s /= A[r * ars + r * acs];
+          X[r * 17 + c] = s;
+        }
+      } else {
+        for (int r = r1 - 1; r >= r0; --r) {
+          T s = X[r * 17 + c];
+          for (int j = r + 1; j < r1; ++j) s -= A[r * ars + j * acs] * X[j * 17 + c];
+          if (!it.unit) s /= A[r * ars + r * acs];
+          X[r * 17 + c] = s;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * 16; e += 256) {
+    int r, cc;
+    if (rfast) {
+      r = e % n;
+      cc = e / n;
+    } else {
+      r = e >> 4;
+      cc = e & 15;
+    }
+    if (cc < nc) B[(int64_t)r * it.b_rs + (int64_t)(c0 + cc) * it.b_cs] = X[r * 17 + cc];
+  }
+}
+
+template <typename T>
+void launch_trsm(const TrsmItem* d_items, int n_items, int max_n, int max_nrhs, hipStream_t s) {
+  if (n_items <= 0 || max_nrhs <= 0 || max_n <= 0) return;
+  if (max_n > kTrsmMaxN) throw std::runtime_error("trsm: n exceeds 512");
+  dim3 grid(cdiv(max_nrhs, 16), n_items);
+  hipLaunchKernelGGL(trsm_kernel<T>, grid, dim3(256), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// Unblocked no-pivot LU of a <= 64 x 64 block: one wave, lane j owns column j.
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(64) void getrf_small_kernel(const GemmItem* __restrict__ items,
+                                                         int32_t* __restrict__ info) {
+  const GemmItem it = items[blockIdx.x];
+  T* A = static_cast<T*>(it.C);
+  const int n = it.m;
+  const int64_t rs = it.c_rs, cs = it.c_cs;
+  const int lane = threadIdx.x;
+  __shared__ T col[64];
+  T a[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) a[i] = (i < n && lane < n) ? A[i * rs + lane * cs] : T(0);
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    if (k < n) {
+      const T piv = __shfl(a[k], k, 64);
+      if (piv == T(0) && bad == 0) bad = k + 1;
+      if (lane == k) {
+#pragma unroll
+        for (int i = k + 1; i < 64; ++i) {
+          a[i] = a[i] / piv;
+          col[i] = a[i];
+        }
+      }
+      __syncthreads();
+      if (lane > k && lane < n) {
+        const T ukj = a[k];
+#pragma unroll
+        for (int i = k + 1; i < 64; ++i) a[i] -= col[i] * ukj;
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 64; ++i)
+    if (i < n && lane < n) A[i * rs + lane * cs] = a[i];
+  if (lane == 0 && info != nullptr) info[blockIdx.x] = bad;
+}
+
+template <typename T>
+void launch_getrf_small(const GemmItem* d_items, int n_items, int32_t* d_info, hipStream_t s) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(getrf_small_kernel<T>, dim3(n_items), dim3(64), 0, s, d_items, d_info);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// Householder QR of [A1 S1 | A2 S2] (m x K, m <= 64 * MR, K <= W * CPW).
+// Optional MGS projection against an orthonormal basis P first; its
+// coefficients occupy rows [0, kp) of the R buffer, R itself rows [kp, kp+K).
+// ===========================================================================
+template <typename T, int MR, int W, int CPW>
+__global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ items) {
+  const QrItem it = items[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int m = it.m;
+  const int K = it.k1 + it.k2;
+  const int nref = min(m, K);
+  __shared__ T vbuf[2][MR * 64];
+  __shared__ T taus[W * CPW];
+  T a[CPW][MR];
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+#pragma unroll
+    for (int t = 0; t < MR; ++t) a[q][t] = T(0);
+    const int c = q * W + w;
+    if (c < K) {
+      const bool first = c < it.k1;
+      const T* Ab = static_cast<const T*>(first ? it.A1 : it.A2);
+      const T* S = static_cast<const T*>(first ? it.S1 : it.S2);
+      const int64_t rs = first ? it.a1_rs : it.a2_rs;
+      const int64_t cs = first ? it.a1_cs : it.a2_cs;
+      const int sld = first ? it.s1_ld : it.s2_ld;
+      const int ka = first ? it.k1a : it.k2a;
+      const int cc = first ? c : c - it.k1;
+      if (S == nullptr) {
+#pragma unroll
+        for (int t = 0; t < MR; ++t) {
+          const int row = lane + 64 * t;
+          if (row < m) a[q][t] = Ab[row * rs + cc * cs];
+        }
+      } else {
+        for (int s = 0; s < ka; ++s) {
+          const T coef = S[s * sld + cc];
+#pragma unroll
+          for (int t = 0; t < MR; ++t) {
+            const int row = lane + 64 * t;
+            if (row < m) a[q][t] += Ab[row * rs + s * cs] * coef;
+          }
+        }
+      }
+    }
+  }
+  T* Rb = static_cast<T*>(it.R);
+  const int ldr = it.ldr;
+  if (it.P != nullptr && it.kp > 0) {
+    const T* P = static_cast<const T*>(it.P);
+    for (int s = 0; s < it.kp; ++s) {
+      T pv[MR];
+#pragma unroll
+      for (int t = 0; t < MR; ++t) {
+        const int row = lane + 64 * t;
+        pv[t] = (row < m) ? P[row * it.p_rs + s * it.p_cs] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) {
+        const int c = q * W + w;
+        if (c < K) {
+          T d = T(0);
+#pragma unroll
+          for (int t = 0; t < MR; ++t) d += pv[t] * a[q][t];
+          d = wave_sum(d);
+#pragma unroll
+          for (int t = 0; t < MR; ++t) a[q][t] -= d * pv[t];
+          if (lane == 0) Rb[s * ldr + c] = d;
+        }
+      }
+    }
+  }
+  const int kp = (it.P != nullptr) ? it.kp : 0;
+#pragma unroll
+  for (int qj = 0; qj < CPW; ++qj) {
+    for (int wj = 0; wj < W; ++wj) {
+      const int j = qj * W + wj;
+      if (j >= nref) break;
+      const int buf = j & 1;
+      if (w == wj) {
+        T x2 = T(0), alpha = T(0);
+#pragma unroll
+        for (int t = 0; t < MR; ++t) {
+          const int row = lane + 64 * t;
+          const T v = a[qj][t];
+          if (row > j && row < m) x2 += v * v;
+          if (row == j) alpha = v;
+        }
+        x2 = wave_sum(x2);
+        alpha = wave_sum(alpha);
+        T beta, tau, scal;
+        if (x2 == T(0)) {
+          tau = T(0);
+          beta = alpha;
+          scal = T(0);
+        } else {
+          const T nrm = sqrt(alpha * alpha + x2);
+          beta = (alpha >= T(0)) ? -nrm : nrm;
+          tau = (beta - alpha) / beta;
+          scal = T(1) / (alpha - beta);
+        }
+#pragma unroll
+        for (int t = 0; t < MR; ++t) {
+          const int row = lane + 64 * t;
+          if (row > j) a[qj][t] *= scal;
+          if (row == j) a[qj][t] = beta;
+          vbuf[buf][lane + 64 * t] = (row == j) ? T(1) : ((row > j && row < m) ? a[qj][t] : T(0));
+        }
+        if (lane == 0) taus[j] = tau;
+      }
+      __syncthreads();
+      const T tj = taus[j];
+      if (tj != T(0)) {
+        T v[MR];
+#pragma unroll
+        for (int t = 0; t < MR; ++t) v[t] = vbuf[buf][lane + 64 * t];
+#pragma unroll
+        for (int q = 0; q < CPW; ++q) {
+          const int c = q * W + w;
+          if (c > j && c < K) {
+            T d = T(0);
+#pragma unroll
+            for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
+            d = wave_sum(d) * tj;
+#pragma unroll
+            for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
+          }
+        }
+      }
+    }
+  }
+  T* Y = static_cast<T*>(it.Y);
+  T* tau_out = static_cast<T*>(it.tau);
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const int c = q * W + w;
+    if (c < K) {
+#pragma unroll
+      for (int t = 0; t < MR; ++t) {
+        const int row = lane + 64 * t;
+        if (row < m && row > c && c < nref) Y[(int64_t)row + (int64_t)c * it.ldy] = a[q][t];
+        if (row < nref)
+          Rb[(kp + row) * ldr + c] = (row <= c) ? a[q][t] : T(0);
+        else if (row < K)
+          Rb[(kp + row) * ldr + c] = T(0);
+      }
+      if (lane == 0 && c < nref) tau_out[c] = taus[c];
+    }
+  }
+}
+
+template <typename T>
+void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s) {
+  if (n_items <= 0) return;
+  if (max_k <= 16) {
+    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 4>), dim3(n_items), dim3(256), 0, s, d_items);
+  } else if (max_k <= 32) {
+    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 8>), dim3(n_items), dim3(256), 0, s, d_items);
+  } else if (max_k <= 64) {
+    hipLaunchKernelGGL((qr_kernel<T, 8, 8, 8>), dim3(n_items), dim3(512), 0, s, d_items);
+  } else if (max_k <= 128) {
+    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 8>), dim3(n_items), dim3(1024), 0, s, d_items);
+  } else {
+    throw std::runtime_error("qr: K exceeds 128");
+  }
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// out = Q [X; 0] (+ Base Xb), Q = H_0 ... H_{nref-1} (trans: Q^T).
+// Columns are independent: no workgroup barrier except before the write when
+// a base term may alias the output.
+// ===========================================================================
+template <typename T, int MR, int W, int CPW>
+__global__ __launch_bounds__(W * 64) void apply_q_kernel(const ApplyQItem* __restrict__ items) {
+  ApplyQItem it = items[blockIdx.x];
+  if (it.ncol_ptr != nullptr) it.ncol = *it.ncol_ptr;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int m = it.m;
+  const T* Y = static_cast<const T*>(it.Y);
+  const T* tau = static_cast<const T*>(it.tau);
+  const T* X = static_cast<const T*>(it.X);
+  T a[CPW][MR];
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const int c = q * W + w;
+#pragma unroll
+    for (int t = 0; t < MR; ++t) {
+      const int row = lane + 64 * t;
+      a[q][t] = (X != nullptr && c < it.ncol && row < it.kx && row < m)
+                    ? X[row * it.x_rs + c * it.x_cs]
+                    : T(0);
+    }
+  }
+  for (int jj = 0; jj < it.nref; ++jj) {
+    const int j = it.trans ? jj : it.nref - 1 - jj;
+    const T tj = tau[j];
+    if (tj == T(0)) continue;
+    T v[MR];
+#pragma unroll
+    for (int t = 0; t < MR; ++t) {
+      const int row = lane + 64 * t;
+      v[t] = (row == j) ? T(1) : ((row > j && row < m) ? Y[row * it.y_rs + j * it.y_cs] : T(0));
+    }
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int c = q * W + w;
+      if (c < it.ncol) {
+        T d = T(0);
+#pragma unroll
+        for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
+        d = wave_sum(d) * tj;
+#pragma unroll
+        for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
+      }
+    }
+  }
+  if (it.Base != nullptr && it.kb > 0) {
+    const T* Bs = static_cast<const T*>(it.Base);
+    const T* Xb = static_cast<const T*>(it.Xb);
+    for (int s = 0; s < it.kb; ++s) {
+      T bv[MR];
+#pragma unroll
+      for (int t = 0; t < MR; ++t) {
+        const int row = lane + 64 * t;
+        bv[t] = (row < m) ? Bs[row * it.base_rs + s * it.base_cs] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) {
+        const int c = q * W + w;
+        if (c < it.ncol) {
+          const T coef = Xb[s * it.xb_rs + c * it.xb_cs];
+#pragma unroll
+          for (int t = 0; t < MR; ++t) a[q][t] += bv[t] * coef;
+        }
+      }
+    }
+    __syncthreads();  // the base may alias the output
+  }
+  T* out = static_cast<T*>(it.out);
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const int c = q * W + w;
+    if (c < it.ncol) {
+#pragma unroll
+      for (int t = 0; t < MR; ++t) {
+        const int row = lane + 64 * t;
+        if (row < m) out[row * it.out_rs + c * it.out_cs] = a[q][t];
+      }
+    }
+  }
+}
+
+template <typename T>
+void launch_apply_q(const ApplyQItem* d_items, int n_items, int max_ncol, hipStream_t s) {
+  if (n_items <= 0) return;
+  if (max_ncol <= 16) {
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 4>), dim3(n_items), dim3(256), 0, s, d_items);
+  } else if (max_ncol <= 32) {
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 8>), dim3(n_items), dim3(256), 0, s, d_items);
+  } else if (max_ncol <= 64) {
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 8, 8>), dim3(n_items), dim3(512), 0, s, d_items);
+  } else if (max_ncol <= 128) {
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 16, 8>), dim3(n_items), dim3(1024), 0, s, d_items);
+  } else {
+    throw std::runtime_error("apply_q: more than 128 columns");
+  }
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// One-sided (Hestenes) Jacobi SVD of the K x K core with truncation.
+// ===========================================================================
+__device__ __forceinline__ int rr_pos(int pos, int r, int k2) {
+  return pos == 0 ? 0 : ((pos - 1 + r) % (k2 - 1)) + 1;
+}
+
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ items) {
+  const SvdItem it = items[blockIdx.x];
+  const int K = it.K;
+  __shared__ T G[KMAX][KMAX + 1];
+  __shared__ T Vs[KMAX][KMAX + 1];
+  __shared__ T sig[KMAX];
+  __shared__ int order[KMAX];
+  __shared__ int flag;
+  __shared__ int rank_sh;
+  const int tid = threadIdx.x;
+  if (K <= 0) {
+    if (tid == 0 && it.rank_out) *it.rank_out = 0;
+    return;
+  }
+  for (int e = tid; e < K * K; e += 256) {
+    const int i = e / K, j = e % K;
+    T v;
+    if (it.mode == 0) {
+      v = static_cast<const T*>(it.M)[i * it.m_rs + j * it.m_cs];
+    } else {
+      const T* Ru = static_cast<const T*>(it.Ru);
+      const T* Rv = static_cast<const T*>(it.Rv);
+      T s = T(0);
+      for (int t = 0; t < it.kx; ++t) s += Ru[i * it.ldr + t] * Rv[j * it.ldr + t];
+      if (i < it.ka && j < it.ka) s += static_cast<const T*>(it.Sa)[i * it.lds_in + j];
+      v = s;
+    }
+    G[j][i] = v;
+    Vs[j][i] = (i == j) ? T(1) : T(0);
+  }
+  __syncthreads();
+  const int K2 = K + (K & 1);
+  const int P = K2 / 2;
+  int gl = 64;
+  while (gl > 1 && P * gl > 256) gl >>= 1;
+  const int p = tid / gl, sub = tid % gl;
+  const T tol = T(K) * Eps<T>::v;
+  if (K >= 2) {
+    for (int sweep = 0; sweep < 60; ++sweep) {
+      if (tid == 0) flag = 0;
+      __syncthreads();
+      for (int r = 0; r < K2 - 1; ++r) {
+        if (p < P) {
+          int i = rr_pos(p, r, K2), j = rr_pos(K2 - 1 - p, r, K2);
+          if (i > j) {
+            const int tmp = i;
+            i = j;
+            j = tmp;
+          }
+          if (j < K) {
+            T al = T(0), be = T(0), ga = T(0);
+            for (int row = sub; row < K; row += gl) {
+              const T x = G[i][row], y = G[j][row];
+              al += x * x;
+              be += y * y;
+              ga += x * y;
+            }
+            al = group_sum(al, gl);
+            be = group_sum(be, gl);
+            ga = group_sum(ga, gl);
+            if (ga != T(0) && fabs(ga) > tol * sqrt(al * be)) {
+              const T zeta = (be - al) / (T(2) * ga);
+              const T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
+              const T cth = T(1) / sqrt(T(1) + t * t);
+              const T sth = cth * t;
+              for (int row = sub; row < K; row += gl) {
+                const T x = G[i][row], y = G[j][row];
+                G[i][row] = cth * x - sth * y;
+                G[j][row] = sth * x + cth * y;
+                const T vx = Vs[i][row], vy = Vs[j][row];
+                Vs[i][row] = cth * vx - sth * vy;
+                Vs[j][row] = sth * vx + cth * vy;
+              }
+              if (sub == 0) flag = 1;
+            }
+          }
+        }
+        __syncthreads();
+      }
+      const int f = flag;
+      __syncthreads();
+      if (f == 0) break;
+    }
+  }
+  for (int j = tid; j < K; j += 256) {
+    T s = T(0);
+    for (int row = 0; row < K; ++row) s += G[j][row] * G[j][row];
+    sig[j] = sqrt(s);
+  }
+  __syncthreads();
+  for (int j = tid; j < K; j += 256) {
+    int rk = 0;
+    const T sj = sig[j];
+    for (int i = 0; i < K; ++i)
+      if (sig[i] > sj || (sig[i] == sj && i < j)) ++rk;
+    order[rk] = j;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int r;
+    if (it.fixed_rank >= 0) {
+      r = min(it.fixed_rank, K);
+    } else {
+      const int md = (it.kdim > 0) ? min(it.kdim, K) : K;
+      T ss = T(0);
+      for (int i = 0; i < md; ++i) {
+        const T v = sig[order[i]];
+        ss += v * v;
+      }
+      const T threshold = static_cast<T>(it.eps) * sqrt(ss);
+      r = 0;
+      T err;
+      do {
+        ++r;
+        err = T(0);
+        if (r < md) {
+          const T sr = sig[order[r]];
+          for (int jj = r; jj < md; ++jj) err += sr * sr;
+        }
+      } while (r < md && sqrt(err) > threshold);
+    }
+    if (r > it.cap) {
+      if (it.rank_out) *it.rank_out = -r;
+      r = it.cap;
+    } else if (it.rank_out) {
+      *it.rank_out = r;
+    }
+    rank_sh = r;
+  }
+  __syncthreads();
+  const int r = rank_sh;
+  // normalise left vectors in place
+  for (int e = tid; e < K * K; e += 256) {
+    const int c = e / K, row = e % K;
+    const T s = sig[c];
+    if (s > T(0)) G[c][row] = G[c][row] / s;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    // orthonormal completion for exactly-zero singular values among the kept ones
+    for (int c = 0; c < r; ++c) {
+      const int src = order[c];
+      if (sig[src] > T(0)) continue;
+      for (int u = 0; u < K; ++u) {
+        for (int row = 0; row < K; ++row) G[src][row] = (row == u) ? T(1) : T(0);
+        for (int pc = 0; pc < c; ++pc) {
+          const int ps = order[pc];
+          T d = T(0);
+          for (int row = 0; row < K; ++row) d += G[ps][row] * G[src][row];
+          for (int row = 0; row < K; ++row) G[src][row] -= d * G[ps][row];
+        }
+        T nn = T(0);
+        for (int row = 0; row < K; ++row) nn += G[src][row] * G[src][row];
+        if (nn > T(0.25)) {
+          const T inv = T(1) / sqrt(nn);
+          for (int row = 0; row < K; ++row) G[src][row] *= inv;
+          break;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  T* S = static_cast<T*>(it.S);
+  T* Um = static_cast<T*>(it.Um);
+  T* Vm = static_cast<T*>(it.Vm);
+  if (S != nullptr)
+    for (int e = tid; e < r * r; e += 256) {
+      const int i = e / r, j = e % r;
+      S[i * it.lds + j] = (i == j) ? sig[order[i]] : T(0);
+    }
+  for (int e = tid; e < K * r; e += 256) {
+    const int row = e / r, c = e % r;
+    const int src = order[c];
+    if (Um != nullptr) Um[row * it.ldu + c] = G[src][row];
+    if (Vm != nullptr) Vm[row * it.ldv + c] = Vs[src][row];
+  }
+}
+
+template <typename T>
+void launch_svd(const SvdItem* d_items, int n_items, int max_k, hipStream_t s) {
+  if (n_items <= 0) return;
+  if (max_k <= 32) {
+    hipLaunchKernelGGL((svd_kernel<T, 32>), dim3(n_items), dim3(256), 0, s, d_items);
+  } else if (max_k <= 64) {
+    hipLaunchKernelGGL((svd_kernel<T, 64>), dim3(n_items), dim3(256), 0, s, d_items);
+  } else {
+    if constexpr (sizeof(T) == 4) {
+      if (max_k <= 128) {
+        hipLaunchKernelGGL((svd_kernel<T, 128>), dim3(n_items), dim3(256), 0, s, d_items);
+        FRK_HIP_CHECK(hipGetLastError());
+        return;
+      }
+    }
+    throw std::runtime_error("svd: K exceeds the LDS capacity for this precision");
+  }
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// Truncated column-pivoted Householder QR (geqp3 with LAWN-176 downdating),
+// one 512-thread workgroup per row-major tile; thread-per-column sweeps.
+// ===========================================================================
+template <typename T>
+__device__ T block_sum512(T v, T* red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = 256; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = red[tid] + red[tid + s];
+    __syncthreads();
+  }
+  const T out = red[0];
+  __syncthreads();
+  return out;
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void geqp3_kernel(const Geqp3Item* __restrict__ items) {
+  const Geqp3Item it = items[blockIdx.x];
+  T* A = static_cast<T*>(it.A);
+  const int m = it.m, n = it.n;
+  const int64_t lda = it.lda;
+  const int tid = threadIdx.x;
+  __shared__ T cn[512], pcn[512], vv[512], red[512];
+  __shared__ int piv[512], redi[512];
+  const T tol = Eps<T>::half;
+  const T tol3z = sqrt(tol);
+  const int min_dim = min(m, n);
+  T part = T(0);
+  for (int c = tid; c < n; c += 512) {
+    T s = T(0);
+    for (int i = 0; i < m; ++i) {
+      const T x = A[i * lda + c];
+      s += x * x;
+    }
+    part += s;
+    cn[c] = sqrt(s);
+    pcn[c] = cn[c];
+    piv[c] = c;
+  }
+  const T total = block_sum512(part, red);
+  const T threshold = static_cast<T>(it.eps) * sqrt(total);
+  T* tau_out = static_cast<T*>(it.tau);
+  // initial max
+  auto block_argmax = [&](int from, T& best, int& besti) {
+    T bv = -T(1);
+    int bi = 0x7fffffff;
+    for (int c = from + tid; c < n; c += 512) {
+      const T x = cn[c];
+      if (x > bv || (x == bv && c < bi)) {
+        bv = x;
+        bi = c;
+      }
+    }
+    red[tid] = bv;
+    redi[tid] = bi;
+    __syncthreads();
+    for (int s = 256; s > 0; s >>= 1) {
+      if (tid < s) {
+        const T o = red[tid + s];
+        const int oi = redi[tid + s];
+        if (o > red[tid] || (o == red[tid] && oi < redi[tid])) {
+          red[tid] = o;
+          redi[tid] = oi;
+        }
+      }
+      __syncthreads();
+    }
+    best = red[0];
+    besti = redi[0];
+    __syncthreads();
+  };
+  T maxv;
+  int kmax;
+  block_argmax(0, maxv, kmax);
+  if (maxv <= tol) {
+    // zero matrix: Q = e1, R = 0 (rank 1)
+    for (int c = tid; c < n; c += 512) {
+      A[c] = T(0);
+      it.piv[c] = c;
+    }
+    if (tid == 0) {
+      tau_out[0] = T(0);
+      *it.rank_out = 1;
+    }
+    return;
+  }
+  int r = 0;
+  while (r < min_dim && maxv > threshold) {
+    // pivot: first maximum of cn[r:] (computed at the end of the previous step)
+    const int k = kmax;
+    if (k != r) {
+      for (int i = tid; i < m; i += 512) {
+        const T x = A[i * lda + r];
+        A[i * lda + r] = A[i * lda + k];
+        A[i * lda + k] = x;
+      }
+    }
+    __syncthreads();
+    if (tid == 0 && k != r) {
+      T x = cn[r];
+      cn[r] = cn[k];
+      cn[k] = x;
+      x = pcn[r];
+      pcn[r] = pcn[k];
+      pcn[k] = x;
+      const int pi = piv[r];
+      piv[r] = piv[k];
+      piv[k] = pi;
+    }
+    // reflector from column r
+    T x2p = T(0);
+    for (int i = r + 1 + tid; i < m; i += 512) {
+      const T x = A[i * lda + r];
+      x2p += x * x;
+    }
+    const T x2 = (r < m - 1) ? block_sum512(x2p, red) : T(0);
+    const T alpha = A[r * lda + r];
+    T beta, tau, scal;
+    if (x2 == T(0)) {
+      tau = T(0);
+      beta = alpha;
+      scal = T(0);
+    } else {
+      const T nrm = sqrt(alpha * alpha + x2);
+      beta = (alpha >= T(0)) ? -nrm : nrm;
+      tau = (beta - alpha) / beta;
+      scal = T(1) / (alpha - beta);
+    }
+    for (int i = r + 1 + tid; i < m; i += 512) {
+      const T x = A[i * lda + r] * scal;
+      A[i * lda + r] = x;
+      vv[i] = x;
+    }
+    if (tid == 0) {
+      vv[r] = T(1);
+      tau_out[r] = tau;
+    }
+    __syncthreads();
+    if (tid == 0) A[r * lda + r] = beta;
+    if (r < min_dim - 1 && tau != T(0)) {
+      for (int c = r + 1 + tid; c < n; c += 512) {
+        T wsum = T(0);
+        for (int i = r; i < m; ++i) wsum += vv[i] * A[i * lda + c];
+        wsum *= tau;
+        for (int i = r; i < m; ++i) A[i * lda + c] -= wsum * vv[i];
+      }
+    }
+    __syncthreads();
+    for (int c = r + 1 + tid; c < n; c += 512) {
+      if (cn[c] != T(0)) {
+        T temp = fabs(A[r * lda + c] / cn[c]);
+        temp = fmax(T(0), (T(1) - temp) * (T(1) + temp));
+        const T ratio = cn[c] / pcn[c];
+        const T temp2 = temp * ratio * ratio;
+        if (temp2 > tol3z) {
+          cn[c] = cn[c] * sqrt(temp);
+        } else if (r < m - 1) {
+          T s = T(0);
+          for (int i = r + 1; i < m; ++i) {
+            const T x = A[i * lda + c];
+            s += x * x;
+          }
+          cn[c] = sqrt(s);
+          pcn[c] = cn[c];
+        } else {
+          cn[c] = T(0);
+          pcn[c] = T(0);
+        }
+      }
+    }
+    __syncthreads();
+    ++r;
+    if (r < n) {
+      block_argmax(r, maxv, kmax);
+    } else {
+      maxv = T(0);
+    }
+  }
+  for (int c = tid; c < n; c += 512) it.piv[c] = piv[c];
+  if (tid == 0) *it.rank_out = (r > it.cap) ? -r : r;
+}
+
+template <typename T>
+void launch_geqp3(const Geqp3Item* d_items, int n_items, hipStream_t s) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(geqp3_kernel<T>, dim3(n_items), dim3(512), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// R (first `rank` rows, upper part) scattered back to original column order.
+template <typename T>
+__global__ __launch_bounds__(256) void permute_r_kernel(const PermuteItem* __restrict__ items) {
+  const PermuteItem it = items[blockIdx.y];
+  const int rk = *it.rank;
+  const int i = blockIdx.x;
+  if (i >= rk) return;
+  const T* A = static_cast<const T*>(it.A);
+  T* out = static_cast<T*>(it.out);
+  for (int j = threadIdx.x; j < it.n; j += 256) {
+    const T v = (j >= i) ? A[(int64_t)i * it.lda + j] : T(0);
+    out[(int64_t)i * it.ldo + it.piv[j]] = v;
+  }
+}
+
+template <typename T>
+void launch_permute_r(const PermuteItem* d_items, int n_items, int max_n, int max_rows, hipStream_t s) {
+  if (n_items <= 0 || max_rows <= 0) return;
+  (void)max_n;
+  hipLaunchKernelGGL(permute_r_kernel<T>, dim3(max_rows, n_items), dim3(256), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void flip_r_kernel(const FlipItem* __restrict__ items) {
+  const FlipItem it = items[blockIdx.x];
+  const int r = *it.rank;
+  const T* R = static_cast<const T*>(it.R);
+  T* S = static_cast<T*>(it.S);
+  for (int e = threadIdx.x; e < r * r; e += 256) {
+    const int i = e / r, j = e % r;
+    S[i * it.lds + j] = (i <= j) ? R[(r - 1 - j) * it.ldr + (r - 1 - i)] : T(0);
+  }
+}
+
+template <typename T>
+void launch_flip_r(const FlipItem* d_items, int n_items, int max_r, hipStream_t s) {
+  if (n_items <= 0 || max_r <= 0) return;
+  hipLaunchKernelGGL(flip_r_kernel<T>, dim3(n_items), dim3(256), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// Kernel-matrix tiles (same operation order as the CPU generators: the
+// squared distance is accumulated without contraction).
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(256) void gen_kernel(KernelSpec ks, const GenItem* __restrict__ items) {
+#pragma clang fp contract(off)
+  const GenItem it = items[blockIdx.y];
+  const int64_t total = (int64_t)it.nr * it.nc;
+  T* out = static_cast<T*>(it.out);
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e / it.nc), j = (int)(e % it.nc);
+    const int64_t gi = it.r0 + i, gj = it.c0 + j;
+    double rij = 0.0;
+    for (int d = 0; d < ks.dim; ++d) {
+      const double dd = ks.pts[d * ks.n_pts + gi] - ks.pts[d * ks.n_pts + gj];
+      rij = rij + dd * dd;
+    }
+    double v;
+    if (ks.kind == 0) {
+      v = 1.0 / (sqrt(rij) + 1e-3);
+    } else {
+      v = exp(-rij / ((2.0 * ks.ell) * ks.ell));
+    }
+    if (gi == gj) v = v + ks.shift;
+    out[(int64_t)i * it.ld + j] = static_cast<T>(v);
+  }
+}
+
+template <typename T>
+void launch_gen(const KernelSpec& ks, const GenItem* d_items, int n_items, int max_nr, int max_nc,
+                hipStream_t s) {
+  if (n_items <= 0) return;
+  const int64_t total = (int64_t)max_nr * max_nc;
+  int gx = (int)std::min<int64_t>((total + 255) / 256, 1024);
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(gen_kernel<T>, dim3(gx, n_items), dim3(256), 0, s, ks, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void fill_identity_kernel(T* p, int64_t rows, int64_t cols, int64_t ld) {
+  const int64_t total = rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e / cols, j = e % cols;
+    p[i * ld + j] = (i == j) ? T(1) : T(0);
+  }
+}
+
+template <typename T>
+void launch_fill_identity(T* p, int64_t rows, int64_t cols, int64_t ld, hipStream_t s) {
+  const int64_t total = rows * cols;
+  if (total <= 0) return;
+  int gx = (int)std::min<int64_t>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(fill_identity_kernel<T>, dim3(gx), dim3(256), 0, s, p, rows, cols, ld);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename TD, typename TS>
+__global__ void convert_kernel(TD* dst, const TS* src, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
+    dst[e] = static_cast<TD>(src[e]);
+}
+
+template <typename TD, typename TS>
+void launch_convert(TD* dst, const TS* src, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int gx = (int)std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL((convert_kernel<TD, TS>), dim3(gx), dim3(256), 0, s, dst, src, n);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// Matrix-free residual R = B - A X (FP64) for the kernel matrices.
+// Row block of 256 per workgroup, columns split over gridDim.y partial sums
+// reduced in a fixed order by a second kernel (deterministic).
+// ===========================================================================
+constexpr int kResMaxRhs = 16;
+constexpr int kResSplits = 16;
+
+__global__ __launch_bounds__(256) void residual_partial_kernel(KernelSpec ks, const double* __restrict__ X,
+                                                               double* __restrict__ part, int64_t n,
+                                                               int nrhs) {
+#pragma clang fp contract(off)
+  __shared__ double xs[3][256];
+  __shared__ double vs[256][kResMaxRhs + 1];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int split = blockIdx.y;
+  const int64_t chunk = (n + kResSplits - 1) / kResSplits;
+  const int64_t j0 = split * chunk, j1 = std::min<int64_t>(n, j0 + chunk);
+  double acc[kResMaxRhs];
+  for (int c = 0; c < kResMaxRhs; ++c) acc[c] = 0.0;
+  double xi[3] = {0.0, 0.0, 0.0};
+  if (i < n)
+    for (int d = 0; d < ks.dim; ++d) xi[d] = ks.pts[d * ks.n_pts + i];
+  for (int64_t jb = j0; jb < j1; jb += 256) {
+    const int64_t j = jb + threadIdx.x;
+    __syncthreads();
+    if (j < j1) {
+      for (int d = 0; d < ks.dim; ++d) xs[d][threadIdx.x] = ks.pts[d * ks.n_pts + j];
+      for (int c = 0; c < nrhs; ++c) vs[threadIdx.x][c] = X[j * nrhs + c];
+    }
+    __syncthreads();
+    const int cnt = (int)std::min<int64_t>(256, j1 - jb);
+    if (i < n) {
+      for (int jj = 0; jj < cnt; ++jj) {
+        double rij = 0.0;
+        for (int d = 0; d < ks.dim; ++d) {
+          const double dd = xi[d] - xs[d][jj];
+          rij = rij + dd * dd;
+        }
+        double a;
+        if (ks.kind == 0)
+          a = 1.0 / (sqrt(rij) + 1e-3);
+        else
+          a = exp(-rij / ((2.0 * ks.ell) * ks.ell));
+        if (jb + jj == i) a = a + ks.shift;
+        for (int c = 0; c < nrhs; ++c) acc[c] = fma(a, vs[jj][c], acc[c]);
+      }
+    }
+  }
+  if (i < n)
+    for (int c = 0; c < nrhs; ++c) part[((int64_t)split * n + i) * nrhs + c] = acc[c];
+}
+
+__global__ void residual_reduce_kernel(const double* __restrict__ part, const double* __restrict__ B,
+                                       double* __restrict__ R, int64_t n, int nrhs) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n * nrhs) return;
+  double s = 0.0;
+  for (int sp = 0; sp < kResSplits; ++sp) s += part[(int64_t)sp * n * nrhs + e];
+  R[e] = B[e] - s;
+}
+
+void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R, int64_t n,
+                            int nrhs, hipStream_t s) {
+  if (nrhs > kResMaxRhs) throw std::runtime_error("residual: at most 16 right-hand sides");
+  static thread_local double* part = nullptr;
+  static thread_local int64_t part_cap = 0;
+  const int64_t need = (int64_t)kResSplits * n * nrhs;
+  if (need > part_cap) {
+    if (part) FRK_HIP_CHECK(hipFree(part));
+    FRK_HIP_CHECK(hipMalloc(&part, need * sizeof(double)));
+    part_cap = need;
+  }
+  dim3 grid((unsigned)((n + 255) / 256), kResSplits);
+  hipLaunchKernelGGL(residual_partial_kernel, grid, dim3(256), 0, s, ks, X, part, n, nrhs);
+  FRK_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(residual_reduce_kernel, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, part,
+                     B, R, n, nrhs);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Explicit instantiations
+// ---------------------------------------------------------------------------
+#define FRK_INST(T)                                                                              \
+  template void launch_gemm<T>(const GemmItem*, int, int, int, hipStream_t);                     \
+  template void launch_trsm<T>(const TrsmItem*, int, int, int, hipStream_t);                     \
+  template void launch_getrf_small<T>(const GemmItem*, int, int32_t*, hipStream_t);              \
+  template void launch_qr<T>(const QrItem*, int, int, hipStream_t);                              \
+  template void launch_apply_q<T>(const ApplyQItem*, int, int, hipStream_t);                     \
+  template void launch_svd<T>(const SvdItem*, int, int, hipStream_t);                            \
+  template void launch_geqp3<T>(const Geqp3Item*, int, hipStream_t);                             \
+  template void launch_permute_r<T>(const PermuteItem*, int, int, int, hipStream_t);             \
+  template void launch_flip_r<T>(const FlipItem*, int, int, hipStream_t);                        \
+  template void launch_gen<T>(const KernelSpec&, const GenItem*, int, int, int, hipStream_t);    \
+  template void launch_fill_identity<T>(T*, int64_t, int64_t, int64_t, hipStream_t);
+
+FRK_INST(double)
+FRK_INST(float)
+template void launch_convert<float, double>(float*, const double*, int64_t, hipStream_t);
+template void launch_convert<double, float>(double*, const float*, int64_t, hipStream_t);
+
+}  // namespace frk

@@ -1,0 +1,87 @@
+// Host-side launchers for the gfx950 kernels (kernels.hip).  All launchers
+// are asynchronous on `stream`; item arrays are device pointers.
+#pragma once
+#include "common.hpp"
+
+namespace frk {
+
+// Strided batched GEMM (one 64x64 output tile per workgroup).
+template <typename T>
+void launch_gemm(const GemmItem* d_items, int n_items, int max_m, int max_n, hipStream_t s);
+
+// Batched triangular solve, left side, in place (strips of 16 right-hand sides).
+template <typename T>
+void launch_trsm(const TrsmItem* d_items, int n_items, int max_n, int max_nrhs, hipStream_t s);
+
+// Unblocked no-pivot LU of small diagonal blocks (n <= 64), one wave each.
+// items: GemmItem reused as {C = block pointer, c_rs, c_cs, m = n}; info[i]
+// receives the first zero pivot (1-based) or 0.
+template <typename T>
+void launch_getrf_small(const GemmItem* d_items, int n_items, int32_t* d_info, hipStream_t s);
+
+// Householder QR of [A1 S1 | A2 S2] (optionally projected), m <= 512.
+template <typename T>
+void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s);
+
+// Apply Householder products to zero-padded blocks, m <= 512.
+template <typename T>
+void launch_apply_q(const ApplyQItem* d_items, int n_items, int max_ncol, hipStream_t s);
+
+// One-sided Jacobi SVD with truncation, K <= 64.
+template <typename T>
+void launch_svd(const SvdItem* d_items, int n_items, int max_k, hipStream_t s);
+
+// Truncated column-pivoted QR (LAWN-176 downdating), one workgroup per tile.
+template <typename T>
+void launch_geqp3(const Geqp3Item* d_items, int n_items, hipStream_t s);
+
+// Gather the first `rank` rows of the triangular factor in original column order.
+struct PermuteItem {
+  const void* A;   // geqp3-processed tile (row-major, ld)
+  const int32_t* piv;
+  const int32_t* rank;
+  void* out;       // rank x n row-major, ld = ldo
+  int32_t n, lda, ldo, pad;
+};
+template <typename T>
+void launch_permute_r(const PermuteItem* d_items, int n_items, int max_n, int max_rows, hipStream_t s);
+
+// S = flip(Rz^T): S[i][j] = Rz[r-1-j][r-1-i] (upper triangular), r from *rank.
+struct FlipItem {
+  const void* R;
+  void* S;
+  const int32_t* rank;
+  int32_t ldr, lds;
+};
+template <typename T>
+void launch_flip_r(const FlipItem* d_items, int n_items, int max_r, hipStream_t s);
+
+// Kernel-matrix tile generation.
+struct GenItem {
+  void* out;
+  int64_t ld;
+  int32_t r0, nr, c0, nc;
+};
+struct KernelSpec {
+  const double* pts;  // dim arrays of N points, structure of arrays
+  int64_t n_pts;
+  int32_t dim;
+  int32_t kind;       // 0 laplace, 1 gaussian
+  double ell;         // gaussian length scale
+  double shift;       // added on the global diagonal
+};
+template <typename T>
+void launch_gen(const KernelSpec& ks, const GenItem* d_items, int n_items, int max_nr, int max_nc,
+                hipStream_t s);
+
+// Matrix-free residual R = B - A X for a kernel matrix (FP64), X/B/R row-major N x nrhs.
+void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R,
+                            int64_t n, int nrhs, hipStream_t s);
+
+// Elementwise helpers.
+template <typename T>
+void launch_fill_identity(T* p, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
+template <typename TD, typename TS>
+void launch_convert(TD* dst, const TS* src, int64_t n, hipStream_t s);
+
+}  // namespace frk

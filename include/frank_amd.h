@@ -1,0 +1,149 @@
+/*
+ * frank_amd: C ABI of the MI355X (gfx950) BLR factorization engine.
+ *
+ * Every entry point returns an int status: 0 on success, < 0 on a bad
+ * argument / HIP failure / capacity overflow, > 0 for LAPACK-style numerical
+ * info (e.g. a zero pivot).  frk_last_error() gives the message of the last
+ * failure on the calling thread.  No entry point aborts the process; the
+ * reference aborts with "name(Types) undefined!" (src/util/omm_error_handler.cpp:15-30)
+ * where this ABI returns FRK_ERR_UNSUPPORTED with the same text.
+ *
+ * Layouts follow FRANK's Dense (include/FRANK/classes/dense.h:43-66): row-major
+ * with a leading dimension.  Device pointers are HIP device memory; host
+ * pointers are plain host memory.  All work is ordered on the context's stream.
+ */
+#ifndef FRANK_AMD_H
+#define FRANK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRK_OK 0
+#define FRK_ERR_ARG (-1)
+#define FRK_ERR_HIP (-2)
+#define FRK_ERR_CAPACITY (-3)
+#define FRK_ERR_UNSUPPORTED (-4)
+
+/* Tile kinds (FRANK Dense / LowRank / Empty) */
+#define FRK_TILE_DENSE 0
+#define FRK_TILE_LOWRANK 1
+#define FRK_TILE_EMPTY 2
+
+/* Low-rank addition algorithm: FRANK_LRA (src/operations/arithmetic/addition.cpp:191-203) */
+#define FRK_LRA_NAIVE 0
+#define FRK_LRA_ROUNDED 1
+#define FRK_LRA_FAST 2
+
+/* Kernel generators (src/functions.cpp:92-109 laplacend; gaussian has no reference counterpart) */
+#define FRK_KERNEL_LAPLACE 0
+#define FRK_KERNEL_GAUSSIAN 1
+
+/* Admissibility (include/FRANK/definitions.h:25 AdmisType) */
+#define FRK_ADMIS_POSITION 0
+#define FRK_ADMIS_GEOMETRY 1
+
+/* Low-rank storage precision */
+#define FRK_PREC_F64 0
+#define FRK_PREC_F32 1
+
+typedef struct frk_ctx_s* frk_ctx;
+typedef struct frk_blr_s* frk_blr;
+
+const char* frk_last_error(void);
+int frk_version(void);
+
+/* Execution context: one HIP stream, staging ring, workspace.
+ * Replaces FRANK::initialize() (src/util/initialize.cpp:9-17). */
+int frk_ctx_create(frk_ctx* out);
+int frk_ctx_destroy(frk_ctx ctx);
+int frk_ctx_synchronize(frk_ctx ctx);
+
+/* ---------------------------------------------------------------------------
+ * BLR matrices (one-level Hierarchical, tiles resident in HBM)
+ * ------------------------------------------------------------------------- */
+
+/* Hierarchical(kernel, params, N, N, nleaf, eps, admis, nb, nb, admis_type)
+ * (src/classes/hierarchical.cpp:89-109) when eps != 0, or the fixed-rank
+ * constructor (hierarchical.cpp:67-87) with `rank` when eps == 0.
+ * pts: host, dim x n structure of arrays.  shift is added on the diagonal. */
+int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
+                          double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
+                          int precision, frk_blr* out);
+/* Empty grid with given block offsets (nb+1 entries each) and tile kinds; fill with set_* below. */
+int frk_blr_create(frk_ctx ctx, int nb_r, int nb_c, const int64_t* roff, const int64_t* coff, const int32_t* kinds,
+                   int kcap, double eps, int rank, int precision, frk_blr* out);
+int frk_blr_destroy(frk_blr a);
+/* FRANK_LRA selection for this matrix (default FRK_LRA_FAST, as the reference). */
+int frk_blr_set_lra(frk_blr a, int lra);
+int frk_blr_shape(frk_blr a, int* nb_r, int* nb_c);
+int frk_blr_tile_info(frk_blr a, int i, int j, int32_t* kind, int32_t* rank, int64_t* rows, int64_t* cols);
+/* Host copies of tiles: dense rows x cols row-major; low-rank U rows x r, S r x r, V r x cols (row-major). */
+int frk_blr_get_dense(frk_blr a, int i, int j, double* host);
+int frk_blr_set_dense(frk_blr a, int i, int j, const double* host);
+int frk_blr_get_lowrank(frk_blr a, int i, int j, double* U, double* S, double* V);
+int frk_blr_set_lowrank(frk_blr a, int i, int j, int rank, const double* U, const double* S, const double* V);
+
+/* getrf(Hierarchical&) (src/operations/LAPACK/getrf.cpp:45-65): tile LU without
+ * pivoting, in place.  Afterwards tiles below the diagonal hold L, tiles above
+ * hold U, diagonal tiles hold the packed unit-L \ U factors.  Returns > 0
+ * (1-based global row) when a zero pivot is met. */
+int frk_blr_getrf(frk_blr a);
+
+/* trsm(L, b, Lower); trsm(U, b, Upper) (src/operations/BLAS/trsm.cpp:40-90,117-125)
+ * on device b (N x nrhs row-major, FP64).  trans != 0 solves A^T x = b with
+ * U^T then L^T sweeps (adjoint; no reference counterpart). */
+int frk_blr_solve(frk_blr f, double* b_dev, int nrhs, int trans);
+
+/* gemm(Hierarchical, Dense, Dense, 1, 1): y += A x (device, N x nrhs row-major). */
+int frk_blr_matvec(frk_blr a, const double* x_dev, double* y_dev, int nrhs);
+
+/* Matrix-free FP64 residual r = b - A x for the kernel matrix the BLR matrix
+ * was built from (exact, uncompressed).  Device vectors, N x nrhs row-major. */
+int frk_blr_kernel_residual(frk_blr a, const double* x_dev, const double* b_dev, double* r_dev, int nrhs);
+
+/* ---------------------------------------------------------------------------
+ * Tile kernels (FP64, device pointers, row-major with leading dimension)
+ * ------------------------------------------------------------------------- */
+
+/* getrf_omm(Dense) (src/operations/LAPACK/getrf.cpp:67-84) without the pivot
+ * search: A (n x n) overwritten by unit-L \ U.  info_host[b] = first zero pivot (1-based). */
+int frk_dgetrf_nopiv_batched(frk_ctx ctx, double* const* tiles_host, int batch, int n, int64_t ld,
+                             int32_t* info_host);
+
+/* trsm_omm(Dense, Dense) (src/operations/BLAS/trsm.cpp:92-104): uplo lower =>
+ * unit diagonal, upper => non-unit.  side 0 = left (A X = B), 1 = right (X A = B).
+ * trans != 0 uses op(A) = A^T (adjoint sweeps; no reference counterpart). */
+int frk_dtrsm(frk_ctx ctx, int side, int lower, int trans, int m, int n, const double* A, int64_t lda, double* B,
+              int64_t ldb);
+
+/* gemm_omm(Dense, Dense, Dense) (src/operations/BLAS/gemm.cpp:126-146):
+ * C = alpha op(A) op(B) + beta C, C m x n. */
+int frk_dgemm(frk_ctx ctx, int trans_a, int trans_b, int m, int n, int k, double alpha, const double* A,
+              int64_t lda, const double* B, int64_t ldb, double beta, double* C, int64_t ldc);
+
+/* LowRank += LowRank (src/operations/arithmetic/addition.cpp:93-203), batched.
+ * Every pair b: A_b = Ua Sa Va (rank ka, capacity cap: Ua m x cap, Sa cap x cap,
+ * Va cap x n row-major, device), B_b = Ub Sb Vb (rank kb).  A is updated in
+ * place; ranks_host[b] holds ka on entry and the new rank on exit.
+ * eps != 0 selects the rounded addition with the truncation rule of
+ * src/operations/misc/misc.cpp:133-146; eps == 0 keeps ka (lra selects rounded or fast). */
+int frk_dlr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const int32_t* n, double eps, int cap,
+                        double* const* Ua, double* const* Sa, double* const* Va, int32_t* ranks_host,
+                        const double* const* Ub, const double* const* Sb, const double* const* Vb,
+                        const int32_t* kb);
+
+/* LowRank(Dense, eps) (src/classes/low_rank.cpp:50-59, truncated_geqp3 + rq)
+ * when eps != 0, else LowRank(Dense, rank) (low_rank.cpp:40-48, rsvd).
+ * A (m x n row-major, device) is overwritten.  Outputs U (m x cap), S (cap x cap),
+ * V (cap x n) row-major, device; *rank_out receives the rank. */
+int frk_dcompress(frk_ctx ctx, int m, int n, double* A, int64_t lda, double eps, int rank, int cap, double* U,
+                  double* S, double* V, int32_t* rank_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRANK_AMD_H */
