@@ -44,6 +44,12 @@ class Context:
     def synchronize(self):
         L.check(L.lib().frk_ctx_synchronize(self.h))
 
+    @property
+    def stream_ptr(self) -> int:
+        p = C.c_void_p()
+        L.check(L.lib().frk_ctx_stream(self.h, C.byref(p)))
+        return p.value or 0
+
     def close(self):
         if self.h:
             L.lib().frk_ctx_destroy(self.h)
@@ -183,6 +189,15 @@ class BlrMatrix:
             r0 += rows[i]
         return out
 
+    def clone(self) -> "BlrMatrix":
+        h = C.c_void_p()
+        L.check(L.lib().frk_blr_clone(self.h, C.byref(h)))
+        return BlrMatrix(h, self.ctx)
+
+    def copy_from(self, src: "BlrMatrix") -> "BlrMatrix":
+        L.check(L.lib().frk_blr_copy(self.h, src.h))
+        return self
+
     # -- hot path -------------------------------------------------------------
     def set_lra(self, lra: str):
         L.check(L.lib().frk_blr_set_lra(self.h, _LRA[lra]))
@@ -203,6 +218,13 @@ class BlrMatrix:
         nrhs = 1 if x.dim() == 1 else x.shape[1]
         L.check(L.lib().frk_blr_matvec(self.h, _dptr(x), _dptr(y), nrhs))
         return y
+
+    def refine(self, b, x, tol: float = 1e-10, max_iter: int = 50):
+        """x <- refined solution of A x = b (A: the exact kernel matrix, FP64). Returns (iters, berr)."""
+        nrhs = 1 if b.dim() == 1 else b.shape[1]
+        it, be = C.c_int(), C.c_double()
+        L.check(L.lib().frk_blr_refine(self.h, _dptr(b), _dptr(x), nrhs, tol, max_iter, C.byref(it), C.byref(be)))
+        return it.value, be.value
 
     def kernel_residual(self, x, b, r):
         """r = b - A x with the exact (uncompressed) kernel matrix, FP64, on the GPU."""

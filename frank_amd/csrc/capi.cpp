@@ -23,6 +23,9 @@ struct frk_blr_s {
   bool has_kernel = false;
   KernelSpec ks{};
   DevBuf pts;
+  std::vector<double> host_pts;
+  KernelDesc kd{};
+  double anorm = -1.0;
 };
 
 namespace {
@@ -199,6 +202,14 @@ int frk_ctx_synchronize(frk_ctx ctx) {
   });
 }
 
+int frk_ctx_stream(frk_ctx ctx, void** stream) {
+  return guard([&] {
+    if (!ctx || !stream) throw std::invalid_argument("null argument");
+    *stream = (void*)ctx->stream;
+    return FRK_OK;
+  });
+}
+
 int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
                           double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
                           int precision, frk_blr* out) {
@@ -220,7 +231,10 @@ int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, cons
       h->a32 = build_blr_kernel<float>(kd, nb, eps, rank, admis, admis_type, kcap, *ctx->ctx);
     else
       throw std::invalid_argument("unknown precision");
-    h->ks = upload_kernel(kd, h->pts, ctx->stream);
+    h->host_pts.assign(pts, pts + (size_t)dim * n);
+    h->kd = kd;
+    h->kd.host_pts = h->host_pts.data();
+    h->ks = upload_kernel(h->kd, h->pts, ctx->stream);
     h->has_kernel = true;
     FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     *out = h.release();
@@ -251,6 +265,62 @@ int frk_blr_create(frk_ctx ctx, int nb_r, int nb_c, const int64_t* roff, const i
       throw std::invalid_argument("unknown precision");
     FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     *out = h.release();
+    return FRK_OK;
+  });
+}
+
+int frk_blr_clone(frk_blr src, frk_blr* out) {
+  return guard([&] {
+    if (!src || !out) throw std::invalid_argument("null argument");
+    auto h = std::make_unique<frk_blr_s>();
+    h->owner = src->owner;
+    h->precision = src->precision;
+    Context& c = *src->owner->ctx;
+    if (src->a64) {
+      h->a64 = src->a64->clone_structure(c);
+      h->a64->copy_from(*src->a64, src->owner->stream);
+    } else {
+      h->a32 = src->a32->clone_structure(c);
+      h->a32->copy_from(*src->a32, src->owner->stream);
+    }
+    if (src->has_kernel) {
+      h->host_pts = src->host_pts;
+      h->kd = src->kd;
+      h->kd.host_pts = h->host_pts.data();
+      h->ks = upload_kernel(h->kd, h->pts, src->owner->stream);
+      h->has_kernel = true;
+      h->anorm = src->anorm;
+    }
+    FRK_HIP_CHECK(hipStreamSynchronize(src->owner->stream));
+    *out = h.release();
+    return FRK_OK;
+  });
+}
+
+int frk_blr_copy(frk_blr dst, frk_blr src) {
+  return guard([&] {
+    if (!dst || !src) throw std::invalid_argument("null argument");
+    if ((dst->a64 != nullptr) != (src->a64 != nullptr)) throw std::invalid_argument("precisions differ");
+    if (dst->a64)
+      dst->a64->copy_from(*src->a64, dst->owner->stream);
+    else
+      dst->a32->copy_from(*src->a32, dst->owner->stream);
+    return FRK_OK;
+  });
+}
+
+int frk_blr_refine(frk_blr f, const double* b_dev, double* x_dev, int nrhs, double tol, int max_iter, int* iters,
+                   double* berr) {
+  return guard([&] {
+    if (!f || !b_dev || !x_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
+    if (!f->has_kernel) throw std::invalid_argument("refinement needs a matrix built from a kernel");
+    Context& c = *f->owner->ctx;
+    const int64_t n = f->a64 ? f->a64->roff.back() : f->a32->roff.back();
+    if (f->anorm < 0) f->anorm = kernel_norm_inf(f->ks, n, c);
+    RefineResult r = f->a64 ? blr_refine<double>(*f->a64, f->ks, f->anorm, b_dev, x_dev, nrhs, tol, max_iter, c)
+                            : blr_refine<float>(*f->a32, f->ks, f->anorm, b_dev, x_dev, nrhs, tol, max_iter, c);
+    if (iters) *iters = r.iters;
+    if (berr) *berr = r.berr;
     return FRK_OK;
   });
 }
@@ -412,35 +482,16 @@ int frk_dtrsm(frk_ctx ctx, int side, int lower, int trans, int m, int n, const d
     if (!ctx || !A || !B || m <= 0 || n <= 0) throw std::invalid_argument("bad argument");
     Context& c = *ctx->ctx;
     c.sync();
-    TrsmItem x{};
     const bool unit = lower != 0;  // trsm.cpp:92-104: Lower => Unit, Upper => NonUnit
     // op(A) is lower-triangular iff (lower XOR trans)
     const bool op_lower = (lower != 0) != (trans != 0);
-    x.unit = unit ? 1 : 0;
     if (side == 0) {  // op(A) X = B, A m x m
       if (m > 512) throw std::invalid_argument("triangle above 512");
-      x.A = A;
-      x.a_rs = trans ? 1 : lda;
-      x.a_cs = trans ? lda : 1;
-      x.B = B;
-      x.b_rs = ldb;
-      x.b_cs = 1;
-      x.n = m;
-      x.nrhs = n;
-      x.lower = op_lower ? 1 : 0;
+      trsm_generic<double>(A, trans ? 1 : lda, trans ? lda : 1, op_lower, unit, B, ldb, 1, m, n, c);
     } else {  // X op(A) = B  <=>  op(A)^T X^T = B^T, A n x n
       if (n > 512) throw std::invalid_argument("triangle above 512");
-      x.A = A;
-      x.a_rs = trans ? lda : 1;
-      x.a_cs = trans ? 1 : lda;
-      x.B = B;
-      x.b_rs = 1;
-      x.b_cs = ldb;
-      x.n = n;
-      x.nrhs = m;
-      x.lower = op_lower ? 0 : 1;
+      trsm_generic<double>(A, trans ? lda : 1, trans ? 1 : lda, !op_lower, unit, B, 1, ldb, n, m, c);
     }
-    launch_trsm<double>(c.staging().push(std::vector<TrsmItem>{x}, ctx->stream), 1, x.n, x.nrhs, ctx->stream);
     c.sync();
     return FRK_OK;
   });
@@ -467,7 +518,8 @@ int frk_dgemm(frk_ctx ctx, int trans_a, int trans_b, int m, int n, int k, double
     g.k = k;
     g.alpha = alpha;
     g.beta = beta;
-    launch_gemm<double>(c.staging().push(std::vector<GemmItem>{g}, ctx->stream), 1, m, n, ctx->stream);
+    const GemmItem* dg = static_cast<const GemmItem*>(c.staging().push_bytes(&g, sizeof(g), ctx->stream));
+    launch_gemm<double>(dg, 1, m, n, ctx->stream);
     c.sync();
     return FRK_OK;
   });

@@ -39,43 +39,58 @@ struct GemmItem {
 
 // ---------------------------------------------------------------------------
 // Batched triangular solve item (left side, op(A) already folded into the
-// strides): solve T X = B in place, T n x n, B n x nrhs.
-// lower: forward substitution; upper: backward substitution.
+// strides): solve T X = B in place, T n x n, B n x nrhs.  lower: forward
+// substitution; upper: backward.  dinv holds the inverses of T's 64 x 64
+// diagonal blocks (block k at dinv + k * 4096, element (i, j) at
+// [i * d_rs + j * d_cs]); the unit flag is already folded into them.
 // ---------------------------------------------------------------------------
+constexpr int kTriBlk = 64;
+
 struct TrsmItem {
   const void* A;
+  const void* dinv;
   void* B;
-  int64_t a_rs, a_cs, b_rs, b_cs;
+  int64_t a_rs, a_cs, b_rs, b_cs, d_rs, d_cs;
   int32_t n, nrhs;
   int32_t lower, unit;
 };
 
+// No-pivot LU of one <= 64 x 64 diagonal block plus the inverses of its
+// unit-lower and upper factors (row-major 64 x 64 outputs).
+struct LuItem {
+  void* a;
+  void* linv;
+  void* uinv;
+  int64_t rs, cs;
+  int32_t n, pad;
+};
+
+// Inverse of one <= 64 x 64 diagonal block of a triangle (strided).
+struct TriInvItem {
+  const void* a;
+  void* out;  // row-major 64 x 64
+  int64_t rs, cs;
+  int32_t n, lower, unit, pad;
+};
+
 // ---------------------------------------------------------------------------
 // Householder QR item.  The m x K input is the column concatenation
-// [A1 * S1 | A2 * S2] formed on load (S == nullptr means identity).  Optional
-// projection: before factorisation every column x is replaced by
-// x - P (P^T x), with P^T x written to Z (K2 x kp row-major per column... see
-// qr kernel).  Outputs: reflectors Y (column-major, m x K, ld = ldy; unit
-// diagonal implicit), tau[K], R (K x K row-major, ld = ldr).
+// [A1 | A2] (k1 + k2 columns, strided).  Optional projection first: every
+// column x becomes x - P (P^T x) (modified Gram-Schmidt against an
+// orthonormal P, m x kp) with the coefficients in rows [0, kp) of the R
+// buffer.  Outputs: reflectors Y (column-major, m x K, ld = ldy; unit
+// diagonal implicit), tau[K], R (rows [kp, kp + K) of the R buffer, ld = ldr).
 // ---------------------------------------------------------------------------
 struct QrItem {
   const void* A1;
-  const void* S1;
   const void* A2;
-  const void* S2;
-  const void* P;  // projection basis (m x kp), may be nullptr
+  const void* P;
   void* Y;
   void* tau;
   void* R;
-  void* Z;        // projection coefficients, row-major K x kp (ldz = kp)
   int64_t a1_rs, a1_cs, a2_rs, a2_cs, p_rs, p_cs;
-  int32_t s1_ld, s2_ld;
-  int32_t m;
-  int32_t k1a, k1;  // A1 has k1a columns, S1 is k1a x k1
-  int32_t k2a, k2;  // A2 has k2a columns, S2 is k2a x k2
-  int32_t kp;
+  int32_t m, k1, k2, kp;
   int32_t ldy, ldr;
-  int32_t pad;
 };
 
 // ---------------------------------------------------------------------------

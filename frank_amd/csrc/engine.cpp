@@ -155,7 +155,8 @@ BlrMatrix<T>::BlrMatrix(std::vector<int64_t> ro, std::vector<int64_t> co, std::v
       else if (kind[t] == kLowRank)
         total += al((size_t)rows(i) * kcap) + al((size_t)kcap * kcap) + al((size_t)kcap * cols(j));
     }
-  arena_.ensure(std::max<size_t>(total, 1) * sizeof(T));
+  arena_bytes_ = std::max<size_t>(total, 1) * sizeof(T);
+  arena_.ensure(arena_bytes_);
   T* p = arena_.as<T>();
   for (int i = 0; i < nr; ++i)
     for (int j = 0; j < nc; ++j) {
@@ -173,6 +174,23 @@ BlrMatrix<T>::BlrMatrix(std::vector<int64_t> ro, std::vector<int64_t> co, std::v
       }
     }
   FRK_HIP_CHECK(hipMemsetAsync(arena_.as<T>(), 0, std::max<size_t>(total, 1) * sizeof(T), ctx.stream()));
+  Linv.assign(nr * nc, nullptr);
+  Uinv.assign(nr * nc, nullptr);
+  size_t dtot = 0;
+  for (int i = 0; i < std::min(nr, nc); ++i)
+    if (kind[idx(i, i)] == kDense && rows(i) == cols(i)) dtot += 2 * (size_t)((rows(i) + 63) / 64) * 4096;
+  if (dtot) {
+    dinv_.ensure(dtot * sizeof(T));
+    T* q = dinv_.as<T>();
+    for (int i = 0; i < std::min(nr, nc); ++i)
+      if (kind[idx(i, i)] == kDense && rows(i) == cols(i)) {
+        const size_t nbk = (size_t)((rows(i) + 63) / 64) * 4096;
+        Linv[idx(i, i)] = q;
+        q += nbk;
+        Uinv[idx(i, i)] = q;
+        q += nbk;
+      }
+  }
   d_ranks_.ensure(sizeof(int32_t) * nr * nc);
   FRK_HIP_CHECK(hipMemsetAsync(d_ranks_.as<int32_t>(), 0, sizeof(int32_t) * nr * nc, ctx.stream()));
 }
@@ -189,6 +207,26 @@ void BlrMatrix<T>::pull_ranks(hipStream_t s) {
 }
 
 template <typename T>
+std::unique_ptr<BlrMatrix<T>> BlrMatrix<T>::clone_structure(Context& ctx) const {
+  auto c = std::make_unique<BlrMatrix<T>>(roff, coff, kind, kcap, eps, fixed_rank, ctx);
+  c->lra = lra;
+  return c;
+}
+
+template <typename T>
+void BlrMatrix<T>::copy_from(const BlrMatrix<T>& src, hipStream_t s) {
+  if (src.arena_bytes_ != arena_bytes_ || src.kind != kind || src.roff != roff || src.coff != coff)
+    throw std::invalid_argument("copy_from: tile structures differ");
+  FRK_HIP_CHECK(hipMemcpyAsync(arena_.as<char>(), src.arena_.as<char>(), arena_bytes_, hipMemcpyDeviceToDevice, s));
+  FRK_HIP_CHECK(hipMemcpyAsync(d_ranks_.as<int32_t>(), src.d_ranks_.as<int32_t>(), sizeof(int32_t) * rank.size(),
+                               hipMemcpyDeviceToDevice, s));
+  if (dinv_.bytes() && src.dinv_.bytes() == dinv_.bytes())
+    FRK_HIP_CHECK(hipMemcpyAsync(dinv_.as<char>(), src.dinv_.as<char>(), dinv_.bytes(), hipMemcpyDeviceToDevice, s));
+  rank = src.rank;
+  lra = src.lra;
+}
+
+template <typename T>
 void BlrMatrix<T>::push_ranks(hipStream_t s) {
   FRK_HIP_CHECK(hipMemcpyAsync(d_ranks_.as<int32_t>(), rank.data(), sizeof(int32_t) * rank.size(),
                                hipMemcpyHostToDevice, s));
@@ -196,8 +234,16 @@ void BlrMatrix<T>::push_ranks(hipStream_t s) {
 }
 
 // ============================================================================
-// Item helpers
+// Item helpers and plans
 // ============================================================================
+void Plan::run() {
+  if (launches_.empty()) return;
+  const char* d = static_cast<const char*>(ctx_.staging().push_bytes(bytes_.data(), bytes_.size(), ctx_.stream()));
+  for (auto& f : launches_) f(d);
+  launches_.clear();
+  bytes_.clear();
+}
+
 static GemmItem gemm_item(const void* A, int64_t a_rs, int64_t a_cs, const void* B, int64_t b_rs, int64_t b_cs,
                           void* C, int64_t c_rs, int64_t c_cs, int m, int n, int k, double alpha, double beta) {
   GemmItem g{};
@@ -218,107 +264,175 @@ static GemmItem gemm_item(const void* A, int64_t a_rs, int64_t a_cs, const void*
   return g;
 }
 
+static TrsmItem trsm_item(const void* A, int64_t a_rs, int64_t a_cs, const void* dinv, int64_t d_rs, int64_t d_cs,
+                          void* B, int64_t b_rs, int64_t b_cs, int n, int nrhs, bool lower) {
+  TrsmItem x{};
+  x.A = A;
+  x.a_rs = a_rs;
+  x.a_cs = a_cs;
+  x.dinv = dinv;
+  x.d_rs = d_rs;
+  x.d_cs = d_cs;
+  x.B = B;
+  x.b_rs = b_rs;
+  x.b_cs = b_cs;
+  x.n = n;
+  x.nrhs = nrhs;
+  x.lower = lower ? 1 : 0;
+  return x;
+}
+
 template <typename T>
-static void run_gemms(const std::vector<GemmItem>& items, Context& ctx) {
+static void plan_gemms(Plan& pl, const std::vector<GemmItem>& items) {
   if (items.empty()) return;
-  int mm = 0, nn = 0;
-  for (const auto& g : items) {
-    mm = std::max(mm, g.m);
-    nn = std::max(nn, g.n);
-  }
-  // grid.y is limited to 65535 items per launch
-  const size_t chunk = 65535;
+  const hipStream_t s = pl.ctx().stream();
+  const size_t chunk = 65535;  // grid.y limit
   for (size_t o = 0; o < items.size(); o += chunk) {
     std::vector<GemmItem> part(items.begin() + o, items.begin() + std::min(items.size(), o + chunk));
-    launch_gemm<T>(ctx.staging().push(part, ctx.stream()), (int)part.size(), mm, nn, ctx.stream());
+    int mm = 0, nn = 0;
+    for (const auto& g : part) {
+      mm = std::max(mm, g.m);
+      nn = std::max(nn, g.n);
+    }
+    const size_t off = pl.add(part);
+    const int cnt = (int)part.size();
+    pl.launch([=](const char* d) { launch_gemm<T>(reinterpret_cast<const GemmItem*>(d + off), cnt, mm, nn, s); });
   }
 }
 
 template <typename T>
-static void run_trsms(const std::vector<TrsmItem>& items, Context& ctx) {
+static void plan_trsms(Plan& pl, const std::vector<TrsmItem>& items) {
   if (items.empty()) return;
+  const hipStream_t s = pl.ctx().stream();
   int n = 0, r = 0;
   for (const auto& t : items) {
     n = std::max(n, t.n);
     r = std::max(r, t.nrhs);
   }
-  launch_trsm<T>(ctx.staging().push(items, ctx.stream()), (int)items.size(), n, r, ctx.stream());
+  const size_t off = pl.add(items);
+  const int cnt = (int)items.size();
+  pl.launch([=](const char* d) { launch_trsm<T>(reinterpret_cast<const TrsmItem*>(d + off), cnt, n, r, s); });
+}
+
+template <typename T>
+static void run_gemms(const std::vector<GemmItem>& items, Context& ctx) {
+  Plan pl(ctx);
+  plan_gemms<T>(pl, items);
+  pl.run();
 }
 
 // ============================================================================
 // Dense tile LU (blocked, no pivoting)
 // ============================================================================
 template <typename T>
-void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_dev, Context& ctx) {
+void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& linv, const std::vector<T*>& uinv,
+                int n, int64_t ld, int32_t* info_dev) {
   if (tiles.empty() || n <= 0) return;
+  const hipStream_t s = pl.ctx().stream();
   const int nt = (int)tiles.size();
-  const int npan = (n + 63) / 64;
   int pi = 0;
   for (int p = 0; p < n; p += 64, ++pi) {
     const int nbk = std::min(64, n - p);
     const int rest = n - p - nbk;
-    std::vector<GemmItem> small(nt);
+    std::vector<LuItem> small(nt);
     for (int t = 0; t < nt; ++t) {
-      small[t] = GemmItem{};
-      small[t].C = tiles[t] + p * ld + p;
-      small[t].c_rs = ld;
-      small[t].c_cs = 1;
-      small[t].m = nbk;
+      LuItem x{};
+      x.a = tiles[t] + p * ld + p;
+      x.rs = ld;
+      x.cs = 1;
+      x.n = nbk;
+      x.linv = linv[t] + (size_t)pi * 4096;
+      x.uinv = uinv[t] + (size_t)pi * 4096;
+      small[t] = x;
     }
-    launch_getrf_small<T>(ctx.staging().push(small, ctx.stream()), nt, info_dev ? info_dev + pi * nt : nullptr,
-                          ctx.stream());
+    const size_t off = pl.add(small);
+    int32_t* inf = info_dev ? info_dev + pi * nt : nullptr;
+    pl.launch([=](const char* d) { launch_getrf_small<T>(reinterpret_cast<const LuItem*>(d + off), nt, inf, s); });
     if (rest <= 0) continue;
-    std::vector<TrsmItem> tr;
-    tr.reserve(2 * nt);
+    // A12 <- inv(L11) A12 ; A21 <- A21 inv(U11)   (in place: one output tile row / column)
+    std::vector<GemmItem> pan;
+    pan.reserve(2 * nt);
     for (int t = 0; t < nt; ++t) {
       T* a = tiles[t];
-      TrsmItem x{};
-      // A12 <- L11^{-1} A12 (unit lower)
-      x.A = a + p * ld + p;
-      x.a_rs = ld;
-      x.a_cs = 1;
-      x.B = a + p * ld + p + nbk;
-      x.b_rs = ld;
-      x.b_cs = 1;
-      x.n = nbk;
-      x.nrhs = rest;
-      x.lower = 1;
-      x.unit = 1;
-      tr.push_back(x);
-      // A21 <- A21 U11^{-1}  <=>  U11^T A21^T = A21^T (lower, non-unit)
-      TrsmItem y{};
-      y.A = a + p * ld + p;
-      y.a_rs = 1;
-      y.a_cs = ld;
-      y.B = a + (p + nbk) * ld + p;
-      y.b_rs = 1;
-      y.b_cs = ld;
-      y.n = nbk;
-      y.nrhs = rest;
-      y.lower = 1;
-      y.unit = 0;
-      tr.push_back(y);
+      pan.push_back(gemm_item(linv[t] + (size_t)pi * 4096, 64, 1, a + p * ld + p + nbk, ld, 1, a + p * ld + p + nbk,
+                              ld, 1, nbk, rest, nbk, 1.0, 0.0));
+      pan.push_back(gemm_item(a + (p + nbk) * ld + p, ld, 1, uinv[t] + (size_t)pi * 4096, 64, 1,
+                              a + (p + nbk) * ld + p, ld, 1, rest, nbk, nbk, 1.0, 0.0));
     }
-    run_trsms<T>(tr, ctx);
+    plan_gemms<T>(pl, pan);
     std::vector<GemmItem> gm(nt);
     for (int t = 0; t < nt; ++t) {
       T* a = tiles[t];
       gm[t] = gemm_item(a + (p + nbk) * ld + p, ld, 1, a + p * ld + p + nbk, ld, 1, a + (p + nbk) * ld + p + nbk,
                         ld, 1, rest, rest, nbk, -1.0, 1.0);
     }
-    run_gemms<T>(gm, ctx);
+    plan_gemms<T>(pl, gm);
   }
-  (void)npan;
+}
+
+template <typename T>
+void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_dev, Context& ctx) {
+  const int nbk = (n + 63) / 64;
+  DevBuf inv(sizeof(T) * tiles.size() * nbk * 4096 * 2);
+  std::vector<T*> li(tiles.size()), ui(tiles.size());
+  for (size_t t = 0; t < tiles.size(); ++t) {
+    li[t] = inv.as<T>() + t * nbk * 4096 * 2;
+    ui[t] = li[t] + nbk * 4096;
+  }
+  Plan pl(ctx);
+  plan_getrf<T>(pl, tiles, li, ui, n, ld, info_dev);
+  pl.run();
+  FRK_HIP_CHECK(hipStreamSynchronize(ctx.stream()));
+}
+
+// Generic trsm without precomputed inverses: invert the diagonal blocks first.
+template <typename T>
+void trsm_generic(const T* A, int64_t a_rs, int64_t a_cs, bool lower, bool unit, T* B, int64_t b_rs, int64_t b_cs,
+                  int n, int nrhs, Context& ctx) {
+  const hipStream_t s = ctx.stream();
+  const int nbk = (n + 63) / 64;
+  DevBuf inv(sizeof(T) * nbk * 4096);
+  std::vector<TriInvItem> ti(nbk);
+  for (int k = 0; k < nbk; ++k) {
+    TriInvItem x{};
+    x.a = A + (int64_t)(64 * k) * a_rs + (int64_t)(64 * k) * a_cs;
+    x.out = inv.as<T>() + (size_t)k * 4096;
+    x.rs = a_rs;
+    x.cs = a_cs;
+    x.n = std::min(64, n - 64 * k);
+    x.lower = lower ? 1 : 0;
+    x.unit = unit ? 1 : 0;
+    ti[k] = x;
+  }
+  Plan pl(ctx);
+  const size_t off = pl.add(ti);
+  pl.launch([=](const char* d) { launch_tri_inv<T>(reinterpret_cast<const TriInvItem*>(d + off), nbk, s); });
+  plan_trsms<T>(pl, {trsm_item(A, a_rs, a_cs, inv.as<T>(), 64, 1, B, b_rs, b_cs, n, nrhs, lower)});
+  pl.run();
+  FRK_HIP_CHECK(hipStreamSynchronize(s));
 }
 
 // ============================================================================
 // Rounded low-rank addition (batched)
 // ============================================================================
 template <typename T>
-void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
+size_t lr_add_workspace(const std::vector<LrAddDesc<T>>& d) {
+  int kmax = 0, mmax = 0, nmax = 0;
+  for (const auto& x : d) {
+    kmax = std::max(kmax, x.kc + x.kx2);
+    mmax = std::max(mmax, x.m);
+    nmax = std::max(nmax, x.n);
+  }
+  const size_t K = kmax;
+  return d.size() * sizeof(T) * ((size_t)(mmax + nmax) * K + 2 * K + 4 * K * K) + 8 * 256 + 4096;
+}
+
+template <typename T>
+void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   if (d.empty()) return;
-  const int nd = (int)d.size();
+  Context& ctx = pl.ctx();
   const hipStream_t s = ctx.stream();
+  const int nd = (int)d.size();
   int kmax = 0, mmax = 0, nmax = 0;
   for (const auto& x : d) {
     kmax = std::max(kmax, x.kc + x.kx2);
@@ -326,7 +440,7 @@ void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
     nmax = std::max(nmax, x.n);
   }
   const int K = kmax;
-  // per-item workspace
+  // Yu doubles as the formed panel [Uc Sc | Ux Sx] (the QR reads it before writing reflectors)
   T* Yu = ctx.take<T>((size_t)nd * mmax * K);
   T* Yv = ctx.take<T>((size_t)nd * nmax * K);
   T* tu = ctx.take<T>((size_t)nd * K);
@@ -335,6 +449,16 @@ void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
   T* Rv = ctx.take<T>((size_t)nd * K * K);
   T* Um = ctx.take<T>((size_t)nd * K * K);
   T* Vm = ctx.take<T>((size_t)nd * K * K);
+  const T* I = nullptr;
+  int ild = 0;
+  for (const auto& x : d)
+    if (x.xs == nullptr) {
+      I = ctx.identity<T>(K + 1);
+      ild = ctx.ident_ld();
+      break;
+    }
+  std::vector<GemmItem> form;
+  form.reserve(2 * nd);
   std::vector<QrItem> qi;
   qi.reserve(2 * nd);
   std::vector<SvdItem> si(nd);
@@ -350,6 +474,8 @@ void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
     T* rv = Rv + (size_t)t * K * K;
     T* um = Um + (size_t)t * K * K;
     T* vm = Vm + (size_t)t * K * K;
+    const T* xs = x.xs ? x.xs : I;
+    const int xsld = x.xs ? x.xs_ld : ild;
     QrItem q{};
     q.Y = yu;
     q.ldy = x.m;
@@ -364,40 +490,31 @@ void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
     v.R = rv;
     v.m = x.n;
     if (!fast) {
-      // Uc = [Uc Sc | Ux Sx] ; Vc^T = [Vc^T | Vx^T]
-      q.A1 = x.cu;
-      q.a1_rs = x.cu_rs;
-      q.a1_cs = x.cu_cs;
-      q.S1 = x.cs;
-      q.s1_ld = x.lds;
-      q.k1a = q.k1 = x.kc;
-      q.A2 = x.xu;
-      q.a2_rs = x.xu_rs;
-      q.a2_cs = x.xu_cs;
-      q.S2 = x.xs;
-      q.s2_ld = x.xs_ld;
-      q.k2a = x.kx1;
-      q.k2 = x.kx2;
+      // Uc = [Uc Sc | Ux Sx] (formed by GEMM), Vc^T = [Vc^T | Vx^T]
+      form.push_back(gemm_item(x.cu, x.cu_rs, x.cu_cs, x.cs, x.lds, 1, yu, 1, x.m, x.m, x.kc, x.kc, 1.0, 0.0));
+      form.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, xs, xsld, 1, yu + (size_t)x.kc * x.m, 1, x.m, x.m, x.kx2,
+                               x.kx1, 1.0, 0.0));
+      q.A1 = yu;
+      q.a1_rs = 1;
+      q.a1_cs = x.m;
+      q.k1 = Kt;
+      q.k2 = 0;
       v.A1 = x.cv;
       v.a1_rs = x.cv_cs;
       v.a1_cs = x.cv_rs;
-      v.S1 = nullptr;
-      v.k1a = v.k1 = x.kc;
+      v.k1 = x.kc;
       v.A2 = x.xv;
       v.a2_rs = x.xv_cs;
       v.a2_cs = x.xv_rs;
-      v.S2 = nullptr;
-      v.k2a = v.k2 = x.kx2;
+      v.k2 = x.kx2;
     } else {
       // project X's bases on the (orthonormal) bases of C, QR the remainders
-      q.A1 = x.xu;
-      q.a1_rs = x.xu_rs;
-      q.a1_cs = x.xu_cs;
-      q.S1 = x.xs;
-      q.s1_ld = x.xs_ld;
-      q.k1a = x.kx1;
+      form.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, xs, xsld, 1, yu, 1, x.m, x.m, x.kx2, x.kx1, 1.0, 0.0));
+      q.A1 = yu;
+      q.a1_rs = 1;
+      q.a1_cs = x.m;
       q.k1 = x.kx2;
-      q.k2a = q.k2 = 0;
+      q.k2 = 0;
       q.P = x.cu;
       q.p_rs = x.cu_rs;
       q.p_cs = x.cu_cs;
@@ -405,9 +522,8 @@ void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
       v.A1 = x.xv;
       v.a1_rs = x.xv_cs;
       v.a1_cs = x.xv_rs;
-      v.S1 = nullptr;
-      v.k1a = v.k1 = x.kx2;
-      v.k2a = v.k2 = 0;
+      v.k1 = x.kx2;
+      v.k2 = 0;
       v.P = x.cv;
       v.p_rs = x.cv_cs;
       v.p_cs = x.cv_rs;
@@ -504,11 +620,22 @@ void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
   }
   int qk = 0;
   for (const auto& q : qi) qk = std::max(qk, q.k1 + q.k2);
-  launch_qr<T>(ctx.staging().push(qi, s), (int)qi.size(), qk, s);
-  launch_svd<T>(ctx.staging().push(si, s), nd, K, s);
   int capmax = 0;
   for (const auto& x : d) capmax = std::max(capmax, std::min(x.cap, x.kc + x.kx2));
-  launch_apply_q<T>(ctx.staging().push(ai, s), (int)ai.size(), capmax, s);
+  plan_gemms<T>(pl, form);
+  const size_t oq = pl.add(qi), os = pl.add(si), oa = pl.add(ai);
+  const int nq = (int)qi.size(), na = (int)ai.size();
+  pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oq), nq, qk, s); });
+  pl.launch([=](const char* dd) { launch_svd<T>(reinterpret_cast<const SvdItem*>(dd + os), nd, K, s); });
+  pl.launch([=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oa), na, capmax, s); });
+}
+
+template <typename T>
+void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
+  if (d.empty()) return;
+  Plan pl(ctx);
+  plan_lr_add<T>(pl, d, lra);
+  pl.run();
 }
 
 // ============================================================================
@@ -521,17 +648,18 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
   std::vector<CompressDesc<T>> de, dr;
   for (const auto& x : d) (x.eps != 0.0 ? de : dr).push_back(x);
   if (!de.empty()) {
+    // truncated_geqp3 (src/operations/LAPACK/geqp3.cpp:71-191) then rq of R (low_rank.cpp:50-59)
     const int nd = (int)de.size();
-    int nmax = 0, mmax = 0;
+    int nmax = 0, mmax = 0, capmax = 0;
     for (const auto& x : de) {
       nmax = std::max(nmax, x.n);
       mmax = std::max(mmax, x.m);
+      capmax = std::max(capmax, x.cap);
     }
     ctx.sync();
-    const int capmax = de[0].cap;
     ctx.reserve((size_t)nd * sizeof(T) * ((size_t)nmax + (size_t)mmax + (size_t)capmax * nmax * 2 + 4 * capmax +
                                             (size_t)capmax * capmax + 64) +
-                (size_t)nd * sizeof(int32_t) * (nmax + 64) + (size_t)nd * 4096);
+                (size_t)nd * sizeof(int32_t) * (nmax + 64) + (size_t)16 * 4096);
     T* tau = ctx.take<T>((size_t)nd * std::max(mmax, nmax));
     int32_t* piv = ctx.take<int32_t>((size_t)nd * nmax);
     std::vector<Geqp3Item> gi(nd);
@@ -548,8 +676,12 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       g.eps = de[t].eps;
       gi[t] = g;
     }
-    launch_geqp3<T>(ctx.staging().push(gi, s), nd, s);
-    // ranks to host
+    {
+      Plan pl(ctx);
+      const size_t og = pl.add(gi);
+      pl.launch([=](const char* dd) { launch_geqp3<T>(reinterpret_cast<const Geqp3Item*>(dd + og), nd, s); });
+      pl.run();
+    }
     std::vector<int32_t> rk(nd);
     for (int t = 0; t < nd; ++t)
       FRK_HIP_CHECK(hipMemcpyAsync(&rk[t], de[t].d_rank, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -560,17 +692,16 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
         throw std::runtime_error("low-rank capacity exceeded during compression: rank " + std::to_string(-rk[t]));
       rmax = std::max(rmax, rk[t]);
     }
-    const T* I = ctx.identity<T>(std::max(rmax, 1) + 1);
+    const T* I = ctx.identity<T>(rmax + 1);
     const int ild = ctx.ident_ld();
     T* RP = ctx.take<T>((size_t)nd * rmax * nmax);
     T* Yz = ctx.take<T>((size_t)nd * nmax * rmax);
     T* tz = ctx.take<T>((size_t)nd * rmax);
     T* Rz = ctx.take<T>((size_t)nd * rmax * rmax);
-    std::vector<ApplyQItem> au;
+    std::vector<ApplyQItem> au, av;
     std::vector<PermuteItem> pi(nd);
     std::vector<QrItem> qz(nd);
     std::vector<FlipItem> fl(nd);
-    std::vector<ApplyQItem> av;
     for (int t = 0; t < nd; ++t) {
       const auto& x = de[t];
       const int r = rk[t];
@@ -591,7 +722,6 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       a.out_cs = x.u_cs;
       au.push_back(a);
       T* rp = RP + (size_t)t * rmax * nmax;
-      FRK_HIP_CHECK(hipMemsetAsync(rp, 0, sizeof(T) * (size_t)r * x.n, s));
       PermuteItem p{};
       p.A = x.a;
       p.piv = gi[t].piv;
@@ -601,11 +731,12 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       p.lda = (int)x.lda;
       p.ldo = x.n;
       pi[t] = p;
+      // RQ of RP via QR of its column-reversed transpose Z(i, c) = RP[r-1-c][i]
       QrItem q{};
       q.A1 = rp + (size_t)(r - 1) * x.n;
       q.a1_rs = 1;
       q.a1_cs = -(int64_t)x.n;
-      q.k1a = q.k1 = r;
+      q.k1 = r;
       q.m = x.n;
       q.Y = Yz + (size_t)t * nmax * rmax;
       q.ldy = x.n;
@@ -637,14 +768,32 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       b.out_cs = x.v_rs;
       av.push_back(b);
     }
-    launch_apply_q<T>(ctx.staging().push(au, s), nd, rmax, s);
-    launch_permute_r<T>(ctx.staging().push(pi, s), nd, nmax, rmax, s);
-    launch_qr<T>(ctx.staging().push(qz, s), nd, rmax, s);
-    launch_flip_r<T>(ctx.staging().push(fl, s), nd, rmax, s);
-    launch_apply_q<T>(ctx.staging().push(av, s), nd, rmax, s);
+    Plan pl(ctx);
+    const size_t oau = pl.add(au), opi = pl.add(pi), oqz = pl.add(qz), ofl = pl.add(fl), oav = pl.add(av);
+    pl.launch([=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oau), nd, rmax, s); });
+    pl.launch([=](const char* dd) {
+      launch_permute_r<T>(reinterpret_cast<const PermuteItem*>(dd + opi), nd, nmax, rmax, s);
+    });
+    pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oqz), nd, rmax, s); });
+    pl.launch([=](const char* dd) { launch_flip_r<T>(reinterpret_cast<const FlipItem*>(dd + ofl), nd, rmax, s); });
+    pl.launch([=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oav), nd, rmax, s); });
+    pl.run();
     ctx.sync();
   }
   if (!dr.empty()) {
+    // rsvd (src/operations/randomized/rsvd.cpp:17-29) at p = min(rank + 5, m, n), truncated to rank.
+    // Every call shares one Omega(n, p): split batches whose tiles differ in (n, p).
+    auto np_of = [](const CompressDesc<T>& x) {
+      return std::make_pair(x.n, std::min(std::min(x.rank + 5, x.m), x.n));
+    };
+    for (const auto& x : dr)
+      if (np_of(x) != np_of(dr[0])) {
+        std::vector<CompressDesc<T>> same, rest;
+        for (const auto& y : dr) (np_of(y) == np_of(dr[0]) ? same : rest).push_back(y);
+        compress_batched<T>(same, ctx);
+        compress_batched<T>(rest, ctx);
+        return;
+      }
     const int nd = (int)dr.size();
     ctx.sync();
     int nmax = 0, mmax = 0, pmax = 0;
@@ -653,16 +802,15 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       mmax = std::max(mmax, x.m);
       pmax = std::max(pmax, std::min(std::min(x.rank + 5, x.m), x.n));
     }
-    ctx.reserve((size_t)nd * sizeof(T) * ((size_t)mmax * pmax * 2 + (size_t)nmax * pmax * 2 + 3 * pmax * pmax + 4 * pmax + 64) +
-                (size_t)nd * 4096);
-    // all tiles share one Omega only when they share n and p
-    std::vector<GemmItem> g1;
-    std::vector<QrItem> q1;
-    std::vector<ApplyQItem> a1;
-    std::vector<GemmItem> g2;
-    std::vector<QrItem> q2;
+    const T* I = ctx.identity<T>(pmax + 1);
+    const int ild = ctx.ident_ld();
+    ctx.reserve((size_t)nd * sizeof(T) *
+                    ((size_t)mmax * pmax * 2 + (size_t)nmax * pmax + 4 * (size_t)pmax * pmax + 4 * pmax + 64) +
+                (size_t)16 * 4096);
+    std::vector<GemmItem> g1, g2;
+    std::vector<QrItem> q1, q2;
+    std::vector<ApplyQItem> a1, a2;
     std::vector<SvdItem> sv;
-    std::vector<ApplyQItem> a2;
     T* Yb = ctx.take<T>((size_t)nd * mmax * pmax);
     T* Qe = ctx.take<T>((size_t)nd * mmax * pmax);
     T* Zb = ctx.take<T>((size_t)nd * nmax * pmax);
@@ -672,8 +820,6 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
     T* Rz = ctx.take<T>((size_t)nd * pmax * pmax);
     T* Um = ctx.take<T>((size_t)nd * pmax * pmax);
     T* Vm = ctx.take<T>((size_t)nd * pmax * pmax);
-    const T* I = ctx.identity<T>(pmax + 1);
-    const int ild = ctx.ident_ld();
     for (int t = 0; t < nd; ++t) {
       const auto& x = dr[t];
       const int p = std::min(std::min(x.rank + 5, x.m), x.n);
@@ -681,13 +827,12 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       T* y = Yb + (size_t)t * mmax * pmax;
       T* qe = Qe + (size_t)t * mmax * pmax;
       T* z = Zb + (size_t)t * nmax * pmax;
-      // Y = A * Omega (m x p, column-major)
-      g1.push_back(gemm_item(x.a, x.lda, 1, Om, p, 1, y, 1, x.m, x.m, p, x.n, 1.0, 0.0));
+      g1.push_back(gemm_item(x.a, x.lda, 1, Om, p, 1, y, 1, x.m, x.m, p, x.n, 1.0, 0.0));  // Y = A Omega
       QrItem q{};
       q.A1 = y;
       q.a1_rs = 1;
       q.a1_cs = x.m;
-      q.k1a = q.k1 = p;
+      q.k1 = p;
       q.m = x.m;
       q.Y = y;
       q.ldy = x.m;
@@ -711,13 +856,12 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       a.out_rs = 1;
       a.out_cs = x.m;
       a1.push_back(a);
-      // Z = A^T Qe (n x p, column-major) = (Q^T A)^T
-      g2.push_back(gemm_item(x.a, 1, x.lda, qe, 1, x.m, z, 1, x.n, x.n, p, x.m, 1.0, 0.0));
+      g2.push_back(gemm_item(x.a, 1, x.lda, qe, 1, x.m, z, 1, x.n, x.n, p, x.m, 1.0, 0.0));  // Z = A^T Q
       QrItem qq{};
       qq.A1 = z;
       qq.a1_rs = 1;
       qq.a1_cs = x.n;
-      qq.k1a = qq.k1 = p;
+      qq.k1 = p;
       qq.m = x.n;
       qq.Y = z;
       qq.ldy = x.n;
@@ -729,7 +873,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       s0.mode = 0;
       s0.M = qq.R;
       s0.m_rs = 1;
-      s0.m_cs = p;  // M = Rz^T
+      s0.m_cs = p;  // M = Rz^T = (Q^T A) Qz
       s0.K = p;
       s0.fixed_rank = x.rank;
       s0.S = x.s;
@@ -774,13 +918,19 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       c.out_cs = x.v_rs;
       a2.push_back(c);
     }
-    run_gemms<T>(g1, ctx);
-    launch_qr<T>(ctx.staging().push(q1, s), nd, pmax, s);
-    launch_apply_q<T>(ctx.staging().push(a1, s), nd, pmax, s);
-    run_gemms<T>(g2, ctx);
-    launch_qr<T>(ctx.staging().push(q2, s), nd, pmax, s);
-    launch_svd<T>(ctx.staging().push(sv, s), nd, pmax, s);
-    launch_apply_q<T>(ctx.staging().push(a2, s), (int)a2.size(), pmax, s);
+    Plan pl(ctx);
+    plan_gemms<T>(pl, g1);
+    const size_t oq1 = pl.add(q1), oa1 = pl.add(a1);
+    pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oq1), nd, pmax, s); });
+    pl.launch([=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oa1), nd, pmax, s); });
+    plan_gemms<T>(pl, g2);
+    const size_t oq2 = pl.add(q2), osv = pl.add(sv), oa2 = pl.add(a2);
+    const int na2 = (int)a2.size();
+    pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oq2), nd, pmax, s); });
+    pl.launch([=](const char* dd) { launch_svd<T>(reinterpret_cast<const SvdItem*>(dd + osv), nd, pmax, s); });
+    pl.launch(
+        [=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oa2), na2, pmax, s); });
+    pl.run();
     ctx.sync();
   }
 }
@@ -789,7 +939,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
 // Construction from a kernel function
 // ============================================================================
 static std::vector<int64_t> index_split(int64_t n, int nsplit) {
-  // src/classes/initialization_helpers/index_range.cpp:15-24
+  // src/classes/initialization_helpers/index_range.cpp:15-24 (ceil split, last block smaller)
   std::vector<int64_t> off(nsplit + 1);
   const int64_t child = (n + nsplit - 1) / nsplit;
   for (int i = 0; i < nsplit; ++i) off[i] = child * i;
@@ -850,7 +1000,6 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, dou
   DevBuf pts;
   KernelSpec ks = upload_kernel(kd, pts, s);
   ctx.sync();
-  // dense tiles straight into their slots
   std::vector<GenItem> gd;
   int mr = 0, mc = 0;
   for (int i = 0; i < nb; ++i)
@@ -867,8 +1016,13 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, dou
         mr = std::max<int>(mr, g.nr);
         mc = std::max<int>(mc, g.nc);
       }
-  launch_gen<T>(ks, ctx.staging().push(gd, s), (int)gd.size(), mr, mc, s);
-  // low-rank tiles: generate a chunk densely, compress into the slots
+  {
+    Plan pl(ctx);
+    const size_t og = pl.add(gd);
+    const int ng = (int)gd.size();
+    pl.launch([=](const char* dd) { launch_gen<T>(ks, reinterpret_cast<const GenItem*>(dd + og), ng, mr, mc, s); });
+    pl.run();
+  }
   std::vector<std::pair<int, int>> lr;
   for (int i = 0; i < nb; ++i)
     for (int j = 0; j < nb; ++j)
@@ -912,7 +1066,14 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, dou
       c.cap = kcap;
       cd.push_back(c);
     }
-    launch_gen<T>(ks, ctx.staging().push(gl, s), (int)gl.size(), (int)bmax, (int)bmax, s);
+    {
+      Plan pl(ctx);
+      const size_t og = pl.add(gl);
+      const int ng = (int)gl.size();
+      const int bm = (int)bmax;
+      pl.launch([=](const char* dd) { launch_gen<T>(ks, reinterpret_cast<const GenItem*>(dd + og), ng, bm, bm, s); });
+      pl.run();
+    }
     compress_batched<T>(cd, ctx);
   }
   ctx.sync();
@@ -937,259 +1098,228 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     A.pull_ranks(s);
     const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
     const int ii = A.idx(i, i);
-    if (A.kind[ii] != kDense) throw std::runtime_error("getrf: diagonal tile is not dense");
+    if (A.kind[ii] != kDense || m_i != n_i) throw std::runtime_error("getrf: diagonal tile is not square dense");
     if (m_i > 512) throw std::runtime_error("getrf: tile size above 512");
-    // workspace sizing for this step
     const int rem = nb - 1 - i;
+    // ---- plan the step ---------------------------------------------------
+    std::vector<LrAddDesc<T>> adds;
+    std::vector<CompressDesc<T>> comp;
+    std::vector<int> comp_add;
     size_t need = 4096;
     {
-      size_t upd = (size_t)rem * rem;
-      int kmax = 0;
+      // count workspace: panel products + per-update temporaries + addition
+      int kmax = 1;
       for (int r = 0; r < (int)A.rank.size(); ++r) kmax = std::max(kmax, A.rank[r]);
-      const size_t K = (size_t)2 * std::max(kmax, 1) + 1;
       const size_t bm = (size_t)std::max<int64_t>(A.rows(0), A.cols(0));
-      need += upd * sizeof(T) * (2 * bm * K + 6 * K * K + 2 * K + (size_t)A.kcap * A.kcap + bm * A.kcap + 64);
-      need += (size_t)rem * 2 * sizeof(T) * (bm * A.kcap + 64);
-      need += upd * sizeof(T) * 4 * 64;
-      // dense products feeding D.D -> LR compression
-      size_t ndd = 0, nld = 0;
+      need += (size_t)rem * 2 * sizeof(T) * (bm * kmax + 256);
       for (int ic = i + 1; ic < nb; ++ic)
         for (int k = i + 1; k < nb; ++k) {
-          if (A.kind[A.idx(ic, i)] == kDense && A.kind[A.idx(i, k)] == kDense && A.kind[A.idx(ic, k)] == kLowRank)
-            ++ndd;
-          if (A.kind[A.idx(ic, k)] == kDense) ++nld;
+          const int kl = A.kind[A.idx(ic, i)], kb = A.kind[A.idx(i, k)], kc = A.kind[A.idx(ic, k)];
+          if (kc == kDense)
+            need += sizeof(T) * (2 * bm * kmax + (size_t)kmax * kmax + 512);
+          else if (kl == kDense && kb == kDense)
+            need += sizeof(T) * (bm * bm + bm * A.kcap * 2 + (size_t)A.kcap * A.kcap + 1024);
+          else
+            need += sizeof(T) * (bm * kmax + (size_t)kmax * kmax + 512);
+          if (kc == kLowRank) {
+            const size_t K = (size_t)kmax + ((kl == kDense && kb == kDense) ? (size_t)A.kcap : (size_t)kmax);
+            need += sizeof(T) * (2 * bm * K + 2 * K + 4 * K * K) + 8 * 256;
+          }
         }
-      need += (ndd + nld) * sizeof(T) * (bm * bm + 64);
     }
     ctx.reserve(need);
+    Plan pl(ctx);
     // 1. diagonal tile
-    getrf_tiles<T>({A.D[ii]}, m_i, n_i, info + i * npan_max, ctx);
+    plan_getrf<T>(pl, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info + i * npan_max);
     const T* Dii = A.D[ii];
     const int64_t ldd = n_i;
-    // 2. panel solves
+    // 2. panel solves (src/operations/LAPACK/getrf.cpp:50-57)
     std::vector<TrsmItem> tr;
     for (int ic = i + 1; ic < nb; ++ic) {
       const int t = A.idx(ic, i);
-      TrsmItem x{};
-      x.A = Dii;
-      x.a_rs = 1;
-      x.a_cs = ldd;
-      x.lower = 1;
-      x.unit = 0;
-      x.n = n_i;
-      if (A.kind[t] == kLowRank) {  // V <- V U^{-1}
-        x.B = A.V[t];
-        x.b_rs = 1;
-        x.b_cs = A.cols(i);
-        x.nrhs = A.rank[t];
-      } else if (A.kind[t] == kDense) {
-        x.B = A.D[t];
-        x.b_rs = 1;
-        x.b_cs = A.cols(i);
-        x.nrhs = (int)A.rows(ic);
-      } else
-        continue;
-      tr.push_back(x);
+      if (A.kind[t] == kLowRank)  // V <- V U^-1  <=>  U^T V^T = V^T
+        tr.push_back(trsm_item(Dii, 1, ldd, A.Uinv[ii], 1, 64, A.V[t], 1, A.cols(i), n_i, A.rank[t], true));
+      else if (A.kind[t] == kDense)
+        tr.push_back(trsm_item(Dii, 1, ldd, A.Uinv[ii], 1, 64, A.D[t], 1, A.cols(i), n_i, (int)A.rows(ic), true));
     }
     for (int j = i + 1; j < nb; ++j) {
       const int t = A.idx(i, j);
-      TrsmItem x{};
-      x.A = Dii;
-      x.a_rs = ldd;
-      x.a_cs = 1;
-      x.lower = 1;
-      x.unit = 1;
-      x.n = m_i;
-      if (A.kind[t] == kLowRank) {  // U <- L^{-1} U
-        x.B = A.U[t];
-        x.b_rs = 1;
-        x.b_cs = A.rows(i);
-        x.nrhs = A.rank[t];
-      } else if (A.kind[t] == kDense) {
-        x.B = A.D[t];
-        x.b_rs = A.cols(j);
-        x.b_cs = 1;
-        x.nrhs = (int)A.cols(j);
-      } else
-        continue;
-      tr.push_back(x);
+      if (A.kind[t] == kLowRank)  // U <- L^-1 U
+        tr.push_back(trsm_item(Dii, ldd, 1, A.Linv[ii], 64, 1, A.U[t], 1, A.rows(i), m_i, A.rank[t], true));
+      else if (A.kind[t] == kDense)
+        tr.push_back(trsm_item(Dii, ldd, 1, A.Linv[ii], 64, 1, A.D[t], A.cols(j), 1, m_i, (int)A.cols(j), true));
     }
-    run_trsms<T>(tr, ctx);
-    if (rem == 0) break;
-    // 3. Schur updates A(ic,k) -= L(ic,i) A(i,k)
-    std::vector<T*> P(nb, nullptr), Q(nb, nullptr);
-    std::vector<GemmItem> gA, gB, gC, gD;
-    for (int ic = i + 1; ic < nb; ++ic) {
-      const int t = A.idx(ic, i);
-      if (A.kind[t] != kLowRank) continue;
-      const int kl = A.rank[t];
-      P[ic] = ctx.take<T>((size_t)kl * n_i + 1);
-      // P = S_L V_L (kl x n_i, row-major)
-      gA.push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, 1.0, 0.0));
-    }
-    for (int k = i + 1; k < nb; ++k) {
-      const int t = A.idx(i, k);
-      if (A.kind[t] != kLowRank) continue;
-      const int kb = A.rank[t];
-      Q[k] = ctx.take<T>((size_t)m_i * kb + 1);
-      // Q = U_B S_B (m_i x kb, column-major)
-      gA.push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
-    }
-    std::vector<LrAddDesc<T>> adds;
-    std::vector<CompressDesc<T>> comp;
-    std::vector<std::pair<int, int>> comp_tiles;
-    std::vector<std::array<T*, 3>> comp_slots;
-    for (int ic = i + 1; ic < nb; ++ic)
+    plan_trsms<T>(pl, tr);
+    if (rem > 0) {
+      // 3. Schur updates A(ic,k) -= L(ic,i) A(i,k) (getrf.cpp:58-62, gemm.cpp:126-272)
+      std::vector<T*> P(nb, nullptr), Q(nb, nullptr);
+      std::vector<GemmItem> gA, gB, gC, gD;
+      for (int ic = i + 1; ic < nb; ++ic) {
+        const int t = A.idx(ic, i);
+        if (A.kind[t] != kLowRank) continue;
+        const int kl = A.rank[t];
+        P[ic] = ctx.take<T>((size_t)kl * n_i + 1);
+        gA.push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, 1.0, 0.0));
+      }
       for (int k = i + 1; k < nb; ++k) {
-        const int tl = A.idx(ic, i), tb = A.idx(i, k), tc = A.idx(ic, k);
-        const int kl = A.kind[tl], kb = A.kind[tb], kc = A.kind[tc];
-        const int m = (int)A.rows(ic), n = (int)A.cols(k);
-        if (kl == kEmpty || kb == kEmpty || kc == kEmpty) continue;
-        if (kc == kDense) {
-          T* C = A.D[tc];
-          if (kl == kLowRank && kb == kLowRank) {
+        const int t = A.idx(i, k);
+        if (A.kind[t] != kLowRank) continue;
+        const int kb = A.rank[t];
+        Q[k] = ctx.take<T>((size_t)m_i * kb + 1);
+        gA.push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
+      }
+      for (int ic = i + 1; ic < nb; ++ic)
+        for (int k = i + 1; k < nb; ++k) {
+          const int tl = A.idx(ic, i), tb = A.idx(i, k), tc = A.idx(ic, k);
+          const int kl = A.kind[tl], kb = A.kind[tb], kc = A.kind[tc];
+          const int m = (int)A.rows(ic), n = (int)A.cols(k);
+          if (kl == kEmpty || kb == kEmpty || kc == kEmpty) continue;
+          if (kc == kDense) {
+            T* C = A.D[tc];
+            if (kl == kLowRank && kb == kLowRank) {
+              const int rl = A.rank[tl], rb = A.rank[tb];
+              T* Sx = ctx.take<T>((size_t)rl * rb + 1);
+              T* z = ctx.take<T>((size_t)m * rb + 1);
+              gB.push_back(gemm_item(P[ic], n_i, 1, Q[k], 1, m_i, Sx, rb, 1, rl, rb, n_i, -1.0, 0.0));
+              gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, rb, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
+              gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+            } else if (kl == kLowRank && kb == kDense) {
+              const int rl = A.rank[tl];
+              T* w = ctx.take<T>((size_t)rl * n + 1);
+              T* w2 = ctx.take<T>((size_t)rl * n + 1);
+              gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
+              gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
+              gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
+            } else if (kl == kDense && kb == kLowRank) {
+              const int rb = A.rank[tb];
+              T* a = ctx.take<T>((size_t)m * rb + 1);
+              T* a2 = ctx.take<T>((size_t)m * rb + 1);
+              gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
+              gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
+              gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+            } else {
+              gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, C, n, 1, m, n, n_i, -1.0, 1.0));
+            }
+            continue;
+          }
+          LrAddDesc<T> ad{};
+          ad.cu = A.U[tc];
+          ad.cs = A.S[tc];
+          ad.cv = A.V[tc];
+          ad.d_rank = A.d_rank(ic, k);
+          ad.kc = A.rank[tc];
+          ad.cu_rs = 1;
+          ad.cu_cs = m;
+          ad.cv_rs = n;
+          ad.cv_cs = 1;
+          ad.lds = A.kcap;
+          ad.m = m;
+          ad.n = n;
+          ad.eps = A.eps;
+          ad.cap = A.kcap;
+          if (kl == kLowRank && kb == kLowRank) {  // gemm.cpp:253-272
             const int rl = A.rank[tl], rb = A.rank[tb];
             T* Sx = ctx.take<T>((size_t)rl * rb + 1);
-            T* z = ctx.take<T>((size_t)m * rb + 1);
             gB.push_back(gemm_item(P[ic], n_i, 1, Q[k], 1, m_i, Sx, rb, 1, rl, rb, n_i, -1.0, 0.0));
-            gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, rb, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
-            gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
-          } else if (kl == kLowRank && kb == kDense) {
+            ad.xu = A.U[tl];
+            ad.xu_rs = 1;
+            ad.xu_cs = A.rows(ic);
+            ad.kx1 = rl;
+            ad.xs = Sx;
+            ad.xs_ld = rb;
+            ad.kx2 = rb;
+            ad.xv = A.V[tb];
+            ad.xv_rs = A.cols(k);
+            ad.xv_cs = 1;
+          } else if (kl == kLowRank && kb == kDense) {  // gemm.cpp:219-234
             const int rl = A.rank[tl];
             T* w = ctx.take<T>((size_t)rl * n + 1);
-            T* w2 = ctx.take<T>((size_t)rl * n + 1);
             gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
-            gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
-            gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
-          } else if (kl == kDense && kb == kLowRank) {
+            ad.xu = A.U[tl];
+            ad.xu_rs = 1;
+            ad.xu_cs = A.rows(ic);
+            ad.kx1 = rl;
+            ad.xs = A.S[tl];
+            ad.xs_ld = A.kcap;
+            ad.kx2 = rl;
+            ad.xv = w;
+            ad.xv_rs = n;
+            ad.xv_cs = 1;
+          } else if (kl == kDense && kb == kLowRank) {  // gemm.cpp:236-251
             const int rb = A.rank[tb];
             T* a = ctx.take<T>((size_t)m * rb + 1);
-            T* a2 = ctx.take<T>((size_t)m * rb + 1);
             gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
-            gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
-            gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
-          } else {
-            gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, C, n, 1, m, n, n_i, -1.0, 1.0));
+            ad.xu = a;
+            ad.xu_rs = 1;
+            ad.xu_cs = m;
+            ad.kx1 = rb;
+            ad.xs = A.S[tb];
+            ad.xs_ld = A.kcap;
+            ad.kx2 = rb;
+            ad.xv = A.V[tb];
+            ad.xv_rs = A.cols(k);
+            ad.xv_cs = 1;
+          } else {  // D.D -> LR: compress the dense product first (gemm.cpp:202-217)
+            T* prod = ctx.take<T>((size_t)m * n + 1);
+            gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, prod, n, 1, m, n, n_i, -1.0, 0.0));
+            T* xu = ctx.take<T>((size_t)m * A.kcap + 1);
+            T* xs = ctx.take<T>((size_t)A.kcap * A.kcap + 1);
+            T* xv = ctx.take<T>((size_t)A.kcap * n + 1);
+            CompressDesc<T> c{};
+            c.a = prod;
+            c.m = m;
+            c.n = n;
+            c.lda = n;
+            c.eps = A.eps;
+            c.rank = A.rank[tc];
+            c.u = xu;
+            c.s = xs;
+            c.v = xv;
+            c.u_rs = 1;
+            c.u_cs = m;
+            c.v_rs = n;
+            c.v_cs = 1;
+            c.lds = A.kcap;
+            c.cap = A.kcap;
+            comp.push_back(c);
+            comp_add.push_back((int)adds.size());
+            ad.xu = xu;
+            ad.xu_rs = 1;
+            ad.xu_cs = m;
+            ad.xs = xs;
+            ad.xs_ld = A.kcap;
+            ad.xv = xv;
+            ad.xv_rs = n;
+            ad.xv_cs = 1;
+            ad.kx1 = ad.kx2 = -1;  // known after compression
           }
-          continue;
+          adds.push_back(ad);
         }
-        // low-rank target: C += X
-        LrAddDesc<T> ad{};
-        ad.cu = A.U[tc];
-        ad.cs = A.S[tc];
-        ad.cv = A.V[tc];
-        ad.d_rank = A.d_rank(ic, k);
-        ad.kc = A.rank[tc];
-        ad.cu_rs = 1;
-        ad.cu_cs = m;
-        ad.cv_rs = n;
-        ad.cv_cs = 1;
-        ad.lds = A.kcap;
-        ad.m = m;
-        ad.n = n;
-        ad.eps = A.eps;
-        ad.cap = A.kcap;
-        if (kl == kLowRank && kb == kLowRank) {
-          const int rl = A.rank[tl], rb = A.rank[tb];
-          T* Sx = ctx.take<T>((size_t)rl * rb + 1);
-          gB.push_back(gemm_item(P[ic], n_i, 1, Q[k], 1, m_i, Sx, rb, 1, rl, rb, n_i, -1.0, 0.0));
-          ad.xu = A.U[tl];
-          ad.xu_rs = 1;
-          ad.xu_cs = A.rows(ic);
-          ad.kx1 = rl;
-          ad.xs = Sx;
-          ad.xs_ld = rb;
-          ad.kx2 = rb;
-          ad.xv = A.V[tb];
-          ad.xv_rs = A.cols(k);
-          ad.xv_cs = 1;
-        } else if (kl == kLowRank && kb == kDense) {
-          const int rl = A.rank[tl];
-          T* w = ctx.take<T>((size_t)rl * n + 1);
-          gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
-          ad.xu = A.U[tl];
-          ad.xu_rs = 1;
-          ad.xu_cs = A.rows(ic);
-          ad.kx1 = rl;
-          ad.xs = A.S[tl];
-          ad.xs_ld = A.kcap;
-          ad.kx2 = rl;
-          ad.xv = w;
-          ad.xv_rs = n;
-          ad.xv_cs = 1;
-        } else if (kl == kDense && kb == kLowRank) {
-          const int rb = A.rank[tb];
-          T* a = ctx.take<T>((size_t)m * rb + 1);
-          gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
-          ad.xu = a;
-          ad.xu_rs = 1;
-          ad.xu_cs = m;
-          ad.kx1 = rb;
-          ad.xs = A.S[tb];
-          ad.xs_ld = A.kcap;
-          ad.kx2 = rb;
-          ad.xv = A.V[tb];
-          ad.xv_rs = A.cols(k);
-          ad.xv_cs = 1;
-        } else {
-          // D.D -> LR: compress the dense product first (gemm.cpp:202-217)
-          T* prod = ctx.take<T>((size_t)m * n + 1);
-          gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, prod, n, 1, m, n, n_i, -1.0, 0.0));
-          T* xu = ctx.take<T>((size_t)m * A.kcap + 1);
-          T* xs = ctx.take<T>((size_t)A.kcap * A.kcap + 1);
-          T* xv = ctx.take<T>((size_t)A.kcap * n + 1);
-          CompressDesc<T> c{};
-          c.a = prod;
-          c.m = m;
-          c.n = n;
-          c.lda = n;
-          c.eps = A.eps;
-          c.rank = A.rank[tc];
-          c.u = xu;
-          c.s = xs;
-          c.v = xv;
-          c.u_rs = 1;
-          c.u_cs = m;
-          c.v_rs = n;
-          c.v_cs = 1;
-          c.lds = A.kcap;
-          c.cap = A.kcap;
-          c.d_rank = nullptr;  // assigned below
-          comp.push_back(c);
-          comp_slots.push_back({xu, xs, xv});
-          comp_tiles.emplace_back((int)adds.size(), 0);
-          ad.xu = xu;
-          ad.xu_rs = 1;
-          ad.xu_cs = m;
-          ad.xs = xs;
-          ad.xs_ld = A.kcap;
-          ad.xv = xv;
-          ad.xv_rs = n;
-          ad.xv_cs = 1;
-          ad.kx1 = ad.kx2 = -1;  // known after compression
-        }
-        adds.push_back(ad);
+      plan_gemms<T>(pl, gA);
+      plan_gemms<T>(pl, gB);
+      plan_gemms<T>(pl, gC);
+      plan_gemms<T>(pl, gD);
+      if (!comp.empty()) {
+        // D.D -> LR needs the compressed ranks on the host: run what is planned,
+        // compress on a side context (own workspace), then plan the additions.
+        pl.run();
+        DevBuf crank(sizeof(int32_t) * comp.size());
+        for (size_t c = 0; c < comp.size(); ++c) comp[c].d_rank = crank.as<int32_t>() + c;
+        Context side(s);
+        compress_batched<T>(comp, side);
+        std::vector<int32_t> cr(comp.size());
+        FRK_HIP_CHECK(
+            hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
+        for (size_t c = 0; c < comp.size(); ++c) adds[comp_add[c]].kx1 = adds[comp_add[c]].kx2 = cr[c];
+        Plan p2(ctx);
+        plan_lr_add<T>(p2, adds, A.lra);
+        p2.run();
+        ctx.sync();
+        continue;
       }
-    run_gemms<T>(gA, ctx);
-    run_gemms<T>(gB, ctx);
-    run_gemms<T>(gC, ctx);
-    run_gemms<T>(gD, ctx);
-    if (!comp.empty()) {
-      // rank table entries for the compressed products live in the info buffer tail
-      DevBuf crank(sizeof(int32_t) * comp.size());
-      for (size_t c = 0; c < comp.size(); ++c) comp[c].d_rank = crank.as<int32_t>() + c;
-      // compression drains the stream and resets the workspace: keep the
-      // already-planned buffers alive by running it on a separate arena
-      Context side(s);
-      compress_batched<T>(comp, side);
-      std::vector<int32_t> cr(comp.size());
-      FRK_HIP_CHECK(hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
-      for (size_t c = 0; c < comp.size(); ++c) {
-        auto& ad = adds[comp_tiles[c].first];
-        ad.kx1 = ad.kx2 = cr[c];
-      }
+      plan_lr_add<T>(pl, adds, A.lra);
     }
-    lr_add_batched<T>(adds, A.lra, ctx);
+    pl.run();
   }
   ctx.sync();
   A.pull_ranks(s);
@@ -1203,8 +1333,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
 }
 
 // ============================================================================
-// Block substitution (column-oriented so every batched update has a distinct
-// target block)
+// Block substitution (column-oriented: every batched update has a distinct
+// target block).  One plan (one descriptor upload) per sweep.
 // ============================================================================
 template <typename T>
 void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
@@ -1213,36 +1343,23 @@ void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
   F.pull_ranks(ctx.stream());
   int kmax = 1;
   for (int r : F.rank) kmax = std::max(kmax, r);
+  ctx.reserve((size_t)nb * nb * 2 * sizeof(T) * ((size_t)kmax * nrhs + 64) + 4096);
   auto bptr = [&](int i) { return b + F.roff[i] * nrhs; };
-  // one sweep: diagonal solves in `order`, updates towards `targets(i)`
-  auto sweep = [&](bool forward, bool lower_factor, bool transposed) {
+  auto sweep = [&](Plan& pl, bool forward, bool lower_factor, bool transposed) {
     for (int step = 0; step < nb; ++step) {
       const int i = forward ? step : nb - 1 - step;
-      ctx.reserve((size_t)nb * 2 * sizeof(T) * ((size_t)kmax * nrhs + 64) + 4096);
       const int ii = F.idx(i, i);
       const int64_t ld = F.cols(i);
-      TrsmItem x{};
-      x.A = F.D[ii];
-      x.B = bptr(i);
-      x.b_rs = nrhs;
-      x.b_cs = 1;
-      x.n = (int)F.rows(i);
-      x.nrhs = nrhs;
-      if (!transposed) {
-        x.a_rs = ld;
-        x.a_cs = 1;
-        x.lower = lower_factor ? 1 : 0;
-      } else {
-        x.a_rs = 1;
-        x.a_cs = ld;
-        x.lower = lower_factor ? 0 : 1;
-      }
-      x.unit = lower_factor ? 1 : 0;
-      run_trsms<T>({x}, ctx);
+      const T* dinv = lower_factor ? F.Linv[ii] : F.Uinv[ii];
+      TrsmItem x;
+      if (!transposed)
+        x = trsm_item(F.D[ii], ld, 1, dinv, 64, 1, bptr(i), nrhs, 1, (int)F.rows(i), nrhs, lower_factor);
+      else
+        x = trsm_item(F.D[ii], 1, ld, dinv, 1, 64, bptr(i), nrhs, 1, (int)F.rows(i), nrhs, !lower_factor);
+      plan_trsms<T>(pl, {x});
       std::vector<GemmItem> g1, g2, g3;
       for (int k = 0; k < nb; ++k) {
         if (forward ? (k <= i) : (k >= i)) continue;
-        // contribution of block i to block k through tile t
         const int t = transposed ? F.idx(i, k) : F.idx(k, i);
         const int mk = (int)F.rows(k), ni = (int)F.rows(i);
         if (F.kind[t] == kEmpty) continue;
@@ -1258,30 +1375,31 @@ void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
         T* t1 = ctx.take<T>((size_t)r * nrhs + 1);
         T* t2 = ctx.take<T>((size_t)r * nrhs + 1);
         if (!transposed) {
-          // b_k -= U (S (V b_i)), tile (k, i): U rows(k) x r, V r x cols(i)
+          // b_k -= U (S (V b_i)), tile (k, i)
           g1.push_back(gemm_item(F.V[t], F.cols(i), 1, bptr(i), nrhs, 1, t1, nrhs, 1, r, nrhs, ni, 1.0, 0.0));
           g2.push_back(gemm_item(F.S[t], F.kcap, 1, t1, nrhs, 1, t2, nrhs, 1, r, nrhs, r, 1.0, 0.0));
           g3.push_back(gemm_item(F.U[t], 1, F.rows(k), t2, nrhs, 1, bptr(k), nrhs, 1, mk, nrhs, r, -1.0, 1.0));
         } else {
-          // b_k -= V^T (S^T (U^T b_i)), tile (i, k): U rows(i) x r, V r x cols(k)
+          // b_k -= V^T (S^T (U^T b_i)), tile (i, k)
           g1.push_back(gemm_item(F.U[t], F.rows(i), 1, bptr(i), nrhs, 1, t1, nrhs, 1, r, nrhs, ni, 1.0, 0.0));
           g2.push_back(gemm_item(F.S[t], 1, F.kcap, t1, nrhs, 1, t2, nrhs, 1, r, nrhs, r, 1.0, 0.0));
           g3.push_back(gemm_item(F.V[t], 1, F.cols(k), t2, nrhs, 1, bptr(k), nrhs, 1, mk, nrhs, r, -1.0, 1.0));
         }
       }
-      run_gemms<T>(g1, ctx);
-      run_gemms<T>(g2, ctx);
-      run_gemms<T>(g3, ctx);
-      ctx.staging();
+      plan_gemms<T>(pl, g1);
+      plan_gemms<T>(pl, g2);
+      plan_gemms<T>(pl, g3);
     }
   };
+  Plan pl(ctx);
   if (!trans) {
-    sweep(true, true, false);    // L y = b
-    sweep(false, false, false);  // U x = y
+    sweep(pl, true, true, false);    // L y = b
+    sweep(pl, false, false, false);  // U x = y
   } else {
-    sweep(true, false, true);   // U^T y = b
-    sweep(false, true, true);   // L^T x = y
+    sweep(pl, true, false, true);   // U^T y = b
+    sweep(pl, false, true, true);   // L^T x = y
   }
+  pl.run();
 }
 
 template <typename T>
@@ -1291,8 +1409,9 @@ void blr_matvec(BlrMatrix<T>& A, const T* x, T* y, int nrhs, Context& ctx) {
   A.pull_ranks(ctx.stream());
   int kmax = 1;
   for (int r : A.rank) kmax = std::max(kmax, r);
+  ctx.reserve((size_t)nr * nc * 2 * sizeof(T) * ((size_t)kmax * nrhs + 64) + 4096);
+  Plan pl(ctx);
   for (int j = 0; j < nc; ++j) {
-    ctx.reserve((size_t)nr * 2 * sizeof(T) * ((size_t)kmax * nrhs + 64) + 4096);
     std::vector<GemmItem> g1, g2, g3;
     const T* xj = x + A.coff[j] * nrhs;
     for (int i = 0; i < nr; ++i) {
@@ -1310,23 +1429,93 @@ void blr_matvec(BlrMatrix<T>& A, const T* x, T* y, int nrhs, Context& ctx) {
         g3.push_back(gemm_item(A.U[t], 1, mi, t2, nrhs, 1, yi, nrhs, 1, mi, nrhs, r, 1.0, 1.0));
       }
     }
-    run_gemms<T>(g1, ctx);
-    run_gemms<T>(g2, ctx);
-    run_gemms<T>(g3, ctx);
+    plan_gemms<T>(pl, g1);
+    plan_gemms<T>(pl, g2);
+    plan_gemms<T>(pl, g3);
   }
+  pl.run();
+}
+
+// ============================================================================
+// Iterative refinement with the exact FP64 kernel matrix
+// ============================================================================
+static std::vector<double> colmax(const double* a, int64_t n, int nrhs, Context& ctx) {
+  double* part = ctx.take<double>((size_t)kColmaxBlocks * nrhs);
+  launch_colmax_abs(a, n, nrhs, part, ctx.stream());
+  std::vector<double> h((size_t)kColmaxBlocks * nrhs);
+  FRK_HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx.stream()));
+  FRK_HIP_CHECK(hipStreamSynchronize(ctx.stream()));
+  std::vector<double> out(nrhs, 0.0);
+  for (int b = 0; b < kColmaxBlocks; ++b)
+    for (int c = 0; c < nrhs; ++c) out[c] = std::max(out[c], h[(size_t)b * nrhs + c]);
+  return out;
+}
+
+double kernel_norm_inf(const KernelSpec& ks, int64_t n, Context& ctx) {
+  ctx.sync();
+  ctx.reserve(sizeof(double) * ((size_t)3 * n + kColmaxBlocks) + 4096);
+  double* ones = ctx.take<double>(n);
+  double* zero = ctx.take<double>(n);
+  double* r = ctx.take<double>(n);
+  launch_fill_value(ones, n, 1.0, ctx.stream());
+  launch_fill_value(zero, n, 0.0, ctx.stream());
+  launch_kernel_residual(ks, ones, zero, r, n, 1, ctx.stream());  // r = -A 1 (non-negative entries)
+  return colmax(r, n, 1, ctx)[0];
+}
+
+template <typename T>
+RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, const double* b, double* x, int nrhs,
+                        double tol, int max_iter, Context& ctx) {
+  const int64_t n = F.roff.back();
+  const hipStream_t s = ctx.stream();
+  DevBuf rbuf(sizeof(double) * n * nrhs), dbuf(sizeof(T) * n * nrhs);
+  double* r = rbuf.as<double>();
+  T* d = dbuf.as<T>();
+  ctx.sync();
+  ctx.reserve(sizeof(double) * kColmaxBlocks * nrhs * 4 + 4096);
+  const std::vector<double> bn = colmax(b, n, nrhs, ctx);
+  // x0 = F^-1 b
+  launch_convert<T, double>(d, b, n * nrhs, s);
+  blr_solve<T>(F, d, nrhs, false, ctx);
+  launch_convert<double, T>(x, d, n * nrhs, s);
+  RefineResult res{0, 0.0};
+  for (int it = 0;; ++it) {
+    launch_kernel_residual(ks, x, b, r, n, nrhs, s);
+    ctx.sync();
+    ctx.reserve(sizeof(double) * kColmaxBlocks * nrhs * 4 + 4096);
+    const std::vector<double> rn = colmax(r, n, nrhs, ctx);
+    const std::vector<double> xn = colmax(x, n, nrhs, ctx);
+    double berr = 0.0;
+    for (int c = 0; c < nrhs; ++c) berr = std::max(berr, rn[c] / (anorm * xn[c] + bn[c]));
+    res.iters = it;
+    res.berr = berr;
+    if (berr <= tol || it >= max_iter) break;
+    launch_convert<T, double>(d, r, n * nrhs, s);
+    blr_solve<T>(F, d, nrhs, false, ctx);
+    launch_axpy_convert<T>(x, d, n * nrhs, s);
+  }
+  return res;
 }
 
 // ----------------------------------------------------------------------------
 #define FRK_ENGINE_INST(T)                                                                                  \
   template class BlrMatrix<T>;                                                                              \
+  template void plan_lr_add<T>(Plan&, const std::vector<LrAddDesc<T>>&, int);                               \
+  template size_t lr_add_workspace<T>(const std::vector<LrAddDesc<T>>&);                                    \
   template void lr_add_batched<T>(const std::vector<LrAddDesc<T>>&, int, Context&);                         \
   template void compress_batched<T>(const std::vector<CompressDesc<T>>&, Context&);                         \
+  template void plan_getrf<T>(Plan&, const std::vector<T*>&, const std::vector<T*>&, const std::vector<T*>&, \
+                              int, int64_t, int32_t*);                                                      \
   template void getrf_tiles<T>(const std::vector<T*>&, int, int64_t, int32_t*, Context&);                   \
+  template void trsm_generic<T>(const T*, int64_t, int64_t, bool, bool, T*, int64_t, int64_t, int, int,     \
+                                Context&);                                                                  \
   template std::unique_ptr<BlrMatrix<T>> build_blr_kernel<T>(const KernelDesc&, int, double, int, double, \
                                                              int, int, Context&);                           \
   template void blr_getrf<T>(BlrMatrix<T>&, Context&);                                                      \
   template void blr_solve<T>(BlrMatrix<T>&, T*, int, bool, Context&);                                       \
-  template void blr_matvec<T>(BlrMatrix<T>&, const T*, T*, int, Context&);
+  template void blr_matvec<T>(BlrMatrix<T>&, const T*, T*, int, Context&);                                  \
+  template RefineResult blr_refine<T>(BlrMatrix<T>&, const KernelSpec&, double, const double*, double*, int,   \
+                                      double, int, Context&);
 
 FRK_ENGINE_INST(double)
 FRK_ENGINE_INST(float)

@@ -9,6 +9,8 @@
 //              V  kcap x n row-major (ld = n)     -> every basis vector contiguous
 // Updates rewrite the slots in place, so a factorization never reallocates.
 #pragma once
+#include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -96,6 +98,30 @@ class Context {
   DevBuf info_;
 };
 
+// A batch of launches whose descriptor arrays travel to the device in one
+// copy: add() item arrays, launch() closures that receive the device base,
+// run() uploads once and issues every launch in order on the context stream.
+class Plan {
+ public:
+  explicit Plan(Context& ctx) : ctx_(ctx) {}
+  template <class Item>
+  size_t add(const std::vector<Item>& v) {
+    const size_t off = (bytes_.size() + 255) & ~size_t(255);
+    bytes_.resize(off + v.size() * sizeof(Item));
+    if (!v.empty()) std::memcpy(bytes_.data() + off, v.data(), v.size() * sizeof(Item));
+    return off;
+  }
+  void launch(std::function<void(const char*)> f) { launches_.push_back(std::move(f)); }
+  void run();
+  bool empty() const { return launches_.empty(); }
+  Context& ctx() { return ctx_; }
+
+ private:
+  Context& ctx_;
+  std::vector<char> bytes_;
+  std::vector<std::function<void(const char*)>> launches_;
+};
+
 enum TileKind : int32_t { kDense = 0, kLowRank = 1, kEmpty = 2 };
 enum Lra : int32_t { kLraNaive = 0, kLraRounded = 1, kLraFast = 2 };
 
@@ -112,6 +138,10 @@ class BlrMatrix {
   int32_t* d_rank(int i, int j) const { return d_ranks_.as<int32_t>() + idx(i, j); }
   void pull_ranks(hipStream_t s);  // synchronous device -> host copy of the rank table
   void push_ranks(hipStream_t s);
+  // Same tile structure, fresh slots; copy_from() restores contents (stream-ordered).
+  std::unique_ptr<BlrMatrix<T>> clone_structure(Context& ctx) const;
+  void copy_from(const BlrMatrix<T>& src, hipStream_t s);
+  size_t arena_bytes() const { return arena_bytes_; }
 
   std::vector<int64_t> roff, coff;
   std::vector<int32_t> kind;
@@ -121,9 +151,14 @@ class BlrMatrix {
   int fixed_rank;  // used when eps == 0
   int lra = kLraFast;
   std::vector<T*> D, U, S, V;  // per tile slots (nullptr when unused)
+  // inverses of the 64 x 64 diagonal blocks of the L and U factors of every
+  // square dense diagonal tile (written by getrf, read by the blocked trsm)
+  std::vector<T*> Linv, Uinv;
 
  private:
   DevBuf arena_;
+  size_t arena_bytes_ = 0;
+  DevBuf dinv_;
   DevBuf d_ranks_;
 };
 
@@ -156,6 +191,11 @@ struct LrAddDesc {
 };
 
 // C += X for every descriptor (reference: rounded_addition / fast_rounded_addition).
+// Workspace comes from the plan's context (reserve lr_add_workspace() first).
+template <typename T>
+void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra);
+template <typename T>
+size_t lr_add_workspace(const std::vector<LrAddDesc<T>>& d);
 template <typename T>
 void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx);
 
@@ -181,9 +221,19 @@ template <typename T>
 void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx);
 
 // No-pivot LU of square row-major tiles (n <= 512) in place; blocked with
-// 64-wide panels.  info_dev[t] receives 1-based first zero pivot (0 if none).
+// 64-wide panels whose diagonal-block inverses go to linv / uinv (nblk x 64 x 64
+// each).  info_dev[p * ntiles + t] receives the 1-based first zero pivot of
+// panel p (0 if none).
+template <typename T>
+void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& linv, const std::vector<T*>& uinv,
+                int n, int64_t ld, int32_t* info_dev);
 template <typename T>
 void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_dev, Context& ctx);
+// Triangular solve T X = B (left, in place) for a triangle without precomputed
+// block inverses (computes them first).  Drains the stream.
+template <typename T>
+void trsm_generic(const T* A, int64_t a_rs, int64_t a_cs, bool lower, bool unit, T* B, int64_t b_rs, int64_t b_cs,
+                  int n, int nrhs, Context& ctx);
 
 // ----------------------------------------------------------------------------
 // BLR drivers.
@@ -214,6 +264,19 @@ void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx);
 // y += A x on an unfactored BLR matrix.
 template <typename T>
 void blr_matvec(BlrMatrix<T>& A, const T* x, T* y, int nrhs, Context& ctx);
+
+// Iterative refinement: x = F^-1 b, then x += F^-1 (b - A x) with the exact
+// FP64 kernel matrix (matrix-free) until the normwise backward error
+// max_c ||b_c - A x_c||_inf / (||A||_inf ||x_c||_inf + ||b_c||_inf) <= tol.
+struct RefineResult {
+  int iters;
+  double berr;
+};
+template <typename T>
+RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm_inf, const double* b, double* x,
+                        int nrhs, double tol, int max_iter, Context& ctx);
+// ||A||_inf of a kernel matrix with non-negative entries (max row sum).
+double kernel_norm_inf(const KernelSpec& ks, int64_t n, Context& ctx);
 
 // Device kernel spec with uploaded points (owned by the returned buffer).
 KernelSpec upload_kernel(const KernelDesc& kd, DevBuf& pts, hipStream_t s);

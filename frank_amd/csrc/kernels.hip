@@ -142,83 +142,144 @@ void launch_gemm(const GemmItem* d_items, int n_items, int max_m, int max_n, hip
 }
 
 // ===========================================================================
-// Batched left triangular solve, in place.  One workgroup per (item, strip of
-// 16 right-hand sides); the strip lives in LDS, rows are solved in blocks of
-// 16: a dot-product update against solved rows, then the 16 x 16 diagonal
-// block by substitution.
+// Batched left triangular solve, in place, blocked by 64 x 64 diagonal blocks
+// whose inverses are given:  X_k = inv(T_kk) (B_k - sum_j T_kj X_j).
+// One workgroup per (item, strip of 16 right-hand sides).  The strip lives in
+// LDS; every T block is staged through LDS with coalesced loads, and the next
+// block is prefetched into registers while the current one is consumed.
 // ===========================================================================
 constexpr int kTrsmMaxN = 512;
+constexpr int kTrsmW = 16;
+
+template <typename T>
+__device__ __forceinline__ void trsm_fetch(T (&pf)[16], const T* __restrict__ A, int64_t ars, int64_t acs, bool acol,
+                                           int r0, int nk, int q0, int nj, int tid) {
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = tid + u * 256;
+    int a, b;
+    if (acol) {
+      a = e >> 6;
+      b = e & 63;
+    } else {
+      b = e >> 6;
+      a = e & 63;
+    }
+    pf[u] = (a < nk && b < nj) ? A[(int64_t)(r0 + a) * ars + (int64_t)(q0 + b) * acs] : T(0);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void trsm_store(const T (&pf)[16], T (*Tb)[kTriBlk + 1], bool acol, int tid) {
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int e = tid + u * 256;
+    int a, b;
+    if (acol) {
+      a = e >> 6;
+      b = e & 63;
+    } else {
+      b = e >> 6;
+      a = e & 63;
+    }
+    Tb[a][b] = pf[u];
+  }
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_kernel(const TrsmItem* __restrict__ items) {
   const TrsmItem it = items[blockIdx.y];
-  const int c0 = blockIdx.x * 16;
+  const int c0 = blockIdx.x * kTrsmW;
   if (c0 >= it.nrhs) return;
-  const int nc = min(16, it.nrhs - c0);
+  const int nc = min(kTrsmW, it.nrhs - c0);
   const int n = it.n;
-  __shared__ T X[kTrsmMaxN * 17];
+  __shared__ T X[kTrsmMaxN][kTrsmW + 1];
+  __shared__ T Tb[kTriBlk][kTriBlk + 1];
+  __shared__ T Yb[kTriBlk][kTrsmW + 1];
   const T* __restrict__ A = static_cast<const T*>(it.A);
+  const T* __restrict__ Dv = static_cast<const T*>(it.dinv);
   T* B = static_cast<T*>(it.B);
   const int tid = threadIdx.x;
   const bool rfast = (it.b_rs == 1);
-  for (int e = tid; e < n * 16; e += 256) {
+  for (int e = tid; e < n * kTrsmW; e += 256) {
     int r, c;
     if (rfast) {
       r = e % n;
       c = e / n;
     } else {
-      r = e >> 4;
-      c = e & 15;
+      r = e / kTrsmW;
+      c = e % kTrsmW;
     }
-    X[r * 17 + c] = (c < nc) ? B[(int64_t)r * it.b_rs + (int64_t)(c0 + c) * it.b_cs] : T(0);
+    X[r][c] = (c < nc) ? B[(int64_t)r * it.b_rs + (int64_t)(c0 + c) * it.b_cs] : T(0);
+  }
+  const int row = tid & 63;
+  const int cg = (tid >> 6) * 4;
+  const int nblk = (n + 63) / 64;
+  const bool acol = (it.a_cs == 1);
+  const bool dcol = (it.d_cs == 1);
+  T pf[16];
+  for (int bb = 0; bb < nblk; ++bb) {
+    const int k = it.lower ? bb : nblk - 1 - bb;
+    const int r0 = k * 64, nk = min(64, n - r0);
+    const int jlo = it.lower ? 0 : k + 1, jhi = it.lower ? k : nblk;
+    __syncthreads();  // X complete from the previous block
+    T acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] = (row < nk) ? X[r0 + row][cg + c] : T(0);
+    if (jlo < jhi) trsm_fetch(pf, A, it.a_rs, it.a_cs, acol, r0, nk, jlo * 64, min(64, n - jlo * 64), tid);
+    for (int j = jlo; j < jhi; ++j) {
+      const int q0 = j * 64, nj = min(64, n - q0);
+      __syncthreads();  // previous Tb consumed
+      trsm_store(pf, Tb, acol, tid);
+      __syncthreads();
+      if (j + 1 < jhi) trsm_fetch(pf, A, it.a_rs, it.a_cs, acol, r0, nk, q0 + 64, min(64, n - q0 - 64), tid);
+      for (int b2 = 0; b2 < nj; ++b2) {
+        const T t = Tb[row][b2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] -= t * X[q0 + b2][cg + c];
+      }
+    }
+    // inverse of the diagonal block
+    const T* D = Dv + (int64_t)k * (kTriBlk * kTriBlk);
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int e = tid + u * 256;
+      int a, b2;
+      if (dcol) {
+        a = e >> 6;
+        b2 = e & 63;
+      } else {
+        b2 = e >> 6;
+        a = e & 63;
+      }
+      pf[u] = (a < nk && b2 < nk) ? D[(int64_t)a * it.d_rs + (int64_t)b2 * it.d_cs] : T(0);
+    }
+    __syncthreads();  // Tb / Yb free
+#pragma unroll
+    for (int c = 0; c < 4; ++c) Yb[row][cg + c] = acc[c];
+    trsm_store(pf, Tb, dcol, tid);
+    __syncthreads();
+    T out[4] = {T(0), T(0), T(0), T(0)};
+    for (int b2 = 0; b2 < nk; ++b2) {
+      const T d = Tb[row][b2];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) out[c] += d * Yb[b2][cg + c];
+    }
+    if (row < nk)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) X[r0 + row][cg + c] = out[c];
   }
   __syncthreads();
-  const int c = tid & 15, g = tid >> 4;
-  const int nblk = (n + 15) >> 4;
-  const int64_t ars = it.a_rs, acs = it.a_cs;
-  for (int bb = 0; bb < nblk; ++bb) {
-    const int blk = it.lower ? bb : nblk - 1 - bb;
-    const int r0 = blk * 16, r1 = min(n, r0 + 16);
-    const int row = r0 + g;
-    if (row < r1 && c < nc) {
-      T s = X[row * 17 + c];
-      if (it.lower) {
-        for (int j = 0; j < r0; ++j) s -= A[row * ars + j * acs] * X[j * 17 + c];
-      } else {
-        for (int j = r1; j < n; ++j) s -= A[row * ars + j * acs] * X[j * 17 + c];
-      }
-      X[row * 17 + c] = s;
-    }
-    __syncthreads();
-    if (g == 0 && c < nc) {
-      if (it.lower) {
-        for (int r = r0; r < r1; ++r) {
-          T s = X[r * 17 + c];
-          for (int j = r0; j < r; ++j) s -= A[r * ars + j * acs] * X[j * 17 + c];
-          if (!it.unit) s /= A[r * ars + r * acs];
-          X[r * 17 + c] = s;
-        }
-      } else {
-        for (int r = r1 - 1; r >= r0; --r) {
-          T s = X[r * 17 + c];
-          for (int j = r + 1; j < r1; ++j) s -= A[r * ars + j * acs] * X[j * 17 + c];
-          if (!it.unit) s /= A[r * ars + r * acs];
-          X[r * 17 + c] = s;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int e = tid; e < n * 16; e += 256) {
-    int r, cc;
+  for (int e = tid; e < n * kTrsmW; e += 256) {
+    int r, c;
     if (rfast) {
       r = e % n;
-      cc = e / n;
+      c = e / n;
     } else {
-      r = e >> 4;
-      cc = e & 15;
+      r = e / kTrsmW;
+      c = e % kTrsmW;
     }
-    if (cc < nc) B[(int64_t)r * it.b_rs + (int64_t)(c0 + cc) * it.b_cs] = X[r * 17 + cc];
+    if (c < nc) B[(int64_t)r * it.b_rs + (int64_t)(c0 + c) * it.b_cs] = X[r][c];
   }
 }
 
@@ -226,56 +287,129 @@ template <typename T>
 void launch_trsm(const TrsmItem* d_items, int n_items, int max_n, int max_nrhs, hipStream_t s) {
   if (n_items <= 0 || max_nrhs <= 0 || max_n <= 0) return;
   if (max_n > kTrsmMaxN) throw std::runtime_error("trsm: n exceeds 512");
-  dim3 grid(cdiv(max_nrhs, 16), n_items);
+  dim3 grid(cdiv(max_nrhs, kTrsmW), n_items);
   hipLaunchKernelGGL(trsm_kernel<T>, grid, dim3(256), 0, s, d_items);
   FRK_HIP_CHECK(hipGetLastError());
 }
 
-// ===========================================================================
-// Unblocked no-pivot LU of a <= 64 x 64 block: one wave, lane j owns column j.
-// ===========================================================================
+// Inverses of the 64 x 64 diagonal blocks of a strided triangle: lane j
+// solves for column j by substitution against the block staged in LDS.
 template <typename T>
-__global__ __launch_bounds__(64) void getrf_small_kernel(const GemmItem* __restrict__ items,
-                                                         int32_t* __restrict__ info) {
-  const GemmItem it = items[blockIdx.x];
-  T* A = static_cast<T*>(it.C);
-  const int n = it.m;
-  const int64_t rs = it.c_rs, cs = it.c_cs;
-  const int lane = threadIdx.x;
-  __shared__ T col[64];
-  T a[64];
+__global__ __launch_bounds__(64) void tri_inv_kernel(const TriInvItem* __restrict__ items) {
+  const TriInvItem it = items[blockIdx.x];
+  const T* A = static_cast<const T*>(it.a);
+  T* out = static_cast<T*>(it.out);
+  const int n = it.n, lane = threadIdx.x;
+  __shared__ T Bk[64][65];
+  for (int i = 0; i < 64; ++i)
+    Bk[i][lane] = (i < n && lane < n) ? A[(int64_t)i * it.rs + (int64_t)lane * it.cs] : T(i == lane ? 1 : 0);
+  __syncthreads();
+  T x[64];
+  if (it.lower) {
 #pragma unroll
-  for (int i = 0; i < 64; ++i) a[i] = (i < n && lane < n) ? A[i * rs + lane * cs] : T(0);
-  int bad = 0;
+    for (int i = 0; i < 64; ++i) {
+      T sacc = (i == lane) ? T(1) : T(0);
 #pragma unroll
-  for (int k = 0; k < 64; ++k) {
-    if (k < n) {
-      const T piv = __shfl(a[k], k, 64);
-      if (piv == T(0) && bad == 0) bad = k + 1;
-      if (lane == k) {
+      for (int kk = 0; kk < i; ++kk) sacc -= Bk[i][kk] * x[kk];
+      x[i] = (it.unit || i >= n) ? sacc : sacc / Bk[i][i];
+    }
+  } else {
 #pragma unroll
-        for (int i = k + 1; i < 64; ++i) {
-          a[i] = a[i] / piv;
-          col[i] = a[i];
-        }
-      }
-      __syncthreads();
-      if (lane > k && lane < n) {
-        const T ukj = a[k];
+    for (int i = 63; i >= 0; --i) {
+      T sacc = (i == lane) ? T(1) : T(0);
 #pragma unroll
-        for (int i = k + 1; i < 64; ++i) a[i] -= col[i] * ukj;
-      }
-      __syncthreads();
+      for (int kk = i + 1; kk < 64; ++kk) sacc -= Bk[i][kk] * x[kk];
+      x[i] = (it.unit || i >= n) ? sacc : sacc / Bk[i][i];
     }
   }
 #pragma unroll
-  for (int i = 0; i < 64; ++i)
-    if (i < n && lane < n) A[i * rs + lane * cs] = a[i];
-  if (lane == 0 && info != nullptr) info[blockIdx.x] = bad;
+  for (int i = 0; i < 64; ++i) out[i * 64 + lane] = x[i];
 }
 
 template <typename T>
-void launch_getrf_small(const GemmItem* d_items, int n_items, int32_t* d_info, hipStream_t s) {
+void launch_tri_inv(const TriInvItem* d_items, int n_items, hipStream_t s) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(tri_inv_kernel<T>, dim3(n_items), dim3(64), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
+// No-pivot LU of a <= 64 x 64 diagonal block (one wave, lane j owns column j;
+// one reciprocal per pivot) followed by the inverses of its unit-lower and
+// upper factors, which the blocked trsm / panel updates consume.
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(64) void getrf_small_kernel(const LuItem* __restrict__ items,
+                                                         int32_t* __restrict__ info) {
+  const LuItem it = items[blockIdx.x];
+  T* A = static_cast<T*>(it.a);
+  const int n = it.n;
+  const int64_t rs = it.rs, cs = it.cs;
+  const int lane = threadIdx.x;
+  __shared__ T col[64];
+  __shared__ T Lsh[64][65];
+  __shared__ T rdiag[64];
+  T a[64];
+#pragma unroll
+  for (int i = 0; i < 64; ++i) a[i] = (i < n && lane < n) ? A[i * rs + lane * cs] : T(i == lane ? 1 : 0);
+  int bad = 0;
+#pragma unroll
+  for (int k = 0; k < 64; ++k) {
+    const T piv = __shfl(a[k], k, 64);
+    if (piv == T(0) && bad == 0 && k < n) bad = k + 1;
+    const T inv = T(1) / piv;
+    if (lane == k) {
+      rdiag[k] = inv;
+#pragma unroll
+      for (int i = k + 1; i < 64; ++i) {
+        a[i] = a[i] * inv;
+        col[i] = a[i];
+      }
+    }
+    __syncthreads();
+    if (lane > k) {
+      const T ukj = a[k];
+#pragma unroll
+      for (int i = k + 1; i < 64; ++i) a[i] -= col[i] * ukj;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 64; ++i) {
+    if (i < n && lane < n) A[i * rs + lane * cs] = a[i];
+    Lsh[i][lane] = a[i];
+  }
+  if (lane == 0 && info != nullptr) info[blockIdx.x] = bad;
+  __syncthreads();
+  T x[64];
+  if (it.linv != nullptr) {
+    T* out = static_cast<T*>(it.linv);
+#pragma unroll
+    for (int i = 0; i < 64; ++i) {
+      T sacc = (i == lane) ? T(1) : T(0);
+#pragma unroll
+      for (int kk = 0; kk < i; ++kk) sacc -= Lsh[i][kk] * x[kk];
+      x[i] = sacc;
+    }
+#pragma unroll
+    for (int i = 0; i < 64; ++i) out[i * 64 + lane] = x[i];
+  }
+  if (it.uinv != nullptr) {
+    T* out = static_cast<T*>(it.uinv);
+#pragma unroll
+    for (int i = 63; i >= 0; --i) {
+      T sacc = (i == lane) ? T(1) : T(0);
+#pragma unroll
+      for (int kk = i + 1; kk < 64; ++kk) sacc -= Lsh[i][kk] * x[kk];
+      x[i] = sacc * rdiag[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 64; ++i) out[i * 64 + lane] = x[i];
+  }
+}
+
+template <typename T>
+void launch_getrf_small(const LuItem* d_items, int n_items, int32_t* d_info, hipStream_t s) {
   if (n_items <= 0) return;
   hipLaunchKernelGGL(getrf_small_kernel<T>, dim3(n_items), dim3(64), 0, s, d_items, d_info);
   FRK_HIP_CHECK(hipGetLastError());
@@ -299,34 +433,16 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
   T a[CPW][MR];
 #pragma unroll
   for (int q = 0; q < CPW; ++q) {
-#pragma unroll
-    for (int t = 0; t < MR; ++t) a[q][t] = T(0);
     const int c = q * W + w;
-    if (c < K) {
-      const bool first = c < it.k1;
-      const T* Ab = static_cast<const T*>(first ? it.A1 : it.A2);
-      const T* S = static_cast<const T*>(first ? it.S1 : it.S2);
-      const int64_t rs = first ? it.a1_rs : it.a2_rs;
-      const int64_t cs = first ? it.a1_cs : it.a2_cs;
-      const int sld = first ? it.s1_ld : it.s2_ld;
-      const int ka = first ? it.k1a : it.k2a;
-      const int cc = first ? c : c - it.k1;
-      if (S == nullptr) {
+    const bool first = c < it.k1;
+    const T* Ab = static_cast<const T*>(first ? it.A1 : it.A2);
+    const int64_t rs = first ? it.a1_rs : it.a2_rs;
+    const int64_t cs = first ? it.a1_cs : it.a2_cs;
+    const int cc = first ? c : c - it.k1;
 #pragma unroll
-        for (int t = 0; t < MR; ++t) {
-          const int row = lane + 64 * t;
-          if (row < m) a[q][t] = Ab[row * rs + cc * cs];
-        }
-      } else {
-        for (int s = 0; s < ka; ++s) {
-          const T coef = S[s * sld + cc];
-#pragma unroll
-          for (int t = 0; t < MR; ++t) {
-            const int row = lane + 64 * t;
-            if (row < m) a[q][t] += Ab[row * rs + s * cs] * coef;
-          }
-        }
-      }
+    for (int t = 0; t < MR; ++t) {
+      const int row = lane + 64 * t;
+      a[q][t] = (c < K && row < m) ? Ab[row * rs + cc * cs] : T(0);
     }
   }
   T* Rb = static_cast<T*>(it.R);
@@ -478,26 +594,41 @@ __global__ __launch_bounds__(W * 64) void apply_q_kernel(const ApplyQItem* __res
                     : T(0);
     }
   }
-  for (int jj = 0; jj < it.nref; ++jj) {
-    const int j = it.trans ? jj : it.nref - 1 - jj;
-    const T tj = tau[j];
-    if (tj == T(0)) continue;
-    T v[MR];
-#pragma unroll
-    for (int t = 0; t < MR; ++t) {
-      const int row = lane + 64 * t;
-      v[t] = (row == j) ? T(1) : ((row > j && row < m) ? Y[row * it.y_rs + j * it.y_cs] : T(0));
+  constexpr int CH = 8;
+  __shared__ T vs[CH][MR * 64];
+  __shared__ T ts[CH];
+  const int nref = it.nref;
+  for (int base = 0; base < nref; base += CH) {
+    const int cnt = min(CH, nref - base);
+    __syncthreads();  // previous chunk consumed
+    for (int e = threadIdx.x; e < cnt * MR * 64; e += W * 64) {
+      const int ci = e / (MR * 64), row = e % (MR * 64);
+      const int jj = base + ci;
+      const int j = it.trans ? jj : nref - 1 - jj;
+      vs[ci][row] = (row == j) ? T(1) : ((row > j && row < m) ? Y[row * it.y_rs + j * it.y_cs] : T(0));
     }
+    if ((int)threadIdx.x < cnt) {
+      const int jj = base + threadIdx.x;
+      ts[threadIdx.x] = tau[it.trans ? jj : nref - 1 - jj];
+    }
+    __syncthreads();
+    for (int ci = 0; ci < cnt; ++ci) {
+      const T tj = ts[ci];
+      if (tj == T(0)) continue;
+      T v[MR];
 #pragma unroll
-    for (int q = 0; q < CPW; ++q) {
-      const int c = q * W + w;
-      if (c < it.ncol) {
-        T d = T(0);
+      for (int t = 0; t < MR; ++t) v[t] = vs[ci][lane + 64 * t];
 #pragma unroll
-        for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
-        d = wave_sum(d) * tj;
+      for (int q = 0; q < CPW; ++q) {
+        const int c = q * W + w;
+        if (c < it.ncol) {
+          T d = T(0);
 #pragma unroll
-        for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
+          for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
+          d = wave_sum(d) * tj;
+#pragma unroll
+          for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
+        }
       }
     }
   }
@@ -576,21 +707,46 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
     if (tid == 0 && it.rank_out) *it.rank_out = 0;
     return;
   }
-  for (int e = tid; e < K * K; e += 256) {
-    const int i = e / K, j = e % K;
-    T v;
-    if (it.mode == 0) {
-      v = static_cast<const T*>(it.M)[i * it.m_rs + j * it.m_cs];
-    } else {
-      const T* Ru = static_cast<const T*>(it.Ru);
-      const T* Rv = static_cast<const T*>(it.Rv);
-      T s = T(0);
-      for (int t = 0; t < it.kx; ++t) s += Ru[i * it.ldr + t] * Rv[j * it.ldr + t];
-      if (i < it.ka && j < it.ka) s += static_cast<const T*>(it.Sa)[i * it.lds_in + j];
-      v = s;
+  if (it.mode == 0) {
+    for (int e = tid; e < K * K; e += 256) {
+      const int i = e / K, j = e % K;
+      G[j][i] = static_cast<const T*>(it.M)[i * it.m_rs + j * it.m_cs];
+      Vs[j][i] = (i == j) ? T(1) : T(0);
     }
-    G[j][i] = v;
-    Vs[j][i] = (i == j) ? T(1) : T(0);
+  } else {
+    // stage Ru (K x J) into G and Rv (K x J) into Vs with independent loads
+    const int J = it.kx;
+    const T* Ru = static_cast<const T*>(it.Ru);
+    const T* Rv = static_cast<const T*>(it.Rv);
+    for (int e = tid; e < K * J; e += 256) {
+      const int i = e / J, t = e % J;
+      G[t][i] = Ru[i * it.ldr + t];
+      Vs[t][i] = Rv[i * it.ldr + t];
+    }
+    __syncthreads();
+    constexpr int PER = (KMAX * KMAX + 255) / 256;
+    T mv[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * 256;
+      T sacc = T(0);
+      if (e < K * K) {
+        const int i = e / K, j = e % K;
+        for (int t = 0; t < J; ++t) sacc += G[t][i] * Vs[t][j];
+        if (i < it.ka && j < it.ka) sacc += static_cast<const T*>(it.Sa)[i * it.lds_in + j];
+      }
+      mv[u] = sacc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = tid + u * 256;
+      if (e < K * K) {
+        const int i = e / K, j = e % K;
+        G[j][i] = mv[u];
+        Vs[j][i] = (i == j) ? T(1) : T(0);
+      }
+    }
   }
   __syncthreads();
   const int K2 = K + (K & 1);
@@ -1147,13 +1303,65 @@ void launch_kernel_residual(const KernelSpec& ks, const double* X, const double*
   FRK_HIP_CHECK(hipGetLastError());
 }
 
+__global__ __launch_bounds__(256) void colmax_kernel(const double* __restrict__ a, int64_t n, int nrhs,
+                                                     double* __restrict__ out) {
+  __shared__ double red[256];
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = blockIdx.x * chunk, r1 = std::min<int64_t>(n, r0 + chunk);
+  for (int c = 0; c < nrhs; ++c) {
+    double m = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += 256) m = fmax(m, fabs(a[i * nrhs + c]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + st]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x * nrhs + c] = red[0];
+    __syncthreads();
+  }
+}
+
+void launch_colmax_abs(const double* a, int64_t n, int nrhs, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(colmax_kernel, dim3(kColmaxBlocks), dim3(256), 0, s, a, n, nrhs, out);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void fill_value_kernel(double* p, int64_t n, double v) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) p[e] = v;
+}
+
+void launch_fill_value(double* p, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_value_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0, s, p,
+                     n, v);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+__global__ void axpy_convert_kernel(double* x, const T* d, int64_t n) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
+    x[e] += static_cast<double>(d[e]);
+}
+
+template <typename T>
+void launch_axpy_convert(double* x, const T* d, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(axpy_convert_kernel<T>, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                     s, x, d, n);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_axpy_convert<double>(double*, const double*, int64_t, hipStream_t);
+template void launch_axpy_convert<float>(double*, const float*, int64_t, hipStream_t);
+
 // ---------------------------------------------------------------------------
 // Explicit instantiations
 // ---------------------------------------------------------------------------
 #define FRK_INST(T)                                                                              \
   template void launch_gemm<T>(const GemmItem*, int, int, int, hipStream_t);                     \
   template void launch_trsm<T>(const TrsmItem*, int, int, int, hipStream_t);                     \
-  template void launch_getrf_small<T>(const GemmItem*, int, int32_t*, hipStream_t);              \
+  template void launch_getrf_small<T>(const LuItem*, int, int32_t*, hipStream_t);                \
+  template void launch_tri_inv<T>(const TriInvItem*, int, hipStream_t);                          \
   template void launch_qr<T>(const QrItem*, int, int, hipStream_t);                              \
   template void launch_apply_q<T>(const ApplyQItem*, int, int, hipStream_t);                     \
   template void launch_svd<T>(const SvdItem*, int, int, hipStream_t);                            \
@@ -1166,6 +1374,7 @@ void launch_kernel_residual(const KernelSpec& ks, const double* X, const double*
 FRK_INST(double)
 FRK_INST(float)
 template void launch_convert<float, double>(float*, const double*, int64_t, hipStream_t);
+template void launch_convert<double, double>(double*, const double*, int64_t, hipStream_t);
 template void launch_convert<double, float>(double*, const float*, int64_t, hipStream_t);
 
 }  // namespace frk

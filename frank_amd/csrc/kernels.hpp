@@ -9,15 +9,19 @@ namespace frk {
 template <typename T>
 void launch_gemm(const GemmItem* d_items, int n_items, int max_m, int max_n, hipStream_t s);
 
-// Batched triangular solve, left side, in place (strips of 16 right-hand sides).
+// Batched triangular solve, left side, in place (strips of 16 right-hand sides),
+// blocked by 64 x 64 diagonal blocks with precomputed inverses.
 template <typename T>
 void launch_trsm(const TrsmItem* d_items, int n_items, int max_n, int max_nrhs, hipStream_t s);
 
-// Unblocked no-pivot LU of small diagonal blocks (n <= 64), one wave each.
-// items: GemmItem reused as {C = block pointer, c_rs, c_cs, m = n}; info[i]
-// receives the first zero pivot (1-based) or 0.
+// Inverses of the 64 x 64 diagonal blocks of triangles.
 template <typename T>
-void launch_getrf_small(const GemmItem* d_items, int n_items, int32_t* d_info, hipStream_t s);
+void launch_tri_inv(const TriInvItem* d_items, int n_items, hipStream_t s);
+
+// No-pivot LU of <= 64 x 64 blocks (one wave each) plus the inverses of the
+// unit-lower and upper factors; info[i] = first zero pivot (1-based) or 0.
+template <typename T>
+void launch_getrf_small(const LuItem* d_items, int n_items, int32_t* d_info, hipStream_t s);
 
 // Householder QR of [A1 S1 | A2 S2] (optionally projected), m <= 512.
 template <typename T>
@@ -77,6 +81,14 @@ void launch_gen(const KernelSpec& ks, const GenItem* d_items, int n_items, int m
 // Matrix-free residual R = B - A X for a kernel matrix (FP64), X/B/R row-major N x nrhs.
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R,
                             int64_t n, int nrhs, hipStream_t s);
+
+// Per-column max |a| over row blocks: out[b * nrhs + c] for b < kColmaxBlocks.
+constexpr int kColmaxBlocks = 64;
+void launch_colmax_abs(const double* a, int64_t n, int nrhs, double* out, hipStream_t s);
+void launch_fill_value(double* p, int64_t n, double v, hipStream_t s);
+// x += (double) d
+template <typename T>
+void launch_axpy_convert(double* x, const T* d, int64_t n, hipStream_t s);
 
 // Elementwise helpers.
 template <typename T>

@@ -60,6 +60,8 @@ int frk_version(void);
 int frk_ctx_create(frk_ctx* out);
 int frk_ctx_destroy(frk_ctx ctx);
 int frk_ctx_synchronize(frk_ctx ctx);
+/* The hipStream_t every call on this context is ordered on (for events / interop). */
+int frk_ctx_stream(frk_ctx ctx, void** stream);
 
 /* ---------------------------------------------------------------------------
  * BLR matrices (one-level Hierarchical, tiles resident in HBM)
@@ -86,6 +88,12 @@ int frk_blr_set_dense(frk_blr a, int i, int j, const double* host);
 int frk_blr_get_lowrank(frk_blr a, int i, int j, double* U, double* S, double* V);
 int frk_blr_set_lowrank(frk_blr a, int i, int j, int rank, const double* U, const double* S, const double* V);
 
+/* Deep copy (the reference's Hierarchical copy constructor clones every block,
+ * src/classes/matrix_proxy.cpp:68-76): new slots with the same structure. */
+int frk_blr_clone(frk_blr src, frk_blr* out);
+/* Restore dst's contents from src (same structure): one device-to-device copy. */
+int frk_blr_copy(frk_blr dst, frk_blr src);
+
 /* getrf(Hierarchical&) (src/operations/LAPACK/getrf.cpp:45-65): tile LU without
  * pivoting, in place.  Afterwards tiles below the diagonal hold L, tiles above
  * hold U, diagonal tiles hold the packed unit-L \ U factors.  Returns > 0
@@ -103,6 +111,14 @@ int frk_blr_matvec(frk_blr a, const double* x_dev, double* y_dev, int nrhs);
 /* Matrix-free FP64 residual r = b - A x for the kernel matrix the BLR matrix
  * was built from (exact, uncompressed).  Device vectors, N x nrhs row-major. */
 int frk_blr_kernel_residual(frk_blr a, const double* x_dev, const double* b_dev, double* r_dev, int nrhs);
+
+/* Solve with iterative refinement (no reference counterpart): x = F^-1 b, then
+ * x += F^-1 (b - A x) with the exact FP64 kernel matrix, until the normwise
+ * backward error max_c ||b_c - A x_c||_inf / (||A||_inf ||x_c||_inf + ||b_c||_inf)
+ * <= tol or max_iter corrections.  F: factorized matrix built from a kernel
+ * (FP64 or FP32 storage).  b, x: device, N x nrhs row-major FP64. */
+int frk_blr_refine(frk_blr f, const double* b_dev, double* x_dev, int nrhs, double tol, int max_iter, int* iters,
+                   double* berr);
 
 /* ---------------------------------------------------------------------------
  * Tile kernels (FP64, device pointers, row-major with leading dimension)

@@ -333,12 +333,10 @@ def geqp3(a: np.ndarray) -> Tuple[np.ndarray, List[int]]:
     """src/operations/LAPACK/geqp3.cpp:38-61: dgeqp3 (jpvt zero-initialised), R n x n upper, 0-based order."""
     a = np.array(a, dtype=np.float64, copy=True)
     m, n = a.shape
-    qr_, jpvt, tau, work, info = lapack.dgeqp3(a, np.zeros(n, dtype=np.int32))
-    order = [int(j) - 1 for j in jpvt] if np.min(jpvt) >= 1 else [int(j) for j in jpvt]
+    r, order = sla.qr(a, mode="r", pivoting=True)  # dgeqp3 with all columns free (jpvt = 0)
     R = np.zeros((n, n))
-    for i in range(min(m, n)):
-        R[i, i:] = qr_[i, i:]
-    return R, order
+    R[:min(m, n), :] = np.triu(r[:min(m, n), :])
+    return R, [int(j) for j in order]
 
 
 # ---------------------------------------------------------------------------
