@@ -53,8 +53,10 @@ _SIGS = {
     "frk_blr_create": ([vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, dbl, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
     "frk_blr_destroy": ([vp], C.c_int),
     "frk_ctx_stream": ([vp, C.POINTER(vp)], C.c_int),
+    "frk_ctx_counters": ([vp, pi64, C.c_int, C.c_int], C.c_int),
     "frk_blr_clone": ([vp, C.POINTER(vp)], C.c_int),
     "frk_blr_copy": ([vp, vp], C.c_int),
+    "frk_blr_convert": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
     "frk_blr_refine": ([vp, vp, vp, C.c_int, dbl, C.c_int, C.POINTER(C.c_int), C.POINTER(dbl)], C.c_int),
     "frk_blr_set_lra": ([vp, C.c_int], C.c_int),
     "frk_blr_shape": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),

@@ -50,6 +50,12 @@ class Context:
         L.check(L.lib().frk_ctx_stream(self.h, C.byref(p)))
         return p.value or 0
 
+    def counters(self, reset: bool = False) -> dict:
+        """Device event counters (drains the stream)."""
+        out = (C.c_int64 * 16)()
+        L.check(L.lib().frk_ctx_counters(self.h, out, 16, int(reset)))
+        return {"svd_problems": out[0], "svd_sweeps": out[1], "svd_max_sweeps": out[2]}
+
     def close(self):
         if self.h:
             L.lib().frk_ctx_destroy(self.h)
@@ -192,6 +198,12 @@ class BlrMatrix:
     def clone(self) -> "BlrMatrix":
         h = C.c_void_p()
         L.check(L.lib().frk_blr_clone(self.h, C.byref(h)))
+        return BlrMatrix(h, self.ctx)
+
+    def convert(self, precision: str) -> "BlrMatrix":
+        """Same structure in another storage precision ('f64' or 'f32')."""
+        h = C.c_void_p()
+        L.check(L.lib().frk_blr_convert(self.h, _PREC[precision], C.byref(h)))
         return BlrMatrix(h, self.ctx)
 
     def copy_from(self, src: "BlrMatrix") -> "BlrMatrix":

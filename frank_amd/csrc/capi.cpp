@@ -210,6 +210,18 @@ int frk_ctx_stream(frk_ctx ctx, void** stream) {
   });
 }
 
+int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset) {
+  return guard([&] {
+    if (!ctx || (!out && n > 0) || n < 0) throw std::invalid_argument("null or negative argument");
+    int32_t c[kNumCounters];
+    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    FRK_HIP_CHECK(hipMemcpy(c, ctx->ctx->counters(), sizeof(c), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n && i < kNumCounters; ++i) out[i] = c[i];
+    if (reset) FRK_HIP_CHECK(hipMemset(ctx->ctx->counters(), 0, sizeof(c)));
+    return FRK_OK;
+  });
+}
+
 int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
                           double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
                           int precision, frk_blr* out) {
@@ -283,6 +295,32 @@ int frk_blr_clone(frk_blr src, frk_blr* out) {
       h->a32 = src->a32->clone_structure(c);
       h->a32->copy_from(*src->a32, src->owner->stream);
     }
+    if (src->has_kernel) {
+      h->host_pts = src->host_pts;
+      h->kd = src->kd;
+      h->kd.host_pts = h->host_pts.data();
+      h->ks = upload_kernel(h->kd, h->pts, src->owner->stream);
+      h->has_kernel = true;
+      h->anorm = src->anorm;
+    }
+    FRK_HIP_CHECK(hipStreamSynchronize(src->owner->stream));
+    *out = h.release();
+    return FRK_OK;
+  });
+}
+
+int frk_blr_convert(frk_blr src, int precision, frk_blr* out) {
+  return guard([&] {
+    if (!src || !out) throw std::invalid_argument("null argument");
+    auto h = std::make_unique<frk_blr_s>();
+    h->owner = src->owner;
+    h->precision = precision;
+    Context& c = *src->owner->ctx;
+    if (precision == FRK_PREC_F32)
+      h->a32 = src->a64 ? convert_blr<float, double>(*src->a64, c) : std::unique_ptr<BlrMatrix<float>>();
+    else
+      h->a64 = src->a32 ? convert_blr<double, float>(*src->a32, c) : std::unique_ptr<BlrMatrix<double>>();
+    if (!h->a32 && !h->a64) throw std::invalid_argument("source already has that precision (use frk_blr_clone)");
     if (src->has_kernel) {
       h->host_pts = src->host_pts;
       h->kd = src->kd;

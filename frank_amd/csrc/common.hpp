@@ -59,9 +59,9 @@ struct TrsmItem {
 // unit-lower and upper factors (row-major 64 x 64 outputs).
 struct LuItem {
   void* a;
-  void* linv;
-  void* uinv;
-  int64_t rs, cs;
+  void* linv;  // inverse of the unit-lower factor, row stride ld_inv (may be nullptr)
+  void* uinv;  // inverse of the upper factor, row stride ld_inv (may be nullptr)
+  int64_t rs, cs, ld_inv;
   int32_t n, pad;
 };
 
@@ -75,21 +75,25 @@ struct TriInvItem {
 
 // ---------------------------------------------------------------------------
 // Householder QR item.  The m x K input is the column concatenation
-// [A1 | A2] (k1 + k2 columns, strided).  Optional projection first: every
-// column x becomes x - P (P^T x) (modified Gram-Schmidt against an
+// [A1 S1 | A2 S2] formed on load (S == nullptr: identity; A1 has k1a columns
+// and S1 is k1a x k1, likewise for block 2).  Optional projection first:
+// every column x becomes x - P (P^T x) (modified Gram-Schmidt against an
 // orthonormal P, m x kp) with the coefficients in rows [0, kp) of the R
 // buffer.  Outputs: reflectors Y (column-major, m x K, ld = ldy; unit
 // diagonal implicit), tau[K], R (rows [kp, kp + K) of the R buffer, ld = ldr).
 // ---------------------------------------------------------------------------
 struct QrItem {
   const void* A1;
+  const void* S1;
   const void* A2;
+  const void* S2;
   const void* P;
   void* Y;
   void* tau;
   void* R;
   int64_t a1_rs, a1_cs, a2_rs, a2_cs, p_rs, p_cs;
-  int32_t m, k1, k2, kp;
+  int32_t s1_ld, s2_ld;
+  int32_t m, k1a, k1, k2a, k2, kp;
   int32_t ldy, ldr;
 };
 
@@ -132,6 +136,7 @@ struct SvdItem {
   void* Um;
   void* Vm;
   int32_t* rank_out;
+  int32_t* stats;      // optional: [0] += 1 problem, [1] += sweeps, [2] = max sweeps
   int64_t m_rs, m_cs;  // mode 0 input strides
   int32_t K, ldr, lds_in, kdim;  // kdim > 0: min-dimension used by the truncation rule
   int32_t ka, kx, fixed_rank, mode;

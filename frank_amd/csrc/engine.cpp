@@ -56,7 +56,11 @@ const void* Staging::push_bytes(const void* src, size_t bytes, hipStream_t s) {
   return dp;
 }
 
-Context::Context(hipStream_t s) : s_(s) { info_.ensure(sizeof(int32_t) * 4096); }
+Context::Context(hipStream_t s) : s_(s) {
+  info_.ensure(sizeof(int32_t) * 4096);
+  counters_.ensure(sizeof(int32_t) * kNumCounters);
+  FRK_HIP_CHECK(hipMemset(counters_.as<void>(), 0, sizeof(int32_t) * kNumCounters));
+}
 
 Context::~Context() = default;
 
@@ -178,17 +182,18 @@ BlrMatrix<T>::BlrMatrix(std::vector<int64_t> ro, std::vector<int64_t> co, std::v
   Uinv.assign(nr * nc, nullptr);
   size_t dtot = 0;
   for (int i = 0; i < std::min(nr, nc); ++i)
-    if (kind[idx(i, i)] == kDense && rows(i) == cols(i)) dtot += 2 * (size_t)((rows(i) + 63) / 64) * 4096;
+    if (kind[idx(i, i)] == kDense && rows(i) == cols(i)) dtot += 2 * al((size_t)rows(i) * rows(i));
   if (dtot) {
     dinv_.ensure(dtot * sizeof(T));
+    FRK_HIP_CHECK(hipMemsetAsync(dinv_.as<T>(), 0, dtot * sizeof(T), ctx.stream()));
     T* q = dinv_.as<T>();
     for (int i = 0; i < std::min(nr, nc); ++i)
       if (kind[idx(i, i)] == kDense && rows(i) == cols(i)) {
-        const size_t nbk = (size_t)((rows(i) + 63) / 64) * 4096;
+        const size_t nn = al((size_t)rows(i) * rows(i));
         Linv[idx(i, i)] = q;
-        q += nbk;
+        q += nn;
         Uinv[idx(i, i)] = q;
-        q += nbk;
+        q += nn;
       }
   }
   d_ranks_.ensure(sizeof(int32_t) * nr * nc);
@@ -224,6 +229,11 @@ void BlrMatrix<T>::copy_from(const BlrMatrix<T>& src, hipStream_t s) {
     FRK_HIP_CHECK(hipMemcpyAsync(dinv_.as<char>(), src.dinv_.as<char>(), dinv_.bytes(), hipMemcpyDeviceToDevice, s));
   rank = src.rank;
   lra = src.lra;
+}
+
+template <typename T>
+void BlrMatrix<T>::set_ranks_from_device(const int32_t* src, hipStream_t s) {
+  FRK_HIP_CHECK(hipMemcpyAsync(d_ranks_.as<int32_t>(), src, sizeof(int32_t) * rank.size(), hipMemcpyDeviceToDevice, s));
 }
 
 template <typename T>
@@ -283,8 +293,7 @@ static TrsmItem trsm_item(const void* A, int64_t a_rs, int64_t a_cs, const void*
 }
 
 template <typename T>
-static void plan_gemms(Plan& pl, const std::vector<GemmItem>& items) {
-  if (items.empty()) return;
+static void plan_gemm_group(Plan& pl, const std::vector<GemmItem>& items) {
   const hipStream_t s = pl.ctx().stream();
   const size_t chunk = 65535;  // grid.y limit
   for (size_t o = 0; o < items.size(); o += chunk) {
@@ -298,6 +307,39 @@ static void plan_gemms(Plan& pl, const std::vector<GemmItem>& items) {
     const int cnt = (int)part.size();
     pl.launch([=](const char* d) { launch_gemm<T>(reinterpret_cast<const GemmItem*>(d + off), cnt, mm, nn, s); });
   }
+}
+
+// Independent GEMMs, launched per output-shape class (tiny, tall-skinny,
+// short-wide, general) so every item runs in the tile shape that fits it.
+template <typename T>
+static void plan_gemms(Plan& pl, const std::vector<GemmItem>& items) {
+  if (items.empty()) return;
+  std::vector<GemmItem> cls[4];
+  for (const auto& g : items) {
+    if (g.m <= 0 || g.n <= 0) continue;
+    const int c = (g.m <= 16 && g.n <= 16) ? 0 : (g.n <= 16 ? 1 : (g.m <= 16 ? 2 : 3));
+    cls[c].push_back(g);
+  }
+  for (auto& v : cls)
+    if (!v.empty()) plan_gemm_group<T>(pl, v);
+}
+
+// GEMMs that must run in exactly the given launch (in-place panel updates
+// rely on the 64 x 64 tiling of one combined launch).
+template <typename T>
+static void plan_gemms_onelaunch(Plan& pl, const std::vector<GemmItem>& items) {
+  if (!items.empty()) plan_gemm_group<T>(pl, items);
+}
+
+template <typename T>
+static void plan_copies(Plan& pl, const std::vector<GemmItem>& items) {
+  if (items.empty()) return;
+  const hipStream_t s = pl.ctx().stream();
+  int64_t mx = 0;
+  for (const auto& g : items) mx = std::max<int64_t>(mx, (int64_t)g.m * g.n);
+  const size_t off = pl.add(items);
+  const int cnt = (int)items.size();
+  pl.launch([=](const char* d) { launch_copy<T>(reinterpret_cast<const GemmItem*>(d + off), cnt, mx, s); });
 }
 
 template <typename T>
@@ -341,25 +383,29 @@ void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& l
       x.rs = ld;
       x.cs = 1;
       x.n = nbk;
-      x.linv = linv[t] + (size_t)pi * 4096;
-      x.uinv = uinv[t] + (size_t)pi * 4096;
+      x.linv = linv[t] + (size_t)p * n + p;
+      x.uinv = uinv[t] + (size_t)p * n + p;
+      x.ld_inv = n;
       small[t] = x;
     }
     const size_t off = pl.add(small);
     int32_t* inf = info_dev ? info_dev + pi * nt : nullptr;
     pl.launch([=](const char* d) { launch_getrf_small<T>(reinterpret_cast<const LuItem*>(d + off), nt, inf, s); });
     if (rest <= 0) continue;
-    // A12 <- inv(L11) A12 ; A21 <- A21 inv(U11)   (in place: one output tile row / column)
+    // A12 <- inv(L11) A12 ; A21 <- A21 inv(U11), in place: the 64 x 64 GEMM tiling gives A12 a single
+    // output row tile and A21 a single output column tile, so every workgroup reads only the region it
+    // writes, and it reads all of it before writing.
     std::vector<GemmItem> pan;
     pan.reserve(2 * nt);
     for (int t = 0; t < nt; ++t) {
       T* a = tiles[t];
-      pan.push_back(gemm_item(linv[t] + (size_t)pi * 4096, 64, 1, a + p * ld + p + nbk, ld, 1, a + p * ld + p + nbk,
+      pan.push_back(gemm_item(linv[t] + (size_t)p * n + p, n, 1, a + p * ld + p + nbk, ld, 1, a + p * ld + p + nbk,
                               ld, 1, nbk, rest, nbk, 1.0, 0.0));
-      pan.push_back(gemm_item(a + (p + nbk) * ld + p, ld, 1, uinv[t] + (size_t)pi * 4096, 64, 1,
+      pan.push_back(gemm_item(a + (p + nbk) * ld + p, ld, 1, uinv[t] + (size_t)p * n + p, n, 1,
                               a + (p + nbk) * ld + p, ld, 1, rest, nbk, nbk, 1.0, 0.0));
     }
-    plan_gemms<T>(pl, pan);
+    if (nbk != 64) throw std::runtime_error("getrf: only the last panel may be narrower than 64");
+    plan_gemms_onelaunch<T>(pl, pan);
     std::vector<GemmItem> gm(nt);
     for (int t = 0; t < nt; ++t) {
       T* a = tiles[t];
@@ -368,17 +414,51 @@ void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& l
     }
     plan_gemms<T>(pl, gm);
   }
+  // full inverses of the unit-lower and upper factors by recursive doubling of the 64 x 64 block
+  // inverses: [[A, 0], [C, D]]^-1 has off-diagonal block -D^-1 (C A^-1); the upper case mirrors it.
+  Context& ctx = pl.ctx();
+  for (int sz = 64; sz < n; sz *= 2) {
+    std::vector<GemmItem> w1, w2;
+    for (int t = 0; t < nt; ++t) {
+      T* a = tiles[t];
+      for (int p = 0; p + sz < n; p += 2 * sz) {
+        const int d0 = p + sz, sd = std::min(sz, n - d0);
+        T* wl = ctx.take<T>((size_t)sz * sd + 1);
+        T* wu = ctx.take<T>((size_t)sz * sd + 1);
+        w1.push_back(gemm_item(a + (int64_t)d0 * ld + p, ld, 1, linv[t] + (size_t)p * n + p, n, 1, wl, sz, 1, sd, sz,
+                               sz, 1.0, 0.0));
+        w1.push_back(gemm_item(a + (int64_t)p * ld + d0, ld, 1, uinv[t] + (size_t)d0 * n + d0, n, 1, wu, sd, 1, sz,
+                               sd, sd, 1.0, 0.0));
+        w2.push_back(gemm_item(linv[t] + (size_t)d0 * n + d0, n, 1, wl, sz, 1, linv[t] + (size_t)d0 * n + p, n, 1,
+                               sd, sz, sd, -1.0, 0.0));
+        w2.push_back(gemm_item(uinv[t] + (size_t)p * n + p, n, 1, wu, sd, 1, uinv[t] + (size_t)p * n + d0, n, 1,
+                               sz, sd, sz, -1.0, 0.0));
+      }
+    }
+    plan_gemms<T>(pl, w1);
+    plan_gemms<T>(pl, w2);
+  }
+}
+
+template <typename T>
+static size_t getrf_workspace(int n, int nt) {
+  size_t w = 4096;
+  for (int sz = 64; sz < n; sz *= 2) w += (size_t)nt * ((n + 2 * sz - 1) / (2 * sz)) * 2 * sizeof(T) * ((size_t)sz * sz + 64);
+  return w;
 }
 
 template <typename T>
 void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_dev, Context& ctx) {
-  const int nbk = (n + 63) / 64;
-  DevBuf inv(sizeof(T) * tiles.size() * nbk * 4096 * 2);
+  const size_t nn = (size_t)n * n;
+  DevBuf inv(sizeof(T) * tiles.size() * nn * 2);
+  FRK_HIP_CHECK(hipMemsetAsync(inv.as<T>(), 0, sizeof(T) * tiles.size() * nn * 2, ctx.stream()));
   std::vector<T*> li(tiles.size()), ui(tiles.size());
   for (size_t t = 0; t < tiles.size(); ++t) {
-    li[t] = inv.as<T>() + t * nbk * 4096 * 2;
-    ui[t] = li[t] + nbk * 4096;
+    li[t] = inv.as<T>() + t * nn * 2;
+    ui[t] = li[t] + nn;
   }
+  ctx.sync();
+  ctx.reserve(getrf_workspace<T>(n, (int)tiles.size()));
   Plan pl(ctx);
   plan_getrf<T>(pl, tiles, li, ui, n, ld, info_dev);
   pl.run();
@@ -440,7 +520,6 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     nmax = std::max(nmax, x.n);
   }
   const int K = kmax;
-  // Yu doubles as the formed panel [Uc Sc | Ux Sx] (the QR reads it before writing reflectors)
   T* Yu = ctx.take<T>((size_t)nd * mmax * K);
   T* Yv = ctx.take<T>((size_t)nd * nmax * K);
   T* tu = ctx.take<T>((size_t)nd * K);
@@ -449,16 +528,6 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   T* Rv = ctx.take<T>((size_t)nd * K * K);
   T* Um = ctx.take<T>((size_t)nd * K * K);
   T* Vm = ctx.take<T>((size_t)nd * K * K);
-  const T* I = nullptr;
-  int ild = 0;
-  for (const auto& x : d)
-    if (x.xs == nullptr) {
-      I = ctx.identity<T>(K + 1);
-      ild = ctx.ident_ld();
-      break;
-    }
-  std::vector<GemmItem> form;
-  form.reserve(2 * nd);
   std::vector<QrItem> qi;
   qi.reserve(2 * nd);
   std::vector<SvdItem> si(nd);
@@ -474,8 +543,9 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     T* rv = Rv + (size_t)t * K * K;
     T* um = Um + (size_t)t * K * K;
     T* vm = Vm + (size_t)t * K * K;
-    const T* xs = x.xs ? x.xs : I;
-    const int xsld = x.xs ? x.xs_ld : ild;
+    // a null S stands for the identity (X = U V): the QR loads U directly
+    const T* xs = x.xs;
+    const int xsld = x.xs_ld;
     QrItem q{};
     q.Y = yu;
     q.ldy = x.m;
@@ -490,29 +560,36 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     v.R = rv;
     v.m = x.n;
     if (!fast) {
-      // Uc = [Uc Sc | Ux Sx] (formed by GEMM), Vc^T = [Vc^T | Vx^T]
-      form.push_back(gemm_item(x.cu, x.cu_rs, x.cu_cs, x.cs, x.lds, 1, yu, 1, x.m, x.m, x.kc, x.kc, 1.0, 0.0));
-      form.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, xs, xsld, 1, yu + (size_t)x.kc * x.m, 1, x.m, x.m, x.kx2,
-                               x.kx1, 1.0, 0.0));
-      q.A1 = yu;
-      q.a1_rs = 1;
-      q.a1_cs = x.m;
-      q.k1 = Kt;
-      q.k2 = 0;
+      // Uc = [Uc Sc | Ux Sx] (formed inside the QR kernel), Vc^T = [Vc^T | Vx^T]
+      q.A1 = x.cu;
+      q.a1_rs = x.cu_rs;
+      q.a1_cs = x.cu_cs;
+      q.S1 = x.cs;
+      q.s1_ld = x.lds;
+      q.k1a = q.k1 = x.kc;
+      q.A2 = x.xu;
+      q.a2_rs = x.xu_rs;
+      q.a2_cs = x.xu_cs;
+      q.S2 = xs;
+      q.s2_ld = xsld;
+      q.k2a = x.kx1;
+      q.k2 = x.kx2;
       v.A1 = x.cv;
       v.a1_rs = x.cv_cs;
       v.a1_cs = x.cv_rs;
-      v.k1 = x.kc;
+      v.k1a = v.k1 = x.kc;
       v.A2 = x.xv;
       v.a2_rs = x.xv_cs;
       v.a2_cs = x.xv_rs;
-      v.k2 = x.kx2;
+      v.k2a = v.k2 = x.kx2;
     } else {
       // project X's bases on the (orthonormal) bases of C, QR the remainders
-      form.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, xs, xsld, 1, yu, 1, x.m, x.m, x.kx2, x.kx1, 1.0, 0.0));
-      q.A1 = yu;
-      q.a1_rs = 1;
-      q.a1_cs = x.m;
+      q.A1 = x.xu;
+      q.a1_rs = x.xu_rs;
+      q.a1_cs = x.xu_cs;
+      q.S1 = xs;
+      q.s1_ld = xsld;
+      q.k1a = x.kx1;
       q.k1 = x.kx2;
       q.k2 = 0;
       q.P = x.cu;
@@ -522,7 +599,7 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
       v.A1 = x.xv;
       v.a1_rs = x.xv_cs;
       v.a1_cs = x.xv_rs;
-      v.k1 = x.kx2;
+      v.k1a = v.k1 = x.kx2;
       v.k2 = 0;
       v.P = x.cv;
       v.p_rs = x.cv_cs;
@@ -557,6 +634,7 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     sv.ldv = K;
     sv.rank_out = x.d_rank;
     sv.cap = x.cap;
+    sv.stats = ctx.counters();
     si[t] = sv;
     ApplyQItem a{};
     a.Y = yu;
@@ -622,7 +700,6 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   for (const auto& q : qi) qk = std::max(qk, q.k1 + q.k2);
   int capmax = 0;
   for (const auto& x : d) capmax = std::max(capmax, std::min(x.cap, x.kc + x.kx2));
-  plan_gemms<T>(pl, form);
   const size_t oq = pl.add(qi), os = pl.add(si), oa = pl.add(ai);
   const int nq = (int)qi.size(), na = (int)ai.size();
   pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oq), nq, qk, s); });
@@ -736,7 +813,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       q.A1 = rp + (size_t)(r - 1) * x.n;
       q.a1_rs = 1;
       q.a1_cs = -(int64_t)x.n;
-      q.k1 = r;
+      q.k1a = q.k1 = r;
       q.m = x.n;
       q.Y = Yz + (size_t)t * nmax * rmax;
       q.ldy = x.n;
@@ -832,7 +909,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       q.A1 = y;
       q.a1_rs = 1;
       q.a1_cs = x.m;
-      q.k1 = p;
+      q.k1a = q.k1 = p;
       q.m = x.m;
       q.Y = y;
       q.ldy = x.m;
@@ -861,7 +938,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       qq.A1 = z;
       qq.a1_rs = 1;
       qq.a1_cs = x.n;
-      qq.k1 = p;
+      qq.k1a = qq.k1 = p;
       qq.m = x.n;
       qq.Y = z;
       qq.ldy = x.n;
@@ -884,6 +961,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       s0.ldv = p;
       s0.rank_out = x.d_rank;
       s0.cap = x.cap;
+      s0.stats = ctx.counters();
       sv.push_back(s0);
       ApplyQItem b{};
       b.Y = y;
@@ -1112,6 +1190,10 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
       for (int r = 0; r < (int)A.rank.size(); ++r) kmax = std::max(kmax, A.rank[r]);
       const size_t bm = (size_t)std::max<int64_t>(A.rows(0), A.cols(0));
       need += (size_t)rem * 2 * sizeof(T) * (bm * kmax + 256);
+      for (int t2 = i + 1; t2 < nb; ++t2) {
+        if (A.kind[A.idx(t2, i)] == kDense) need += sizeof(T) * (bm * bm + 64);
+        if (A.kind[A.idx(i, t2)] == kDense) need += sizeof(T) * (bm * bm + 64);
+      }
       for (int ic = i + 1; ic < nb; ++ic)
         for (int k = i + 1; k < nb; ++k) {
           const int kl = A.kind[A.idx(ic, i)], kb = A.kind[A.idx(i, k)], kc = A.kind[A.idx(ic, k)];
@@ -1127,29 +1209,68 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           }
         }
     }
+    if (sizeof(T) == 4) need += sizeof(double) * ((size_t)3 * m_i * n_i + 768);
+    need += getrf_workspace<double>(m_i, 1);
     ctx.reserve(need);
     Plan pl(ctx);
-    // 1. diagonal tile
-    plan_getrf<T>(pl, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info + i * npan_max);
+    // 1. diagonal tile (always factorized in FP64)
+    if constexpr (sizeof(T) == 8) {
+      plan_getrf<T>(pl, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info + i * npan_max);
+    } else {
+      const int64_t nn = (int64_t)m_i * n_i, ni = nn;
+      double* d64 = ctx.take<double>(nn);
+      double* l64 = ctx.take<double>(ni);
+      double* u64 = ctx.take<double>(ni);
+      T* dt = A.D[ii];
+      T* lt = A.Linv[ii];
+      T* ut = A.Uinv[ii];
+      pl.launch([=](const char*) {
+        launch_convert<double, T>(d64, dt, nn, s);
+        launch_fill_value(l64, ni, 0.0, s);
+        launch_fill_value(u64, ni, 0.0, s);
+      });
+      plan_getrf<double>(pl, {d64}, {l64}, {u64}, m_i, n_i, info + i * npan_max);
+      pl.launch([=](const char*) {
+        launch_convert<T, double>(dt, d64, nn, s);
+        launch_convert<T, double>(lt, l64, ni, s);
+        launch_convert<T, double>(ut, u64, ni, s);
+      });
+    }
     const T* Dii = A.D[ii];
     const int64_t ldd = n_i;
-    // 2. panel solves (src/operations/LAPACK/getrf.cpp:50-57)
-    std::vector<TrsmItem> tr;
+    // 2. panel solves (src/operations/LAPACK/getrf.cpp:50-57) as products with the explicit
+    //    triangular inverses, written to workspace and copied back into the slots
+    std::vector<GemmItem> tg, tcp;
     for (int ic = i + 1; ic < nb; ++ic) {
       const int t = A.idx(ic, i);
-      if (A.kind[t] == kLowRank)  // V <- V U^-1  <=>  U^T V^T = V^T
-        tr.push_back(trsm_item(Dii, 1, ldd, A.Uinv[ii], 1, 64, A.V[t], 1, A.cols(i), n_i, A.rank[t], true));
-      else if (A.kind[t] == kDense)
-        tr.push_back(trsm_item(Dii, 1, ldd, A.Uinv[ii], 1, 64, A.D[t], 1, A.cols(i), n_i, (int)A.rows(ic), true));
+      if (A.kind[t] == kLowRank) {  // V <- V U^-1
+        const int r = A.rank[t];
+        T* w = ctx.take<T>((size_t)r * n_i + 1);
+        tg.push_back(gemm_item(A.V[t], A.cols(i), 1, A.Uinv[ii], n_i, 1, w, n_i, 1, r, n_i, n_i, 1.0, 0.0));
+        tcp.push_back(gemm_item(w, n_i, 1, nullptr, 0, 0, A.V[t], A.cols(i), 1, r, n_i, 0, 0.0, 0.0));
+      } else if (A.kind[t] == kDense) {
+        const int m = (int)A.rows(ic);
+        T* w = ctx.take<T>((size_t)m * n_i + 1);
+        tg.push_back(gemm_item(A.D[t], A.cols(i), 1, A.Uinv[ii], n_i, 1, w, n_i, 1, m, n_i, n_i, 1.0, 0.0));
+        tcp.push_back(gemm_item(w, n_i, 1, nullptr, 0, 0, A.D[t], A.cols(i), 1, m, n_i, 0, 0.0, 0.0));
+      }
     }
     for (int j = i + 1; j < nb; ++j) {
       const int t = A.idx(i, j);
-      if (A.kind[t] == kLowRank)  // U <- L^-1 U
-        tr.push_back(trsm_item(Dii, ldd, 1, A.Linv[ii], 64, 1, A.U[t], 1, A.rows(i), m_i, A.rank[t], true));
-      else if (A.kind[t] == kDense)
-        tr.push_back(trsm_item(Dii, ldd, 1, A.Linv[ii], 64, 1, A.D[t], A.cols(j), 1, m_i, (int)A.cols(j), true));
+      if (A.kind[t] == kLowRank) {  // U <- L^-1 U
+        const int r = A.rank[t];
+        T* w = ctx.take<T>((size_t)m_i * r + 1);
+        tg.push_back(gemm_item(A.Linv[ii], m_i, 1, A.U[t], 1, A.rows(i), w, 1, m_i, m_i, r, m_i, 1.0, 0.0));
+        tcp.push_back(gemm_item(w, 1, m_i, nullptr, 0, 0, A.U[t], 1, A.rows(i), m_i, r, 0, 0.0, 0.0));
+      } else if (A.kind[t] == kDense) {
+        const int n = (int)A.cols(j);
+        T* w = ctx.take<T>((size_t)m_i * n + 1);
+        tg.push_back(gemm_item(A.Linv[ii], m_i, 1, A.D[t], n, 1, w, n, 1, m_i, n, m_i, 1.0, 0.0));
+        tcp.push_back(gemm_item(w, n, 1, nullptr, 0, 0, A.D[t], n, 1, m_i, n, 0, 0.0, 0.0));
+      }
     }
-    plan_trsms<T>(pl, tr);
+    plan_gemms<T>(pl, tg);
+    plan_copies<T>(pl, tcp);
     if (rem > 0) {
       // 3. Schur updates A(ic,k) -= L(ic,i) A(i,k) (getrf.cpp:58-62, gemm.cpp:126-272)
       std::vector<T*> P(nb, nullptr), Q(nb, nullptr);
@@ -1343,20 +1464,23 @@ void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
   F.pull_ranks(ctx.stream());
   int kmax = 1;
   for (int r : F.rank) kmax = std::max(kmax, r);
-  ctx.reserve((size_t)nb * nb * 2 * sizeof(T) * ((size_t)kmax * nrhs + 64) + 4096);
+  ctx.reserve((size_t)nb * nb * 2 * sizeof(T) * ((size_t)kmax * nrhs + 64) +
+              (size_t)2 * nb * sizeof(T) * ((size_t)F.rows(0) * nrhs + 64) + 4096);
   auto bptr = [&](int i) { return b + F.roff[i] * nrhs; };
   auto sweep = [&](Plan& pl, bool forward, bool lower_factor, bool transposed) {
     for (int step = 0; step < nb; ++step) {
       const int i = forward ? step : nb - 1 - step;
       const int ii = F.idx(i, i);
       const int64_t ld = F.cols(i);
-      const T* dinv = lower_factor ? F.Linv[ii] : F.Uinv[ii];
-      TrsmItem x;
+      const T* tinv = lower_factor ? F.Linv[ii] : F.Uinv[ii];
+      const int ni_ = (int)F.rows(i);
+      T* w = ctx.take<T>((size_t)ni_ * nrhs + 1);
+      // b_i <- op(T)^-1 b_i with the explicit inverse (op = transpose for the adjoint sweeps)
       if (!transposed)
-        x = trsm_item(F.D[ii], ld, 1, dinv, 64, 1, bptr(i), nrhs, 1, (int)F.rows(i), nrhs, lower_factor);
+        plan_gemms<T>(pl, {gemm_item(tinv, ld, 1, bptr(i), nrhs, 1, w, nrhs, 1, ni_, nrhs, ni_, 1.0, 0.0)});
       else
-        x = trsm_item(F.D[ii], 1, ld, dinv, 1, 64, bptr(i), nrhs, 1, (int)F.rows(i), nrhs, !lower_factor);
-      plan_trsms<T>(pl, {x});
+        plan_gemms<T>(pl, {gemm_item(tinv, 1, ld, bptr(i), nrhs, 1, w, nrhs, 1, ni_, nrhs, ni_, 1.0, 0.0)});
+      plan_copies<T>(pl, {gemm_item(w, nrhs, 1, nullptr, 0, 0, bptr(i), nrhs, 1, ni_, nrhs, 0, 0.0, 0.0)});
       std::vector<GemmItem> g1, g2, g3;
       for (int k = 0; k < nb; ++k) {
         if (forward ? (k <= i) : (k >= i)) continue;
@@ -1435,6 +1559,25 @@ void blr_matvec(BlrMatrix<T>& A, const T* x, T* y, int nrhs, Context& ctx) {
   }
   pl.run();
 }
+
+// ============================================================================
+// Precision conversion of a whole BLR matrix (same structure)
+// ============================================================================
+template <typename TD, typename TS>
+std::unique_ptr<BlrMatrix<TD>> convert_blr(const BlrMatrix<TS>& src, Context& ctx) {
+  auto dst = std::make_unique<BlrMatrix<TD>>(src.roff, src.coff, src.kind, src.kcap, src.eps, src.fixed_rank, ctx);
+  dst->lra = src.lra;
+  const hipStream_t s = ctx.stream();
+  launch_convert<TD, TS>(dst->arena_data(), src.arena_data(), (int64_t)(src.arena_bytes() / sizeof(TS)), s);
+  if (src.dinv_count() && dst->dinv_count() == src.dinv_count())
+    launch_convert<TD, TS>(dst->dinv_data(), src.dinv_data(), (int64_t)src.dinv_count(), s);
+  dst->set_ranks_from_device(src.d_rank(0, 0), s);
+  FRK_HIP_CHECK(hipStreamSynchronize(s));
+  dst->rank = src.rank;
+  return dst;
+}
+template std::unique_ptr<BlrMatrix<float>> convert_blr<float, double>(const BlrMatrix<double>&, Context&);
+template std::unique_ptr<BlrMatrix<double>> convert_blr<double, float>(const BlrMatrix<float>&, Context&);
 
 // ============================================================================
 // Iterative refinement with the exact FP64 kernel matrix

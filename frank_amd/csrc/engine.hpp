@@ -85,6 +85,8 @@ class Context {
   template <class X>
   const X* omega(int n, int p);
   int32_t* info() { return info_.as<int32_t>(); }
+  // Device event counters (kCounter* slots), zero at creation.
+  int32_t* counters() { return counters_.as<int32_t>(); }
 
  private:
   hipStream_t s_;
@@ -96,7 +98,11 @@ class Context {
   DevBuf omega64_, omega32_;
   int omega_n_ = 0, omega_p_ = 0;
   DevBuf info_;
+  DevBuf counters_;
 };
+
+// Slots of Context::counters().
+enum Counter : int32_t { kCounterSvdProblems = 0, kCounterSvdSweeps = 1, kCounterSvdMaxSweeps = 2, kNumCounters = 16 };
 
 // A batch of launches whose descriptor arrays travel to the device in one
 // copy: add() item arrays, launch() closures that receive the device base,
@@ -142,6 +148,11 @@ class BlrMatrix {
   std::unique_ptr<BlrMatrix<T>> clone_structure(Context& ctx) const;
   void copy_from(const BlrMatrix<T>& src, hipStream_t s);
   size_t arena_bytes() const { return arena_bytes_; }
+  // raw slot storage (element-wise conversion between precisions)
+  T* arena_data() const { return arena_.as<T>(); }
+  T* dinv_data() const { return dinv_.as<T>(); }
+  size_t dinv_count() const { return dinv_.bytes() / sizeof(T); }
+  void set_ranks_from_device(const int32_t* src, hipStream_t s);
 
   std::vector<int64_t> roff, coff;
   std::vector<int32_t> kind;
@@ -277,6 +288,10 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm_inf,
                         int nrhs, double tol, int max_iter, Context& ctx);
 // ||A||_inf of a kernel matrix with non-negative entries (max row sum).
 double kernel_norm_inf(const KernelSpec& ks, int64_t n, Context& ctx);
+
+// Same tile structure in another storage precision, contents converted element-wise.
+template <typename TD, typename TS>
+std::unique_ptr<BlrMatrix<TD>> convert_blr(const BlrMatrix<TS>& src, Context& ctx);
 
 // Device kernel spec with uploaded points (owned by the returned buffer).
 KernelSpec upload_kernel(const KernelDesc& kd, DevBuf& pts, hipStream_t s);

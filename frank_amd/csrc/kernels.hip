@@ -3,8 +3,8 @@
 // Design notes (CDNA4, wave64):
 //  * Small dense factorizations (Householder QR, reflector application) keep
 //    the whole m x K operand in VGPRs: wave w owns columns c = q*W + w and
-//    lane l holds rows l + 64*t.  Column reductions are wave shuffles
-//    (xor butterfly, bitwise identical in every lane); the only workgroup
+//    lane l holds rows l + 64*t.  Column reductions are DPP / swizzle /
+//    permlane butterflies (bitwise identical in every lane); the only workgroup
 //    barrier per Householder step publishes the reflector through LDS.
 //  * Jacobi SVD of the K x K core runs in LDS with a round-robin pairing so
 //    K/2 rotations proceed in parallel, one lane group per pair.
@@ -32,98 +32,152 @@ struct Eps<float> {
   static constexpr float half = 5.9604644775390625e-08f;
 };
 
+// Lane exchanges of the 64-wide wavefront without the LDS crossbar: DPP for
+// partners inside a row of 16 lanes, ds_swizzle for xor 16 inside a half
+// wave, v_permlane32_swap (gfx950) across the two halves.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  return __hiloint2double(dpp_i<CTRL>(__double2hiint(v)), dpp_i<CTRL>(__double2loint(v)));
+}
+__device__ __forceinline__ int swz16_i(int v) { return __builtin_amdgcn_ds_swizzle(v, 0x401F); }
+__device__ __forceinline__ float xor16(float v) { return __int_as_float(swz16_i(__float_as_int(v))); }
+__device__ __forceinline__ double xor16(double v) {
+  return __hiloint2double(swz16_i(__double2hiint(v)), swz16_i(__double2loint(v)));
+}
+// v[l] + v[l ^ 32] with the operands in the same order in both halves
+__device__ __forceinline__ float half_sum(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ __forceinline__ double half_sum(double v) {
+  const auto h = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const auto l = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+
+// Sum over aligned groups of g lanes (g a power of two <= 64, uniform over
+// the wave).  Every step adds a value that is uniform over one sub-group to
+// the value of its partner sub-group, so all lanes of a group end with the
+// same bits.
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+__device__ __forceinline__ T group_sum(T v, int g) {
+  if (g >= 2) v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  if (g >= 4) v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  if (g >= 8) v += dpp_mov<0x141>(v);  // row_half_mirror
+  if (g >= 16) v += dpp_mov<0x140>(v); // row_mirror
+  if (g >= 32) v += xor16(v);
+  if (g >= 64) v = half_sum(v);
   return v;
 }
 
 template <typename T>
-__device__ __forceinline__ T group_sum(T v, int g) {
-  for (int off = g >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+__device__ __forceinline__ T wave_sum(T v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  v += xor16(v);
+  return half_sum(v);
 }
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ===========================================================================
-// Strided batched GEMM: 64 x 64 output tile per workgroup, 4 x 4 per thread.
+// Strided batched GEMM.  One TM x TN output tile per workgroup (256 threads,
+// RM x RN outputs each), K staged through LDS in chunks of TK with loads
+// mapped to whichever operand stride is unit (coalesced).  Variants:
+//   64 x 64 x 16 (square-ish), 64 x 16 x 32 (tall-skinny: m large, n <= 16),
+//   16 x 64 x 32 (short-wide), 16 x 16 x 64 (tiny output, long K).
 // ===========================================================================
-template <typename T>
+template <typename T, int TM, int TN, int TK, int RM, int RN>
 __global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ items) {
+  static_assert((TM / RM) * (TN / RN) == 256, "256 threads");
+  constexpr int TX = TN / RN;  // threads along n
   const GemmItem it = items[blockIdx.y];
-  const int tiles_n = (it.n + 63) >> 6;
-  const int tiles_m = (it.m + 63) >> 6;
+  const int tiles_n = (it.n + TN - 1) / TN;
+  const int tiles_m = (it.m + TM - 1) / TM;
   if ((int)blockIdx.x >= tiles_m * tiles_n) return;
-  const int row0 = (blockIdx.x / tiles_n) * 64;
-  const int col0 = (blockIdx.x % tiles_n) * 64;
-  __shared__ T As[16][65];
-  __shared__ T Bs[16][65];
+  const int row0 = (blockIdx.x / tiles_n) * TM;
+  const int col0 = (blockIdx.x % tiles_n) * TN;
+  __shared__ T As[TK][TM + 1];
+  __shared__ T Bs[TK][TN + 1];
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
-  T acc[4][4];
+  const int tx = tid % TX, ty = tid / TX;
+  T acc[RM][RN];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < RM; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+    for (int b = 0; b < RN; ++b) acc[a][b] = T(0);
   const T* __restrict__ A = static_cast<const T*>(it.A);
   const T* __restrict__ B = static_cast<const T*>(it.B);
   const bool a_kfast = (it.a_cs == 1 && it.a_rs != 1);
   const bool b_kfast = (it.b_rs == 1 && it.b_cs != 1);
-  for (int k0 = 0; k0 < it.k; k0 += 16) {
+  constexpr int AE = (TM * TK + 255) / 256, BE = (TN * TK + 255) / 256;
+  for (int k0 = 0; k0 < it.k; k0 += TK) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < AE; ++e) {
       const int idx = tid + e * 256;
-      int i, kk;
-      if (a_kfast) {
-        kk = idx & 15;
-        i = idx >> 4;
-      } else {
-        i = idx & 63;
-        kk = idx >> 6;
+      if (idx < TM * TK) {
+        int i, kk;
+        if (a_kfast) {
+          kk = idx % TK;
+          i = idx / TK;
+        } else {
+          i = idx % TM;
+          kk = idx / TM;
+        }
+        const int gi = row0 + i, gk = k0 + kk;
+        As[kk][i] = (gi < it.m && gk < it.k) ? A[(int64_t)gi * it.a_rs + (int64_t)gk * it.a_cs] : T(0);
       }
-      const int gi = row0 + i, gk = k0 + kk;
-      As[kk][i] = (gi < it.m && gk < it.k) ? A[(int64_t)gi * it.a_rs + (int64_t)gk * it.a_cs] : T(0);
     }
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < BE; ++e) {
       const int idx = tid + e * 256;
-      int j, kk;
-      if (b_kfast) {
-        kk = idx & 15;
-        j = idx >> 4;
-      } else {
-        j = idx & 63;
-        kk = idx >> 6;
+      if (idx < TN * TK) {
+        int j, kk;
+        if (b_kfast) {
+          kk = idx % TK;
+          j = idx / TK;
+        } else {
+          j = idx % TN;
+          kk = idx / TN;
+        }
+        const int gj = col0 + j, gk = k0 + kk;
+        Bs[kk][j] = (gj < it.n && gk < it.k) ? B[(int64_t)gk * it.b_rs + (int64_t)gj * it.b_cs] : T(0);
       }
-      const int gj = col0 + j, gk = k0 + kk;
-      Bs[kk][j] = (gj < it.n && gk < it.k) ? B[(int64_t)gk * it.b_rs + (int64_t)gj * it.b_cs] : T(0);
     }
     __syncthreads();
+#pragma unroll 8
+    for (int kk = 0; kk < TK; ++kk) {
+      T av[RM], bv[RN];
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      T av[4], bv[4];
+      for (int a = 0; a < RM; ++a) av[a] = As[kk][ty + (TM / RM) * a];
 #pragma unroll
-      for (int a = 0; a < 4; ++a) av[a] = As[kk][ty + 16 * a];
+      for (int b = 0; b < RN; ++b) bv[b] = Bs[kk][tx + TX * b];
 #pragma unroll
-      for (int b = 0; b < 4; ++b) bv[b] = Bs[kk][tx + 16 * b];
+      for (int a = 0; a < RM; ++a)
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] += av[a] * bv[b];
+        for (int b = 0; b < RN; ++b) acc[a][b] += av[a] * bv[b];
     }
     __syncthreads();
   }
   T* C = static_cast<T*>(it.C);
   const T alpha = static_cast<T>(it.alpha), beta = static_cast<T>(it.beta);
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int i = row0 + ty + 16 * a;
+  for (int a = 0; a < RM; ++a) {
+    const int i = row0 + ty + (TM / RM) * a;
     if (i >= it.m) continue;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int j = col0 + tx + 16 * b;
+    for (int b = 0; b < RN; ++b) {
+      const int j = col0 + tx + TX * b;
       if (j >= it.n) continue;
       const int64_t idx = (int64_t)i * it.c_rs + (int64_t)j * it.c_cs;
       T v = alpha * acc[a][b];
@@ -136,8 +190,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ 
 template <typename T>
 void launch_gemm(const GemmItem* d_items, int n_items, int max_m, int max_n, hipStream_t s) {
   if (n_items <= 0 || max_m <= 0 || max_n <= 0) return;
-  dim3 grid(cdiv(max_m, 64) * cdiv(max_n, 64), n_items);
-  hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(256), 0, s, d_items);
+  if (max_m <= 16 && max_n <= 16) {
+    dim3 grid(1, n_items);
+    hipLaunchKernelGGL((gemm_kernel<T, 16, 16, 64, 1, 1>), grid, dim3(256), 0, s, d_items);
+  } else if (max_n <= 16) {
+    dim3 grid(cdiv(max_m, 64), n_items);
+    hipLaunchKernelGGL((gemm_kernel<T, 64, 16, 32, 4, 1>), grid, dim3(256), 0, s, d_items);
+  } else if (max_m <= 16) {
+    dim3 grid(cdiv(max_n, 64), n_items);
+    hipLaunchKernelGGL((gemm_kernel<T, 16, 64, 32, 1, 4>), grid, dim3(256), 0, s, d_items);
+  } else {
+    dim3 grid(cdiv(max_m, 64) * cdiv(max_n, 64), n_items);
+    hipLaunchKernelGGL((gemm_kernel<T, 64, 64, 16, 4, 4>), grid, dim3(256), 0, s, d_items);
+  }
   FRK_HIP_CHECK(hipGetLastError());
 }
 
@@ -334,84 +399,181 @@ void launch_tri_inv(const TriInvItem* d_items, int n_items, hipStream_t s) {
 }
 
 // ===========================================================================
-// No-pivot LU of a <= 64 x 64 diagonal block (one wave, lane j owns column j;
-// one reciprocal per pivot) followed by the inverses of its unit-lower and
-// upper factors, which the blocked trsm / panel updates consume.
+// No-pivot LU of a <= 64 x 64 diagonal block by 256 threads (thread t owns row
+// t / 4 and columns (t % 4) + 4 q), two barriers per pivot; then the inverses
+// of the unit-lower and upper factors by recursive doubling in LDS: 16 x 16
+// diagonal blocks by substitution, combined 16 -> 32 -> 64 with small products
+// ([[A,0],[C,D]]^-1 has off-diagonal block -D^-1 C A^-1; the upper case is
+// symmetric).  Padding beyond n is the identity.
 // ===========================================================================
 template <typename T>
-__global__ __launch_bounds__(64) void getrf_small_kernel(const LuItem* __restrict__ items,
+__global__ __launch_bounds__(256) void getrf_small_kernel(const LuItem* __restrict__ items,
                                                          int32_t* __restrict__ info) {
   const LuItem it = items[blockIdx.x];
   T* A = static_cast<T*>(it.a);
   const int n = it.n;
   const int64_t rs = it.rs, cs = it.cs;
-  const int lane = threadIdx.x;
-  __shared__ T col[64];
-  __shared__ T Lsh[64][65];
-  __shared__ T rdiag[64];
-  T a[64];
+  const int tid = threadIdx.x;
+  const int row = tid >> 2, cq = tid & 3;
+  __shared__ T rowk[2][64];
+  __shared__ T colk[2][64];
+  __shared__ T M[64][65];
+  __shared__ T Li[64][65];
+  __shared__ T Ui[64][65];
+  __shared__ T W[2][32][33];
+  __shared__ int bad_sh;
+  T a[16];
 #pragma unroll
-  for (int i = 0; i < 64; ++i) a[i] = (i < n && lane < n) ? A[i * rs + lane * cs] : T(i == lane ? 1 : 0);
-  int bad = 0;
+  for (int q = 0; q < 16; ++q) {
+    const int j = cq + 4 * q;
+    a[q] = (row < n && j < n) ? A[row * rs + j * cs] : T(row == j ? 1 : 0);
+  }
+  if (tid == 0) bad_sh = 0;
 #pragma unroll
   for (int k = 0; k < 64; ++k) {
-    const T piv = __shfl(a[k], k, 64);
-    if (piv == T(0) && bad == 0 && k < n) bad = k + 1;
-    const T inv = T(1) / piv;
-    if (lane == k) {
-      rdiag[k] = inv;
+    const int buf = k & 1;
+    if (row == k) {
 #pragma unroll
-      for (int i = k + 1; i < 64; ++i) {
-        a[i] = a[i] * inv;
-        col[i] = a[i];
+      for (int q = 0; q < 16; ++q) rowk[buf][cq + 4 * q] = a[q];
+    }
+    __syncthreads();
+    const T piv = rowk[buf][k];
+    if (cq == (k & 3) && row > k) {
+      const T l = a[k >> 2] / piv;
+      a[k >> 2] = l;
+      colk[buf][row] = l;
+    }
+    if (tid == 0 && piv == T(0) && k < n && bad_sh == 0) bad_sh = k + 1;
+    __syncthreads();
+    if (row > k) {
+      const T l = colk[buf][row];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int j = cq + 4 * q;
+        if (j > k) a[q] -= l * rowk[buf][j];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int j = cq + 4 * q;
+    if (row < n && j < n) A[row * rs + j * cs] = a[q];
+    M[row][j] = a[q];
+    Li[row][j] = T(0);
+    Ui[row][j] = T(0);
+  }
+  __syncthreads();
+  if (tid == 0 && info != nullptr) info[blockIdx.x] = bad_sh;
+  if (it.linv == nullptr && it.uinv == nullptr) return;
+  // level 0: eight 16 x 16 substitutions (4 unit-lower, 4 upper), 16 columns each
+  if (tid < 128) {
+    const int prob = tid >> 4, col = tid & 15;
+    const int o = 16 * (prob & 3);
+    T x[16];
+    if (prob < 4) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        T sacc = (r == col) ? T(1) : T(0);
+#pragma unroll
+        for (int kk = 0; kk < r; ++kk) sacc -= M[o + r][o + kk] * x[kk];
+        x[r] = sacc;
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Li[o + r][o + col] = x[r];
+    } else {
+#pragma unroll
+      for (int r = 15; r >= 0; --r) {
+        T sacc = (r == col) ? T(1) : T(0);
+#pragma unroll
+        for (int kk = r + 1; kk < 16; ++kk) sacc -= M[o + r][o + kk] * x[kk];
+        x[r] = sacc / M[o + r][o + r];
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) Ui[o + r][o + col] = x[r];
+    }
+  }
+  __syncthreads();
+  // levels 1 (s = 16, two pairs) and 2 (s = 32, one pair)
+#pragma unroll
+  for (int lev = 0; lev < 2; ++lev) {
+    const int sz = 16 << lev;
+    const int npair = 2 >> lev;  // pairs per factor
+    // W = C A^-1 (lower) and B D^-1 (upper), sz x sz each, for every pair
+    for (int e = tid; e < 2 * npair * sz * sz; e += 256) {
+      const int f = e / (npair * sz * sz);         // 0: L, 1: U
+      const int pr = (e / (sz * sz)) % npair;
+      const int r = (e / sz) % sz, c = e % sz;
+      const int o0 = 2 * sz * pr, o1 = o0 + sz;
+      T sacc = T(0);
+      if (f == 0) {
+        for (int t = 0; t < sz; ++t) sacc += M[o1 + r][o0 + t] * Li[o0 + t][o0 + c];
+      } else {
+        for (int t = 0; t < sz; ++t) sacc += M[o0 + r][o1 + t] * Ui[o1 + t][o1 + c];
+      }
+      W[f][pr * sz + r][c] = sacc;
+    }
+    __syncthreads();
+    for (int e = tid; e < 2 * npair * sz * sz; e += 256) {
+      const int f = e / (npair * sz * sz);
+      const int pr = (e / (sz * sz)) % npair;
+      const int r = (e / sz) % sz, c = e % sz;
+      const int o0 = 2 * sz * pr, o1 = o0 + sz;
+      T sacc = T(0);
+      if (f == 0) {
+        for (int t = 0; t < sz; ++t) sacc += Li[o1 + r][o1 + t] * W[0][pr * sz + t][c];
+        Li[o1 + r][o0 + c] = -sacc;
+      } else {
+        for (int t = 0; t < sz; ++t) sacc += Ui[o0 + r][o0 + t] * W[1][pr * sz + t][c];
+        Ui[o0 + r][o1 + c] = -sacc;
       }
     }
     __syncthreads();
-    if (lane > k) {
-      const T ukj = a[k];
-#pragma unroll
-      for (int i = k + 1; i < 64; ++i) a[i] -= col[i] * ukj;
-    }
-    __syncthreads();
   }
-#pragma unroll
-  for (int i = 0; i < 64; ++i) {
-    if (i < n && lane < n) A[i * rs + lane * cs] = a[i];
-    Lsh[i][lane] = a[i];
-  }
-  if (lane == 0 && info != nullptr) info[blockIdx.x] = bad;
-  __syncthreads();
-  T x[64];
-  if (it.linv != nullptr) {
-    T* out = static_cast<T*>(it.linv);
-#pragma unroll
-    for (int i = 0; i < 64; ++i) {
-      T sacc = (i == lane) ? T(1) : T(0);
-#pragma unroll
-      for (int kk = 0; kk < i; ++kk) sacc -= Lsh[i][kk] * x[kk];
-      x[i] = sacc;
+  T* lo = static_cast<T*>(it.linv);
+  T* uo = static_cast<T*>(it.uinv);
+  const int64_t ld = it.ld_inv;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (r < n && c < n) {
+      if (lo) lo[r * ld + c] = Li[r][c];
+      if (uo) uo[r * ld + c] = Ui[r][c];
     }
-#pragma unroll
-    for (int i = 0; i < 64; ++i) out[i * 64 + lane] = x[i];
-  }
-  if (it.uinv != nullptr) {
-    T* out = static_cast<T*>(it.uinv);
-#pragma unroll
-    for (int i = 63; i >= 0; --i) {
-      T sacc = (i == lane) ? T(1) : T(0);
-#pragma unroll
-      for (int kk = i + 1; kk < 64; ++kk) sacc -= Lsh[i][kk] * x[kk];
-      x[i] = sacc * rdiag[i];
-    }
-#pragma unroll
-    for (int i = 0; i < 64; ++i) out[i * 64 + lane] = x[i];
   }
 }
 
 template <typename T>
 void launch_getrf_small(const LuItem* d_items, int n_items, int32_t* d_info, hipStream_t s) {
   if (n_items <= 0) return;
-  hipLaunchKernelGGL(getrf_small_kernel<T>, dim3(n_items), dim3(64), 0, s, d_items, d_info);
+  hipLaunchKernelGGL(getrf_small_kernel<T>, dim3(n_items), dim3(256), 0, s, d_items, d_info);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// Batched strided copy: C <- A (m x n), GemmItem fields A/a_rs/a_cs, C/c_rs/c_cs, m, n.
+template <typename T>
+__global__ __launch_bounds__(256) void copy_kernel(const GemmItem* __restrict__ items) {
+  const GemmItem it = items[blockIdx.y];
+  const T* src = static_cast<const T*>(it.A);
+  T* dst = static_cast<T*>(it.C);
+  const int64_t total = (int64_t)it.m * it.n;
+  const bool rfast = (it.c_rs == 1);
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    int64_t i, j;
+    if (rfast) {
+      i = e % it.m;
+      j = e / it.m;
+    } else {
+      i = e / it.n;
+      j = e % it.n;
+    }
+    dst[i * it.c_rs + j * it.c_cs] = src[i * it.a_rs + j * it.a_cs];
+  }
+}
+
+template <typename T>
+void launch_copy(const GemmItem* d_items, int n_items, int64_t max_elems, hipStream_t s) {
+  if (n_items <= 0 || max_elems <= 0) return;
+  const int gx = (int)std::min<int64_t>((max_elems + 255) / 256, 64);
+  hipLaunchKernelGGL(copy_kernel<T>, dim3(gx, n_items), dim3(256), 0, s, d_items);
   FRK_HIP_CHECK(hipGetLastError());
 }
 
@@ -432,17 +594,54 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
   __shared__ T taus[W * CPW];
   T a[CPW][MR];
 #pragma unroll
-  for (int q = 0; q < CPW; ++q) {
-    const int c = q * W + w;
-    const bool first = c < it.k1;
-    const T* Ab = static_cast<const T*>(first ? it.A1 : it.A2);
-    const int64_t rs = first ? it.a1_rs : it.a2_rs;
-    const int64_t cs = first ? it.a1_cs : it.a2_cs;
-    const int cc = first ? c : c - it.k1;
+  for (int q = 0; q < CPW; ++q)
 #pragma unroll
-    for (int t = 0; t < MR; ++t) {
-      const int row = lane + 64 * t;
-      a[q][t] = (c < K && row < m) ? Ab[row * rs + cc * cs] : T(0);
+    for (int t = 0; t < MR; ++t) a[q][t] = T(0);
+  constexpr int CS = 8;
+  __shared__ T stage[CS][MR * 64];
+  for (int blk = 0; blk < 2; ++blk) {
+    const T* Ab = static_cast<const T*>(blk ? it.A2 : it.A1);
+    const T* S = static_cast<const T*>(blk ? it.S2 : it.S1);
+    const int64_t rs = blk ? it.a2_rs : it.a1_rs, cs = blk ? it.a2_cs : it.a1_cs;
+    const int sld = blk ? it.s2_ld : it.s1_ld;
+    const int kc = blk ? it.k2 : it.k1, ka = blk ? it.k2a : it.k1a;
+    const int cbase = blk ? it.k1 : 0;
+    if (kc <= 0) continue;
+    if (S == nullptr) {
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) {
+        const int c = q * W + w;
+        if (c >= cbase && c < cbase + kc) {
+#pragma unroll
+          for (int t = 0; t < MR; ++t) {
+            const int row = lane + 64 * t;
+            a[q][t] = (row < m) ? Ab[row * rs + (c - cbase) * cs] : T(0);
+          }
+        }
+      }
+    } else {
+      // column c of A S = sum_s A[:, s] S[s, c]: stage CS columns of A at a time (coalesced)
+      for (int s0 = 0; s0 < ka; s0 += CS) {
+        const int cnt = min(CS, ka - s0);
+        __syncthreads();
+        for (int e = threadIdx.x; e < cnt * MR * 64; e += W * 64) {
+          const int si = e / (MR * 64), row = e % (MR * 64);
+          stage[si][row] = (row < m) ? Ab[row * rs + (int64_t)(s0 + si) * cs] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < CPW; ++q) {
+          const int c = q * W + w;
+          if (c >= cbase && c < cbase + kc) {
+            const int cc = c - cbase;
+            for (int si = 0; si < cnt; ++si) {
+              const T coef = S[(s0 + si) * sld + cc];
+#pragma unroll
+              for (int t = 0; t < MR; ++t) a[q][t] += stage[si][lane + 64 * t] * coef;
+            }
+          }
+        }
+      }
     }
   }
   T* Rb = static_cast<T*>(it.R);
@@ -692,15 +891,14 @@ __device__ __forceinline__ int rr_pos(int pos, int r, int k2) {
   return pos == 0 ? 0 : ((pos - 1 + r) % (k2 - 1)) + 1;
 }
 
-template <typename T, int KMAX>
-__global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ items) {
+template <typename T, int KMAX, int NT>
+__global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ items) {
   const SvdItem it = items[blockIdx.x];
   const int K = it.K;
   __shared__ T G[KMAX][KMAX + 1];
   __shared__ T Vs[KMAX][KMAX + 1];
   __shared__ T sig[KMAX];
   __shared__ int order[KMAX];
-  __shared__ int flag;
   __shared__ int rank_sh;
   const int tid = threadIdx.x;
   if (K <= 0) {
@@ -708,7 +906,7 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
     return;
   }
   if (it.mode == 0) {
-    for (int e = tid; e < K * K; e += 256) {
+    for (int e = tid; e < K * K; e += NT) {
       const int i = e / K, j = e % K;
       G[j][i] = static_cast<const T*>(it.M)[i * it.m_rs + j * it.m_cs];
       Vs[j][i] = (i == j) ? T(1) : T(0);
@@ -718,17 +916,17 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
     const int J = it.kx;
     const T* Ru = static_cast<const T*>(it.Ru);
     const T* Rv = static_cast<const T*>(it.Rv);
-    for (int e = tid; e < K * J; e += 256) {
+    for (int e = tid; e < K * J; e += NT) {
       const int i = e / J, t = e % J;
       G[t][i] = Ru[i * it.ldr + t];
       Vs[t][i] = Rv[i * it.ldr + t];
     }
     __syncthreads();
-    constexpr int PER = (KMAX * KMAX + 255) / 256;
+    constexpr int PER = (KMAX * KMAX + NT - 1) / NT;
     T mv[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * 256;
+      const int e = tid + u * NT;
       T sacc = T(0);
       if (e < K * K) {
         const int i = e / K, j = e % K;
@@ -740,7 +938,7 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
-      const int e = tid + u * 256;
+      const int e = tid + u * NT;
       if (e < K * K) {
         const int i = e / K, j = e % K;
         G[j][i] = mv[u];
@@ -752,13 +950,15 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
   const int K2 = K + (K & 1);
   const int P = K2 / 2;
   int gl = 64;
-  while (gl > 1 && P * gl > 256) gl >>= 1;
+  while (gl > 1 && P * gl > NT) gl >>= 1;
   const int p = tid / gl, sub = tid % gl;
-  const T tol = T(K) * Eps<T>::v;
+  // rotation threshold: above the rounding noise of a length-K dot product
+  const T tol = T(8 * max(K, 8)) * Eps<T>::v;
+  int sweeps = 0;
   if (K >= 2) {
     for (int sweep = 0; sweep < 60; ++sweep) {
-      if (tid == 0) flag = 0;
-      __syncthreads();
+      ++sweeps;
+      int rotated = 0;
       for (int r = 0; r < K2 - 1; ++r) {
         if (p < P) {
           int i = rr_pos(p, r, K2), j = rr_pos(K2 - 1 - p, r, K2);
@@ -791,24 +991,27 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
                 Vs[i][row] = cth * vx - sth * vy;
                 Vs[j][row] = sth * vx + cth * vy;
               }
-              if (sub == 0) flag = 1;
+              rotated = 1;
             }
           }
         }
         __syncthreads();
       }
-      const int f = flag;
-      __syncthreads();
-      if (f == 0) break;
+      if (!__syncthreads_or(rotated)) break;
     }
   }
-  for (int j = tid; j < K; j += 256) {
+  if (tid == 0 && it.stats != nullptr) {
+    atomicAdd(it.stats, 1);
+    atomicAdd(it.stats + 1, sweeps);
+    atomicMax(it.stats + 2, sweeps);
+  }
+  for (int j = tid; j < K; j += NT) {
     T s = T(0);
     for (int row = 0; row < K; ++row) s += G[j][row] * G[j][row];
     sig[j] = sqrt(s);
   }
   __syncthreads();
-  for (int j = tid; j < K; j += 256) {
+  for (int j = tid; j < K; j += NT) {
     int rk = 0;
     const T sj = sig[j];
     for (int i = 0; i < K; ++i)
@@ -850,7 +1053,7 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
   __syncthreads();
   const int r = rank_sh;
   // normalise left vectors in place
-  for (int e = tid; e < K * K; e += 256) {
+  for (int e = tid; e < K * K; e += NT) {
     const int c = e / K, row = e % K;
     const T s = sig[c];
     if (s > T(0)) G[c][row] = G[c][row] / s;
@@ -884,11 +1087,11 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
   T* Um = static_cast<T*>(it.Um);
   T* Vm = static_cast<T*>(it.Vm);
   if (S != nullptr)
-    for (int e = tid; e < r * r; e += 256) {
+    for (int e = tid; e < r * r; e += NT) {
       const int i = e / r, j = e % r;
       S[i * it.lds + j] = (i == j) ? sig[order[i]] : T(0);
     }
-  for (int e = tid; e < K * r; e += 256) {
+  for (int e = tid; e < K * r; e += NT) {
     const int row = e / r, c = e % r;
     const int src = order[c];
     if (Um != nullptr) Um[row * it.ldu + c] = G[src][row];
@@ -899,14 +1102,16 @@ __global__ __launch_bounds__(256) void svd_kernel(const SvdItem* __restrict__ it
 template <typename T>
 void launch_svd(const SvdItem* d_items, int n_items, int max_k, hipStream_t s) {
   if (n_items <= 0) return;
-  if (max_k <= 32) {
-    hipLaunchKernelGGL((svd_kernel<T, 32>), dim3(n_items), dim3(256), 0, s, d_items);
+  if (max_k <= 16) {
+    hipLaunchKernelGGL((svd_kernel<T, 16, 64>), dim3(n_items), dim3(64), 0, s, d_items);
+  } else if (max_k <= 32) {
+    hipLaunchKernelGGL((svd_kernel<T, 32, 64>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 64) {
-    hipLaunchKernelGGL((svd_kernel<T, 64>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((svd_kernel<T, 64, 256>), dim3(n_items), dim3(256), 0, s, d_items);
   } else {
     if constexpr (sizeof(T) == 4) {
       if (max_k <= 128) {
-        hipLaunchKernelGGL((svd_kernel<T, 128>), dim3(n_items), dim3(256), 0, s, d_items);
+        hipLaunchKernelGGL((svd_kernel<T, 128, 256>), dim3(n_items), dim3(256), 0, s, d_items);
         FRK_HIP_CHECK(hipGetLastError());
         return;
       }
@@ -1362,6 +1567,7 @@ template void launch_axpy_convert<float>(double*, const float*, int64_t, hipStre
   template void launch_trsm<T>(const TrsmItem*, int, int, int, hipStream_t);                     \
   template void launch_getrf_small<T>(const LuItem*, int, int32_t*, hipStream_t);                \
   template void launch_tri_inv<T>(const TriInvItem*, int, hipStream_t);                          \
+  template void launch_copy<T>(const GemmItem*, int, int64_t, hipStream_t);                      \
   template void launch_qr<T>(const QrItem*, int, int, hipStream_t);                              \
   template void launch_apply_q<T>(const ApplyQItem*, int, int, hipStream_t);                     \
   template void launch_svd<T>(const SvdItem*, int, int, hipStream_t);                            \
@@ -1376,5 +1582,6 @@ FRK_INST(float)
 template void launch_convert<float, double>(float*, const double*, int64_t, hipStream_t);
 template void launch_convert<double, double>(double*, const double*, int64_t, hipStream_t);
 template void launch_convert<double, float>(double*, const float*, int64_t, hipStream_t);
+template void launch_convert<float, float>(float*, const float*, int64_t, hipStream_t);
 
 }  // namespace frk

@@ -23,6 +23,10 @@ void launch_tri_inv(const TriInvItem* d_items, int n_items, hipStream_t s);
 template <typename T>
 void launch_getrf_small(const LuItem* d_items, int n_items, int32_t* d_info, hipStream_t s);
 
+// Batched strided copy C <- A (GemmItem: A, a_rs, a_cs, C, c_rs, c_cs, m, n).
+template <typename T>
+void launch_copy(const GemmItem* d_items, int n_items, int64_t max_elems, hipStream_t s);
+
 // Householder QR of [A1 S1 | A2 S2] (optionally projected), m <= 512.
 template <typename T>
 void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s);

@@ -62,6 +62,12 @@ int frk_ctx_destroy(frk_ctx ctx);
 int frk_ctx_synchronize(frk_ctx ctx);
 /* The hipStream_t every call on this context is ordered on (for events / interop). */
 int frk_ctx_stream(frk_ctx ctx, void** stream);
+/* Device event counters of this context (no reference counterpart; the
+ * reference's timers in src/util/timer.cpp cover host time only).  Copies
+ * min(n, 16) slots after draining the stream, then clears them if reset != 0.
+ * Slot 0: small SVD problems solved, 1: Jacobi sweeps summed over them,
+ * 2: largest sweep count of one problem. */
+int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset);
 
 /* ---------------------------------------------------------------------------
  * BLR matrices (one-level Hierarchical, tiles resident in HBM)
@@ -93,6 +99,9 @@ int frk_blr_set_lowrank(frk_blr a, int i, int j, int rank, const double* U, cons
 int frk_blr_clone(frk_blr src, frk_blr* out);
 /* Restore dst's contents from src (same structure): one device-to-device copy. */
 int frk_blr_copy(frk_blr dst, frk_blr src);
+/* Same structure in another storage precision (FRK_PREC_F64 / FRK_PREC_F32),
+ * tiles converted element-wise (no reference counterpart). */
+int frk_blr_convert(frk_blr src, int precision, frk_blr* out);
 
 /* getrf(Hierarchical&) (src/operations/LAPACK/getrf.cpp:45-65): tile LU without
  * pivoting, in place.  Afterwards tiles below the diagonal hold L, tiles above

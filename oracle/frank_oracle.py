@@ -796,3 +796,134 @@ def l2_error(a, b) -> float:
 def lu_product(L: Hier, U: Hier) -> np.ndarray:
     """Dense L @ U for the factorization-error metric ||A - LU|| / ||A||."""
     return L.dense() @ U.dense()
+
+
+# ---------------------------------------------------------------------------
+# Tiled Householder BLR-QR (src/operations/LAPACK/qr.cpp:213-257 and the
+# geqrt / larfb / tpqrt / tpmqrt / trmm tile operations)
+# ---------------------------------------------------------------------------
+def trmm(A: np.ndarray, B, side: str, uplo: str, trans: str = "n", diag: str = "n", alpha: float = 1.0):
+    """src/operations/BLAS/trmm.cpp:53-89: B = alpha op(tri(A)) B (left) or B op(tri(A)) (right).
+
+    LowRank B: left acts on B.U, right on B.V.  Returns the updated B.
+    """
+    t = np.tril(A) if uplo == "lower" else np.triu(A)
+    if diag == "u":
+        t = t - np.diag(np.diag(t)) + np.eye(t.shape[0])
+    op = t.T if trans == "t" else t
+    if isinstance(B, LowRank):
+        if side == "left":
+            return LowRank(alpha * (op @ B.U), B.S, B.V, B.eps)
+        return LowRank(B.U, B.S, alpha * (B.V @ op), B.eps)
+    B[...] = alpha * (op @ B) if side == "left" else alpha * (B @ op)
+    return B
+
+
+def geqrt(A: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """src/operations/LAPACK/geqrt.cpp:34-43: dgeqrt3 (one block): A -> R above / V below, T (n x n)."""
+    n = A.shape[1]
+    a, t, info = lapack.dgeqrt(n, np.array(A, dtype=np.float64, copy=True))
+    return a, t
+
+
+def larfb(V: np.ndarray, T: np.ndarray, C, trans: bool):
+    """src/operations/LAPACK/larfb.cpp:37-56: C <- (I - V T V^T)^{T|N} C ('L', 'F', 'C'); LowRank -> on C.U."""
+    if isinstance(C, LowRank):
+        return LowRank(larfb(V, T, C.U.copy(), trans), C.S, C.V, C.eps)
+    out, info = lapack.dgemqrt(V, T, np.array(C, dtype=np.float64, copy=True), side="L",
+                               trans="T" if trans else "N")
+    C[...] = out
+    return C
+
+
+def tpqrt(A: np.ndarray, B, T_out=None):
+    """src/operations/LAPACK/tpqrt.cpp:35-53: dtpqrt2 (l = 0) of [A; B].
+
+    A: R (upper triangle used); B dense m x n or LowRank (V <- S V, S <- I, QR of [R; V]).
+    Returns (A, B, T).
+    """
+    if isinstance(B, LowRank):
+        bv = B.S @ B.V
+        A2, bv2, T = tpqrt(A, bv)
+        return A2, LowRank(B.U, np.eye(B.rank), bv2, B.eps), T
+    n = A.shape[1]
+    a, b, t, info = lapack.dtpqrt(0, n, np.array(A, dtype=np.float64, copy=True),
+                                  np.array(B, dtype=np.float64, copy=True))
+    return a, b, t
+
+
+def _transpose_any(X):
+    return X.transpose() if isinstance(X, LowRank) else np.ascontiguousarray(X.T)
+
+
+def tpmqrt(V, T: np.ndarray, A, B, trans: bool, lra: str = "rounded_addition"):
+    """src/operations/LAPACK/tpmqrt.cpp:45-170: apply the pentagonal reflector block to [A; B].
+
+    All-dense: dtprfb ('L', T/N, 'F', 'C', l = 0).  Any LowRank operand:
+    C = A + V^T B; C = T^{T|N} C; A -= C; B -= V C through the typed gemm /
+    trmm / += of the reference.  Returns (A, B).
+    """
+    if all(isinstance(x, np.ndarray) for x in (V, A, B)):
+        a, b, info = lapack.dtpmqrt(0, V, T, np.array(A, dtype=np.float64, copy=True),
+                                    np.array(B, dtype=np.float64, copy=True), side="L",
+                                    trans="T" if trans else "N")
+        return a, b
+    # C takes A's type when V is LowRank, else it is Dense(A) unless the
+    # specialisation keeps a LowRank C (tpmqrt.cpp:61-170)
+    if isinstance(A, LowRank) and (isinstance(V, LowRank) or isinstance(B, LowRank)):
+        C = A.copy()
+    else:
+        C = A.dense() if isinstance(A, LowRank) else A.copy()
+    Vt = _transpose_any(V)
+    C = gemm(Vt, B, C, 1.0, 1.0, lra=lra)                       # C = A + V^T B
+    C = trmm(T, C, "left", "upper", "t" if trans else "n", "n", 1.0)  # C = T^{T|N} C
+    I = np.eye(T.shape[0])
+    A = gemm(I, C, A, -1.0, 1.0, lra=lra)                       # A -= C
+    B = gemm(V, C, B, -1.0, 1.0, lra=lra)                       # B -= V C
+    return A, B
+
+
+def tiled_householder_blr_qr(A: Hier, lra: str = "rounded_addition") -> Tuple[Hier, dict]:
+    """src/operations/LAPACK/qr.cpp:213-228 (in place on A; returns A and the T tiles)."""
+    Tt = {}
+    for k in range(A.nb_c):
+        A[k, k], Tt[k, k] = geqrt(A[k, k])
+        for j in range(k + 1, A.nb_c):
+            A[k, j] = larfb(A[k, k], Tt[k, k], A[k, j], True)
+        for i in range(k + 1, A.nb_r):
+            A[k, k], A[i, k], Tt[i, k] = tpqrt(A[k, k], A[i, k])
+            for j in range(k + 1, A.nb_c):
+                A[k, j], A[i, j] = tpmqrt(A[i, k], Tt[i, k], A[k, j], A[i, j], True, lra)
+    return A, Tt
+
+
+def left_multiply_tiled_reflector(Y: Hier, Tt: dict, C: Hier, trans: bool, lra: str = "rounded_addition") -> Hier:
+    """src/operations/LAPACK/qr.cpp:230-257: C <- Q C (trans=False) or Q^T C from the stored reflectors."""
+    if trans:
+        for k in range(Y.nb_c):
+            for j in range(k, Y.nb_c):
+                C[k, j] = larfb(Y[k, k], Tt[k, k], C[k, j], True)
+            for i in range(k + 1, Y.nb_r):
+                for j in range(k, Y.nb_c):
+                    C[k, j], C[i, j] = tpmqrt(Y[i, k], Tt[i, k], C[k, j], C[i, j], True, lra)
+    else:
+        for k in range(Y.nb_c - 1, -1, -1):
+            for i in range(Y.nb_r - 1, k, -1):
+                for j in range(k, Y.nb_c):
+                    C[k, j], C[i, j] = tpmqrt(Y[i, k], Tt[i, k], C[k, j], C[i, j], False, lra)
+            for j in range(k, Y.nb_c):
+                C[k, j] = larfb(Y[k, k], Tt[k, k], C[k, j], False)
+    return C
+
+
+def blr_r_factor(A: Hier) -> np.ndarray:
+    """Dense R from the upper tiles of the factored A (upper triangle of diagonal tiles)."""
+    R = np.zeros((sum(r[1] for r in A.row_ranges[:A.nb_c]), sum(c[1] for c in A.col_ranges)))
+    for i, (r0, rn) in enumerate(A.row_ranges[:A.nb_c]):
+        for j, (c0, cn) in enumerate(A.col_ranges):
+            if j < i:
+                continue
+            b = A[i, j]
+            blk = b.dense() if isinstance(b, LowRank) else b
+            R[r0:r0 + rn, c0:c0 + cn] = np.triu(blk) if i == j else blk
+    return R
