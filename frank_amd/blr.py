@@ -56,6 +56,27 @@ class Context:
         L.check(L.lib().frk_ctx_counters(self.h, out, 16, int(reset)))
         return {"svd_problems": out[0], "svd_sweeps": out[1], "svd_max_sweeps": out[2]}
 
+    def time_class(self, cls: Optional[str]):
+        """Bracket every launch of kernel class ``cls`` with HIP events (None: none)."""
+        L.check(L.lib().frk_ctx_time_class(self.h, cls.encode() if cls else None))
+
+    def op_stats(self) -> dict:
+        """Op log per kernel class: launches, algorithmic flops/bytes, event-timed ms."""
+        buf = C.create_string_buffer(4096)
+        L.check(L.lib().frk_ctx_op_classes(self.h, buf, 4096))
+        out = {}
+        for cls in filter(None, buf.value.decode().split(",")):
+            n, t = C.c_int64(), C.c_int64()
+            f, b, ms = C.c_double(), C.c_double(), C.c_double()
+            L.check(L.lib().frk_ctx_op_stats(self.h, cls.encode(), C.byref(n), C.byref(f), C.byref(b), C.byref(ms),
+                                             C.byref(t)))
+            out[cls] = {"launches": n.value, "flops": f.value, "bytes": b.value, "ms": ms.value,
+                        "timed_launches": t.value}
+        return out
+
+    def reset_op_stats(self):
+        L.check(L.lib().frk_ctx_reset_op_stats(self.h))
+
     def close(self):
         if self.h:
             L.lib().frk_ctx_destroy(self.h)

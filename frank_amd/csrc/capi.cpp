@@ -222,6 +222,55 @@ int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset) {
   });
 }
 
+int frk_ctx_time_class(frk_ctx ctx, const char* cls) {
+  return guard([&] {
+    if (!ctx) throw std::invalid_argument("ctx is null");
+    ctx->ctx->sync();
+    ctx->ctx->time_class(cls ? std::string(cls) : std::string());
+    return FRK_OK;
+  });
+}
+
+int frk_ctx_op_classes(frk_ctx ctx, char* buf, int len) {
+  return guard([&] {
+    if (!ctx || !buf || len <= 0) throw std::invalid_argument("null argument or empty buffer");
+    std::string all;
+    for (const auto& kv : ctx->ctx->op_stats()) {
+      if (!all.empty()) all += ",";
+      all += kv.first;
+    }
+    const size_t n = std::min(all.size(), (size_t)len - 1);
+    std::memcpy(buf, all.data(), n);
+    buf[n] = 0;
+    return FRK_OK;
+  });
+}
+
+int frk_ctx_op_stats(frk_ctx ctx, const char* cls, int64_t* launches, double* flops, double* bytes, double* ms,
+                     int64_t* timed_launches) {
+  return guard([&] {
+    if (!ctx || !cls) throw std::invalid_argument("null argument");
+    const auto& st = ctx->ctx->op_stats();
+    const auto it = st.find(cls);
+    OpStat z;
+    const OpStat& o = (it == st.end()) ? z : it->second;
+    if (launches) *launches = o.launches;
+    if (flops) *flops = o.flops;
+    if (bytes) *bytes = o.bytes;
+    if (ms) *ms = o.ms;
+    if (timed_launches) *timed_launches = o.timed;
+    return FRK_OK;
+  });
+}
+
+int frk_ctx_reset_op_stats(frk_ctx ctx) {
+  return guard([&] {
+    if (!ctx) throw std::invalid_argument("ctx is null");
+    ctx->ctx->reset_op_stats();
+    return FRK_OK;
+  });
+}
+
 int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
                           double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
                           int precision, frk_blr* out) {

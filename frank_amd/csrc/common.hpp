@@ -79,8 +79,10 @@ struct TriInvItem {
 // and S1 is k1a x k1, likewise for block 2).  Optional projection first:
 // every column x becomes x - P (P^T x) (modified Gram-Schmidt against an
 // orthonormal P, m x kp) with the coefficients in rows [0, kp) of the R
-// buffer.  Outputs: reflectors Y (column-major, m x K, ld = ldy; unit
-// diagonal implicit), tau[K], R (rows [kp, kp + K) of the R buffer, ld = ldr).
+// buffer.  Outputs: reflectors Y (element (row, c) at Y[row*y_rs + c*y_cs],
+// rows below the diagonal only; unit diagonal implicit), tau[K], R (rows
+// [kp, kp + K) of the R buffer, row-major with ld = ldr; with r_upper_only
+// only the upper triangle is written, so R and Y may share one tile).
 // ---------------------------------------------------------------------------
 struct QrItem {
   const void* A1;
@@ -94,7 +96,35 @@ struct QrItem {
   int64_t a1_rs, a1_cs, a2_rs, a2_cs, p_rs, p_cs;
   int32_t s1_ld, s2_ld;
   int32_t m, k1a, k1, k2a, k2, kp;
-  int32_t ldy, ldr;
+  int64_t y_rs, y_cs;
+  int32_t ldr, r_upper_only;
+};
+
+// ---------------------------------------------------------------------------
+// Pentagonal QR (LAPACK tpqrt, l = 0) of [Rt; B]: Rt is the K x K upper
+// triangle on top (strided, upper part read and updated in place), B the
+// m x K block below (overwritten by the reflectors' lower parts).  Reflector
+// j is [e_j; v_j]; tau[K] receives the scalars.
+// ---------------------------------------------------------------------------
+struct TpqrtItem {
+  void* R;
+  void* B;
+  void* tau;
+  int64_t r_rs, r_cs, b_rs, b_cs;
+  int32_t m, K;
+};
+
+// ---------------------------------------------------------------------------
+// Triangular factor of a block of K <= 64 reflectors (LAPACK larft, forward,
+// columnwise): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j] with
+// G[c][j] = w_c . w_j (strided, only c < j read).  T written upper, zeros below.
+// ---------------------------------------------------------------------------
+struct LarftItem {
+  const void* G;
+  const void* tau;
+  void* T;
+  int64_t g_rs, g_cs, t_rs, t_cs;
+  int32_t K, pad;
 };
 
 // ---------------------------------------------------------------------------

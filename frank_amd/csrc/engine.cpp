@@ -1,5 +1,6 @@
 // Host orchestration of the batched BLR operations (see engine.hpp).
 #include "engine.hpp"
+#include "plan.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -62,7 +63,14 @@ Context::Context(hipStream_t s) : s_(s) {
   FRK_HIP_CHECK(hipMemset(counters_.as<void>(), 0, sizeof(int32_t) * kNumCounters));
 }
 
-Context::~Context() = default;
+Context::~Context() {
+  (void)hipStreamSynchronize(s_);
+  for (auto& pr : ev_pending_) {
+    ev_pool_.push_back(pr.first);
+    ev_pool_.push_back(pr.second);
+  }
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+}
 
 void Context::reserve(size_t bytes) {
   if (bytes > ws_.bytes()) {
@@ -85,6 +93,65 @@ void Context::sync() {
   FRK_HIP_CHECK(hipStreamSynchronize(s_));
   stage_.reset();
   ws_off_ = 0;
+  resolve_events();
+}
+
+void Context::account(const char* cls, double flops, double bytes) {
+  OpStat& st = owner_->stats_[cls];
+  st.launches += 1;
+  st.flops += flops;
+  st.bytes += bytes;
+}
+
+void Context::event_begin() {
+  Context& o = *owner_;
+  if (o.ev_pool_.empty()) {
+    hipEvent_t e;
+    FRK_HIP_CHECK(hipEventCreate(&e));
+    o.ev_pool_.push_back(e);
+  }
+  ev_open_ = o.ev_pool_.back();
+  o.ev_pool_.pop_back();
+  FRK_HIP_CHECK(hipEventRecord(ev_open_, s_));
+}
+
+void Context::event_end() {
+  Context& o = *owner_;
+  if (o.ev_pool_.empty()) {
+    hipEvent_t e;
+    FRK_HIP_CHECK(hipEventCreate(&e));
+    o.ev_pool_.push_back(e);
+  }
+  hipEvent_t e1 = o.ev_pool_.back();
+  o.ev_pool_.pop_back();
+  FRK_HIP_CHECK(hipEventRecord(e1, s_));
+  o.ev_pending_.emplace_back(ev_open_, e1);
+  ev_open_ = nullptr;
+}
+
+void Context::resolve_events() {
+  if (ev_pending_.empty()) return;
+  OpStat& st = stats_[timed_cls_];
+  for (auto& pr : ev_pending_) {
+    FRK_HIP_CHECK(hipEventSynchronize(pr.second));
+    float ms = 0.f;
+    FRK_HIP_CHECK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    st.ms += ms;
+    st.timed += 1;
+    ev_pool_.push_back(pr.first);
+    ev_pool_.push_back(pr.second);
+  }
+  ev_pending_.clear();
+}
+
+const std::map<std::string, OpStat>& Context::op_stats() {
+  resolve_events();
+  return stats_;
+}
+
+void Context::reset_op_stats() {
+  resolve_events();
+  stats_.clear();
 }
 
 template <class X>
@@ -249,118 +316,17 @@ void BlrMatrix<T>::push_ranks(hipStream_t s) {
 void Plan::run() {
   if (launches_.empty()) return;
   const char* d = static_cast<const char*>(ctx_.staging().push_bytes(bytes_.data(), bytes_.size(), ctx_.stream()));
-  for (auto& f : launches_) f(d);
+  for (size_t i = 0; i < launches_.size(); ++i) {
+    const Meta& m = meta_[i];
+    const bool timed = ctx_.is_timed(m.cls);
+    if (timed) ctx_.event_begin();
+    launches_[i](d);
+    if (timed) ctx_.event_end();
+    if (m.cls) ctx_.account(m.cls, m.flops, m.bytes);
+  }
   launches_.clear();
+  meta_.clear();
   bytes_.clear();
-}
-
-static GemmItem gemm_item(const void* A, int64_t a_rs, int64_t a_cs, const void* B, int64_t b_rs, int64_t b_cs,
-                          void* C, int64_t c_rs, int64_t c_cs, int m, int n, int k, double alpha, double beta) {
-  GemmItem g{};
-  g.A = A;
-  g.B = B;
-  g.C = C;
-  g.a_rs = a_rs;
-  g.a_cs = a_cs;
-  g.b_rs = b_rs;
-  g.b_cs = b_cs;
-  g.c_rs = c_rs;
-  g.c_cs = c_cs;
-  g.m = m;
-  g.n = n;
-  g.k = k;
-  g.alpha = alpha;
-  g.beta = beta;
-  return g;
-}
-
-static TrsmItem trsm_item(const void* A, int64_t a_rs, int64_t a_cs, const void* dinv, int64_t d_rs, int64_t d_cs,
-                          void* B, int64_t b_rs, int64_t b_cs, int n, int nrhs, bool lower) {
-  TrsmItem x{};
-  x.A = A;
-  x.a_rs = a_rs;
-  x.a_cs = a_cs;
-  x.dinv = dinv;
-  x.d_rs = d_rs;
-  x.d_cs = d_cs;
-  x.B = B;
-  x.b_rs = b_rs;
-  x.b_cs = b_cs;
-  x.n = n;
-  x.nrhs = nrhs;
-  x.lower = lower ? 1 : 0;
-  return x;
-}
-
-template <typename T>
-static void plan_gemm_group(Plan& pl, const std::vector<GemmItem>& items) {
-  const hipStream_t s = pl.ctx().stream();
-  const size_t chunk = 65535;  // grid.y limit
-  for (size_t o = 0; o < items.size(); o += chunk) {
-    std::vector<GemmItem> part(items.begin() + o, items.begin() + std::min(items.size(), o + chunk));
-    int mm = 0, nn = 0;
-    for (const auto& g : part) {
-      mm = std::max(mm, g.m);
-      nn = std::max(nn, g.n);
-    }
-    const size_t off = pl.add(part);
-    const int cnt = (int)part.size();
-    pl.launch([=](const char* d) { launch_gemm<T>(reinterpret_cast<const GemmItem*>(d + off), cnt, mm, nn, s); });
-  }
-}
-
-// Independent GEMMs, launched per output-shape class (tiny, tall-skinny,
-// short-wide, general) so every item runs in the tile shape that fits it.
-template <typename T>
-static void plan_gemms(Plan& pl, const std::vector<GemmItem>& items) {
-  if (items.empty()) return;
-  std::vector<GemmItem> cls[4];
-  for (const auto& g : items) {
-    if (g.m <= 0 || g.n <= 0) continue;
-    const int c = (g.m <= 16 && g.n <= 16) ? 0 : (g.n <= 16 ? 1 : (g.m <= 16 ? 2 : 3));
-    cls[c].push_back(g);
-  }
-  for (auto& v : cls)
-    if (!v.empty()) plan_gemm_group<T>(pl, v);
-}
-
-// GEMMs that must run in exactly the given launch (in-place panel updates
-// rely on the 64 x 64 tiling of one combined launch).
-template <typename T>
-static void plan_gemms_onelaunch(Plan& pl, const std::vector<GemmItem>& items) {
-  if (!items.empty()) plan_gemm_group<T>(pl, items);
-}
-
-template <typename T>
-static void plan_copies(Plan& pl, const std::vector<GemmItem>& items) {
-  if (items.empty()) return;
-  const hipStream_t s = pl.ctx().stream();
-  int64_t mx = 0;
-  for (const auto& g : items) mx = std::max<int64_t>(mx, (int64_t)g.m * g.n);
-  const size_t off = pl.add(items);
-  const int cnt = (int)items.size();
-  pl.launch([=](const char* d) { launch_copy<T>(reinterpret_cast<const GemmItem*>(d + off), cnt, mx, s); });
-}
-
-template <typename T>
-static void plan_trsms(Plan& pl, const std::vector<TrsmItem>& items) {
-  if (items.empty()) return;
-  const hipStream_t s = pl.ctx().stream();
-  int n = 0, r = 0;
-  for (const auto& t : items) {
-    n = std::max(n, t.n);
-    r = std::max(r, t.nrhs);
-  }
-  const size_t off = pl.add(items);
-  const int cnt = (int)items.size();
-  pl.launch([=](const char* d) { launch_trsm<T>(reinterpret_cast<const TrsmItem*>(d + off), cnt, n, r, s); });
-}
-
-template <typename T>
-static void run_gemms(const std::vector<GemmItem>& items, Context& ctx) {
-  Plan pl(ctx);
-  plan_gemms<T>(pl, items);
-  pl.run();
 }
 
 // ============================================================================
@@ -390,7 +356,9 @@ void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& l
     }
     const size_t off = pl.add(small);
     int32_t* inf = info_dev ? info_dev + pi * nt : nullptr;
-    pl.launch([=](const char* d) { launch_getrf_small<T>(reinterpret_cast<const LuItem*>(d + off), nt, inf, s); });
+    const double nb3 = (double)nbk * nbk * nbk;
+    pl.launch([=](const char* d) { launch_getrf_small<T>(reinterpret_cast<const LuItem*>(d + off), nt, inf, s); },
+              "getrf_small", nt * (4.0 / 3.0) * nb3, nt * 4.0 * sizeof(T) * nbk * nbk);
     if (rest <= 0) continue;
     // A12 <- inv(L11) A12 ; A21 <- A21 inv(U11), in place: the 64 x 64 GEMM tiling gives A12 a single
     // output row tile and A21 a single output column tile, so every workgroup reads only the region it
@@ -405,14 +373,14 @@ void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& l
                               a + (p + nbk) * ld + p, ld, 1, rest, nbk, nbk, 1.0, 0.0));
     }
     if (nbk != 64) throw std::runtime_error("getrf: only the last panel may be narrower than 64");
-    plan_gemms_onelaunch<T>(pl, pan);
+    plan_gemms_onelaunch<T>(pl, pan, "getrf_gemm");
     std::vector<GemmItem> gm(nt);
     for (int t = 0; t < nt; ++t) {
       T* a = tiles[t];
       gm[t] = gemm_item(a + (p + nbk) * ld + p, ld, 1, a + p * ld + p + nbk, ld, 1, a + (p + nbk) * ld + p + nbk,
                         ld, 1, rest, rest, nbk, -1.0, 1.0);
     }
-    plan_gemms<T>(pl, gm);
+    plan_gemms<T>(pl, gm, "getrf_gemm");
   }
   // full inverses of the unit-lower and upper factors by recursive doubling of the 64 x 64 block
   // inverses: [[A, 0], [C, D]]^-1 has off-diagonal block -D^-1 (C A^-1); the upper case mirrors it.
@@ -435,8 +403,8 @@ void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& l
                                sz, sd, sz, -1.0, 0.0));
       }
     }
-    plan_gemms<T>(pl, w1);
-    plan_gemms<T>(pl, w2);
+    plan_gemms<T>(pl, w1, "getrf_gemm");
+    plan_gemms<T>(pl, w2, "getrf_gemm");
   }
 }
 
@@ -548,14 +516,16 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     const int xsld = x.xs_ld;
     QrItem q{};
     q.Y = yu;
-    q.ldy = x.m;
+    q.y_rs = 1;
+    q.y_cs = x.m;
     q.tau = tu + (size_t)t * K;
     q.R = ru;
     q.ldr = K;
     q.m = x.m;
     QrItem v = q;
     v.Y = yv;
-    v.ldy = x.n;
+    v.y_rs = 1;
+    v.y_cs = x.n;
     v.tau = tv + (size_t)t * K;
     v.R = rv;
     v.m = x.n;
@@ -700,11 +670,34 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   for (const auto& q : qi) qk = std::max(qk, q.k1 + q.k2);
   int capmax = 0;
   for (const auto& x : d) capmax = std::max(capmax, std::min(x.cap, x.kc + x.kx2));
+  // op log: QR (geqrf + orgqr 4mK^2 - 4/3 K^3, panel formation, projection), SVD 22 K^3, reflector
+  // application 4 m K ncol with ncol <= min(K, cap) (the kept rank is known on the device only).
+  double qf = 0, qb = 0, sf = 0, sb = 0, af = 0, ab = 0;
+  for (const auto& q : qi) {
+    const double m = q.m, Kq = q.k1 + q.k2;
+    qf += 4.0 * m * Kq * Kq - 4.0 / 3.0 * Kq * Kq * Kq + 4.0 * m * q.kp * Kq;
+    if (q.S1) qf += 2.0 * m * q.k1a * q.k1;
+    if (q.S2) qf += 2.0 * m * q.k2a * q.k2;
+    qb += sizeof(T) * (m * ((q.S1 ? q.k1a : q.k1) + (q.S2 ? q.k2a : q.k2) + q.kp + Kq) + Kq * Kq);
+  }
+  for (const auto& v : si) {
+    const double Kv = v.K;
+    sf += 24.0 * Kv * Kv * Kv;
+    sb += sizeof(T) * 5.0 * Kv * Kv;
+  }
+  for (const auto& a : ai) {
+    const double m = a.m, nc = std::min(a.ncol, capmax);
+    af += 4.0 * m * a.nref * nc;
+    ab += sizeof(T) * m * (a.nref + nc);
+  }
   const size_t oq = pl.add(qi), os = pl.add(si), oa = pl.add(ai);
   const int nq = (int)qi.size(), na = (int)ai.size();
-  pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oq), nq, qk, s); });
-  pl.launch([=](const char* dd) { launch_svd<T>(reinterpret_cast<const SvdItem*>(dd + os), nd, K, s); });
-  pl.launch([=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oa), na, capmax, s); });
+  pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oq), nq, qk, s); }, "lr_qr", qf,
+            qb);
+  pl.launch([=](const char* dd) { launch_svd<T>(reinterpret_cast<const SvdItem*>(dd + os), nd, K, s); }, "lr_svd", sf,
+            sb);
+  pl.launch([=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oa), na, capmax, s); },
+            "lr_apply_q", af, ab);
 }
 
 template <typename T>
@@ -816,7 +809,8 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       q.k1a = q.k1 = r;
       q.m = x.n;
       q.Y = Yz + (size_t)t * nmax * rmax;
-      q.ldy = x.n;
+      q.y_rs = 1;
+    q.y_cs = x.n;
       q.tau = tz + (size_t)t * rmax;
       q.R = Rz + (size_t)t * rmax * rmax;
       q.ldr = r;
@@ -912,7 +906,8 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       q.k1a = q.k1 = p;
       q.m = x.m;
       q.Y = y;
-      q.ldy = x.m;
+      q.y_rs = 1;
+    q.y_cs = x.m;
       q.tau = ty + (size_t)t * pmax;
       q.R = Ry + (size_t)t * pmax * pmax;
       q.ldr = p;
@@ -941,7 +936,8 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       qq.k1a = qq.k1 = p;
       qq.m = x.n;
       qq.Y = z;
-      qq.ldy = x.n;
+      qq.y_rs = 1;
+    qq.y_cs = x.n;
       qq.tau = tz + (size_t)t * pmax;
       qq.R = Rz + (size_t)t * pmax * pmax;
       qq.ldr = p;
@@ -1162,6 +1158,59 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, dou
 // ============================================================================
 // BLR-LU
 // ============================================================================
+// Factorization of the dense diagonal tile i (always in FP64): plan into p on
+// context c (its stream and workspace).
+template <typename T>
+static void plan_diag_getrf(Plan& p, Context& c, BlrMatrix<T>& A, int i, int32_t* info) {
+  const int ii = A.idx(i, i);
+  const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
+  if constexpr (sizeof(T) == 8) {
+    plan_getrf<T>(p, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info);
+  } else {
+    const hipStream_t s = c.stream();
+    const int64_t nn = (int64_t)m_i * n_i;
+    double* d64 = c.take<double>(nn);
+    double* l64 = c.take<double>(nn);
+    double* u64 = c.take<double>(nn);
+    T* dt = A.D[ii];
+    T* lt = A.Linv[ii];
+    T* ut = A.Uinv[ii];
+    p.launch([=](const char*) {
+      launch_convert<double, T>(d64, dt, nn, s);
+      launch_fill_value(l64, nn, 0.0, s);
+      launch_fill_value(u64, nn, 0.0, s);
+    });
+    plan_getrf<double>(p, {d64}, {l64}, {u64}, m_i, n_i, info);
+    p.launch([=](const char*) {
+      launch_convert<T, double>(dt, d64, nn, s);
+      launch_convert<T, double>(lt, l64, nn, s);
+      launch_convert<T, double>(ut, u64, nn, s);
+    });
+  }
+}
+
+template <typename T>
+static size_t diag_getrf_workspace(int n) {
+  return (sizeof(T) == 4 ? sizeof(double) * ((size_t)3 * n * n + 768) : 0) + getrf_workspace<double>(n, 1);
+}
+
+// RAII pair of a non-blocking stream and two events for the lookahead
+struct Lookahead {
+  hipStream_t s = nullptr;
+  hipEvent_t upd = nullptr, diag = nullptr;
+  Lookahead() {
+    FRK_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&upd, hipEventDisableTiming));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&diag, hipEventDisableTiming));
+  }
+  ~Lookahead() {
+    (void)hipStreamSynchronize(s);
+    (void)hipEventDestroy(upd);
+    (void)hipEventDestroy(diag);
+    (void)hipStreamDestroy(s);
+  }
+};
+
 template <typename T>
 void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   const hipStream_t s = ctx.stream();
@@ -1170,14 +1219,25 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   int32_t* info = ctx.info();
   const int npan_max = 8;
   if (nb * npan_max > 4096) throw std::invalid_argument("too many diagonal tiles for the info buffer");
+  for (int i = 0; i < nb; ++i) {
+    const int ii = A.idx(i, i);
+    if (A.kind[ii] != kDense || A.rows(i) != A.cols(i))
+      throw std::runtime_error("getrf: diagonal tile is not square dense");
+    if (A.rows(i) > 512) throw std::runtime_error("getrf: tile size above 512");
+  }
   FRK_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int32_t) * 4096, s));
+  // Lookahead: the diagonal tile i+1 is final once step i's Schur GEMMs have
+  // run, so its factorization proceeds on a second stream while step i's
+  // low-rank recompressions still run on the first.
+  Lookahead la;
+  Context side(la.s);
+  side.set_stats_owner(&ctx);
+  bool diag_pending = false;
   for (int i = 0; i < nb; ++i) {
     ctx.sync();
     A.pull_ranks(s);
     const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
     const int ii = A.idx(i, i);
-    if (A.kind[ii] != kDense || m_i != n_i) throw std::runtime_error("getrf: diagonal tile is not square dense");
-    if (m_i > 512) throw std::runtime_error("getrf: tile size above 512");
     const int rem = nb - 1 - i;
     // ---- plan the step ---------------------------------------------------
     std::vector<LrAddDesc<T>> adds;
@@ -1209,35 +1269,16 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           }
         }
     }
-    if (sizeof(T) == 4) need += sizeof(double) * ((size_t)3 * m_i * n_i + 768);
-    need += getrf_workspace<double>(m_i, 1);
+    if (!diag_pending) need += diag_getrf_workspace<T>(m_i);
     ctx.reserve(need);
     Plan pl(ctx);
-    // 1. diagonal tile (always factorized in FP64)
-    if constexpr (sizeof(T) == 8) {
-      plan_getrf<T>(pl, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info + i * npan_max);
+    // 1. diagonal tile: already factorized on the lookahead stream, or now
+    if (diag_pending) {
+      FRK_HIP_CHECK(hipStreamWaitEvent(s, la.diag, 0));
+      diag_pending = false;
     } else {
-      const int64_t nn = (int64_t)m_i * n_i, ni = nn;
-      double* d64 = ctx.take<double>(nn);
-      double* l64 = ctx.take<double>(ni);
-      double* u64 = ctx.take<double>(ni);
-      T* dt = A.D[ii];
-      T* lt = A.Linv[ii];
-      T* ut = A.Uinv[ii];
-      pl.launch([=](const char*) {
-        launch_convert<double, T>(d64, dt, nn, s);
-        launch_fill_value(l64, ni, 0.0, s);
-        launch_fill_value(u64, ni, 0.0, s);
-      });
-      plan_getrf<double>(pl, {d64}, {l64}, {u64}, m_i, n_i, info + i * npan_max);
-      pl.launch([=](const char*) {
-        launch_convert<T, double>(dt, d64, nn, s);
-        launch_convert<T, double>(lt, l64, ni, s);
-        launch_convert<T, double>(ut, u64, ni, s);
-      });
+      plan_diag_getrf<T>(pl, ctx, A, i, info + i * npan_max);
     }
-    const T* Dii = A.D[ii];
-    const int64_t ldd = n_i;
     // 2. panel solves (src/operations/LAPACK/getrf.cpp:50-57) as products with the explicit
     //    triangular inverses, written to workspace and copied back into the slots
     std::vector<GemmItem> tg, tcp;
@@ -1269,7 +1310,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         tcp.push_back(gemm_item(w, n, 1, nullptr, 0, 0, A.D[t], n, 1, m_i, n, 0, 0.0, 0.0));
       }
     }
-    plan_gemms<T>(pl, tg);
+    plan_gemms<T>(pl, tg, "trsm_gemm");
     plan_copies<T>(pl, tcp);
     if (rem > 0) {
       // 3. Schur updates A(ic,k) -= L(ic,i) A(i,k) (getrf.cpp:58-62, gemm.cpp:126-272)
@@ -1416,18 +1457,31 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           }
           adds.push_back(ad);
         }
-      plan_gemms<T>(pl, gA);
-      plan_gemms<T>(pl, gB);
-      plan_gemms<T>(pl, gC);
-      plan_gemms<T>(pl, gD);
+      plan_gemms<T>(pl, gA, "schur_gemm");
+      plan_gemms<T>(pl, gB, "schur_gemm");
+      plan_gemms<T>(pl, gC, "schur_gemm");
+      plan_gemms<T>(pl, gD, "schur_gemm");
+      pl.run();
+      // tile (i+1, i+1) has received its last update
+      FRK_HIP_CHECK(hipEventRecord(la.upd, s));
+      auto lookahead = [&]() {
+        side.sync();
+        side.reserve(diag_getrf_workspace<T>((int)A.rows(i + 1)) + 4096);
+        FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.upd, 0));
+        Plan pd(side);
+        plan_diag_getrf<T>(pd, side, A, i + 1, info + (i + 1) * npan_max);
+        pd.run();
+        FRK_HIP_CHECK(hipEventRecord(la.diag, la.s));
+        diag_pending = true;
+      };
       if (!comp.empty()) {
         // D.D -> LR needs the compressed ranks on the host: run what is planned,
         // compress on a side context (own workspace), then plan the additions.
         pl.run();
         DevBuf crank(sizeof(int32_t) * comp.size());
         for (size_t c = 0; c < comp.size(); ++c) comp[c].d_rank = crank.as<int32_t>() + c;
-        Context side(s);
-        compress_batched<T>(comp, side);
+        Context cside(s);
+        compress_batched<T>(comp, cside);
         std::vector<int32_t> cr(comp.size());
         FRK_HIP_CHECK(
             hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
@@ -1435,14 +1489,17 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         Plan p2(ctx);
         plan_lr_add<T>(p2, adds, A.lra);
         p2.run();
-        ctx.sync();
+        lookahead();
         continue;
       }
       plan_lr_add<T>(pl, adds, A.lra);
+      pl.run();
+      lookahead();
     }
     pl.run();
   }
   ctx.sync();
+  side.sync();
   A.pull_ranks(s);
   std::vector<int32_t> hinfo(nb * npan_max);
   FRK_HIP_CHECK(hipMemcpy(hinfo.data(), info, sizeof(int32_t) * hinfo.size(), hipMemcpyDeviceToHost));
@@ -1477,9 +1534,11 @@ void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
       T* w = ctx.take<T>((size_t)ni_ * nrhs + 1);
       // b_i <- op(T)^-1 b_i with the explicit inverse (op = transpose for the adjoint sweeps)
       if (!transposed)
-        plan_gemms<T>(pl, {gemm_item(tinv, ld, 1, bptr(i), nrhs, 1, w, nrhs, 1, ni_, nrhs, ni_, 1.0, 0.0)});
+        plan_gemms<T>(pl, {gemm_item(tinv, ld, 1, bptr(i), nrhs, 1, w, nrhs, 1, ni_, nrhs, ni_, 1.0, 0.0)},
+                      "solve_gemm");
       else
-        plan_gemms<T>(pl, {gemm_item(tinv, 1, ld, bptr(i), nrhs, 1, w, nrhs, 1, ni_, nrhs, ni_, 1.0, 0.0)});
+        plan_gemms<T>(pl, {gemm_item(tinv, 1, ld, bptr(i), nrhs, 1, w, nrhs, 1, ni_, nrhs, ni_, 1.0, 0.0)},
+                      "solve_gemm");
       plan_copies<T>(pl, {gemm_item(w, nrhs, 1, nullptr, 0, 0, bptr(i), nrhs, 1, ni_, nrhs, 0, 0.0, 0.0)});
       std::vector<GemmItem> g1, g2, g3;
       for (int k = 0; k < nb; ++k) {
@@ -1510,9 +1569,9 @@ void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
           g3.push_back(gemm_item(F.V[t], 1, F.cols(k), t2, nrhs, 1, bptr(k), nrhs, 1, mk, nrhs, r, -1.0, 1.0));
         }
       }
-      plan_gemms<T>(pl, g1);
-      plan_gemms<T>(pl, g2);
-      plan_gemms<T>(pl, g3);
+      plan_gemms<T>(pl, g1, "solve_gemm");
+      plan_gemms<T>(pl, g2, "solve_gemm");
+      plan_gemms<T>(pl, g3, "solve_gemm");
     }
   };
   Plan pl(ctx);

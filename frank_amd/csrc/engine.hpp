@@ -11,6 +11,7 @@
 #pragma once
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -62,6 +63,15 @@ class Staging {
 };
 
 // Execution context: stream, staging ring, workspace arena, constants.
+// Per kernel-class op log: launches, algorithmic flops and bytes (computed on
+// the host from the launch descriptors) and, for the one class selected with
+// Context::time_class(), device time from HIP events around each launch.
+struct OpStat {
+  int64_t launches = 0;
+  double flops = 0, bytes = 0, ms = 0;
+  int64_t timed = 0;
+};
+
 class Context {
  public:
   explicit Context(hipStream_t s);
@@ -87,6 +97,18 @@ class Context {
   int32_t* info() { return info_.as<int32_t>(); }
   // Device event counters (kCounter* slots), zero at creation.
   int32_t* counters() { return counters_.as<int32_t>(); }
+  // op log
+  void account(const char* cls, double flops, double bytes);
+  void time_class(const std::string& cls) { timed_cls_ = cls; }
+  bool is_timed(const char* cls) const {
+    return cls && !owner_->timed_cls_.empty() && owner_->timed_cls_ == cls;
+  }
+  void event_begin();
+  void event_end();
+  // Route this context's op log (and event timing) into another context's.
+  void set_stats_owner(Context* owner) { owner_ = owner ? owner : this; }
+  const std::map<std::string, OpStat>& op_stats();  // resolves pending events (drains the stream)
+  void reset_op_stats();
 
  private:
   hipStream_t s_;
@@ -99,6 +121,13 @@ class Context {
   int omega_n_ = 0, omega_p_ = 0;
   DevBuf info_;
   DevBuf counters_;
+  std::map<std::string, OpStat> stats_;
+  std::string timed_cls_;
+  std::vector<hipEvent_t> ev_pool_;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending_;
+  hipEvent_t ev_open_ = nullptr;
+  Context* owner_ = this;
+  void resolve_events();
 };
 
 // Slots of Context::counters().
@@ -117,7 +146,11 @@ class Plan {
     if (!v.empty()) std::memcpy(bytes_.data() + off, v.data(), v.size() * sizeof(Item));
     return off;
   }
-  void launch(std::function<void(const char*)> f) { launches_.push_back(std::move(f)); }
+  // cls (a string literal) names the kernel class for the op log
+  void launch(std::function<void(const char*)> f, const char* cls = nullptr, double flops = 0, double bytes = 0) {
+    launches_.push_back(std::move(f));
+    meta_.push_back({cls, flops, bytes});
+  }
   void run();
   bool empty() const { return launches_.empty(); }
   Context& ctx() { return ctx_; }
@@ -126,6 +159,11 @@ class Plan {
   Context& ctx_;
   std::vector<char> bytes_;
   std::vector<std::function<void(const char*)>> launches_;
+  struct Meta {
+    const char* cls;
+    double flops, bytes;
+  };
+  std::vector<Meta> meta_;
 };
 
 enum TileKind : int32_t { kDense = 0, kLowRank = 1, kEmpty = 2 };

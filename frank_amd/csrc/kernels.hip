@@ -14,6 +14,8 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstring>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -118,13 +120,19 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ 
     for (int b = 0; b < RN; ++b) acc[a][b] = T(0);
   const T* __restrict__ A = static_cast<const T*>(it.A);
   const T* __restrict__ B = static_cast<const T*>(it.B);
+  T* C = static_cast<T*>(it.C);
+  const T alpha = static_cast<T>(it.alpha), beta = static_cast<T>(it.beta);
   const bool a_kfast = (it.a_cs == 1 && it.a_rs != 1);
   const bool b_kfast = (it.b_rs == 1 && it.b_cs != 1);
   constexpr int AE = (TM * TK + 255) / 256, BE = (TN * TK + 255) / 256;
-  for (int k0 = 0; k0 < it.k; k0 += TK) {
+  // chunk loads go to registers first: the next chunk is in flight while the
+  // current one is multiplied out of LDS
+  T ra[AE], rb[BE];
+  auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int e = 0; e < AE; ++e) {
       const int idx = tid + e * 256;
+      ra[e] = T(0);
       if (idx < TM * TK) {
         int i, kk;
         if (a_kfast) {
@@ -135,12 +143,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ 
           kk = idx / TM;
         }
         const int gi = row0 + i, gk = k0 + kk;
-        As[kk][i] = (gi < it.m && gk < it.k) ? A[(int64_t)gi * it.a_rs + (int64_t)gk * it.a_cs] : T(0);
+        if (gi < it.m && gk < it.k) ra[e] = A[(int64_t)gi * it.a_rs + (int64_t)gk * it.a_cs];
       }
     }
 #pragma unroll
     for (int e = 0; e < BE; ++e) {
       const int idx = tid + e * 256;
+      rb[e] = T(0);
       if (idx < TN * TK) {
         int j, kk;
         if (b_kfast) {
@@ -151,10 +160,43 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ 
           kk = idx / TN;
         }
         const int gj = col0 + j, gk = k0 + kk;
-        Bs[kk][j] = (gj < it.n && gk < it.k) ? B[(int64_t)gk * it.b_rs + (int64_t)gj * it.b_cs] : T(0);
+        if (gj < it.n && gk < it.k) rb[e] = B[(int64_t)gk * it.b_rs + (int64_t)gj * it.b_cs];
       }
     }
+  };
+  auto store_chunk = [&]() {
+#pragma unroll
+    for (int e = 0; e < AE; ++e) {
+      const int idx = tid + e * 256;
+      if (idx < TM * TK) {
+        const int i = a_kfast ? idx / TK : idx % TM, kk = a_kfast ? idx % TK : idx / TM;
+        As[kk][i] = ra[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < BE; ++e) {
+      const int idx = tid + e * 256;
+      if (idx < TN * TK) {
+        const int j = b_kfast ? idx / TK : idx % TN, kk = b_kfast ? idx % TK : idx / TN;
+        Bs[kk][j] = rb[e];
+      }
+    }
+  };
+  // the C tile is read early so its latency overlaps the K loop
+  T cv[RM][RN];
+#pragma unroll
+  for (int a = 0; a < RM; ++a)
+#pragma unroll
+    for (int b = 0; b < RN; ++b) {
+      cv[a][b] = T(0);
+      const int i = row0 + ty + (TM / RM) * a, j = col0 + tx + TX * b;
+      if (beta != T(0) && i < it.m && j < it.n) cv[a][b] = C[(int64_t)i * it.c_rs + (int64_t)j * it.c_cs];
+    }
+  if (it.k > 0) load_chunk(0);
+  for (int k0 = 0; k0 < it.k; k0 += TK) {
+    store_chunk();
     __syncthreads();
+    if (k0 + TK < it.k) load_chunk(k0 + TK);
 #pragma unroll 8
     for (int kk = 0; kk < TK; ++kk) {
       T av[RM], bv[RN];
@@ -169,8 +211,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ 
     }
     __syncthreads();
   }
-  T* C = static_cast<T*>(it.C);
-  const T alpha = static_cast<T>(it.alpha), beta = static_cast<T>(it.beta);
 #pragma unroll
   for (int a = 0; a < RM; ++a) {
     const int i = row0 + ty + (TM / RM) * a;
@@ -181,7 +221,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(const GemmItem* __restrict__ 
       if (j >= it.n) continue;
       const int64_t idx = (int64_t)i * it.c_rs + (int64_t)j * it.c_cs;
       T v = alpha * acc[a][b];
-      if (beta != T(0)) v += beta * C[idx];
+      if (beta != T(0)) v += beta * cv[a][b];
       C[idx] = v;
     }
   }
@@ -599,6 +639,7 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
     for (int t = 0; t < MR; ++t) a[q][t] = T(0);
   constexpr int CS = 8;
   __shared__ T stage[CS][MR * 64];
+  __shared__ T sstage[CS][W * CPW];
   for (int blk = 0; blk < 2; ++blk) {
     const T* Ab = static_cast<const T*>(blk ? it.A2 : it.A1);
     const T* S = static_cast<const T*>(blk ? it.S2 : it.S1);
@@ -620,7 +661,8 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
         }
       }
     } else {
-      // column c of A S = sum_s A[:, s] S[s, c]: stage CS columns of A at a time (coalesced)
+      // column c of A S = sum_s A[:, s] S[s, c]: stage CS columns of A and the
+      // matching CS rows of S at a time (coalesced loads, LDS broadcasts)
       for (int s0 = 0; s0 < ka; s0 += CS) {
         const int cnt = min(CS, ka - s0);
         __syncthreads();
@@ -628,16 +670,22 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
           const int si = e / (MR * 64), row = e % (MR * 64);
           stage[si][row] = (row < m) ? Ab[row * rs + (int64_t)(s0 + si) * cs] : T(0);
         }
+        for (int e = threadIdx.x; e < cnt * kc; e += W * 64) {
+          const int si = e / kc, cc = e % kc;
+          sstage[si][cc] = S[(s0 + si) * sld + cc];
+        }
         __syncthreads();
+        for (int si = 0; si < cnt; ++si) {
+          T av[MR];
 #pragma unroll
-        for (int q = 0; q < CPW; ++q) {
-          const int c = q * W + w;
-          if (c >= cbase && c < cbase + kc) {
-            const int cc = c - cbase;
-            for (int si = 0; si < cnt; ++si) {
-              const T coef = S[(s0 + si) * sld + cc];
+          for (int t = 0; t < MR; ++t) av[t] = stage[si][lane + 64 * t];
 #pragma unroll
-              for (int t = 0; t < MR; ++t) a[q][t] += stage[si][lane + 64 * t] * coef;
+          for (int q = 0; q < CPW; ++q) {
+            const int c = q * W + w;
+            if (c >= cbase && c < cbase + kc) {
+              const T coef = sstage[si][c - cbase];
+#pragma unroll
+              for (int t = 0; t < MR; ++t) a[q][t] += av[t] * coef;
             }
           }
         }
@@ -671,60 +719,68 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
     }
   }
   const int kp = (it.P != nullptr) ? it.kp : 0;
+  for (int j = 0; j < nref; ++j) {
+    const int qj = j / W, wj = j - qj * W;
+    const int buf = j & 1;
+    if (w == wj) {
+      // column j by predicated selection (keeps a[][] in registers)
+      T col[MR];
 #pragma unroll
-  for (int qj = 0; qj < CPW; ++qj) {
-    for (int wj = 0; wj < W; ++wj) {
-      const int j = qj * W + wj;
-      if (j >= nref) break;
-      const int buf = j & 1;
-      if (w == wj) {
-        T x2 = T(0), alpha = T(0);
+      for (int t = 0; t < MR; ++t) col[t] = a[0][t];
 #pragma unroll
-        for (int t = 0; t < MR; ++t) {
-          const int row = lane + 64 * t;
-          const T v = a[qj][t];
-          if (row > j && row < m) x2 += v * v;
-          if (row == j) alpha = v;
-        }
-        x2 = wave_sum(x2);
-        alpha = wave_sum(alpha);
-        T beta, tau, scal;
-        if (x2 == T(0)) {
-          tau = T(0);
-          beta = alpha;
-          scal = T(0);
-        } else {
-          const T nrm = sqrt(alpha * alpha + x2);
-          beta = (alpha >= T(0)) ? -nrm : nrm;
-          tau = (beta - alpha) / beta;
-          scal = T(1) / (alpha - beta);
-        }
+      for (int q = 1; q < CPW; ++q)
 #pragma unroll
-        for (int t = 0; t < MR; ++t) {
-          const int row = lane + 64 * t;
-          if (row > j) a[qj][t] *= scal;
-          if (row == j) a[qj][t] = beta;
-          vbuf[buf][lane + 64 * t] = (row == j) ? T(1) : ((row > j && row < m) ? a[qj][t] : T(0));
-        }
-        if (lane == 0) taus[j] = tau;
+        for (int t = 0; t < MR; ++t) col[t] = (q == qj) ? a[q][t] : col[t];
+      T x2 = T(0), alpha = T(0);
+#pragma unroll
+      for (int t = 0; t < MR; ++t) {
+        const int row = lane + 64 * t;
+        const T v = col[t];
+        if (row > j && row < m) x2 += v * v;
+        if (row == j) alpha = v;
       }
-      __syncthreads();
-      const T tj = taus[j];
-      if (tj != T(0)) {
-        T v[MR];
+      x2 = wave_sum(x2);
+      alpha = wave_sum(alpha);
+      T beta, tau, scal;
+      if (x2 == T(0)) {
+        tau = T(0);
+        beta = alpha;
+        scal = T(0);
+      } else {
+        const T nrm = sqrt(alpha * alpha + x2);
+        beta = (alpha >= T(0)) ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        scal = T(1) / (alpha - beta);
+      }
 #pragma unroll
-        for (int t = 0; t < MR; ++t) v[t] = vbuf[buf][lane + 64 * t];
+      for (int t = 0; t < MR; ++t) {
+        const int row = lane + 64 * t;
+        if (row > j) col[t] *= scal;
+        if (row == j) col[t] = beta;
+        vbuf[buf][lane + 64 * t] = (row == j) ? T(1) : ((row > j && row < m) ? col[t] : T(0));
+      }
 #pragma unroll
-        for (int q = 0; q < CPW; ++q) {
-          const int c = q * W + w;
-          if (c > j && c < K) {
-            T d = T(0);
+      for (int q = 0; q < CPW; ++q)
 #pragma unroll
-            for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
-            d = wave_sum(d) * tj;
+        for (int t = 0; t < MR; ++t) a[q][t] = (q == qj) ? col[t] : a[q][t];
+      if (lane == 0) taus[j] = tau;
+    }
+    __syncthreads();
+    const T tj = taus[j];
+    if (tj != T(0)) {
+      T v[MR];
 #pragma unroll
-            for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
-          }
+      for (int t = 0; t < MR; ++t) v[t] = vbuf[buf][lane + 64 * t];
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) {
+        const int c = q * W + w;
+        if (c > j && c < K) {
+          T d = T(0);
+#pragma unroll
+          for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
+          d = wave_sum(d) * tj;
+#pragma unroll
+          for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
         }
       }
     }
@@ -738,11 +794,14 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
 #pragma unroll
       for (int t = 0; t < MR; ++t) {
         const int row = lane + 64 * t;
-        if (row < m && row > c && c < nref) Y[(int64_t)row + (int64_t)c * it.ldy] = a[q][t];
-        if (row < nref)
-          Rb[(kp + row) * ldr + c] = (row <= c) ? a[q][t] : T(0);
-        else if (row < K)
-          Rb[(kp + row) * ldr + c] = T(0);
+        if (row < m && row > c && c < nref) Y[row * it.y_rs + c * it.y_cs] = a[q][t];
+        if (it.r_upper_only) {
+          if (row <= c && row < nref) Rb[(int64_t)(kp + row) * ldr + c] = a[q][t];
+        } else if (row < nref) {
+          Rb[(int64_t)(kp + row) * ldr + c] = (row <= c) ? a[q][t] : T(0);
+        } else if (row < K) {
+          Rb[(int64_t)(kp + row) * ldr + c] = T(0);
+        }
       }
       if (lane == 0 && c < nref) tau_out[c] = taus[c];
     }
@@ -757,7 +816,8 @@ void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s) {
   } else if (max_k <= 32) {
     hipLaunchKernelGGL((qr_kernel<T, 8, 4, 8>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_k <= 64) {
-    hipLaunchKernelGGL((qr_kernel<T, 8, 8, 8>), dim3(n_items), dim3(512), 0, s, d_items);
+    // 16 waves x 4 columns: 32 register values per lane stay within the 1024-thread budget
+    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 4>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else if (max_k <= 128) {
     hipLaunchKernelGGL((qr_kernel<T, 8, 16, 8>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else {
@@ -885,6 +945,211 @@ void launch_apply_q(const ApplyQItem* d_items, int n_items, int max_ncol, hipStr
 }
 
 // ===========================================================================
+// Pentagonal Householder QR of [Rt; B] (LAPACK tpqrt with l = 0): reflector j
+// is [e_j; v_j] with v_j in B's rows, so step j touches only row j of the top
+// triangle.  B (m <= 64 * MR rows, K <= W * CPW columns) lives in VGPRs
+// (wave w owns columns q * W + w); rows of Rt are staged through LDS RC at a
+// time and written back once final.
+// ===========================================================================
+template <typename T, int MR, int W, int CPW>
+__global__ __launch_bounds__(W * 64) void tpqrt_kernel(const TpqrtItem* __restrict__ items) {
+  const TpqrtItem it = items[blockIdx.x];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int m = it.m, K = it.K;
+  constexpr int KC = W * CPW;
+  constexpr int RC = 8;
+  __shared__ T vbuf[2][MR * 64];
+  __shared__ T taus[KC];
+  __shared__ T Rst[RC][KC];
+  T* R = static_cast<T*>(it.R);
+  T* B = static_cast<T*>(it.B);
+  T a[CPW][MR];
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const int c = q * W + w;
+#pragma unroll
+    for (int t = 0; t < MR; ++t) {
+      const int row = lane + 64 * t;
+      a[q][t] = (row < m && c < K) ? B[row * it.b_rs + c * it.b_cs] : T(0);
+    }
+  }
+  int j0 = -RC;  // first row of the staged chunk
+  for (int j = 0; j < K; ++j) {
+    const int qj = j / W, wj = j - qj * W;
+    if (j - j0 >= RC) {
+      // write back the finished chunk, stage the next RC rows (columns >= their row)
+      __syncthreads();
+      if (j0 >= 0)
+        for (int e = threadIdx.x; e < RC * K; e += W * 64) {
+          const int rr = e / K, c = e % K, row = j0 + rr;
+          if (row < K && c >= row) R[row * it.r_rs + c * it.r_cs] = Rst[rr][c];
+        }
+      j0 = j;
+      __syncthreads();
+      for (int e = threadIdx.x; e < RC * K; e += W * 64) {
+        const int rr = e / K, c = e % K, row = j0 + rr;
+        Rst[rr][c] = (row < K && c >= row) ? R[row * it.r_rs + c * it.r_cs] : T(0);
+      }
+    }
+    const int jj = j - j0;
+    const int buf = j & 1;
+    if (w == wj) {
+      // column j by predicated selection (keeps a[][] in registers)
+      T col[MR];
+#pragma unroll
+      for (int t = 0; t < MR; ++t) col[t] = a[0][t];
+#pragma unroll
+      for (int q = 1; q < CPW; ++q)
+#pragma unroll
+        for (int t = 0; t < MR; ++t) col[t] = (q == qj) ? a[q][t] : col[t];
+      T x2 = T(0);
+#pragma unroll
+      for (int t = 0; t < MR; ++t) x2 += col[t] * col[t];
+      x2 = wave_sum(x2);
+      const T alpha = Rst[jj][j];
+      T beta, tau, scal;
+      if (x2 == T(0)) {
+        tau = T(0);
+        beta = alpha;
+        scal = T(1);
+      } else {
+        const T nrm = sqrt(alpha * alpha + x2);
+        beta = (alpha >= T(0)) ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        scal = T(1) / (alpha - beta);
+      }
+#pragma unroll
+      for (int t = 0; t < MR; ++t) {
+        col[t] *= scal;
+        vbuf[buf][lane + 64 * t] = col[t];
+      }
+#pragma unroll
+      for (int q = 0; q < CPW; ++q)
+#pragma unroll
+        for (int t = 0; t < MR; ++t) a[q][t] = (q == qj) ? col[t] : a[q][t];
+      // every lane has read alpha (wave-uniform) before lane 0 overwrites it
+      if (lane == 0) {
+        Rst[jj][j] = beta;
+        taus[j] = tau;
+      }
+    }
+    __syncthreads();
+    const T tj = taus[j];
+    if (tj != T(0)) {
+      T v[MR];
+#pragma unroll
+      for (int t = 0; t < MR; ++t) v[t] = vbuf[buf][lane + 64 * t];
+#pragma unroll
+      for (int q = 0; q < CPW; ++q) {
+        const int c = q * W + w;
+        if (c > j && c < K) {
+          T d = T(0);
+#pragma unroll
+          for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
+          const T rjc = Rst[jj][c];
+          d = (rjc + wave_sum(d)) * tj;
+#pragma unroll
+          for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
+          if (lane == 0) Rst[jj][c] = rjc - d;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < RC * K; e += W * 64) {
+    const int rr = e / K, c = e % K, row = j0 + rr;
+    if (row < K && c >= row) R[row * it.r_rs + c * it.r_cs] = Rst[rr][c];
+  }
+  T* tau_out = static_cast<T*>(it.tau);
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const int c = q * W + w;
+    if (c < K) {
+#pragma unroll
+      for (int t = 0; t < MR; ++t) {
+        const int row = lane + 64 * t;
+        if (row < m) B[row * it.b_rs + c * it.b_cs] = a[q][t];
+      }
+      if (lane == 0) tau_out[c] = taus[c];
+    }
+  }
+}
+
+template <typename T>
+void launch_tpqrt(const TpqrtItem* d_items, int n_items, int max_m, int max_k, hipStream_t s) {
+  if (n_items <= 0) return;
+  if (max_m <= 64 && max_k <= 512) {
+    hipLaunchKernelGGL((tpqrt_kernel<T, 1, 16, 32>), dim3(n_items), dim3(1024), 0, s, d_items);
+  } else if (max_m <= 512 && max_k <= 64) {
+    hipLaunchKernelGGL((tpqrt_kernel<T, 8, 8, 8>), dim3(n_items), dim3(512), 0, s, d_items);
+  } else {
+    throw std::runtime_error("tpqrt: block exceeds 64 rows x 512 columns or 512 rows x 64 columns");
+  }
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// LAPACK larft (forward, columnwise) for K <= 64 reflectors, one wave per item.
+template <typename T>
+__global__ __launch_bounds__(64) void larft_kernel(const LarftItem* __restrict__ items) {
+  const LarftItem it = items[blockIdx.x];
+  const int K = it.K, r = threadIdx.x;
+  const T* G = static_cast<const T*>(it.G);
+  const T* tau = static_cast<const T*>(it.tau);
+  __shared__ T Ts[64][65];
+  __shared__ T gcol[64];
+  for (int j = 0; j < K; ++j) {
+    if (r < j) gcol[r] = G[r * it.g_rs + j * it.g_cs];
+    __syncthreads();
+    const T tj = tau[j];
+    if (r < j) {
+      T sacc = T(0);
+      for (int c = r; c < j; ++c) sacc += Ts[r][c] * gcol[c];
+      Ts[r][j] = -tj * sacc;
+    } else if (r == j) {
+      Ts[j][j] = tj;
+    }
+    __syncthreads();
+  }
+  T* Tout = static_cast<T*>(it.T);
+  for (int c = 0; c < K; ++c)
+    if (r < K) Tout[r * it.t_rs + c * it.t_cs] = (c >= r) ? Ts[r][c] : T(0);
+}
+
+template <typename T>
+void launch_larft(const LarftItem* d_items, int n_items, hipStream_t s) {
+  if (n_items <= 0) return;
+  hipLaunchKernelGGL(larft_kernel<T>, dim3(n_items), dim3(64), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// Copy of a triangle of a strided n x n view into a dense row-major n x n
+// buffer: mode 0 = unit lower (strict lower part, ones on the diagonal,
+// zeros above), mode 1 = upper with diagonal (zeros below).
+template <typename T>
+__global__ __launch_bounds__(256) void tri_extract_kernel(const T* __restrict__ src, int64_t rs, int64_t cs,
+                                                          T* __restrict__ dst, int64_t ld, int n, int mode) {
+  const int64_t total = (int64_t)n * n;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e / n), j = (int)(e % n);
+    T v;
+    if (mode == 0)
+      v = (i > j) ? src[i * rs + j * cs] : (i == j ? T(1) : T(0));
+    else
+      v = (i <= j) ? src[i * rs + j * cs] : T(0);
+    dst[(int64_t)i * ld + j] = v;
+  }
+}
+
+template <typename T>
+void launch_tri_extract(const T* src, int64_t rs, int64_t cs, T* dst, int64_t ld, int n, int mode, hipStream_t s) {
+  if (n <= 0) return;
+  const int g = (int)std::min<int64_t>(((int64_t)n * n + 255) / 256, 1024);
+  hipLaunchKernelGGL(tri_extract_kernel<T>, dim3(g), dim3(256), 0, s, src, rs, cs, dst, ld, n, mode);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+// ===========================================================================
 // One-sided (Hestenes) Jacobi SVD of the K x K core with truncation.
 // ===========================================================================
 __device__ __forceinline__ int rr_pos(int pos, int r, int k2) {
@@ -947,6 +1212,45 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
     }
   }
   __syncthreads();
+  // Scale by a power of two so that max |M_ij| lies in [1, 2): exact, and it
+  // keeps squared column norms clear of underflow in the rotation tests.
+  using UB = typename std::conditional<sizeof(T) == 8, unsigned long long, unsigned int>::type;
+  __shared__ UB amax_bits;
+  __shared__ T tiny_sh;
+  if (tid == 0) amax_bits = 0;
+  __syncthreads();
+  {
+    T mx = T(0);
+    for (int e = tid; e < K * K; e += NT) mx = fmax(mx, fabs(G[e / K][e % K]));
+    UB bits;
+    memcpy(&bits, &mx, sizeof(T));
+    atomicMax(&amax_bits, bits);
+  }
+  __syncthreads();
+  T amax;
+  {
+    const UB bits = amax_bits;
+    memcpy(&amax, &bits, sizeof(T));
+  }
+  int sexp = 0;
+  if (amax > T(0) && isfinite(amax)) sexp = ilogb(amax);
+  const T scale = ldexp(T(1), -sexp), unscale = ldexp(T(1), sexp);
+  for (int e = tid; e < K * K; e += NT) G[e / K][e % K] *= scale;
+  __syncthreads();
+  for (int j = tid; j < K; j += NT) {
+    T c2 = T(0);
+    for (int row = 0; row < K; ++row) c2 += G[j][row] * G[j][row];
+    sig[j] = c2;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    T fro2 = T(0);
+    for (int j = 0; j < K; ++j) fro2 += sig[j];
+    // columns below eps * ||M||_F are numerically zero: rotating them only churns rounding noise
+    tiny_sh = Eps<T>::v * Eps<T>::v * fro2;
+  }
+  __syncthreads();
+  const T tiny = tiny_sh;
   const int K2 = K + (K & 1);
   const int P = K2 / 2;
   int gl = 64;
@@ -978,7 +1282,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
             al = group_sum(al, gl);
             be = group_sum(be, gl);
             ga = group_sum(ga, gl);
-            if (ga != T(0) && fabs(ga) > tol * sqrt(al * be)) {
+            if (al > tiny && be > tiny && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
               const T zeta = (be - al) / (T(2) * ga);
               const T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
               const T cth = T(1) / sqrt(T(1) + t * t);
@@ -1089,7 +1393,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
   if (S != nullptr)
     for (int e = tid; e < r * r; e += NT) {
       const int i = e / r, j = e % r;
-      S[i * it.lds + j] = (i == j) ? sig[order[i]] : T(0);
+      S[i * it.lds + j] = (i == j) ? sig[order[i]] * unscale : T(0);
     }
   for (int e = tid; e < K * r; e += NT) {
     const int row = e / r, c = e % r;
@@ -1569,6 +1873,9 @@ template void launch_axpy_convert<float>(double*, const float*, int64_t, hipStre
   template void launch_tri_inv<T>(const TriInvItem*, int, hipStream_t);                          \
   template void launch_copy<T>(const GemmItem*, int, int64_t, hipStream_t);                      \
   template void launch_qr<T>(const QrItem*, int, int, hipStream_t);                              \
+  template void launch_tpqrt<T>(const TpqrtItem*, int, int, int, hipStream_t);                   \
+  template void launch_larft<T>(const LarftItem*, int, hipStream_t);                             \
+  template void launch_tri_extract<T>(const T*, int64_t, int64_t, T*, int64_t, int, int, hipStream_t); \
   template void launch_apply_q<T>(const ApplyQItem*, int, int, hipStream_t);                     \
   template void launch_svd<T>(const SvdItem*, int, int, hipStream_t);                            \
   template void launch_geqp3<T>(const Geqp3Item*, int, hipStream_t);                             \

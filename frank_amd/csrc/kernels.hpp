@@ -30,6 +30,12 @@ void launch_copy(const GemmItem* d_items, int n_items, int64_t max_elems, hipStr
 // Householder QR of [A1 S1 | A2 S2] (optionally projected), m <= 512.
 template <typename T>
 void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s);
+template <typename T>
+void launch_tpqrt(const TpqrtItem* d_items, int n_items, int max_m, int max_k, hipStream_t s);
+template <typename T>
+void launch_larft(const LarftItem* d_items, int n_items, hipStream_t s);
+template <typename T>
+void launch_tri_extract(const T* src, int64_t rs, int64_t cs, T* dst, int64_t ld, int n, int mode, hipStream_t s);
 
 // Apply Householder products to zero-padded blocks, m <= 512.
 template <typename T>

@@ -68,6 +68,18 @@ int frk_ctx_stream(frk_ctx ctx, void** stream);
  * Slot 0: small SVD problems solved, 1: Jacobi sweeps summed over them,
  * 2: largest sweep count of one problem. */
 int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset);
+/* Op log (no reference counterpart).  Every launch is accounted under its
+ * kernel class ("gemm", "getrf_small", "getrf_gemm", "trsm_gemm",
+ * "schur_gemm", "solve_gemm", "lr_qr", "lr_svd", "lr_apply_q") with its
+ * algorithmic flops and bytes; launches of the one class named by
+ * frk_ctx_time_class (NULL or "" = none) are bracketed by HIP events on the
+ * context stream.  frk_ctx_op_classes writes the class names, comma
+ * separated, into buf (NUL-terminated, truncated to len). */
+int frk_ctx_time_class(frk_ctx ctx, const char* cls);
+int frk_ctx_op_classes(frk_ctx ctx, char* buf, int len);
+int frk_ctx_op_stats(frk_ctx ctx, const char* cls, int64_t* launches, double* flops, double* bytes, double* ms,
+                     int64_t* timed_launches);
+int frk_ctx_reset_op_stats(frk_ctx ctx);
 
 /* ---------------------------------------------------------------------------
  * BLR matrices (one-level Hierarchical, tiles resident in HBM)

@@ -686,14 +686,16 @@ def build_blr(kernel, n_rows: int, n_cols: int, nb_r: int, nb_c: int, *, eps: fl
 
 
 def blr_getrf(A: Hier, lra: str = LRA_DEFAULT, check_pivots: bool = True,
-              op_log: Optional[list] = None) -> Tuple[Hier, Hier]:
+              op_log: Optional[list] = None, max_steps: Optional[int] = None) -> Tuple[Hier, Hier]:
     """src/operations/LAPACK/getrf.cpp:45-65: right-looking tile LU without pivoting.
 
-    Consumes ``A`` (blocks moved into L), returns (L, U).
+    Consumes ``A`` (blocks moved into L), returns (L, U).  ``max_steps`` stops
+    after that many elimination steps (a timing sample; the result is then a
+    partial factorization).
     """
     nb = A.nb_r
     L = Hier(nb, A.nb_c, [None] * (nb * A.nb_c), list(A.row_ranges), list(A.col_ranges))
-    for i in range(nb):
+    for i in range(nb if max_steps is None else min(nb, max_steps)):
         Lii, Uii = getrf_dense(A[i, i], check_pivots)
         L[i, i] = Lii
         A[i, i] = Uii
