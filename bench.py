@@ -88,9 +88,11 @@ def main():
     ap.add_argument("--mode", choices=["both", "f64", "mixed"], default="both")
     ap.add_argument("--eps-mixed", type=float, default=1e-6,
                     help="compression / recompression accuracy of the FP32-storage factorization")
-    ap.add_argument("--roofline-class", default="schur_gemm",
+    ap.add_argument("--roofline-class", default="getrf_gemm",
                     help="kernel class whose launches are event-timed for the roofline object")
     ap.add_argument("--cpu-steps", type=int, default=3, help="elimination steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 256 MiB device copy first (calibrates HBM byte counters under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
@@ -103,6 +105,12 @@ def main():
     from frank_amd.inputs import normal_rhs, sorted_uniform_points
 
     print(f"[bench] N={args.n} nb={args.nb} eps={args.eps} nrhs={args.nrhs}", file=sys.stderr, flush=True)
+    if args.copy_probe:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ctx = fa.default_context()
     pts = [sorted_uniform_points(args.n)]
     b = torch.from_numpy(normal_rhs(args.n, args.nrhs)).cuda()

@@ -21,7 +21,7 @@ FRK_ERR_UNSUPPORTED = -4
 
 TILE_DENSE, TILE_LOWRANK, TILE_EMPTY = 0, 1, 2
 LRA_NAIVE, LRA_ROUNDED, LRA_FAST = 0, 1, 2
-KERNEL_LAPLACE, KERNEL_GAUSSIAN = 0, 1
+KERNEL_LAPLACE, KERNEL_GAUSSIAN, KERNEL_IDENTITY = 0, 1, 2
 ADMIS_POSITION, ADMIS_GEOMETRY = 0, 1
 PREC_F64, PREC_F32 = 0, 1
 
@@ -52,6 +52,12 @@ _SIGS = {
                                C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
     "frk_blr_create": ([vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, dbl, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
     "frk_blr_destroy": ([vp], C.c_int),
+    "frk_blr_create_kernel_rect": ([vp, C.c_int, C.c_int, i64, vp, i64, i64, C.c_int, C.c_int, dbl, dbl, dbl, C.c_int,
+                                    dbl, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
+    "frk_blr_qr": ([vp, C.POINTER(vp)], C.c_int),
+    "frk_qrf_destroy": ([vp], C.c_int),
+    "frk_qrf_get_t": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
+    "frk_blr_apply_q": ([vp, vp, vp, C.c_int], C.c_int),
     "frk_ctx_stream": ([vp, C.POINTER(vp)], C.c_int),
     "frk_ctx_counters": ([vp, pi64, C.c_int, C.c_int], C.c_int),
     "frk_ctx_time_class": ([vp, C.c_char_p], C.c_int),

@@ -99,6 +99,36 @@ def default_context() -> Context:
     return _default_ctx
 
 
+_KERNEL = {"laplace": L.KERNEL_LAPLACE, "gaussian": L.KERNEL_GAUSSIAN, "identity": L.KERNEL_IDENTITY}
+
+
+class QrFactors:
+    """T factors of a tiled Householder BLR-QR (the reference's Hierarchical T)."""
+
+    def __init__(self, h, owner: "BlrMatrix"):
+        self.h = h
+        self.owner = owner
+
+    def t(self, i: int, k: int):
+        """(T(i, k), tau(i, k)) as host arrays."""
+        _, _, rows, cols = self.owner.tile_info(k, k)
+        T = np.empty((cols, cols))
+        tau = np.empty(cols)
+        L.check(L.lib().frk_qrf_get_t(self.h, i, k, _hptr(T), _hptr(tau)))
+        return T, tau
+
+    def close(self):
+        if self.h:
+            L.lib().frk_qrf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BlrMatrix:
     """One-level Hierarchical matrix of Dense / LowRank tiles resident on the GPU."""
 
@@ -116,10 +146,24 @@ class BlrMatrix:
         ctx = ctx or default_context()
         pts = np.ascontiguousarray(np.vstack([np.asarray(p, dtype=np.float64) for p in points]))
         dim, n = pts.shape
-        kind = L.KERNEL_LAPLACE if kernel == "laplace" else L.KERNEL_GAUSSIAN
         h = C.c_void_p()
-        L.check(L.lib().frk_blr_create_kernel(ctx.h, kind, dim, n, _hptr(pts), ell, shift, nb, eps, rank, admis,
-                                              _ADMIS[admis_type], kcap, _PREC[precision], C.byref(h)))
+        L.check(L.lib().frk_blr_create_kernel(ctx.h, _KERNEL[kernel], dim, n, _hptr(pts), ell, shift, nb, eps, rank,
+                                              admis, _ADMIS[admis_type], kcap, _PREC[precision], C.byref(h)))
+        return cls(h, ctx)
+
+    @classmethod
+    def from_kernel_rect(cls, points: Sequence[np.ndarray], n_rows: int, n_cols: int, nb_r: int, nb_c: int, *,
+                         eps: float = 0.0, rank: int = 0, admis: float = 0.0, admis_type: str = "position",
+                         kcap: int = 0, kernel: str = "laplace", ell: float = 0.0, shift: float = 0.0,
+                         precision: str = "f64", ctx: Optional[Context] = None) -> "BlrMatrix":
+        """Hierarchical(kernel, params, n_rows, n_cols, nleaf, eps|rank, admis, nb_r, nb_c, admis_type)."""
+        ctx = ctx or default_context()
+        pts = np.ascontiguousarray(np.vstack([np.asarray(p, dtype=np.float64) for p in points]))
+        dim, n = pts.shape
+        h = C.c_void_p()
+        L.check(L.lib().frk_blr_create_kernel_rect(ctx.h, _KERNEL[kernel], dim, n, _hptr(pts), n_rows, n_cols, nb_r,
+                                                   nb_c, ell, shift, eps, rank, admis, _ADMIS[admis_type], kcap,
+                                                   _PREC[precision], C.byref(h)))
         return cls(h, ctx)
 
     @classmethod
@@ -239,6 +283,17 @@ class BlrMatrix:
         """In-place tile LU: lower tiles -> L, upper tiles -> U (getrf.cpp:45-65)."""
         L.check(L.lib().frk_blr_getrf(self.h))
         return self
+
+    def qr(self) -> QrFactors:
+        """tiled_householder_blr_qr(A, T) in place (qr.cpp:213-228); returns the T factors."""
+        h = C.c_void_p()
+        L.check(L.lib().frk_blr_qr(self.h, C.byref(h)))
+        return QrFactors(h, self)
+
+    def apply_q(self, F: QrFactors, C_: "BlrMatrix", trans: bool = False) -> "BlrMatrix":
+        """left_multiply_tiled_reflector(self, T, C, trans): C <- Q C or Q^T C (qr.cpp:230-257)."""
+        L.check(L.lib().frk_blr_apply_q(self.h, F.h, C_.h, int(trans)))
+        return C_
 
     def solve(self, b, trans: bool = False):
         """In place on a GPU tensor b (N x nrhs, float64): L then U sweeps (U^T, L^T if trans)."""

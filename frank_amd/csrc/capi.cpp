@@ -28,6 +28,11 @@ struct frk_blr_s {
   double anorm = -1.0;
 };
 
+struct frk_qrf_s {
+  frk_blr_s* a = nullptr;
+  std::unique_ptr<QrFactors<double>> f64;
+};
+
 namespace {
 thread_local std::string g_err;
 
@@ -271,36 +276,50 @@ int frk_ctx_reset_op_stats(frk_ctx ctx) {
   });
 }
 
-int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
-                          double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
-                          int precision, frk_blr* out) {
+int frk_blr_create_kernel_rect(frk_ctx ctx, int kernel_kind, int dim, int64_t n_pts, const double* pts,
+                               int64_t n_rows, int64_t n_cols, int nb_r, int nb_c, double ell, double shift,
+                               double eps, int rank, double admis, int admis_type, int kcap, int precision,
+                               frk_blr* out) {
   return guard([&] {
     if (!ctx || !out || !pts) throw std::invalid_argument("null argument");
     if (dim < 1 || dim > 3) throw std::invalid_argument("dim must be 1, 2 or 3");
-    if (nb < 1 || n < nb) throw std::invalid_argument("bad block count");
-    if ((n + nb - 1) / nb > 512) throw std::invalid_argument("tile size above 512");
+    if (kernel_kind < 0 || kernel_kind > 2) throw std::invalid_argument("unknown kernel");
+    if (n_rows <= 0 || n_cols <= 0 || n_rows > n_pts || n_cols > n_pts)
+      throw std::invalid_argument("matrix size must be positive and at most the point count");
+    if (nb_r < 1 || nb_c < 1 || n_rows < nb_r || n_cols < nb_c) throw std::invalid_argument("bad block count");
+    if ((n_rows + nb_r - 1) / nb_r > 512 || (n_cols + nb_c - 1) / nb_c > 512)
+      throw std::invalid_argument("tile size above 512");
     if (eps == 0.0 && rank <= 0) throw std::invalid_argument("need eps != 0 or rank > 0");
     if (kcap <= 0) kcap = (eps == 0.0) ? rank : 64;
     if (eps == 0.0 && rank > kcap) throw std::invalid_argument("rank exceeds kcap");
-    KernelDesc kd{kernel_kind, dim, n, pts, ell, shift};
+    KernelDesc kd{kernel_kind, dim, n_pts, pts, ell, shift};
+    kd.n_rows = n_rows;
+    kd.n_cols = n_cols;
     auto h = std::make_unique<frk_blr_s>();
     h->owner = ctx;
     h->precision = precision;
     if (precision == FRK_PREC_F64)
-      h->a64 = build_blr_kernel<double>(kd, nb, eps, rank, admis, admis_type, kcap, *ctx->ctx);
+      h->a64 = build_blr_kernel<double>(kd, nb_r, nb_c, eps, rank, admis, admis_type, kcap, *ctx->ctx);
     else if (precision == FRK_PREC_F32)
-      h->a32 = build_blr_kernel<float>(kd, nb, eps, rank, admis, admis_type, kcap, *ctx->ctx);
+      h->a32 = build_blr_kernel<float>(kd, nb_r, nb_c, eps, rank, admis, admis_type, kcap, *ctx->ctx);
     else
       throw std::invalid_argument("unknown precision");
-    h->host_pts.assign(pts, pts + (size_t)dim * n);
+    h->host_pts.assign(pts, pts + (size_t)dim * n_pts);
     h->kd = kd;
     h->kd.host_pts = h->host_pts.data();
     h->ks = upload_kernel(h->kd, h->pts, ctx->stream);
-    h->has_kernel = true;
+    h->has_kernel = n_rows == n_cols && n_rows == n_pts;
     FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     *out = h.release();
     return FRK_OK;
   });
+}
+
+int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
+                          double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
+                          int precision, frk_blr* out) {
+  return frk_blr_create_kernel_rect(ctx, kernel_kind, dim, n, pts, n, n, nb, nb, ell, shift, eps, rank, admis,
+                                    admis_type, kcap, precision, out);
 }
 
 int frk_blr_create(frk_ctx ctx, int nb_r, int nb_c, const int64_t* roff, const int64_t* coff, const int32_t* kinds,
@@ -500,6 +519,52 @@ int frk_blr_getrf(frk_blr a) {
       blr_getrf<double>(*a->a64, *a->owner->ctx);
     else
       blr_getrf<float>(*a->a32, *a->owner->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_blr_qr(frk_blr a, frk_qrf* out) {
+  return guard([&] {
+    if (!a || !out) throw std::invalid_argument("null argument");
+    if (!a->a64) throw std::runtime_error("tiled_householder_blr_qr(FP32 Hierarchical) undefined!");
+    auto f = std::make_unique<frk_qrf_s>();
+    f->a = a;
+    f->f64 = std::make_unique<QrFactors<double>>(*a->a64);
+    blr_tiled_qr<double>(*a->a64, *f->f64, *a->owner->ctx);
+    *out = f.release();
+    return FRK_OK;
+  });
+}
+
+int frk_qrf_destroy(frk_qrf f) {
+  return guard([&] {
+    if (!f) return FRK_OK;
+    FRK_HIP_CHECK(hipStreamSynchronize(f->a->owner->stream));
+    delete f;
+    return FRK_OK;
+  });
+}
+
+int frk_qrf_get_t(frk_qrf f, int i, int k, double* t_out, double* tau_out) {
+  return guard([&] {
+    if (!f) throw std::invalid_argument("null factors");
+    const QrFactors<double>& F = *f->f64;
+    if (i < 0 || k < 0 || i >= F.nb_r || k >= F.nb_c || i < k || k >= (int)F.bsize.size())
+      throw std::invalid_argument("T(i, k) exists for i >= k only");
+    const size_t b = F.bsize[k];
+    FRK_HIP_CHECK(hipStreamSynchronize(f->a->owner->stream));
+    if (t_out) FRK_HIP_CHECK(hipMemcpy(t_out, F.tmat(i, k), sizeof(double) * b * b, hipMemcpyDeviceToHost));
+    if (tau_out) FRK_HIP_CHECK(hipMemcpy(tau_out, F.tau_of(i, k), sizeof(double) * b, hipMemcpyDeviceToHost));
+    return FRK_OK;
+  });
+}
+
+int frk_blr_apply_q(frk_blr y, frk_qrf f, frk_blr c, int trans) {
+  return guard([&] {
+    if (!y || !f || !c) throw std::invalid_argument("null argument");
+    if (!y->a64 || !c->a64) throw std::runtime_error("left_multiply_tiled_reflector(FP32) undefined!");
+    if (f->a != y) throw std::invalid_argument("factors belong to another matrix");
+    blr_apply_q<double>(*y->a64, *f->f64, *c->a64, trans != 0, *y->owner->ctx);
     return FRK_OK;
   });
 }

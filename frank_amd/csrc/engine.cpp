@@ -1059,32 +1059,37 @@ KernelSpec upload_kernel(const KernelDesc& kd, DevBuf& pts, hipStream_t s) {
 }
 
 template <typename T>
-std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, double eps, int rank, double admis,
-                                               int admis_type, int kcap, Context& ctx) {
+std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb_r, int nb_c, double eps, int rank,
+                                               double admis, int admis_type, int kcap, Context& ctx) {
   const hipStream_t s = ctx.stream();
-  auto off = index_split(kd.n, nb);
-  std::vector<int32_t> kinds(nb * nb);
-  for (int i = 0; i < nb; ++i)
-    for (int j = 0; j < nb; ++j)
-      kinds[i * nb + j] = admissible(kd, off[i], off[i + 1] - off[i], off[j], off[j + 1] - off[j], i, j, admis,
-                                     admis_type)
-                              ? kLowRank
-                              : kDense;
-  auto A = std::make_unique<BlrMatrix<T>>(off, off, kinds, kcap, eps, rank, ctx);
+  const int64_t n_rows = kd.n_rows > 0 ? kd.n_rows : kd.n, n_cols = kd.n_cols > 0 ? kd.n_cols : kd.n;
+  if (n_rows > kd.n || n_cols > kd.n) throw std::invalid_argument("matrix larger than the point set");
+  auto roff = index_split(n_rows, nb_r);
+  auto coff = index_split(n_cols, nb_c);
+  std::vector<int32_t> kinds(nb_r * nb_c);
+  for (int i = 0; i < nb_r; ++i)
+    for (int j = 0; j < nb_c; ++j)
+      kinds[i * nb_c + j] = admissible(kd, roff[i], roff[i + 1] - roff[i], coff[j], coff[j + 1] - coff[j], i, j,
+                                       admis, admis_type)
+                                ? kLowRank
+                                : kDense;
+  const int nb = nb_c;
+  auto A = std::make_unique<BlrMatrix<T>>(roff, coff, kinds, kcap, eps, rank, ctx);
+  const auto& off = roff;
   DevBuf pts;
   KernelSpec ks = upload_kernel(kd, pts, s);
   ctx.sync();
   std::vector<GenItem> gd;
   int mr = 0, mc = 0;
-  for (int i = 0; i < nb; ++i)
-    for (int j = 0; j < nb; ++j)
+  for (int i = 0; i < nb_r; ++i)
+    for (int j = 0; j < nb_c; ++j)
       if (kinds[i * nb + j] == kDense) {
         GenItem g{};
         g.out = A->D[A->idx(i, j)];
         g.ld = A->cols(j);
         g.r0 = (int)off[i];
         g.nr = (int)A->rows(i);
-        g.c0 = (int)off[j];
+        g.c0 = (int)coff[j];
         g.nc = (int)A->cols(j);
         gd.push_back(g);
         mr = std::max<int>(mr, g.nr);
@@ -1098,10 +1103,12 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, dou
     pl.run();
   }
   std::vector<std::pair<int, int>> lr;
-  for (int i = 0; i < nb; ++i)
-    for (int j = 0; j < nb; ++j)
+  for (int i = 0; i < nb_r; ++i)
+    for (int j = 0; j < nb_c; ++j)
       if (kinds[i * nb + j] == kLowRank) lr.emplace_back(i, j);
-  const int64_t bmax = off[1] - off[0];
+  int64_t bmax = 1;
+  for (int i = 0; i < nb_r; ++i) bmax = std::max<int64_t>(bmax, A->rows(i));
+  for (int j = 0; j < nb_c; ++j) bmax = std::max<int64_t>(bmax, A->cols(j));
   const size_t per = (size_t)bmax * bmax;
   const int chunk = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (int64_t(2) << 30) / (per * sizeof(T))));
   DevBuf dense;
@@ -1118,7 +1125,7 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, dou
       g.ld = A->cols(j);
       g.r0 = (int)off[i];
       g.nr = (int)A->rows(i);
-      g.c0 = (int)off[j];
+      g.c0 = (int)coff[j];
       g.nc = (int)A->cols(j);
       gl.push_back(g);
       CompressDesc<T> c{};
@@ -1711,8 +1718,8 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
   template void getrf_tiles<T>(const std::vector<T*>&, int, int64_t, int32_t*, Context&);                   \
   template void trsm_generic<T>(const T*, int64_t, int64_t, bool, bool, T*, int64_t, int64_t, int, int,     \
                                 Context&);                                                                  \
-  template std::unique_ptr<BlrMatrix<T>> build_blr_kernel<T>(const KernelDesc&, int, double, int, double, \
-                                                             int, int, Context&);                           \
+  template std::unique_ptr<BlrMatrix<T>> build_blr_kernel<T>(const KernelDesc&, int, int, double, int,    \
+                                                             double, int, int, Context&);                   \
   template void blr_getrf<T>(BlrMatrix<T>&, Context&);                                                      \
   template void blr_solve<T>(BlrMatrix<T>&, T*, int, bool, Context&);                                       \
   template void blr_matvec<T>(BlrMatrix<T>&, const T*, T*, int, Context&);                                  \

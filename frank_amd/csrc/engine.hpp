@@ -288,16 +288,17 @@ void trsm_generic(const T* A, int64_t a_rs, int64_t a_cs, bool lower, bool unit,
 // BLR drivers.
 // ----------------------------------------------------------------------------
 struct KernelDesc {
-  int kind;  // 0 laplace, 1 gaussian
+  int kind;  // 0 laplace, 1 gaussian, 2 identity
   int dim;
-  int64_t n;
+  int64_t n;               // number of points
   const double* host_pts;  // dim x n, structure of arrays
   double ell, shift;
+  int64_t n_rows = 0, n_cols = 0;  // matrix size (0: n); rows and columns index the same points
 };
 
-// Hierarchical(kernel, ..., eps | rank, admis, nb, nb) for a one-level tree.
+// Hierarchical(kernel, ..., eps | rank, admis, nb_r, nb_c) for a one-level tree.
 template <typename T>
-std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb, double eps, int rank,
+std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb_r, int nb_c, double eps, int rank,
                                                double admis, int admis_type, int kcap, Context& ctx);
 
 // getrf(Hierarchical&): right-looking tile LU without pivoting, in place
@@ -333,5 +334,97 @@ std::unique_ptr<BlrMatrix<TD>> convert_blr(const BlrMatrix<TS>& src, Context& ct
 
 // Device kernel spec with uploaded points (owned by the returned buffer).
 KernelSpec upload_kernel(const KernelDesc& kd, DevBuf& pts, hipStream_t s);
+
+// ----------------------------------------------------------------------------
+// Typed (Dense / LowRank) operands and batched typed gemm
+// (src/operations/BLAS/gemm.cpp:126-272).
+// ----------------------------------------------------------------------------
+template <typename T>
+struct Op {  // read-only view, possibly transposed through its strides
+  int kind;
+  int m, n;
+  const T* d;
+  int64_t d_rs, d_cs;
+  const T* u;
+  int64_t u_rs, u_cs;
+  const T* s;
+  int64_t s_rs, s_cs;
+  const T* v;
+  int64_t v_rs, v_cs;
+  int r;
+};
+
+template <typename T>
+struct Target {  // dense view, or low-rank slots in the BlrMatrix layout
+  int kind;
+  int m, n;
+  T* d;
+  int64_t d_rs, d_cs;
+  T* u;  // m x cap column-major
+  T* s;  // cap x cap row-major
+  T* v;  // cap x n row-major
+  int cap, rank;
+  int32_t* d_rank;
+  double eps;
+  int fixed_rank;
+};
+
+template <typename T>
+struct TypedBatch {
+  std::vector<GemmItem> cp;    // copies, run first
+  std::vector<GemmItem> g[5];  // dependent product stages
+  std::vector<LrAddDesc<T>> adds;
+  std::vector<CompressDesc<T>> comp;
+  std::vector<int> comp_add;
+};
+
+template <typename T>
+Op<T> tile_op(const BlrMatrix<T>& A, int i, int j);
+template <typename T>
+Op<T> transpose_op(const Op<T>& a);
+template <typename T>
+Target<T> tile_target(BlrMatrix<T>& A, int i, int j);
+template <typename T>
+size_t typed_gemm_workspace(const Op<T>& A, const Op<T>& B, const Target<T>& C);
+// C = alpha A B + beta C (beta = 1 for low-rank targets); products go to the
+// batch's stages, additions to its list; the ranks of low-rank operands and
+// targets must be current on the host.
+template <typename T>
+void typed_gemm(TypedBatch<T>& b, Context& ctx, const Op<T>& A, const Op<T>& B, const Target<T>& C, double alpha,
+                double beta);
+template <typename T>
+void run_typed_batch(TypedBatch<T>& b, Context& ctx, int lra);
+
+// Workspace bound of plan_lr_add for nd additions of at most m x n with K columns.
+template <typename T>
+size_t lr_add_bound(int nd, int m, int n, int K) {
+  return (size_t)nd * sizeof(T) * ((size_t)(m + n) * K + 2 * (size_t)K + 4 * (size_t)K * K) + 8 * 256 + 4096;
+}
+
+// ----------------------------------------------------------------------------
+// Tiled Householder BLR-QR (src/operations/LAPACK/qr.cpp:213-257).
+// ----------------------------------------------------------------------------
+template <typename T>
+struct QrFactors {
+  explicit QrFactors(const BlrMatrix<T>& A);
+  int nb_r, nb_c;
+  std::vector<T*> Tt;   // T(i, k), i >= k: b x b row-major
+  std::vector<T*> tau;  // tau(i, k): b
+  std::vector<T*> Yx;   // explicit unit-lower reflectors of diagonal tile k (b x b)
+  std::vector<int> bsize;
+  T* tmat(int i, int k) const { return Tt[i * nb_c + k]; }
+  T* tau_of(int i, int k) const { return tau[i * nb_c + k]; }
+
+ private:
+  DevBuf buf_;
+};
+
+// tiled_householder_blr_qr(A, T): in place, R in the upper tiles, reflectors
+// in the diagonal tiles' lower parts and in the tiles below the diagonal.
+template <typename T>
+void blr_tiled_qr(BlrMatrix<T>& A, QrFactors<T>& F, Context& ctx);
+// left_multiply_tiled_reflector(Y, T, C, trans): C <- Q C (trans = false) or Q^T C.
+template <typename T>
+void blr_apply_q(const BlrMatrix<T>& Y, QrFactors<T>& F, BlrMatrix<T>& C, bool trans, Context& ctx);
 
 }  // namespace frk

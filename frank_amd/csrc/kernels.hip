@@ -1680,8 +1680,10 @@ __global__ __launch_bounds__(256) void gen_kernel(KernelSpec ks, const GenItem* 
     double v;
     if (ks.kind == 0) {
       v = 1.0 / (sqrt(rij) + 1e-3);
-    } else {
+    } else if (ks.kind == 1) {
       v = exp(-rij / ((2.0 * ks.ell) * ks.ell));
+    } else {  // identity (src/functions.cpp:25-34)
+      v = (gi == gj) ? 1.0 : 0.0;
     }
     if (gi == gj) v = v + ks.shift;
     out[(int64_t)i * it.ld + j] = static_cast<T>(v);

@@ -80,7 +80,7 @@ struct KernelSpec {
   const double* pts;  // dim arrays of N points, structure of arrays
   int64_t n_pts;
   int32_t dim;
-  int32_t kind;       // 0 laplace, 1 gaussian
+  int32_t kind;       // 0 laplace, 1 gaussian, 2 identity
   double ell;         // gaussian length scale
   double shift;       // added on the global diagonal
 };

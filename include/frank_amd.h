@@ -37,9 +37,11 @@ extern "C" {
 #define FRK_LRA_ROUNDED 1
 #define FRK_LRA_FAST 2
 
-/* Kernel generators (src/functions.cpp:92-109 laplacend; gaussian has no reference counterpart) */
+/* Kernel generators (src/functions.cpp:92-109 laplacend, :25-34 identity;
+ * gaussian has no reference counterpart) */
 #define FRK_KERNEL_LAPLACE 0
 #define FRK_KERNEL_GAUSSIAN 1
+#define FRK_KERNEL_IDENTITY 2
 
 /* Admissibility (include/FRANK/definitions.h:25 AdmisType) */
 #define FRK_ADMIS_POSITION 0
@@ -51,6 +53,7 @@ extern "C" {
 
 typedef struct frk_ctx_s* frk_ctx;
 typedef struct frk_blr_s* frk_blr;
+typedef struct frk_qrf_s* frk_qrf;
 
 const char* frk_last_error(void);
 int frk_version(void);
@@ -92,6 +95,12 @@ int frk_ctx_reset_op_stats(frk_ctx ctx);
 int frk_blr_create_kernel(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell,
                           double shift, int nb, double eps, int rank, double admis, int admis_type, int kcap,
                           int precision, frk_blr* out);
+/* Rectangular form: Hierarchical(kernel, params, n_rows, n_cols, nleaf, eps, admis, nb_r, nb_c, admis_type)
+ * (hierarchical.cpp:89-109); rows and columns index the same n_pts points. */
+int frk_blr_create_kernel_rect(frk_ctx ctx, int kernel_kind, int dim, int64_t n_pts, const double* pts,
+                               int64_t n_rows, int64_t n_cols, int nb_r, int nb_c, double ell, double shift,
+                               double eps, int rank, double admis, int admis_type, int kcap, int precision,
+                               frk_blr* out);
 /* Empty grid with given block offsets (nb+1 entries each) and tile kinds; fill with set_* below. */
 int frk_blr_create(frk_ctx ctx, int nb_r, int nb_c, const int64_t* roff, const int64_t* coff, const int32_t* kinds,
                    int kcap, double eps, int rank, int precision, frk_blr* out);
@@ -120,6 +129,21 @@ int frk_blr_convert(frk_blr src, int precision, frk_blr* out);
  * hold U, diagonal tiles hold the packed unit-L \ U factors.  Returns > 0
  * (1-based global row) when a zero pivot is met. */
 int frk_blr_getrf(frk_blr a);
+
+/* tiled_householder_blr_qr(A, T) (include/FRANK/operations/LAPACK.h:450,
+ * src/operations/LAPACK/qr.cpp:213-228): FP64, in place.  Afterwards the
+ * diagonal tiles hold R above / reflectors below the diagonal (geqrt layout),
+ * the tiles above the diagonal R, the tiles below the diagonal the reflector
+ * blocks of tpqrt (a low-rank block keeps U and gets S = I).  *out receives
+ * the triangular factors T(i, k), i >= k. */
+int frk_blr_qr(frk_blr a, frk_qrf* out);
+int frk_qrf_destroy(frk_qrf f);
+/* T(i, k) (b x b row-major) and the reflector scalars tau(i, k) (b) to host;
+ * either output may be NULL. */
+int frk_qrf_get_t(frk_qrf f, int i, int k, double* t_out, double* tau_out);
+/* left_multiply_tiled_reflector(Y, T, C, trans) (LAPACK.h:470-472, qr.cpp:230-257):
+ * C <- Q C (trans = 0) or Q^T C, with Y the factored matrix of frk_blr_qr. */
+int frk_blr_apply_q(frk_blr y, frk_qrf f, frk_blr c, int trans);
 
 /* trsm(L, b, Lower); trsm(U, b, Upper) (src/operations/BLAS/trsm.cpp:40-90,117-125)
  * on device b (N x nrhs row-major, FP64).  trans != 0 solves A^T x = b with

@@ -84,6 +84,11 @@ def random_uniform(rows: int, cols: int) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # Kernel generators
 # ---------------------------------------------------------------------------
+def identity(points, r0: int, nr: int, c0: int, nc: int) -> np.ndarray:
+    """src/functions.cpp:25-34: A_ij = 1 where the global row equals the global column."""
+    return (np.arange(r0, r0 + nr)[:, None] == np.arange(c0, c0 + nc)[None, :]).astype(np.float64)
+
+
 def laplacend(points: Sequence[np.ndarray], r0: int, nr: int, c0: int, nc: int) -> np.ndarray:
     """src/functions.cpp:92-109: A_ij = 1 / (sqrt(sum_k (x_k[i]-x_k[j])^2) + 1e-3)."""
     rij = np.zeros((nr, nc))
