@@ -21,7 +21,7 @@ FRK_ERR_UNSUPPORTED = -4
 
 TILE_DENSE, TILE_LOWRANK, TILE_EMPTY = 0, 1, 2
 LRA_NAIVE, LRA_ROUNDED, LRA_FAST = 0, 1, 2
-KERNEL_LAPLACE, KERNEL_GAUSSIAN, KERNEL_IDENTITY = 0, 1, 2
+KERNEL_LAPLACE, KERNEL_GAUSSIAN, KERNEL_IDENTITY, KERNEL_GAUSSIAN_DELL = 0, 1, 2, 3
 ADMIS_POSITION, ADMIS_GEOMETRY = 0, 1
 PREC_F64, PREC_F32 = 0, 1
 
@@ -55,6 +55,7 @@ _SIGS = {
     "frk_blr_create_kernel_rect": ([vp, C.c_int, C.c_int, i64, vp, i64, i64, C.c_int, C.c_int, dbl, dbl, dbl, C.c_int,
                                     dbl, C.c_int, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
     "frk_blr_qr": ([vp, C.POINTER(vp)], C.c_int),
+    "frk_kernel_apply": ([vp, C.c_int, C.c_int, i64, vp, dbl, dbl, vp, vp, C.c_int], C.c_int),
     "frk_qrf_destroy": ([vp], C.c_int),
     "frk_qrf_get_t": ([vp, C.c_int, C.c_int, vp, vp], C.c_int),
     "frk_blr_apply_q": ([vp, vp, vp, C.c_int], C.c_int),
@@ -84,6 +85,14 @@ _SIGS = {
     "frk_dgemm": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dbl, vp, i64, vp, i64, dbl, vp, i64], C.c_int),
     "frk_dlr_add_batched": ([vp, C.c_int, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp],
                             C.c_int),
+    "frk_slr_add_batched": ([vp, C.c_int, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp],
+                            C.c_int),
+    "frk_dqr_batched": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
+    "frk_drq_batched": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
+    "frk_dsvd_batched": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
+    "frk_dtruncated_geqp3_batched": ([vp, C.c_int, vp, vp, vp, vp, dbl, C.c_int, vp, vp, vp, vp, vp], C.c_int),
+    "frk_drsvd": ([vp, C.c_int, C.c_int, vp, i64, C.c_int, vp, i64, vp, i64, vp, i64], C.c_int),
+    "frk_did": ([vp, C.c_int, C.c_int, vp, i64, C.c_int, vp, i64, vp], C.c_int),
     "frk_dcompress": ([vp, C.c_int, C.c_int, vp, i64, dbl, C.c_int, C.c_int, vp, vp, vp, vp], C.c_int),
 }
 

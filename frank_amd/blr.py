@@ -371,10 +371,11 @@ def lr_add_batched(As: list, Bs: list, *, eps: float = 0.0, lra: str = "fast_rou
     def arr(xs):
         return (C.c_void_p * nb)(*[_dptr(x) for x in xs])
 
-    L.check(L.lib().frk_dlr_add_batched(ctx.h, _LRA[lra], nb, _hptr(m), _hptr(n), eps, cap,
-                                        arr([a[0] for a in As]), arr([a[1] for a in As]), arr([a[2] for a in As]),
-                                        _hptr(ranks), arr([b[0] for b in Bs]), arr([b[1] for b in Bs]),
-                                        arr([b[2] for b in Bs]), _hptr(kb)))
+    fn = L.lib().frk_slr_add_batched if As[0][0].dtype == torch.float32 else L.lib().frk_dlr_add_batched
+    L.check(fn(ctx.h, _LRA[lra], nb, _hptr(m), _hptr(n), eps, cap,
+               arr([a[0] for a in As]), arr([a[1] for a in As]), arr([a[2] for a in As]),
+               _hptr(ranks), arr([b[0] for b in Bs]), arr([b[1] for b in Bs]),
+               arr([b[2] for b in Bs]), _hptr(kb)))
     return [int(r) for r in ranks]
 
 

@@ -18,7 +18,7 @@ BUILD = HERE / "_build"
 LIB = HERE / "libfrank_amd.so"
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "engine.cpp", "qr.cpp", "capi.cpp", "cxx_surface.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "qr.cpp", "small.cpp", "capi.cpp", "cxx_surface.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -443,8 +443,8 @@ int frk_blr_destroy(frk_blr a) {
 int frk_blr_set_lra(frk_blr a, int lra) {
   return guard([&] {
     if (!a) throw std::invalid_argument("null matrix");
-    if (lra == FRK_LRA_NAIVE) throw std::runtime_error("operator+=(LowRank, LowRank) naive addition undefined! (GPU)");
-    if (lra != FRK_LRA_ROUNDED && lra != FRK_LRA_FAST) throw std::invalid_argument("unknown FRANK_LRA value");
+    if (lra != FRK_LRA_NAIVE && lra != FRK_LRA_ROUNDED && lra != FRK_LRA_FAST)
+      throw std::invalid_argument("unknown FRANK_LRA value");
     if (a->a64) a->a64->lra = lra;
     if (a->a32) a->a32->lra = lra;
     return FRK_OK;
@@ -600,6 +600,21 @@ int frk_blr_kernel_residual(frk_blr a, const double* x_dev, const double* b_dev,
   });
 }
 
+int frk_kernel_apply(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell, double shift,
+                     const double* x_dev, double* y_dev, int nrhs) {
+  return guard([&] {
+    if (!ctx || !pts || !x_dev || !y_dev || nrhs <= 0 || n <= 0) throw std::invalid_argument("bad argument");
+    if (dim < 1 || dim > 3) throw std::invalid_argument("dim must be 1, 2 or 3");
+    if (kernel_kind < 0 || kernel_kind > 3) throw std::invalid_argument("unknown kernel");
+    KernelDesc kd{kernel_kind, dim, n, pts, ell, shift};
+    DevBuf dpts;
+    const KernelSpec ks = upload_kernel(kd, dpts, ctx->stream);
+    launch_kernel_residual(ks, x_dev, nullptr, y_dev, n, nrhs, ctx->stream);
+    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return FRK_OK;
+  });
+}
+
 int frk_dgetrf_nopiv_batched(frk_ctx ctx, double* const* tiles_host, int batch, int n, int64_t ld,
                              int32_t* info_host) {
   return guard([&] {
@@ -677,65 +692,158 @@ int frk_dgemm(frk_ctx ctx, int trans_a, int trans_b, int m, int n, int k, double
   });
 }
 
+extern "C++" {
+namespace {
+template <typename T>
+void lr_add_batched_impl(frk_ctx ctx, int lra, int batch, const int32_t* m, const int32_t* n, double eps, int cap,
+                         T* const* Ua, T* const* Sa, T* const* Va, int32_t* ranks_host, const T* const* Ub,
+                         const T* const* Sb, const T* const* Vb, const int32_t* kb) {
+  if (!ctx || batch <= 0 || !m || !n || !Ua || !Sa || !Va || !ranks_host || !Ub || !Sb || !Vb || !kb)
+    throw std::invalid_argument("null argument");
+  if (lra != FRK_LRA_NAIVE && lra != FRK_LRA_ROUNDED && lra != FRK_LRA_FAST)
+    throw std::invalid_argument("unknown FRANK_LRA value");
+  Context& c = *ctx->ctx;
+  c.sync();
+  DevBuf dr(sizeof(int32_t) * batch);
+  FRK_HIP_CHECK(hipMemcpyAsync(dr.as<int32_t>(), ranks_host, sizeof(int32_t) * batch, hipMemcpyHostToDevice,
+                               ctx->stream));
+  std::vector<LrAddDesc<T>> d(batch);
+  size_t need = 4096;
+  int kmax = 0, mmax = 0, nmax = 0;
+  for (int b = 0; b < batch; ++b) {
+    if (m[b] <= 0 || n[b] <= 0 || m[b] > 512 || n[b] > 512) throw std::invalid_argument("tile dims must be in [1, 512]");
+    if (ranks_host[b] <= 0 || ranks_host[b] > cap || kb[b] <= 0) throw std::invalid_argument("bad ranks");
+    kmax = std::max(kmax, ranks_host[b] + kb[b]);
+    mmax = std::max(mmax, m[b]);
+    nmax = std::max(nmax, n[b]);
+  }
+  need += (size_t)batch * sizeof(T) * ((size_t)(mmax + nmax) * kmax + 6 * (size_t)kmax * kmax + 2 * kmax + 512);
+  c.reserve(need);
+  for (int b = 0; b < batch; ++b) {
+    LrAddDesc<T> x{};
+    x.cu = Ua[b];
+    x.cs = Sa[b];
+    x.cv = Va[b];
+    x.d_rank = dr.as<int32_t>() + b;
+    x.kc = ranks_host[b];
+    x.cu_rs = cap;
+    x.cu_cs = 1;
+    x.cv_rs = n[b];
+    x.cv_cs = 1;
+    x.lds = cap;
+    x.m = m[b];
+    x.n = n[b];
+    x.eps = eps;
+    x.cap = cap;
+    x.xu = Ub[b];
+    x.xu_rs = kb[b];
+    x.xu_cs = 1;
+    x.kx1 = kb[b];
+    x.xs = Sb[b];
+    x.xs_ld = kb[b];
+    x.kx2 = kb[b];
+    x.xv = Vb[b];
+    x.xv_rs = n[b];
+    x.xv_cs = 1;
+    d[b] = x;
+  }
+  lr_add_batched<T>(d, lra, c);
+  FRK_HIP_CHECK(hipMemcpyAsync(ranks_host, dr.as<int32_t>(), sizeof(int32_t) * batch, hipMemcpyDeviceToHost,
+                               ctx->stream));
+  c.sync();
+  for (int b = 0; b < batch; ++b)
+    if (ranks_host[b] < 0) throw std::runtime_error("low-rank capacity exceeded in addition");
+}
+}  // namespace
+}  // extern "C++"
+
 int frk_dlr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const int32_t* n, double eps, int cap,
                         double* const* Ua, double* const* Sa, double* const* Va, int32_t* ranks_host,
                         const double* const* Ub, const double* const* Sb, const double* const* Vb,
                         const int32_t* kb) {
   return guard([&] {
-    if (!ctx || batch <= 0 || !m || !n || !Ua || !Sa || !Va || !ranks_host || !Ub || !Sb || !Vb || !kb)
+    lr_add_batched_impl<double>(ctx, lra, batch, m, n, eps, cap, Ua, Sa, Va, ranks_host, Ub, Sb, Vb, kb);
+    return FRK_OK;
+  });
+}
+
+int frk_slr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const int32_t* n, double eps, int cap,
+                        float* const* Ua, float* const* Sa, float* const* Va, int32_t* ranks_host,
+                        const float* const* Ub, const float* const* Sb, const float* const* Vb,
+                        const int32_t* kb) {
+  return guard([&] {
+    lr_add_batched_impl<float>(ctx, lra, batch, m, n, eps, cap, Ua, Sa, Va, ranks_host, Ub, Sb, Vb, kb);
+    return FRK_OK;
+  });
+}
+
+int frk_dqr_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                    const int64_t* lda, double* const* Q, const int64_t* ldq, double* const* R, const int64_t* ldr) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || !m || !n || !A || !lda || !Q || !ldq || !R || !ldr) throw std::invalid_argument("null argument");
+    std::vector<SmallJob<double>> jobs(batch);
+    for (int b = 0; b < batch; ++b) jobs[b] = {A[b], lda[b], 1, Q[b], ldq[b], 1, R[b], ldr[b], 1, m[b], n[b]};
+    qr_small_batched<double>(jobs, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_drq_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                    const int64_t* lda, double* const* R, const int64_t* ldr, double* const* Q, const int64_t* ldq) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || !m || !n || !A || !lda || !Q || !ldq || !R || !ldr) throw std::invalid_argument("null argument");
+    std::vector<SmallJob<double>> jobs(batch);
+    for (int b = 0; b < batch; ++b) {
+      if (m[b] > 128 || n[b] > 512) throw std::invalid_argument("rq: m <= 128, n <= 512");
+      jobs[b] = {A[b], lda[b], 1, R[b], ldr[b], 1, Q[b], ldq[b], 1, m[b], n[b]};
+    }
+    rq_small_batched<double>(jobs, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_dsvd_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                     const int64_t* lda, double* const* U, const int64_t* ldu, double* const* S, const int64_t* lds,
+                     double* const* V, const int64_t* ldv) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || !m || !n || !A || !lda || !U || !ldu || !S || !lds || !V || !ldv)
       throw std::invalid_argument("null argument");
-    if (lra == FRK_LRA_NAIVE) throw std::runtime_error("operator+=(LowRank, LowRank) naive addition undefined! (GPU)");
-    Context& c = *ctx->ctx;
-    c.sync();
-    DevBuf dr(sizeof(int32_t) * batch);
-    FRK_HIP_CHECK(hipMemcpyAsync(dr.as<int32_t>(), ranks_host, sizeof(int32_t) * batch, hipMemcpyHostToDevice,
-                                 ctx->stream));
-    std::vector<LrAddDesc<double>> d(batch);
-    size_t need = 4096;
-    int kmax = 0, mmax = 0, nmax = 0;
-    for (int b = 0; b < batch; ++b) {
-      if (m[b] <= 0 || n[b] <= 0 || m[b] > 512 || n[b] > 512) throw std::invalid_argument("tile dims must be in [1, 512]");
-      if (ranks_host[b] <= 0 || ranks_host[b] > cap || kb[b] <= 0) throw std::invalid_argument("bad ranks");
-      kmax = std::max(kmax, ranks_host[b] + kb[b]);
-      mmax = std::max(mmax, m[b]);
-      nmax = std::max(nmax, n[b]);
-    }
-    need += (size_t)batch * sizeof(double) * ((size_t)(mmax + nmax) * kmax + 6 * (size_t)kmax * kmax + 2 * kmax + 512);
-    c.reserve(need);
-    for (int b = 0; b < batch; ++b) {
-      LrAddDesc<double> x{};
-      x.cu = Ua[b];
-      x.cs = Sa[b];
-      x.cv = Va[b];
-      x.d_rank = dr.as<int32_t>() + b;
-      x.kc = ranks_host[b];
-      x.cu_rs = cap;
-      x.cu_cs = 1;
-      x.cv_rs = n[b];
-      x.cv_cs = 1;
-      x.lds = cap;
-      x.m = m[b];
-      x.n = n[b];
-      x.eps = eps;
-      x.cap = cap;
-      x.xu = Ub[b];
-      x.xu_rs = kb[b];
-      x.xu_cs = 1;
-      x.kx1 = kb[b];
-      x.xs = Sb[b];
-      x.xs_ld = kb[b];
-      x.kx2 = kb[b];
-      x.xv = Vb[b];
-      x.xv_rs = n[b];
-      x.xv_cs = 1;
-      d[b] = x;
-    }
-    lr_add_batched<double>(d, lra, c);
-    FRK_HIP_CHECK(hipMemcpyAsync(ranks_host, dr.as<int32_t>(), sizeof(int32_t) * batch, hipMemcpyDeviceToHost,
-                                 ctx->stream));
-    c.sync();
+    std::vector<SvdJob<double>> jobs(batch);
+    for (int b = 0; b < batch; ++b) jobs[b] = {A[b], lda[b], 1, U[b], ldu[b], 1, S[b], lds[b], V[b], ldv[b], 1, m[b], n[b]};
+    svd_small_batched<double>(jobs, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_dtruncated_geqp3_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                                 const int64_t* lda, double eps, int cap, double* const* Q, const int64_t* ldq,
+                                 double* const* R, const int64_t* ldr, int32_t* ranks_host) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || !m || !n || !A || !lda || !Q || !ldq || !R || !ldr || !ranks_host || cap <= 0)
+      throw std::invalid_argument("bad argument");
+    if (eps < 0.0) throw std::invalid_argument("eps must be >= 0");
+    std::vector<Geqp3Job<double>> jobs(batch);
     for (int b = 0; b < batch; ++b)
-      if (ranks_host[b] < 0) throw std::runtime_error("low-rank capacity exceeded in addition");
+      jobs[b] = {A[b], lda[b], 1, m[b], n[b], eps, cap, Q[b], ldq[b], 1, R[b], ldr[b], ranks_host + b};
+    geqp3_small_batched<double>(jobs, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_drsvd(frk_ctx ctx, int m, int n, const double* A, int64_t lda, int p, double* U, int64_t ldu, double* S,
+              int64_t lds, double* V, int64_t ldv) {
+  return guard([&] {
+    if (!ctx || !A || !U || !S || !V) throw std::invalid_argument("null argument");
+    rsvd_small<double>(A, lda, m, n, p, U, ldu, S, lds, V, ldv, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_did(frk_ctx ctx, int m, int n, const double* A, int64_t lda, int k, double* V, int64_t ldv,
+            int32_t* cols_host) {
+  return guard([&] {
+    if (!ctx || !A || !V || !cols_host) throw std::invalid_argument("null argument");
+    id_small<double>(A, lda, 1, m, n, k, V, ldv, cols_host, *ctx->ctx);
     return FRK_OK;
   });
 }

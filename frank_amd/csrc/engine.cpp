@@ -703,9 +703,120 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
 template <typename T>
 void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx) {
   if (d.empty()) return;
+  if (lra == kLraNaive) {
+    lr_add_naive<T>(d, ctx);
+    return;
+  }
   Plan pl(ctx);
   plan_lr_add<T>(pl, d, lra);
   pl.run();
+}
+
+// naive_addition (addition.cpp:68-89): C <- [Uc | Ux] diag(Sc, Sx) [Vc; Vx]
+// (a rectangular Sx is folded into U: [Uc | Ux Sx], diag(Sc, I)); when
+// kc + kx >= min(m, n) the dense sum is recompressed at rank kc instead
+// (LowRank(Dense(A) + Dense(B), A.rank)).  Drains the stream.
+template <typename T>
+void lr_add_naive(const std::vector<LrAddDesc<T>>& d, Context& ctx) {
+  if (d.empty()) return;
+  const hipStream_t s = ctx.stream();
+  ctx.sync();
+  std::vector<int> cat, dns;
+  for (int i = 0; i < (int)d.size(); ++i)
+    (d[i].kc + d[i].kx2 >= std::min(d[i].m, d[i].n) ? dns : cat).push_back(i);
+  if (!cat.empty()) {
+    ctx.reserve(4096);
+    const T* I = ctx.identity<T>(std::max(1, d[0].cap));
+    const int ild = ctx.ident_ld();
+    Plan pl(ctx);
+    std::vector<GemmItem> cp, g1, zero;
+    std::vector<SetIntItem> sr;
+    for (int i : cat) {
+      const auto& x = d[i];
+      const int kc = x.kc, k1 = x.kx1, k2 = x.kx2;
+      if (kc + k2 > x.cap) throw std::runtime_error("naive addition: rank " + std::to_string(kc + k2) +
+                                                    " exceeds the low-rank capacity " + std::to_string(x.cap));
+      T* ublk = x.cu + (int64_t)kc * x.cu_cs;
+      if (x.xs != nullptr && k1 == k2) {
+        cp.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, nullptr, 0, 0, ublk, x.cu_rs, x.cu_cs, x.m, k1, 0, 0.0, 0.0));
+        cp.push_back(gemm_item(x.xs, x.xs_ld, 1, nullptr, 0, 0, x.cs + (int64_t)kc * x.lds + kc, x.lds, 1, k1, k2, 0,
+                               0.0, 0.0));
+      } else if (x.xs != nullptr) {
+        g1.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, x.xs, x.xs_ld, 1, ublk, x.cu_rs, x.cu_cs, x.m, k2, k1, 1.0, 0.0));
+        cp.push_back(gemm_item(I, ild, 1, nullptr, 0, 0, x.cs + (int64_t)kc * x.lds + kc, x.lds, 1, k2, k2, 0, 0.0,
+                               0.0));
+      } else {
+        cp.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, nullptr, 0, 0, ublk, x.cu_rs, x.cu_cs, x.m, k2, 0, 0.0, 0.0));
+        cp.push_back(gemm_item(I, ild, 1, nullptr, 0, 0, x.cs + (int64_t)kc * x.lds + kc, x.lds, 1, k2, k2, 0, 0.0,
+                               0.0));
+      }
+      // off-diagonal blocks of S are zero (k = 0 products write beta * C = 0)
+      zero.push_back(gemm_item(nullptr, 0, 0, nullptr, 0, 0, x.cs + kc, x.lds, 1, kc, k2, 0, 1.0, 0.0));
+      zero.push_back(gemm_item(nullptr, 0, 0, nullptr, 0, 0, x.cs + (int64_t)kc * x.lds, x.lds, 1, k2, kc, 0, 1.0, 0.0));
+      cp.push_back(gemm_item(x.xv, x.xv_rs, x.xv_cs, nullptr, 0, 0, x.cv + (int64_t)kc * x.cv_rs, x.cv_rs, x.cv_cs, k2,
+                             x.n, 0, 0.0, 0.0));
+      sr.push_back({x.d_rank, kc + k2, 0});
+    }
+    plan_copies<T>(pl, cp);
+    plan_gemms<T>(pl, g1, "lr_naive");
+    plan_gemms<T>(pl, zero, "lr_naive");
+    const size_t osr = pl.add(sr);
+    const int nsr = (int)sr.size();
+    pl.launch([=](const char* dd) { launch_set_int(reinterpret_cast<const SetIntItem*>(dd + osr), nsr, s); });
+    pl.run();
+    ctx.sync();
+  }
+  if (!dns.empty()) {
+    size_t need = 8192;
+    for (int i : dns) need += sizeof(T) * ((size_t)d[i].m * d[i].n + (size_t)d[i].m * (d[i].kc + d[i].kx2) + 256);
+    ctx.reserve(need);
+    Plan pl(ctx);
+    std::vector<GemmItem> g1, g2, g3;
+    std::vector<CompressDesc<T>> cd;
+    for (int i : dns) {
+      const auto& x = d[i];
+      T* W = ctx.take<T>((size_t)x.m * x.n + 1);
+      T* t1 = ctx.take<T>((size_t)x.m * x.kc + 1);
+      T* t2 = ctx.take<T>((size_t)x.m * x.kx2 + 1);
+      g1.push_back(gemm_item(x.cu, x.cu_rs, x.cu_cs, x.cs, x.lds, 1, t1, x.kc, 1, x.m, x.kc, x.kc, 1.0, 0.0));
+      if (x.xs != nullptr)
+        g1.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, x.xs, x.xs_ld, 1, t2, x.kx2, 1, x.m, x.kx2, x.kx1, 1.0, 0.0));
+      else
+        g1.push_back(gemm_item(x.xu, x.xu_rs, x.xu_cs, nullptr, 0, 0, t2, x.kx2, 1, x.m, x.kx2, 0, 0.0, 0.0));
+      g2.push_back(gemm_item(t1, x.kc, 1, x.cv, x.cv_rs, x.cv_cs, W, x.n, 1, x.m, x.n, x.kc, 1.0, 0.0));
+      g3.push_back(gemm_item(t2, x.kx2, 1, x.xv, x.xv_rs, x.xv_cs, W, x.n, 1, x.m, x.n, x.kx2, 1.0, 1.0));
+      CompressDesc<T> c{};
+      c.a = W;
+      c.m = x.m;
+      c.n = x.n;
+      c.lda = x.n;
+      c.eps = 0.0;
+      c.rank = x.kc;
+      c.u = x.cu;
+      c.s = x.cs;
+      c.v = x.cv;
+      c.u_rs = x.cu_rs;
+      c.u_cs = x.cu_cs;
+      c.v_rs = x.cv_rs;
+      c.v_cs = x.cv_cs;
+      c.lds = x.lds;
+      c.d_rank = x.d_rank;
+      c.cap = x.cap;
+      cd.push_back(c);
+    }
+    // the copies of a plain xu (no S) are GEMM items with k = 0: route them as copies
+    std::vector<GemmItem> c1, p1;
+    for (const auto& g : g1) (g.k == 0 && g.B == nullptr ? c1 : p1).push_back(g);
+    plan_copies<T>(pl, c1);
+    plan_gemms<T>(pl, p1, "lr_naive");
+    plan_gemms<T>(pl, g2, "lr_naive");
+    plan_gemms<T>(pl, g3, "lr_naive");
+    pl.run();
+    Context cside(s);
+    cside.set_stats_owner(&ctx);
+    compress_batched<T>(cd, cside);
+    ctx.sync();
+  }
 }
 
 // ============================================================================
@@ -1493,14 +1604,23 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         FRK_HIP_CHECK(
             hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
         for (size_t c = 0; c < comp.size(); ++c) adds[comp_add[c]].kx1 = adds[comp_add[c]].kx2 = cr[c];
-        Plan p2(ctx);
-        plan_lr_add<T>(p2, adds, A.lra);
-        p2.run();
+        if (A.lra == kLraNaive) {
+          lr_add_naive<T>(adds, ctx);
+        } else {
+          Plan p2(ctx);
+          plan_lr_add<T>(p2, adds, A.lra);
+          p2.run();
+        }
         lookahead();
         continue;
       }
-      plan_lr_add<T>(pl, adds, A.lra);
-      pl.run();
+      if (A.lra == kLraNaive) {
+        pl.run();
+        lr_add_naive<T>(adds, ctx);
+      } else {
+        plan_lr_add<T>(pl, adds, A.lra);
+        pl.run();
+      }
       lookahead();
     }
     pl.run();
@@ -1712,6 +1832,7 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
   template void plan_lr_add<T>(Plan&, const std::vector<LrAddDesc<T>>&, int);                               \
   template size_t lr_add_workspace<T>(const std::vector<LrAddDesc<T>>&);                                    \
   template void lr_add_batched<T>(const std::vector<LrAddDesc<T>>&, int, Context&);                         \
+  template void lr_add_naive<T>(const std::vector<LrAddDesc<T>>&, Context&);                                \
   template void compress_batched<T>(const std::vector<CompressDesc<T>>&, Context&);                         \
   template void plan_getrf<T>(Plan&, const std::vector<T*>&, const std::vector<T*>&, const std::vector<T*>&, \
                               int, int64_t, int32_t*);                                                      \

@@ -247,6 +247,9 @@ template <typename T>
 size_t lr_add_workspace(const std::vector<LrAddDesc<T>>& d);
 template <typename T>
 void lr_add_batched(const std::vector<LrAddDesc<T>>& d, int lra, Context& ctx);
+// naive_addition: concatenation, dense recompression past min(m, n).  Synchronous.
+template <typename T>
+void lr_add_naive(const std::vector<LrAddDesc<T>>& d, Context& ctx);
 
 template <typename T>
 struct CompressDesc {
@@ -400,6 +403,65 @@ template <typename T>
 size_t lr_add_bound(int nd, int m, int n, int K) {
   return (size_t)nd * sizeof(T) * ((size_t)(m + n) * K + 2 * (size_t)K + 4 * (size_t)K * K) + 8 * 256 + 4096;
 }
+
+// ----------------------------------------------------------------------------
+// Batched small factorizations of the operator surface (small.cpp).
+// ----------------------------------------------------------------------------
+template <typename T>
+struct SmallJob {  // A (m x n, strided) -> o1, o2 (strided)
+  const T* a;
+  int64_t a_rs, a_cs;
+  T* o1;
+  int64_t o1_rs, o1_cs;
+  T* o2;
+  int64_t o2_rs, o2_cs;
+  int m, n;
+};
+template <typename T>
+struct SvdJob {
+  const T* a;
+  int64_t a_rs, a_cs;
+  T* u;
+  int64_t u_rs, u_cs;
+  T* s;
+  int64_t s_ld;
+  T* v;
+  int64_t v_rs, v_cs;
+  int m, n;
+};
+template <typename T>
+struct Geqp3Job {
+  const T* a;
+  int64_t a_rs, a_cs;
+  int m, n;
+  double eps;
+  int cap;
+  T* q;
+  int64_t q_rs, q_cs;
+  T* r;
+  int64_t r_ld;
+  int32_t* rank_host;
+};
+// qr(A, Q, R): o1 = Q (m x k), o2 = R (k x n), k = min(m, n); m <= 512, n <= 128.
+template <typename T>
+void qr_small_batched(const std::vector<SmallJob<T>>& jobs, Context& ctx);
+// rq(A, R, Q): o1 = R (m x m), o2 = Q (m x n); m <= n, m <= 128, n <= 512.
+template <typename T>
+void rq_small_batched(const std::vector<SmallJob<T>>& jobs, Context& ctx);
+// svd(A): U (m x K), S (K x K diagonal, descending), V (K x n); K = min(m, n) <= 64.
+template <typename T>
+void svd_small_batched(const std::vector<SvdJob<T>>& jobs, Context& ctx);
+// truncated_geqp3(A, eps): Q (m x r), R (r x n), r <= cap.
+template <typename T>
+void geqp3_small_batched(const std::vector<Geqp3Job<T>>& jobs, Context& ctx);
+// rsvd(A, p): U (m x p), S (p x p), V (p x n), all row-major.
+template <typename T>
+void rsvd_small(const T* A, int64_t lda, int m, int n, int p, T* U, int64_t ldu, T* S, int64_t lds, T* V, int64_t ldv,
+                Context& ctx);
+// one_sided_id(A, k): V (k x n, ld ldv), the k selected columns to cols_host.
+template <typename T>
+void id_small(const T* A, int64_t a_rs, int64_t a_cs, int m, int n, int k, T* V, int64_t ldv, int32_t* cols_host,
+              Context& ctx);
 
 // ----------------------------------------------------------------------------
 // Tiled Householder BLR-QR (src/operations/LAPACK/qr.cpp:213-257).

@@ -1514,8 +1514,11 @@ __global__ __launch_bounds__(512) void geqp3_kernel(const Geqp3Item* __restrict_
     }
     return;
   }
+  // eps < 0: exactly min(cap, min_dim) pivot steps (dgeqp3 prefix, used by the ID)
+  const bool fixed = it.eps < 0.0;
+  const int steps = fixed ? min(it.cap, min_dim) : min_dim;
   int r = 0;
-  while (r < min_dim && maxv > threshold) {
+  while (r < steps && (fixed || maxv > threshold)) {
     // pivot: first maximum of cn[r:] (computed at the end of the previous step)
     const int k = kmax;
     if (k != r) {
@@ -1607,7 +1610,7 @@ __global__ __launch_bounds__(512) void geqp3_kernel(const Geqp3Item* __restrict_
     }
   }
   for (int c = tid; c < n; c += 512) it.piv[c] = piv[c];
-  if (tid == 0) *it.rank_out = (r > it.cap) ? -r : r;
+  if (tid == 0) *it.rank_out = (!fixed && r > it.cap) ? -r : r;
 }
 
 template <typename T>
@@ -1682,8 +1685,10 @@ __global__ __launch_bounds__(256) void gen_kernel(KernelSpec ks, const GenItem* 
       v = 1.0 / (sqrt(rij) + 1e-3);
     } else if (ks.kind == 1) {
       v = exp(-rij / ((2.0 * ks.ell) * ks.ell));
-    } else {  // identity (src/functions.cpp:25-34)
+    } else if (ks.kind == 2) {  // identity (src/functions.cpp:25-34)
       v = (gi == gj) ? 1.0 : 0.0;
+    } else {  // d/d ell of the gaussian kernel
+      v = exp(-rij / ((2.0 * ks.ell) * ks.ell)) * (rij / (ks.ell * ks.ell * ks.ell));
     }
     if (gi == gj) v = v + ks.shift;
     out[(int64_t)i * it.ld + j] = static_cast<T>(v);
@@ -1698,6 +1703,17 @@ void launch_gen(const KernelSpec& ks, const GenItem* d_items, int n_items, int m
   int gx = (int)std::min<int64_t>((total + 255) / 256, 1024);
   if (gx < 1) gx = 1;
   hipLaunchKernelGGL(gen_kernel<T>, dim3(gx, n_items), dim3(256), 0, s, ks, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void set_int_kernel(const SetIntItem* __restrict__ items, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) *items[i].p = items[i].v;
+}
+
+void launch_set_int(const SetIntItem* d_items, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(set_int_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_items, n);
   FRK_HIP_CHECK(hipGetLastError());
 }
 
@@ -1773,10 +1789,15 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(KernelSpec ks, co
           rij = rij + dd * dd;
         }
         double a;
-        if (ks.kind == 0)
+        if (ks.kind == 0) {
           a = 1.0 / (sqrt(rij) + 1e-3);
-        else
+        } else if (ks.kind == 1) {
           a = exp(-rij / ((2.0 * ks.ell) * ks.ell));
+        } else if (ks.kind == 2) {
+          a = (jb + jj == i) ? 1.0 : 0.0;
+        } else {  // d/d ell of the gaussian kernel: exp(-r^2 / (2 ell^2)) r^2 / ell^3
+          a = exp(-rij / ((2.0 * ks.ell) * ks.ell)) * (rij / (ks.ell * ks.ell * ks.ell));
+        }
         if (jb + jj == i) a = a + ks.shift;
         for (int c = 0; c < nrhs; ++c) acc[c] = fma(a, vs[jj][c], acc[c]);
       }
@@ -1792,7 +1813,7 @@ __global__ void residual_reduce_kernel(const double* __restrict__ part, const do
   if (e >= n * nrhs) return;
   double s = 0.0;
   for (int sp = 0; sp < kResSplits; ++sp) s += part[(int64_t)sp * n * nrhs + e];
-  R[e] = B[e] - s;
+  R[e] = (B != nullptr) ? B[e] - s : s;
 }
 
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R, int64_t n,

@@ -80,7 +80,7 @@ struct KernelSpec {
   const double* pts;  // dim arrays of N points, structure of arrays
   int64_t n_pts;
   int32_t dim;
-  int32_t kind;       // 0 laplace, 1 gaussian, 2 identity
+  int32_t kind;       // 0 laplace, 1 gaussian, 2 identity, 3 d/d ell of the gaussian
   double ell;         // gaussian length scale
   double shift;       // added on the global diagonal
 };
@@ -88,7 +88,16 @@ template <typename T>
 void launch_gen(const KernelSpec& ks, const GenItem* d_items, int n_items, int max_nr, int max_nc,
                 hipStream_t s);
 
-// Matrix-free residual R = B - A X for a kernel matrix (FP64), X/B/R row-major N x nrhs.
+// *items[i].p = items[i].v (rank updates written from the host plan).
+struct SetIntItem {
+  int32_t* p;
+  int32_t v;
+  int32_t pad;
+};
+void launch_set_int(const SetIntItem* d_items, int n, hipStream_t s);
+
+// Matrix-free residual R = B - A X for a kernel matrix (FP64), X/B/R row-major N x nrhs;
+// with B == nullptr, R = A X.
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R,
                             int64_t n, int nrhs, hipStream_t s);
 

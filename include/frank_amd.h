@@ -42,6 +42,8 @@ extern "C" {
 #define FRK_KERNEL_LAPLACE 0
 #define FRK_KERNEL_GAUSSIAN 1
 #define FRK_KERNEL_IDENTITY 2
+/* d/d ell of the gaussian kernel, exp(-r^2 / (2 ell^2)) r^2 / ell^3 (adjoint; no reference counterpart) */
+#define FRK_KERNEL_GAUSSIAN_DELL 3
 
 /* Admissibility (include/FRANK/definitions.h:25 AdmisType) */
 #define FRK_ADMIS_POSITION 0
@@ -156,6 +158,12 @@ int frk_blr_matvec(frk_blr a, const double* x_dev, double* y_dev, int nrhs);
 /* Matrix-free FP64 residual r = b - A x for the kernel matrix the BLR matrix
  * was built from (exact, uncompressed).  Device vectors, N x nrhs row-major. */
 int frk_blr_kernel_residual(frk_blr a, const double* x_dev, const double* b_dev, double* r_dev, int nrhs);
+/* y = K x for the exact kernel matrix K of n host points (dim x n, structure of
+ * arrays), matrix-free, FP64 device vectors (n x nrhs row-major, nrhs <= 16).
+ * With FRK_KERNEL_GAUSSIAN_DELL this is (dA/d ell) x, the contraction the
+ * adjoint of the solve needs (no reference counterpart). */
+int frk_kernel_apply(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const double* pts, double ell, double shift,
+                     const double* x_dev, double* y_dev, int nrhs);
 
 /* Solve with iterative refinement (no reference counterpart): x = F^-1 b, then
  * x += F^-1 (b - A x) with the exact FP64 kernel matrix, until the normwise
@@ -190,11 +198,48 @@ int frk_dgemm(frk_ctx ctx, int trans_a, int trans_b, int m, int n, int k, double
  * Va cap x n row-major, device), B_b = Ub Sb Vb (rank kb).  A is updated in
  * place; ranks_host[b] holds ka on entry and the new rank on exit.
  * eps != 0 selects the rounded addition with the truncation rule of
- * src/operations/misc/misc.cpp:133-146; eps == 0 keeps ka (lra selects rounded or fast). */
+ * src/operations/misc/misc.cpp:133-146; eps == 0 keeps ka (lra selects rounded or fast);
+ * lra == FRK_LRA_NAIVE concatenates (addition.cpp:68-89). */
 int frk_dlr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const int32_t* n, double eps, int cap,
                         double* const* Ua, double* const* Sa, double* const* Va, int32_t* ranks_host,
                         const double* const* Ub, const double* const* Sb, const double* const* Vb,
                         const int32_t* kb);
+/* The same in FP32 storage and arithmetic (no reference counterpart: the
+ * reference is FP64 only). */
+int frk_slr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const int32_t* n, double eps, int cap,
+                        float* const* Ua, float* const* Sa, float* const* Va, int32_t* ranks_host,
+                        const float* const* Ub, const float* const* Sb, const float* const* Vb,
+                        const int32_t* kb);
+
+/* ---------------------------------------------------------------------------
+ * Batched small dense factorizations (operator surface).  Device pointers,
+ * row-major with leading dimensions; one entry per problem.
+ * ------------------------------------------------------------------------- */
+/* qr(A, Q, R) (include/FRANK/operations/LAPACK.h:232, src/operations/LAPACK/qr.cpp:65-90):
+ * Q m x k, R k x n, k = min(m, n); m <= 512, n <= 128. */
+int frk_dqr_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                    const int64_t* lda, double* const* Q, const int64_t* ldq, double* const* R, const int64_t* ldr);
+/* rq(A, R, Q) (LAPACK.h:488, rq.cpp:43-70): R m x m upper, Q m x n; m <= min(n, 128), n <= 512. */
+int frk_drq_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                    const int64_t* lda, double* const* R, const int64_t* ldr, double* const* Q, const int64_t* ldq);
+/* svd(A) (LAPACK.h:99, svd.cpp:23-44): U m x K, S K x K diagonal descending, V K x n;
+ * K = min(m, n) <= 64, max(m, n) <= 512. */
+int frk_dsvd_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                     const int64_t* lda, double* const* U, const int64_t* ldu, double* const* S, const int64_t* lds,
+                     double* const* V, const int64_t* ldv);
+/* truncated_geqp3(A, eps) (LAPACK.h:170, geqp3.cpp:71-191): Q m x r, R r x n (columns in the
+ * original order), r <= cap written to ranks_host[b]. */
+int frk_dtruncated_geqp3_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
+                                 const int64_t* lda, double eps, int cap, double* const* Q, const int64_t* ldq,
+                                 double* const* R, const int64_t* ldr, int32_t* ranks_host);
+/* rsvd(A, p) (include/FRANK/operations/randomized_factorizations.h:110, rsvd.cpp:17-29),
+ * Omega from std::mt19937(0): U m x p, S p x p, V p x n; p <= min(m, n, 64). */
+int frk_drsvd(frk_ctx ctx, int m, int n, const double* A, int64_t lda, int p, double* U, int64_t ldu, double* S,
+              int64_t lds, double* V, int64_t ldv);
+/* one_sided_id(A, k) (LAPACK.h:61, id.cpp:47-72): V k x n with A ~ A[:, cols] V, the k
+ * selected columns (0-based) to cols_host. */
+int frk_did(frk_ctx ctx, int m, int n, const double* A, int64_t lda, int k, double* V, int64_t ldv,
+            int32_t* cols_host);
 
 /* LowRank(Dense, eps) (src/classes/low_rank.cpp:50-59, truncated_geqp3 + rq)
  * when eps != 0, else LowRank(Dense, rank) (low_rank.cpp:40-48, rsvd).

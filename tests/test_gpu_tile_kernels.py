@@ -195,3 +195,41 @@ def test_lr_add_batched_uneven(torch_gpu, fa):
         assert r == ref.rank
         prod = (a[0][:, :r].cpu().numpy() @ a[1][:r, :r].cpu().numpy()) @ a[2][:r].cpu().numpy()
         np.testing.assert_allclose(prod, ref.dense(), rtol=0, atol=1e-12 * np.abs(dd).max())
+
+
+@pytest.mark.parametrize("ka,kb", [(4, 4), (8, 16), (40, 30)])
+def test_naive_addition(torch_gpu, fa, ka, kb):
+    """naive_addition (addition.cpp:68-89): concatenation below min(m, n), dense
+    recompression at A's rank (rsvd) at or above it."""
+    torch = torch_gpu
+    d = _separated_tile(64, 64)
+    A = fo.LowRank.from_rank(d, ka)
+    B = fo.LowRank.from_rank(0.5 * d, kb)
+    ref = fo.naive_addition(A, B)
+    cap = 80
+    sa = _lr_slot(torch, A, cap)
+    ranks = fa.lr_add_batched([sa], [(dev(torch, B.U), dev(torch, B.S), dev(torch, B.V))], eps=0.0, lra="naive",
+                              cap=cap)
+    assert ranks == [ref.rank]
+    k = ranks[0]
+    prod = (sa[0][:, :k].cpu().numpy() @ sa[1][:k, :k].cpu().numpy()) @ sa[2][:k].cpu().numpy()
+    np.testing.assert_allclose(prod, ref.dense(), rtol=0, atol=1e-10 * np.abs(d).max())
+
+
+@pytest.mark.parametrize("eps", [1e-4, 1e-6])
+def test_rounded_addition_fp32(torch_gpu, fa, eps):
+    """FP32 rounded addition against the FP64 oracle: error within eps + FP32 rounding."""
+    torch = torch_gpu
+    d = _separated_tile(128, 96)
+    d2 = 0.25 * _separated_tile(128, 96)[::-1, ::-1].copy()
+    A = fo.LowRank.from_eps(d, eps)
+    B = fo.LowRank.from_eps(d2, eps)
+    ref = fo.rounded_addition(A, B)
+    cap = 48
+    sa = tuple(x.float() if hasattr(x, "float") else x for x in _lr_slot(torch, A, cap)[:3]) + (A.rank,)
+    Bt = (dev(torch, B.U).float(), dev(torch, B.S).float(), dev(torch, B.V).float())
+    ranks = fa.lr_add_batched([sa], [Bt], eps=eps, lra="rounded_addition", cap=cap)
+    k = ranks[0]
+    assert abs(k - ref.rank) <= 1
+    prod = (sa[0][:, :k].double().cpu().numpy() @ sa[1][:k, :k].double().cpu().numpy()) @ sa[2][:k].double().cpu().numpy()
+    assert fo.l2_error(d + d2, prod) <= 10 * eps + 1e-5

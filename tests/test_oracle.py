@@ -336,3 +336,29 @@ def test_index_split_and_admissibility():
     assert fo.is_admissible((0, 4), (8, 4), 0, 2, 1.0, "position", [])
     assert not fo.is_admissible((0, 4), (4, 4), 0, 1, 1.0, "position", [])
     assert not fo.is_admissible((0, 1), (8, 4), 0, 2, 0.0, "position", [])  # vectors never admissible
+
+
+def test_morton_sort_matches_bitwise_box_order():
+    """Morton order (morton_index.cpp:75-98) against an independent bit interleave."""
+    from frank_amd.inputs import morton_sort, uniform_points
+    pts = uniform_points(300, 2)
+    level = 4
+    xs, perm = morton_sort(pts, level)
+    assert sorted(perm.tolist()) == list(range(300))
+    lo = pts.min(axis=1) - 0.5
+    size = (pts.max(axis=1) - pts.min(axis=1)).max() + 1.0
+
+    def box(p):
+        cell = [int((p[d] - lo[d]) / size * (1 << level)) for d in range(2)]
+        bits = ""
+        for lev in reversed(range(level)):
+            for d in reversed(range(2)):
+                bits += str((cell[d] >> lev) & 1)
+        return int(bits, 2)
+
+    keys = [box(xs[:, i]) for i in range(300)]
+    assert keys == sorted(keys)
+    # stable: equal keys keep the input order
+    for a, b in zip(range(299), range(1, 300)):
+        if keys[a] == keys[b]:
+            assert perm[a] < perm[b]
