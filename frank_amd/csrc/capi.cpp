@@ -782,7 +782,12 @@ int frk_dqr_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, 
   return guard([&] {
     if (!ctx || batch <= 0 || !m || !n || !A || !lda || !Q || !ldq || !R || !ldr) throw std::invalid_argument("null argument");
     std::vector<SmallJob<double>> jobs(batch);
-    for (int b = 0; b < batch; ++b) jobs[b] = {A[b], lda[b], 1, Q[b], ldq[b], 1, R[b], ldr[b], 1, m[b], n[b]};
+    for (int b = 0; b < batch; ++b) {
+      const int k = std::min(m[b], n[b]);
+      if (!A[b] || !Q[b] || !R[b] || lda[b] < n[b] || ldq[b] < k || ldr[b] < n[b])
+        throw std::invalid_argument("qr: null pointer or leading dimension below the row length");
+      jobs[b] = {A[b], lda[b], 1, Q[b], ldq[b], 1, R[b], ldr[b], 1, m[b], n[b]};
+    }
     qr_small_batched<double>(jobs, *ctx->ctx);
     return FRK_OK;
   });
@@ -794,7 +799,9 @@ int frk_drq_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, 
     if (!ctx || batch <= 0 || !m || !n || !A || !lda || !Q || !ldq || !R || !ldr) throw std::invalid_argument("null argument");
     std::vector<SmallJob<double>> jobs(batch);
     for (int b = 0; b < batch; ++b) {
-      if (m[b] > 128 || n[b] > 512) throw std::invalid_argument("rq: m <= 128, n <= 512");
+      if (m[b] < 1 || m[b] > 128 || n[b] > 512 || m[b] > n[b]) throw std::invalid_argument("rq: 1 <= m <= min(n, 128), n <= 512");
+      if (!A[b] || !Q[b] || !R[b] || lda[b] < n[b] || ldr[b] < m[b] || ldq[b] < n[b])
+        throw std::invalid_argument("rq: null pointer or leading dimension below the row length");
       jobs[b] = {A[b], lda[b], 1, R[b], ldr[b], 1, Q[b], ldq[b], 1, m[b], n[b]};
     }
     rq_small_batched<double>(jobs, *ctx->ctx);
@@ -809,7 +816,13 @@ int frk_dsvd_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n,
     if (!ctx || batch <= 0 || !m || !n || !A || !lda || !U || !ldu || !S || !lds || !V || !ldv)
       throw std::invalid_argument("null argument");
     std::vector<SvdJob<double>> jobs(batch);
-    for (int b = 0; b < batch; ++b) jobs[b] = {A[b], lda[b], 1, U[b], ldu[b], 1, S[b], lds[b], V[b], ldv[b], 1, m[b], n[b]};
+    for (int b = 0; b < batch; ++b) {
+      const int K = std::min(m[b], n[b]);
+      if (m[b] < 1 || n[b] < 1 || !A[b] || !U[b] || !S[b] || !V[b] || lda[b] < n[b] || ldu[b] < K || lds[b] < K ||
+          ldv[b] < n[b])
+        throw std::invalid_argument("svd: bad shape, null pointer or leading dimension below the row length");
+      jobs[b] = {A[b], lda[b], 1, U[b], ldu[b], 1, S[b], lds[b], V[b], ldv[b], 1, m[b], n[b]};
+    }
     svd_small_batched<double>(jobs, *ctx->ctx);
     return FRK_OK;
   });
@@ -823,8 +836,12 @@ int frk_dtruncated_geqp3_batched(frk_ctx ctx, int batch, const int32_t* m, const
       throw std::invalid_argument("bad argument");
     if (eps < 0.0) throw std::invalid_argument("eps must be >= 0");
     std::vector<Geqp3Job<double>> jobs(batch);
-    for (int b = 0; b < batch; ++b)
+    if (cap > 128) throw std::invalid_argument("truncated_geqp3: cap <= 128");
+    for (int b = 0; b < batch; ++b) {
+      if (!A[b] || !Q[b] || !R[b] || lda[b] < n[b] || ldq[b] < cap || ldr[b] < n[b])
+        throw std::invalid_argument("truncated_geqp3: null pointer or leading dimension below the row length");
       jobs[b] = {A[b], lda[b], 1, m[b], n[b], eps, cap, Q[b], ldq[b], 1, R[b], ldr[b], ranks_host + b};
+    }
     geqp3_small_batched<double>(jobs, *ctx->ctx);
     return FRK_OK;
   });
@@ -834,6 +851,7 @@ int frk_drsvd(frk_ctx ctx, int m, int n, const double* A, int64_t lda, int p, do
               int64_t lds, double* V, int64_t ldv) {
   return guard([&] {
     if (!ctx || !A || !U || !S || !V) throw std::invalid_argument("null argument");
+    if (lda < n || ldu < p || lds < p || ldv < n) throw std::invalid_argument("rsvd: leading dimension below the row length");
     rsvd_small<double>(A, lda, m, n, p, U, ldu, S, lds, V, ldv, *ctx->ctx);
     return FRK_OK;
   });
@@ -843,6 +861,7 @@ int frk_did(frk_ctx ctx, int m, int n, const double* A, int64_t lda, int k, doub
             int32_t* cols_host) {
   return guard([&] {
     if (!ctx || !A || !V || !cols_host) throw std::invalid_argument("null argument");
+    if (lda < n || ldv < n) throw std::invalid_argument("id: leading dimension below the row length");
     id_small<double>(A, lda, 1, m, n, k, V, ldv, cols_host, *ctx->ctx);
     return FRK_OK;
   });

@@ -165,6 +165,7 @@ struct SvdItem {
   void* S;
   void* Um;
   void* Vm;
+  void* vwork;         // K x K scratch for the right vectors when they do not fit LDS (FP64, K > 64)
   int32_t* rank_out;
   int32_t* stats;      // optional: [0] += 1 problem, [1] += sweeps, [2] = max sweeps
   int64_t m_rs, m_cs;  // mode 0 input strides

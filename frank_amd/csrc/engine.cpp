@@ -472,7 +472,8 @@ size_t lr_add_workspace(const std::vector<LrAddDesc<T>>& d) {
     nmax = std::max(nmax, x.n);
   }
   const size_t K = kmax;
-  return d.size() * sizeof(T) * ((size_t)(mmax + nmax) * K + 2 * K + 4 * K * K) + 8 * 256 + 4096;
+  const size_t vw = (sizeof(T) == 8 && K > 64) ? K * K : 0;  // Jacobi right-vector scratch
+  return d.size() * sizeof(T) * ((size_t)(mmax + nmax) * K + 2 * K + 4 * K * K + vw) + 8 * 256 + 4096;
 }
 
 template <typename T>
@@ -496,6 +497,8 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   T* Rv = ctx.take<T>((size_t)nd * K * K);
   T* Um = ctx.take<T>((size_t)nd * K * K);
   T* Vm = ctx.take<T>((size_t)nd * K * K);
+  // FP64 cores above 64 keep their right vectors in global scratch
+  T* Vw = (sizeof(T) == 8 && K > 64) ? ctx.take<T>((size_t)nd * K * K) : nullptr;
   std::vector<QrItem> qi;
   qi.reserve(2 * nd);
   std::vector<SvdItem> si(nd);
@@ -605,6 +608,7 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     sv.rank_out = x.d_rank;
     sv.cap = x.cap;
     sv.stats = ctx.counters();
+    sv.vwork = Vw ? Vw + (size_t)t * K * K : nullptr;
     si[t] = sv;
     ApplyQItem a{};
     a.Y = yu;
@@ -987,7 +991,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
     const T* I = ctx.identity<T>(pmax + 1);
     const int ild = ctx.ident_ld();
     ctx.reserve((size_t)nd * sizeof(T) *
-                    ((size_t)mmax * pmax * 2 + (size_t)nmax * pmax + 4 * (size_t)pmax * pmax + 4 * pmax + 64) +
+                    ((size_t)mmax * pmax * 2 + (size_t)nmax * pmax + 5 * (size_t)pmax * pmax + 4 * pmax + 64) +
                 (size_t)16 * 4096);
     std::vector<GemmItem> g1, g2;
     std::vector<QrItem> q1, q2;
@@ -1002,6 +1006,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
     T* Rz = ctx.take<T>((size_t)nd * pmax * pmax);
     T* Um = ctx.take<T>((size_t)nd * pmax * pmax);
     T* Vm = ctx.take<T>((size_t)nd * pmax * pmax);
+    T* Vw = (sizeof(T) == 8 && pmax > 64) ? ctx.take<T>((size_t)nd * pmax * pmax) : nullptr;
     for (int t = 0; t < nd; ++t) {
       const auto& x = dr[t];
       const int p = std::min(std::min(x.rank + 5, x.m), x.n);
@@ -1069,6 +1074,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       s0.rank_out = x.d_rank;
       s0.cap = x.cap;
       s0.stats = ctx.counters();
+      s0.vwork = Vw ? Vw + (size_t)t * pmax * pmax : nullptr;
       sv.push_back(s0);
       ApplyQItem b{};
       b.Y = y;
@@ -1383,7 +1389,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
             need += sizeof(T) * (bm * kmax + (size_t)kmax * kmax + 512);
           if (kc == kLowRank) {
             const size_t K = (size_t)kmax + ((kl == kDense && kb == kDense) ? (size_t)A.kcap : (size_t)kmax);
-            need += sizeof(T) * (2 * bm * K + 2 * K + 4 * K * K) + 8 * 256;
+            need += sizeof(T) * (2 * bm * K + 2 * K + 5 * K * K) + 8 * 256;
           }
         }
     }

@@ -1156,12 +1156,22 @@ __device__ __forceinline__ int rr_pos(int pos, int r, int k2) {
   return pos == 0 ? 0 : ((pos - 1 + r) % (k2 - 1)) + 1;
 }
 
-template <typename T, int KMAX, int NT>
+template <typename T, int KMAX, int NT, bool VG>
 __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ items) {
   const SvdItem it = items[blockIdx.x];
   const int K = it.K;
   __shared__ T G[KMAX][KMAX + 1];
-  __shared__ T Vs[KMAX][KMAX + 1];
+  // right vectors: LDS, or a K x K global scratch (column j at vg + j * K) when they do not fit
+  __shared__ T Vs[VG ? 1 : KMAX][KMAX + 1];
+  __shared__ T coef[KMAX];
+  __shared__ int accept_sh;
+  T* vg = static_cast<T*>(it.vwork);
+  auto VA = [&](int j, int row) -> T& {
+    if constexpr (VG)
+      return vg[(int64_t)j * K + row];
+    else
+      return Vs[j][row];
+  };
   __shared__ T sig[KMAX];
   __shared__ int order[KMAX];
   __shared__ int rank_sh;
@@ -1170,11 +1180,17 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
     if (tid == 0 && it.rank_out) *it.rank_out = 0;
     return;
   }
+  if constexpr (VG) {
+    if (vg == nullptr) {  // missing scratch: report an impossible rank instead of touching memory
+      if (tid == 0 && it.rank_out) *it.rank_out = -(1 << 20);
+      return;
+    }
+  }
   if (it.mode == 0) {
     for (int e = tid; e < K * K; e += NT) {
       const int i = e / K, j = e % K;
       G[j][i] = static_cast<const T*>(it.M)[i * it.m_rs + j * it.m_cs];
-      Vs[j][i] = (i == j) ? T(1) : T(0);
+      VA(j, i) = (i == j) ? T(1) : T(0);
     }
   } else {
     // stage Ru (K x J) into G and Rv (K x J) into Vs with independent loads
@@ -1184,7 +1200,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
     for (int e = tid; e < K * J; e += NT) {
       const int i = e / J, t = e % J;
       G[t][i] = Ru[i * it.ldr + t];
-      Vs[t][i] = Rv[i * it.ldr + t];
+      VA(t, i) = Rv[i * it.ldr + t];
     }
     __syncthreads();
     constexpr int PER = (KMAX * KMAX + NT - 1) / NT;
@@ -1195,7 +1211,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
       T sacc = T(0);
       if (e < K * K) {
         const int i = e / K, j = e % K;
-        for (int t = 0; t < J; ++t) sacc += G[t][i] * Vs[t][j];
+        for (int t = 0; t < J; ++t) sacc += G[t][i] * VA(t, j);
         if (i < it.ka && j < it.ka) sacc += static_cast<const T*>(it.Sa)[i * it.lds_in + j];
       }
       mv[u] = sacc;
@@ -1207,7 +1223,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
       if (e < K * K) {
         const int i = e / K, j = e % K;
         G[j][i] = mv[u];
-        Vs[j][i] = (i == j) ? T(1) : T(0);
+        VA(j, i) = (i == j) ? T(1) : T(0);
       }
     }
   }
@@ -1291,9 +1307,9 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
                 const T x = G[i][row], y = G[j][row];
                 G[i][row] = cth * x - sth * y;
                 G[j][row] = sth * x + cth * y;
-                const T vx = Vs[i][row], vy = Vs[j][row];
-                Vs[i][row] = cth * vx - sth * vy;
-                Vs[j][row] = sth * vx + cth * vy;
+                const T vx = VA(i, row), vy = VA(j, row);
+                VA(i, row) = cth * vx - sth * vy;
+                VA(j, row) = sth * vx + cth * vy;
               }
               rotated = 1;
             }
@@ -1363,30 +1379,49 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
     if (s > T(0)) G[c][row] = G[c][row] / s;
   }
   __syncthreads();
-  if (tid == 0) {
-    // orthonormal completion for exactly-zero singular values among the kept ones
-    for (int c = 0; c < r; ++c) {
-      const int src = order[c];
-      if (sig[src] > T(0)) continue;
-      for (int u = 0; u < K; ++u) {
-        for (int row = 0; row < K; ++row) G[src][row] = (row == u) ? T(1) : T(0);
-        for (int pc = 0; pc < c; ++pc) {
-          const int ps = order[pc];
+  // Orthonormal completion of kept left vectors whose singular value is
+  // numerically zero (below eps * ||M||_F: Jacobi leaves those columns
+  // unrotated): two passes of classical Gram-Schmidt of e_u against the
+  // earlier kept vectors, first u whose remainder keeps norm^2 > 1/4.
+  const T sig_floor = sqrt(tiny);
+  for (int c = 0; c < r; ++c) {
+    const int src = order[c];
+    if (sig[src] > sig_floor) continue;
+    for (int u = 0; u < K; ++u) {
+      for (int row = tid; row < K; row += NT) G[src][row] = (row == u) ? T(1) : T(0);
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int pp = tid; pp < c; pp += NT) {
+          const int ps = order[pp];
           T d = T(0);
           for (int row = 0; row < K; ++row) d += G[ps][row] * G[src][row];
-          for (int row = 0; row < K; ++row) G[src][row] -= d * G[ps][row];
+          coef[pp] = d;
         }
+        __syncthreads();
+        for (int row = tid; row < K; row += NT) {
+          T x = G[src][row];
+          for (int pp = 0; pp < c; ++pp) x -= coef[pp] * G[order[pp]][row];
+          G[src][row] = x;
+        }
+        __syncthreads();
+      }
+      if (tid == 0) {
         T nn = T(0);
         for (int row = 0; row < K; ++row) nn += G[src][row] * G[src][row];
-        if (nn > T(0.25)) {
-          const T inv = T(1) / sqrt(nn);
-          for (int row = 0; row < K; ++row) G[src][row] *= inv;
-          break;
-        }
+        accept_sh = nn > T(0.25);
+        coef[0] = (nn > T(0)) ? T(1) / sqrt(nn) : T(0);
+      }
+      __syncthreads();
+      const int ok = accept_sh;
+      const T inv = coef[0];
+      __syncthreads();
+      if (ok) {
+        for (int row = tid; row < K; row += NT) G[src][row] *= inv;
+        __syncthreads();
+        break;
       }
     }
   }
-  __syncthreads();
   T* S = static_cast<T*>(it.S);
   T* Um = static_cast<T*>(it.Um);
   T* Vm = static_cast<T*>(it.Vm);
@@ -1399,7 +1434,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
     const int row = e / r, c = e % r;
     const int src = order[c];
     if (Um != nullptr) Um[row * it.ldu + c] = G[src][row];
-    if (Vm != nullptr) Vm[row * it.ldv + c] = Vs[src][row];
+    if (Vm != nullptr) Vm[row * it.ldv + c] = VA(src, row);
   }
 }
 
@@ -1407,20 +1442,20 @@ template <typename T>
 void launch_svd(const SvdItem* d_items, int n_items, int max_k, hipStream_t s) {
   if (n_items <= 0) return;
   if (max_k <= 16) {
-    hipLaunchKernelGGL((svd_kernel<T, 16, 64>), dim3(n_items), dim3(64), 0, s, d_items);
+    hipLaunchKernelGGL((svd_kernel<T, 16, 64, false>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 32) {
-    hipLaunchKernelGGL((svd_kernel<T, 32, 64>), dim3(n_items), dim3(64), 0, s, d_items);
+    hipLaunchKernelGGL((svd_kernel<T, 32, 64, false>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 64) {
-    hipLaunchKernelGGL((svd_kernel<T, 64, 256>), dim3(n_items), dim3(256), 0, s, d_items);
-  } else {
+    hipLaunchKernelGGL((svd_kernel<T, 64, 256, false>), dim3(n_items), dim3(256), 0, s, d_items);
+  } else if (max_k <= 128) {
     if constexpr (sizeof(T) == 4) {
-      if (max_k <= 128) {
-        hipLaunchKernelGGL((svd_kernel<T, 128, 256>), dim3(n_items), dim3(256), 0, s, d_items);
-        FRK_HIP_CHECK(hipGetLastError());
-        return;
-      }
+      hipLaunchKernelGGL((svd_kernel<T, 128, 256, false>), dim3(n_items), dim3(256), 0, s, d_items);
+    } else {
+      // every item must carry a K x K scratch (SvdItem::vwork) for its right vectors
+      hipLaunchKernelGGL((svd_kernel<T, 128, 256, true>), dim3(n_items), dim3(256), 0, s, d_items);
     }
-    throw std::runtime_error("svd: K exceeds the LDS capacity for this precision");
+  } else {
+    throw std::runtime_error("svd: K above 128");
   }
   FRK_HIP_CHECK(hipGetLastError());
 }

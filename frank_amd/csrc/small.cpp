@@ -85,8 +85,9 @@ void qr_small_batched(const std::vector<SmallJob<T>>& jobs, Context& ctx) {
   ctx.sync();
 }
 
-// A = R Q through the QR of the column-reversed transpose Z = A^T P:
-// Z = Qz Rz gives R = P Rz^T P (upper) and Q = P Qz^T (orthonormal rows).
+// A = R Q through the QR of the reversed transpose Z = J_n A^T J_m (J: index
+// reversal): Z = Qz Rz gives R = J_m Rz^T J_m (upper) and Q = J_m Qz^T J_n,
+// the same reflectors as LAPACK's dgerqf.
 template <typename T>
 void rq_small_batched(const std::vector<SmallJob<T>>& jobs, Context& ctx) {
   std::vector<SmallJob<T>> z;
@@ -103,11 +104,13 @@ void rq_small_batched(const std::vector<SmallJob<T>>& jobs, Context& ctx) {
     SmallJob<T> q{};
     q.m = j.n;
     q.n = j.m;
-    q.a = j.a + (int64_t)(j.m - 1) * j.a_rs;  // Z(i, c) = A(m-1-c, i)
-    q.a_rs = j.a_cs;
+    // Z(i, c) = A(m-1-c, n-1-i): Householder on Z's column c is dgerqf's on row m-1-c
+    q.a = j.a + (int64_t)(j.m - 1) * j.a_rs + (int64_t)(j.n - 1) * j.a_cs;
+    q.a_rs = -j.a_cs;
     q.a_cs = -j.a_rs;
-    q.o1 = j.o2 + (int64_t)(j.m - 1) * j.o2_rs;  // Q(m-1-c, i) = Qz(i, c)
-    q.o1_rs = j.o2_cs;
+    // Q(m-1-c, n-1-i) = Qz(i, c)
+    q.o1 = j.o2 + (int64_t)(j.m - 1) * j.o2_rs + (int64_t)(j.n - 1) * j.o2_cs;
+    q.o1_rs = -j.o2_cs;
     q.o1_cs = -j.o2_rs;
     rz[t] = ctx.take<T>((size_t)j.m * j.m + 1);
     q.o2 = rz[t];
@@ -273,7 +276,8 @@ void geqp3_small_batched(const std::vector<Geqp3Job<T>>& jobs, Context& ctx) {
     pl.run();
   }
   std::vector<int32_t> rk(nj);
-  FRK_HIP_CHECK(hipMemcpy(rk.data(), dr.as<int32_t>(), sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
+  FRK_HIP_CHECK(hipMemcpyAsync(rk.data(), dr.as<int32_t>(), sizeof(int32_t) * nj, hipMemcpyDeviceToHost, s));
+  FRK_HIP_CHECK(hipStreamSynchronize(s));
   int rmax = 1;
   for (int t = 0; t < nj; ++t) {
     if (rk[t] < 0) throw std::runtime_error("truncated_geqp3: rank above the capacity");

@@ -21,12 +21,14 @@ def _arr(ts):
     return (C.c_void_p * len(ts))(*[_dptr(t) for t in ts])
 
 
-def _i32(v):
-    return np.ascontiguousarray(np.asarray(v, dtype=np.int32))
+def _i32(*v):
+    """ctypes int32 array: passed as an argument object, it lives through the call
+    (a bare address of a temporary numpy array would not)."""
+    return (C.c_int32 * len(v))(*v)
 
 
-def _i64(v):
-    return np.ascontiguousarray(np.asarray(v, dtype=np.int64))
+def _i64(*v):
+    return (C.c_int64 * len(v))(*v)
 
 
 def _check(A: torch.Tensor):
@@ -43,8 +45,8 @@ def qr(A: torch.Tensor, ctx: Optional[Context] = None) -> Tuple[torch.Tensor, to
     k = min(m, n)
     Q = torch.empty((m, k), dtype=A.dtype, device=A.device)
     R = torch.empty((k, n), dtype=A.dtype, device=A.device)
-    L.check(L.lib().frk_dqr_batched(ctx.h, 1, _hptr(_i32([m])), _hptr(_i32([n])), _arr([A]), _hptr(_i64([n])),
-                                    _arr([Q]), _hptr(_i64([k])), _arr([R]), _hptr(_i64([n]))))
+    L.check(L.lib().frk_dqr_batched(ctx.h, 1, _i32(m), _i32(n), _arr([A]), _i64(n),
+                                    _arr([Q]), _i64(k), _arr([R]), _i64(n)))
     return Q, R
 
 
@@ -55,8 +57,8 @@ def rq(A: torch.Tensor, ctx: Optional[Context] = None) -> Tuple[torch.Tensor, to
     m, n = A.shape
     R = torch.empty((m, m), dtype=A.dtype, device=A.device)
     Q = torch.empty((m, n), dtype=A.dtype, device=A.device)
-    L.check(L.lib().frk_drq_batched(ctx.h, 1, _hptr(_i32([m])), _hptr(_i32([n])), _arr([A]), _hptr(_i64([n])),
-                                    _arr([R]), _hptr(_i64([m])), _arr([Q]), _hptr(_i64([n]))))
+    L.check(L.lib().frk_drq_batched(ctx.h, 1, _i32(m), _i32(n), _arr([A]), _i64(n),
+                                    _arr([R]), _i64(m), _arr([Q]), _i64(n)))
     return R, Q
 
 
@@ -69,9 +71,9 @@ def svd(A: torch.Tensor, ctx: Optional[Context] = None):
     U = torch.empty((m, K), dtype=A.dtype, device=A.device)
     S = torch.empty((K, K), dtype=A.dtype, device=A.device)
     V = torch.empty((K, n), dtype=A.dtype, device=A.device)
-    L.check(L.lib().frk_dsvd_batched(ctx.h, 1, _hptr(_i32([m])), _hptr(_i32([n])), _arr([A]), _hptr(_i64([n])),
-                                     _arr([U]), _hptr(_i64([K])), _arr([S]), _hptr(_i64([K])), _arr([V]),
-                                     _hptr(_i64([n]))))
+    L.check(L.lib().frk_dsvd_batched(ctx.h, 1, _i32(m), _i32(n), _arr([A]), _i64(n),
+                                     _arr([U]), _i64(K), _arr([S]), _i64(K), _arr([V]),
+                                     _i64(n)))
     return U, S, V
 
 
@@ -82,10 +84,10 @@ def truncated_geqp3(A: torch.Tensor, eps: float, cap: int = 64, ctx: Optional[Co
     m, n = A.shape
     Q = torch.empty((m, cap), dtype=A.dtype, device=A.device)
     R = torch.zeros((cap, n), dtype=A.dtype, device=A.device)
-    r = _i32([0])
-    L.check(L.lib().frk_dtruncated_geqp3_batched(ctx.h, 1, _hptr(_i32([m])), _hptr(_i32([n])), _arr([A]),
-                                                 _hptr(_i64([n])), eps, cap, _arr([Q]), _hptr(_i64([cap])),
-                                                 _arr([R]), _hptr(_i64([n])), _hptr(r)))
+    r = _i32(0)
+    L.check(L.lib().frk_dtruncated_geqp3_batched(ctx.h, 1, _i32(m), _i32(n), _arr([A]),
+                                                 _i64(n), eps, cap, _arr([Q]), _i64(cap),
+                                                 _arr([R]), _i64(n), r))
     k = int(r[0])
     return Q[:, :k], R[:k]
 

@@ -24,3 +24,17 @@ def test_version_and_error_string_without_gpu():
     rc = lib.frk_ctx_synchronize(None)
     assert rc == L.FRK_ERR_ARG
     assert b"null" in lib.frk_last_error()
+
+
+def test_no_host_pointer_to_temporaries():
+    """Host arrays handed to the C ABI must outlive the call: an address taken
+    from a temporary numpy array (``_hptr(np.array(...))``) dangles."""
+    import re
+    from pathlib import Path
+    pkg = Path(L.__file__).resolve().parent
+    bad = []
+    for f in pkg.glob("*.py"):
+        for i, line in enumerate(f.read_text().splitlines(), 1):
+            if re.search(r"_hptr\((np\.|_i32\(|_i64\(|\[)", line):
+                bad.append(f"{f.name}:{i}")
+    assert bad == []

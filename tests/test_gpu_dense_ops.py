@@ -91,7 +91,10 @@ def test_one_sided_id_matches_oracle(torch_gpu):
     V, cols = dense.one_sided_id(torch.from_numpy(A).cuda(), 16)
     Vr, colsr = fo.one_sided_id(A, 16)
     assert cols == [int(c) for c in colsr]
-    np.testing.assert_allclose(V.cpu().numpy(), Vr, rtol=0, atol=1e-9 * np.abs(Vr).max())
+    # V = R11^-1 R12 inherits the conditioning of R11: compare to 1e-6 and by reconstruction
+    np.testing.assert_allclose(V.cpu().numpy(), Vr, rtol=0, atol=1e-6 * np.abs(Vr).max())
+    err = fo.l2_error(A, A[:, cols] @ V.cpu().numpy())
+    assert err <= 10 * max(fo.l2_error(A, A[:, colsr] @ Vr), 1e-15)
 
 
 def test_id_and_rid_contract(torch_gpu):
