@@ -13,12 +13,14 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent
+BIN = HERE / "bin"
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libfrank_amd.so"
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "engine.cpp", "qr.cpp", "small.cpp", "capi.cpp", "cxx_surface.cpp"]
+SOURCES = ["kernels.hip", "engine.cpp", "qr.cpp", "small.cpp", "capi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -57,6 +59,19 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    # the C++ operator-surface driver test links against the library
+    src = ROOT / "tests" / "cxx" / "surface_test.cpp"
+    if src.exists():
+        BIN.mkdir(exist_ok=True)
+        exe = BIN / "surface_test"
+        cmd = [hipcc(), "-std=c++17", "-O2", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+               "-packed-fp32-ops", f"-I{ROOT / 'include'}", str(src), "-o",
+               str(exe), f"-L{HERE}", "-lfrank_amd", "-Wl,-rpath,$ORIGIN/.."]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"C++ surface test build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+        if verbose:
+            print(f"built {exe}")
     return LIB
 
 

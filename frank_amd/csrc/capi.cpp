@@ -569,6 +569,16 @@ int frk_blr_apply_q(frk_blr y, frk_qrf f, frk_blr c, int trans) {
   });
 }
 
+int frk_blr_trsm(frk_blr f, double* b_dev, int nrhs, int upper, int trans) {
+  return guard([&] {
+    if (!f || !b_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
+    if (!f->a64) throw std::runtime_error("trsm(FP32 Hierarchical, FP64 Dense) undefined!");
+    blr_solve<double>(*f->a64, b_dev, nrhs, trans != 0, *f->owner->ctx, upper ? 2 : 1);
+    FRK_HIP_CHECK(hipStreamSynchronize(f->owner->stream));
+    return FRK_OK;
+  });
+}
+
 int frk_blr_solve(frk_blr f, double* b_dev, int nrhs, int trans) {
   return guard([&] {
     if (!f || !b_dev || nrhs <= 0) throw std::invalid_argument("bad argument");

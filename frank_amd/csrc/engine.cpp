@@ -1648,7 +1648,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
 // target block).  One plan (one descriptor upload) per sweep.
 // ============================================================================
 template <typename T>
-void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
+void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx, int part) {
   const int nb = F.nb_r();
   ctx.sync();
   F.pull_ranks(ctx.stream());
@@ -1708,12 +1708,13 @@ void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx) {
     }
   };
   Plan pl(ctx);
+  const bool do_l = part != 2, do_u = part != 1;
   if (!trans) {
-    sweep(pl, true, true, false);    // L y = b
-    sweep(pl, false, false, false);  // U x = y
+    if (do_l) sweep(pl, true, true, false);    // L y = b
+    if (do_u) sweep(pl, false, false, false);  // U x = y
   } else {
-    sweep(pl, true, false, true);   // U^T y = b
-    sweep(pl, false, true, true);   // L^T x = y
+    if (do_u) sweep(pl, true, false, true);   // U^T y = b
+    if (do_l) sweep(pl, false, true, true);   // L^T x = y
   }
   pl.run();
 }
@@ -1848,7 +1849,7 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
   template std::unique_ptr<BlrMatrix<T>> build_blr_kernel<T>(const KernelDesc&, int, int, double, int,    \
                                                              double, int, int, Context&);                   \
   template void blr_getrf<T>(BlrMatrix<T>&, Context&);                                                      \
-  template void blr_solve<T>(BlrMatrix<T>&, T*, int, bool, Context&);                                       \
+  template void blr_solve<T>(BlrMatrix<T>&, T*, int, bool, Context&, int);                                  \
   template void blr_matvec<T>(BlrMatrix<T>&, const T*, T*, int, Context&);                                  \
   template RefineResult blr_refine<T>(BlrMatrix<T>&, const KernelSpec&, double, const double*, double*, int,   \
                                       double, int, Context&);

@@ -311,8 +311,9 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx);
 
 // Forward/backward block substitution on device vectors b (N x nrhs row-major).
 // trans = false: L then U (solves A x = b); trans = true: U^T then L^T.
+// part: 0 both sweeps, 1 the L (or L^T) sweep only, 2 the U (or U^T) sweep only.
 template <typename T>
-void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx);
+void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx, int part = 0);
 
 // y += A x on an unfactored BLR matrix.
 template <typename T>

@@ -151,6 +151,10 @@ int frk_blr_apply_q(frk_blr y, frk_qrf f, frk_blr c, int trans);
  * on device b (N x nrhs row-major, FP64).  trans != 0 solves A^T x = b with
  * U^T then L^T sweeps (adjoint; no reference counterpart). */
 int frk_blr_solve(frk_blr f, double* b_dev, int nrhs, int trans);
+/* One sweep: trsm(L, b, Mode::Lower) (upper = 0: unit-lower forward substitution)
+ * or trsm(U, b, Mode::Upper) (upper = 1: backward), trsm.cpp:40-90; trans = 1
+ * uses L^T / U^T (no reference counterpart). */
+int frk_blr_trsm(frk_blr f, double* b_dev, int nrhs, int upper, int trans);
 
 /* gemm(Hierarchical, Dense, Dense, 1, 1): y += A x (device, N x nrhs row-major). */
 int frk_blr_matvec(frk_blr a, const double* x_dev, double* y_dev, int nrhs);
