@@ -188,4 +188,25 @@ struct Geqp3Item {
   double eps;
 };
 
+// ---------------------------------------------------------------------------
+// Dataflow LU of one square tile (n <= 512) without pivoting: one workgroup
+// per 64 x 64 block, ordered by per-block flags in global memory.  Outputs
+// the packed unit-L \ U factors in place and the full inverses of both
+// factors (row stride ldi).  Flags (3 nbk^2 ints, zero at allocation) are
+// compared against targets derived from epoch, so no reset is needed between
+// launches as long as epoch grows.  A wait that exceeds spin_limit polls sets
+// *err and abandons the factorization (every wave still exits).
+// ---------------------------------------------------------------------------
+struct TileLuItem {
+  void* a;
+  void* linv;
+  void* uinv;
+  int32_t* flags;
+  int32_t* info;  // first zero pivot (1-based row), 0 if none
+  int32_t* err;   // set when a wait timed out
+  int64_t ld, ldi;
+  int32_t n, epoch;
+  int64_t spin_limit;
+};
+
 }  // namespace frk

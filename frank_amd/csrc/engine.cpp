@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 
@@ -60,7 +61,23 @@ const void* Staging::push_bytes(const void* src, size_t bytes, hipStream_t s) {
 Context::Context(hipStream_t s) : s_(s) {
   info_.ensure(sizeof(int32_t) * 4096);
   counters_.ensure(sizeof(int32_t) * kNumCounters);
-  FRK_HIP_CHECK(hipMemset(counters_.as<void>(), 0, sizeof(int32_t) * kNumCounters));
+  FRK_HIP_CHECK(hipMemsetAsync(counters_.as<void>(), 0, sizeof(int32_t) * kNumCounters, s_));
+}
+
+int32_t* Context::tile_flags() {
+  if (tile_flags_.bytes() == 0) {
+    tile_flags_.ensure(sizeof(int32_t) * kTileFlags);
+    FRK_HIP_CHECK(hipMemsetAsync(tile_flags_.as<void>(), 0, sizeof(int32_t) * kTileFlags, s_));
+  }
+  return tile_flags_.as<int32_t>();
+}
+
+void Context::check_tile_lu(int32_t since) {
+  if (tile_flags_.bytes() == 0 || tile_epoch_ <= since) return;
+  int32_t e = 0;
+  FRK_HIP_CHECK(hipMemcpyAsync(&e, tile_flags_.as<int32_t>() + kTileFlags - 1, sizeof(e), hipMemcpyDeviceToHost, s_));
+  FRK_HIP_CHECK(hipStreamSynchronize(s_));
+  if (e > since) throw std::runtime_error("tile LU: a block wait timed out (workgroups not co-resident?)");
 }
 
 Context::~Context() {
@@ -408,6 +425,37 @@ void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& l
   }
 }
 
+// The dataflow kernel (one launch) unless FRK_TILE_LU=0 selects the blocked
+// sequence of getrf_small / panel GEMM / inverse-doubling launches.
+static bool use_tile_lu() {
+  static const bool on = [] {
+    const char* e = std::getenv("FRK_TILE_LU");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// One-launch FP64 LU of an n x n tile (n <= 512) with full inverses of both
+// factors (row stride n, the opposite triangles zeroed).
+static void plan_tile_lu(Plan& pl, double* a, double* linv, double* uinv, int n, int64_t ld, int32_t* info) {
+  Context& c = pl.ctx();
+  TileLuItem it{};
+  it.a = a;
+  it.linv = linv;
+  it.uinv = uinv;
+  it.flags = c.tile_flags();
+  it.err = it.flags + Context::kTileFlags - 1;
+  it.info = info;
+  it.ld = ld;
+  it.ldi = n;
+  it.n = n;
+  it.epoch = c.next_tile_epoch();
+  it.spin_limit = int64_t(1) << 22;
+  const hipStream_t s = c.stream();
+  const double n3 = (double)n * n * n;
+  pl.launch([=](const char*) { launch_tile_lu(it, s); }, "getrf_tile", 2.0 * n3, 5.0 * sizeof(double) * n * n);
+}
+
 template <typename T>
 static size_t getrf_workspace(int n, int nt) {
   size_t w = 4096;
@@ -428,8 +476,17 @@ void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_
   ctx.sync();
   ctx.reserve(getrf_workspace<T>(n, (int)tiles.size()));
   Plan pl(ctx);
-  plan_getrf<T>(pl, tiles, li, ui, n, ld, info_dev);
+  int32_t e0 = ctx.tile_epoch();
+  if constexpr (sizeof(T) == 8) {
+    if (tiles.size() == 1 && use_tile_lu())
+      plan_tile_lu(pl, tiles[0], li[0], ui[0], n, ld, info_dev);
+    else
+      plan_getrf<T>(pl, tiles, li, ui, n, ld, info_dev);
+  } else {
+    plan_getrf<T>(pl, tiles, li, ui, n, ld, info_dev);
+  }
   pl.run();
+  ctx.check_tile_lu(e0);
   FRK_HIP_CHECK(hipStreamSynchronize(ctx.stream()));
 }
 
@@ -1288,8 +1345,12 @@ template <typename T>
 static void plan_diag_getrf(Plan& p, Context& c, BlrMatrix<T>& A, int i, int32_t* info) {
   const int ii = A.idx(i, i);
   const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
+  const bool tile = use_tile_lu() && m_i == n_i && m_i <= 512;
   if constexpr (sizeof(T) == 8) {
-    plan_getrf<T>(p, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info);
+    if (tile)
+      plan_tile_lu(p, A.D[ii], A.Linv[ii], A.Uinv[ii], n_i, n_i, info);
+    else
+      plan_getrf<T>(p, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info);
   } else {
     const hipStream_t s = c.stream();
     const int64_t nn = (int64_t)m_i * n_i;
@@ -1301,10 +1362,15 @@ static void plan_diag_getrf(Plan& p, Context& c, BlrMatrix<T>& A, int i, int32_t
     T* ut = A.Uinv[ii];
     p.launch([=](const char*) {
       launch_convert<double, T>(d64, dt, nn, s);
-      launch_fill_value(l64, nn, 0.0, s);
-      launch_fill_value(u64, nn, 0.0, s);
+      if (!tile) {
+        launch_fill_value(l64, nn, 0.0, s);
+        launch_fill_value(u64, nn, 0.0, s);
+      }
     });
-    plan_getrf<double>(p, {d64}, {l64}, {u64}, m_i, n_i, info);
+    if (tile)
+      plan_tile_lu(p, d64, l64, u64, n_i, n_i, info);
+    else
+      plan_getrf<double>(p, {d64}, {l64}, {u64}, m_i, n_i, info);
     p.launch([=](const char*) {
       launch_convert<T, double>(dt, d64, nn, s);
       launch_convert<T, double>(lt, l64, nn, s);
@@ -1350,6 +1416,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     if (A.rows(i) > 512) throw std::runtime_error("getrf: tile size above 512");
   }
   FRK_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int32_t) * 4096, s));
+  const int32_t epoch0 = ctx.tile_epoch();
   // Lookahead: the diagonal tile i+1 is final once step i's Schur GEMMs have
   // run, so its factorization proceeds on a second stream while step i's
   // low-rank recompressions still run on the first.
@@ -1633,6 +1700,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   }
   ctx.sync();
   side.sync();
+  ctx.check_tile_lu(epoch0);
+  side.check_tile_lu(0);
   A.pull_ranks(s);
   std::vector<int32_t> hinfo(nb * npan_max);
   FRK_HIP_CHECK(hipMemcpy(hinfo.data(), info, sizeof(int32_t) * hinfo.size(), hipMemcpyDeviceToHost));

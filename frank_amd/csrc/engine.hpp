@@ -95,6 +95,15 @@ class Context {
   template <class X>
   const X* omega(int n, int p);
   int32_t* info() { return info_.as<int32_t>(); }
+  // Per-block flags of the dataflow tile LU (zero at allocation, kTileFlags
+  // ints; the last one is the timeout marker) and the launch epochs that
+  // index them: every launch on this context takes the next epoch.
+  static constexpr int kTileFlags = 160;
+  int32_t* tile_flags();
+  int32_t next_tile_epoch() { return ++tile_epoch_; }
+  int32_t tile_epoch() const { return tile_epoch_; }
+  // Throw if a tile LU launched at an epoch > since timed out (drains the stream).
+  void check_tile_lu(int32_t since);
   // Device event counters (kCounter* slots), zero at creation.
   int32_t* counters() { return counters_.as<int32_t>(); }
   // op log
@@ -121,6 +130,8 @@ class Context {
   int omega_n_ = 0, omega_p_ = 0;
   DevBuf info_;
   DevBuf counters_;
+  DevBuf tile_flags_;
+  int32_t tile_epoch_ = 0;
   std::map<std::string, OpStat> stats_;
   std::string timed_cls_;
   std::vector<hipEvent_t> ev_pool_;

@@ -581,6 +581,292 @@ __global__ __launch_bounds__(256) void getrf_small_kernel(const LuItem* __restri
   }
 }
 
+// ===========================================================================
+// Dataflow LU of one tile (n <= 512): workgroup (bi, bj) owns the 64 x 64
+// block (bi, bj) in registers (thread (rb, cb) = (tid / 16, tid % 16) holds
+// rows 4 rb + i, columns 4 cb + j).  It waits for the blocks it needs through
+// acquire loads of per-block flags, applies its Schur updates, factorizes
+// (diagonal block: LU + both 64 x 64 triangular inverses) or solves (panel
+// blocks: products with the diagonal inverses), and finally computes its
+// block of the full inverses of L and U:
+//   Linv_ij = -Linv_ii sum_{k=j}^{i-1} L_ik Linv_kj        (i > j)
+//   Uinv_ij = -(sum_{k=i}^{j-1} Uinv_ik U_kj) Uinv_jj      (i < j)
+// Flags: F(i,j) = factor block final, I(i,j) = inverse block final; both
+// hold the launch epoch once set.
+// ===========================================================================
+__device__ __forceinline__ bool tile_wait(const int32_t* flag, int32_t target, int64_t limit, int32_t* err,
+                                          int* bail_sh) {
+  // err holds the latest epoch whose factorization timed out: a wait gives up
+  // once its own epoch is marked failed, so one timeout ends every waiter.
+  if (threadIdx.x == 0) {
+    int64_t spins = 0;
+    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (++spins > limit) {
+        *bail_sh = 1;
+        __hip_atomic_fetch_max(err, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      if ((spins & 31) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) {
+        *bail_sh = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+  }
+  __syncthreads();
+  __threadfence();
+  return *bail_sh == 0;
+}
+
+__device__ __forceinline__ void tile_signal(int32_t* flag, int32_t value) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_stage(T (*dst)[65], const T* src, int64_t ld, int nr, int nc) {
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    dst[r][c] = (r < nr && c < nc) ? src[(int64_t)r * ld + c] : T(0);
+  }
+}
+
+// acc (+)= sign * X Y for LDS operands X, Y (64 x 64)
+template <typename T>
+__device__ __forceinline__ void tile_mma(T (&acc)[4][4], T (*X)[65], T (*Y)[65], int rb, int cb, T sign) {
+#pragma unroll 4
+  for (int k = 0; k < 64; ++k) {
+    T l[4], u[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) l[i] = sign * X[4 * rb + i][k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) u[j] = Y[k][4 * cb + j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] += l[i] * u[j];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_to_lds(const T (&acc)[4][4], T (*M)[65], int rb, int cb) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) M[4 * rb + i][4 * cb + j] = acc[i][j];
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_store(const T (&acc)[4][4], T* dst, int64_t ld, int nr, int nc, int rb, int cb) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = 4 * rb + i, c = 4 * cb + j;
+      if (r < nr && c < nc) dst[(int64_t)r * ld + c] = acc[i][j];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_zero(T* dst, int64_t ld, int nr, int nc) {
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = T(0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
+  const int n = it.n;
+  const int nbk = (n + 63) / 64;
+  const int bi = blockIdx.x / nbk, bj = blockIdx.x % nbk;
+  const int tid = threadIdx.x, rb = tid >> 4, cb = tid & 15;
+  const int64_t ld = it.ld, ldi = it.ldi;
+  T* A = static_cast<T*>(it.a);
+  T* Li = static_cast<T*>(it.linv);
+  T* Ui = static_cast<T*>(it.uinv);
+  int32_t* F = it.flags;
+  int32_t* I = it.flags + nbk * nbk;
+  const int32_t ep = it.epoch;
+  const int64_t lim = it.spin_limit;
+  auto rows = [&](int b) { return min(64, n - 64 * b); };
+  __shared__ T Ls[64][65];
+  __shared__ T Us[64][65];
+  __shared__ T M[64][65];
+  __shared__ T rowb[2][64];
+  __shared__ T colb[2][64];
+  __shared__ int bail_sh;
+  if (tid == 0) bail_sh = 0;
+  __syncthreads();
+  const int mi = rows(bi), nj = rows(bj);
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = 4 * rb + i, c = 4 * cb + j;
+      if (r < mi && c < nj)
+        acc[i][j] = A[(int64_t)(64 * bi + r) * ld + 64 * bj + c];
+      else
+        acc[i][j] = (bi == bj && r == c) ? T(1) : T(0);  // identity padding of the last diagonal block
+    }
+  // 1. Schur updates from the earlier block columns / rows
+  const int pmax = min(bi, bj);
+  for (int p = 0; p < pmax; ++p) {
+    if (!tile_wait(&F[bi * nbk + p], ep, lim, it.err, &bail_sh)) return;
+    if (!tile_wait(&F[p * nbk + bj], ep, lim, it.err, &bail_sh)) return;
+    tile_stage(Ls, A + (int64_t)(64 * bi) * ld + 64 * p, ld, mi, 64);
+    tile_stage(Us, A + (int64_t)(64 * p) * ld + 64 * bj, ld, 64, nj);
+    __syncthreads();
+    tile_mma(acc, Ls, Us, rb, cb, T(-1));
+    __syncthreads();
+  }
+  T* Ablk = A + (int64_t)(64 * bi) * ld + 64 * bj;
+  if (bi == bj) {
+    // 2a. LU of the diagonal block in registers, one barrier per pivot
+    for (int k = 0; k < 64; ++k) {
+      const int buf = k & 1;
+      if (rb == (k >> 2)) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          if (4 * rb + i == k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) rowb[buf][4 * cb + j] = acc[i][j];
+      }
+      if (cb == (k >> 2)) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (4 * cb + j == k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) colb[buf][4 * rb + i] = acc[i][j];
+      }
+      __syncthreads();
+      const T piv = rowb[buf][k];
+      if (tid == 0 && piv == T(0) && k < mi) atomicCAS(it.info, 0, 64 * bi + k + 1);
+      const T rp = T(1) / piv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = 4 * rb + i;
+        if (r > k) {
+          const T l = colb[buf][r] * rp;
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int c = 4 * cb + j;
+            if (c > k) acc[i][j] -= l * rowb[buf][c];
+            if (c == k) acc[i][j] = l;
+          }
+        }
+      }
+    }
+    tile_store(acc, Ablk, ld, mi, nj, rb, cb);
+    tile_signal(&F[bi * nbk + bj], ep);
+    // 2b. inverses of the unit-lower and upper factors: wave 0 / wave 1, one column per lane
+    tile_to_lds(acc, M, rb, cb);
+    __syncthreads();
+    if (tid < 64) {
+      const int c = tid;
+      for (int r = 0; r < 64; ++r) {
+        T sacc = (r == c) ? T(1) : T(0);
+        for (int k = c; k < r; ++k) sacc -= M[r][k] * Ls[k][c];
+        Ls[r][c] = (r < c) ? T(0) : sacc;
+      }
+    } else if (tid < 128) {
+      const int c = tid - 64;
+      for (int r = 63; r >= 0; --r) {
+        T sacc = (r == c) ? T(1) : T(0);
+        for (int k = r + 1; k <= c; ++k) sacc -= M[r][k] * Us[k][c];
+        Us[r][c] = (r > c) ? T(0) : sacc / M[r][r];
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < 4096; e += 256) {
+      const int r = e >> 6, c = e & 63;
+      if (r < mi && c < mi) {
+        Li[(int64_t)(64 * bi + r) * ldi + 64 * bi + c] = Ls[r][c];
+        Ui[(int64_t)(64 * bi + r) * ldi + 64 * bi + c] = Us[r][c];
+      }
+    }
+    tile_signal(&I[bi * nbk + bj], ep);
+    return;
+  }
+  // 2c. panel blocks: L_ij = A_ij Uinv_jj (i > j), U_ij = Linv_ii A_ij (i < j)
+  const int dgb = (bi > bj) ? bj : bi;
+  if (!tile_wait(&I[dgb * nbk + dgb], ep, lim, it.err, &bail_sh)) return;
+  tile_to_lds(acc, M, rb, cb);
+  if (bi > bj)
+    tile_stage(Us, Ui + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
+  else
+    tile_stage(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+  if (bi > bj)
+    tile_mma(acc, M, Us, rb, cb, T(1));
+  else
+    tile_mma(acc, Ls, M, rb, cb, T(1));
+  tile_store(acc, Ablk, ld, mi, nj, rb, cb);
+  tile_signal(&F[bi * nbk + bj], ep);
+  // 3. block (bi, bj) of the full inverse of L (below the diagonal) or U (above)
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+  if (bi > bj) {
+    for (int k = bj; k < bi; ++k) {
+      if (!tile_wait(&F[bi * nbk + k], ep, lim, it.err, &bail_sh)) return;
+      if (!tile_wait(&I[k * nbk + bj], ep, lim, it.err, &bail_sh)) return;
+      tile_stage(Ls, A + (int64_t)(64 * bi) * ld + 64 * k, ld, mi, rows(k));
+      tile_stage(Us, Li + (int64_t)(64 * k) * ldi + 64 * bj, ldi, rows(k), nj);
+      __syncthreads();
+      tile_mma(acc, Ls, Us, rb, cb, T(1));
+      __syncthreads();
+    }
+    if (!tile_wait(&I[bi * nbk + bi], ep, lim, it.err, &bail_sh)) return;
+    tile_to_lds(acc, M, rb, cb);
+    tile_stage(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+    tile_mma(acc, Ls, M, rb, cb, T(-1));
+    tile_store(acc, Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj, rb, cb);
+    tile_zero(Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+  } else {
+    for (int k = bi; k < bj; ++k) {
+      if (!tile_wait(&I[bi * nbk + k], ep, lim, it.err, &bail_sh)) return;
+      if (!tile_wait(&F[k * nbk + bj], ep, lim, it.err, &bail_sh)) return;
+      tile_stage(Ls, Ui + (int64_t)(64 * bi) * ldi + 64 * k, ldi, mi, rows(k));
+      tile_stage(Us, A + (int64_t)(64 * k) * ld + 64 * bj, ld, rows(k), nj);
+      __syncthreads();
+      tile_mma(acc, Ls, Us, rb, cb, T(1));
+      __syncthreads();
+    }
+    if (!tile_wait(&I[bj * nbk + bj], ep, lim, it.err, &bail_sh)) return;
+    tile_to_lds(acc, M, rb, cb);
+    tile_stage(Us, Ui + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+    tile_mma(acc, M, Us, rb, cb, T(-1));
+    tile_store(acc, Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj, rb, cb);
+    tile_zero(Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+  }
+  tile_signal(&I[bi * nbk + bj], ep);
+}
+
+void launch_tile_lu(const TileLuItem& it, hipStream_t s) {
+  if (it.n <= 0 || it.n > 512) throw std::runtime_error("tile LU: n must be in [1, 512]");
+  const int nbk = (it.n + 63) / 64;
+  hipLaunchKernelGGL(tile_lu_kernel<double>, dim3(nbk * nbk), dim3(256), 0, s, it);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T>
 void launch_getrf_small(const LuItem* d_items, int n_items, int32_t* d_info, hipStream_t s) {
   if (n_items <= 0) return;

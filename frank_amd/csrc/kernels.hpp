@@ -28,6 +28,8 @@ template <typename T>
 void launch_copy(const GemmItem* d_items, int n_items, int64_t max_elems, hipStream_t s);
 
 // Householder QR of [A1 S1 | A2 S2] (optionally projected), m <= 512.
+// Dataflow LU + full triangular inverses of one FP64 tile (n <= 512), nbk^2 workgroups.
+void launch_tile_lu(const TileLuItem& it, hipStream_t s);
 template <typename T>
 void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s);
 template <typename T>

@@ -18,16 +18,19 @@ def dev(torch, a):
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
 
 
-@pytest.mark.parametrize("n", [8, 16, 32, 65, 256, 512])
-def test_getrf_nopiv_matches_lu(torch_gpu, fa, n):
+@pytest.mark.parametrize("n", [8, 16, 32, 65, 256, 333, 512])
+@pytest.mark.parametrize("batch", [1, 2])
+def test_getrf_nopiv_matches_lu(torch_gpu, fa, n, batch):
+    """batch 1 runs the one-launch dataflow tile LU, batch 2 the blocked launch sequence."""
     torch = torch_gpu
-    print(f"getrf n={n}", flush=True)
+    print(f"getrf n={n} batch={batch}", flush=True)
     x = fo.get_sorted_random_vector(2 * n)
     a = fo.laplacend([x], 0, n, 0, n) + 1000.0 * np.eye(n)  # diagonal boost as in dense_getrf_test
-    tiles = [dev(torch, a), dev(torch, 2.0 * a)]
+    scales = (1.0, 2.0)[:batch]
+    tiles = [dev(torch, s * a) for s in scales]
     info = fa.getrf_nopiv_batched(tiles)
-    assert list(info) == [0, 0]
-    for t, scale in zip(tiles, (1.0, 2.0)):
+    assert list(info) == [0] * batch
+    for t, scale in zip(tiles, scales):
         lu = t.cpu().numpy()
         Lf = np.tril(lu, -1) + np.eye(n)
         Uf = np.triu(lu)
