@@ -594,13 +594,17 @@ __global__ __launch_bounds__(256) void getrf_small_kernel(const LuItem* __restri
 // Flags: F(i,j) = factor block final, I(i,j) = inverse block final; both
 // hold the launch epoch once set.
 // ===========================================================================
+// Cross-workgroup handoff.  The waiting thread polls with relaxed agent-scope
+// loads (no cache maintenance per poll) and issues one acquire fence once the
+// flag is seen; the producer's workgroup barrier followed by one release store
+// publishes every wave's stores (release is cumulative through the barrier).
 __device__ __forceinline__ bool tile_wait(const int32_t* flag, int32_t target, int64_t limit, int32_t* err,
                                           int* bail_sh) {
   // err holds the latest epoch whose factorization timed out: a wait gives up
   // once its own epoch is marked failed, so one timeout ends every waiter.
   if (threadIdx.x == 0) {
     int64_t spins = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       if (++spins > limit) {
         *bail_sh = 1;
         __hip_atomic_fetch_max(err, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -610,16 +614,15 @@ __device__ __forceinline__ bool tile_wait(const int32_t* flag, int32_t target, i
         *bail_sh = 1;
         break;
       }
-      __builtin_amdgcn_s_sleep(2);
+      __builtin_amdgcn_s_sleep(1);
     }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-  __threadfence();
   return *bail_sh == 0;
 }
 
 __device__ __forceinline__ void tile_signal(int32_t* flag, int32_t value) {
-  __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -699,6 +702,10 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
   __shared__ int bail_sh;
   if (tid == 0) bail_sh = 0;
   __syncthreads();
+  auto stamp = [&](int slot) {
+    if (it.trace && tid == 0) it.trace[8 * blockIdx.x + slot] = slot < 4 ? wall_clock64() : clock64();
+  };
+  stamp(0);
   const int mi = rows(bi), nj = rows(bj);
   T acc[4][4];
 #pragma unroll
@@ -722,9 +729,22 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     tile_mma(acc, Ls, Us, rb, cb, T(-1));
     __syncthreads();
   }
+  stamp(1);
   T* Ablk = A + (int64_t)(64 * bi) * ld + 64 * bj;
   if (bi == bj) {
-    // 2a. LU of the diagonal block in registers, one barrier per pivot
+    // 2a. LU of the diagonal block in registers, one barrier per pivot.  The
+    // same rank-1 steps build both inverses: E <- M_k E (row operations, E ends
+    // as L^-1) and X <- X C_k (column operations that reduce U to I, X ends as
+    // U^-1); step k needs U's row k, A's column k, E's row k and X's column k.
+    T E[4][4], X[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) E[i][j] = X[i][j] = (4 * rb + i == 4 * cb + j) ? T(1) : T(0);
+    T (*erow)[64] = reinterpret_cast<T (*)[64]>(&M[0][0]);
+    T (*xcol)[64] = reinterpret_cast<T (*)[64]>(&M[2][0]);
+    const int wrow = 16 * __builtin_amdgcn_readfirstlane(tid >> 6);  // first row of this wave's blocks
+    stamp(4);
     for (int k = 0; k < 64; ++k) {
       const int buf = k & 1;
       if (rb == (k >> 2)) {
@@ -732,62 +752,77 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
         for (int i = 0; i < 4; ++i)
           if (4 * rb + i == k)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) rowb[buf][4 * cb + j] = acc[i][j];
+            for (int j = 0; j < 4; ++j) {
+              rowb[buf][4 * cb + j] = acc[i][j];
+              erow[buf][4 * cb + j] = E[i][j];
+            }
       }
       if (cb == (k >> 2)) {
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (4 * cb + j == k)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) colb[buf][4 * rb + i] = acc[i][j];
+            for (int i = 0; i < 4; ++i) {
+              colb[buf][4 * rb + i] = acc[i][j];
+              xcol[buf][4 * rb + i] = X[i][j];
+            }
       }
       __syncthreads();
       const T piv = rowb[buf][k];
       if (tid == 0 && piv == T(0) && k < mi) atomicCAS(it.info, 0, 64 * bi + k + 1);
       const T rp = T(1) / piv;
+      // Masked broadcast vectors make the three rank-1 updates unconditional:
+      // l = 0 on rows <= k, u = 0 on columns <= k (E's row k and X's column k
+      // already vanish outside their triangles).  Column k is set afterwards.
+      T u[4], er[4], l[4], xs[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = 4 * cb + j;
+        const T v = rowb[buf][c];
+        u[j] = (c > k) ? v : T(0);
+        er[j] = erow[buf][c];
+      }
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = 4 * rb + i;
-        if (r > k) {
-          const T l = colb[buf][r] * rp;
+        const T v = colb[buf][r] * rp;
+        l[i] = (r > k) ? v : T(0);
+        xs[i] = xcol[buf][r] * rp;
+      }
+      if (k < wrow + 15) {  // this wave still has rows below the pivot
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const int c = 4 * cb + j;
-            if (c > k) acc[i][j] -= l * rowb[buf][c];
-            if (c == k) acc[i][j] = l;
+            acc[i][j] -= l[i] * u[j];
+            E[i][j] -= l[i] * er[j];
           }
-        }
+      }
+      if (k >= wrow) {  // this wave has rows at or above the pivot
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) X[i][j] -= xs[i] * u[j];
+      }
+      if (cb == (k >> 2)) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (4 * cb + j == k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              if (4 * rb + i > k) acc[i][j] = l[i];
+              X[i][j] = xs[i];
+            }
       }
     }
+    stamp(5);
     tile_store(acc, Ablk, ld, mi, nj, rb, cb);
     tile_signal(&F[bi * nbk + bj], ep);
-    // 2b. inverses of the unit-lower and upper factors: wave 0 / wave 1, one column per lane
-    tile_to_lds(acc, M, rb, cb);
-    __syncthreads();
-    if (tid < 64) {
-      const int c = tid;
-      for (int r = 0; r < 64; ++r) {
-        T sacc = (r == c) ? T(1) : T(0);
-        for (int k = c; k < r; ++k) sacc -= M[r][k] * Ls[k][c];
-        Ls[r][c] = (r < c) ? T(0) : sacc;
-      }
-    } else if (tid < 128) {
-      const int c = tid - 64;
-      for (int r = 63; r >= 0; --r) {
-        T sacc = (r == c) ? T(1) : T(0);
-        for (int k = r + 1; k <= c; ++k) sacc -= M[r][k] * Us[k][c];
-        Us[r][c] = (r > c) ? T(0) : sacc / M[r][r];
-      }
-    }
-    __syncthreads();
-    for (int e = tid; e < 4096; e += 256) {
-      const int r = e >> 6, c = e & 63;
-      if (r < mi && c < mi) {
-        Li[(int64_t)(64 * bi + r) * ldi + 64 * bi + c] = Ls[r][c];
-        Ui[(int64_t)(64 * bi + r) * ldi + 64 * bi + c] = Us[r][c];
-      }
-    }
+    stamp(2);
+    tile_store(E, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, rb, cb);
+    tile_store(X, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, rb, cb);
     tile_signal(&I[bi * nbk + bj], ep);
+    stamp(3);
     return;
   }
   // 2c. panel blocks: L_ij = A_ij Uinv_jj (i > j), U_ij = Linv_ii A_ij (i < j)
@@ -809,6 +844,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     tile_mma(acc, Ls, M, rb, cb, T(1));
   tile_store(acc, Ablk, ld, mi, nj, rb, cb);
   tile_signal(&F[bi * nbk + bj], ep);
+  stamp(2);
   // 3. block (bi, bj) of the full inverse of L (below the diagonal) or U (above)
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -858,6 +894,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     tile_zero(Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
   }
   tile_signal(&I[bi * nbk + bj], ep);
+  stamp(3);
 }
 
 void launch_tile_lu(const TileLuItem& it, hipStream_t s) {
