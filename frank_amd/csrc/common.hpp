@@ -207,8 +207,8 @@ struct TileLuItem {
   int64_t ld, ldi;
   int32_t n, epoch;
   int64_t spin_limit;
-  int64_t* trace;  // optional: per workgroup 8 stamps (wall clock: start, updates done, factor and inverse
-                   // published; shader clock: diagonal elimination start / end)
+  int64_t* trace;  // optional: per workgroup 4 wall-clock stamps (start, updates done, factor published,
+                   // inverse published)
 };
 
 }  // namespace frk

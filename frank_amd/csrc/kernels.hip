@@ -18,6 +18,7 @@
 #include <type_traits>
 
 #include "kernels.hpp"
+#include "diag64.hpp"
 
 namespace frk {
 
@@ -583,12 +584,11 @@ __global__ __launch_bounds__(256) void getrf_small_kernel(const LuItem* __restri
 
 // ===========================================================================
 // Dataflow LU of one tile (n <= 512): workgroup (bi, bj) owns the 64 x 64
-// block (bi, bj) in registers (thread (rb, cb) = (tid / 16, tid % 16) holds
-// rows 4 rb + i, columns 4 cb + j).  It waits for the blocks it needs through
-// acquire loads of per-block flags, applies its Schur updates, factorizes
-// (diagonal block: LU + both 64 x 64 triangular inverses) or solves (panel
-// blocks: products with the diagonal inverses), and finally computes its
-// block of the full inverses of L and U:
+// block (bi, bj), held in LDS.  It waits for the blocks it needs through
+// per-block flags, applies its Schur updates (FP64 MFMA), factorizes
+// (diagonal block: 16-blocked LU plus both 64 x 64 triangular inverses) or
+// solves (panel blocks: products with the diagonal inverses), and finally
+// computes its block of the full inverses of L and U:
 //   Linv_ij = -Linv_ii sum_{k=j}^{i-1} L_ik Linv_kj        (i > j)
 //   Uinv_ij = -(sum_{k=i}^{j-1} Uinv_ik U_kj) Uinv_jj      (i < j)
 // Flags: F(i,j) = factor block final, I(i,j) = inverse block final; both
@@ -627,271 +627,157 @@ __device__ __forceinline__ void tile_signal(int32_t* flag, int32_t value) {
   if (threadIdx.x == 0) __hip_atomic_store(flag, value, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <typename T>
-__device__ __forceinline__ void tile_stage(T (*dst)[65], const T* src, int64_t ld, int nr, int nc) {
+// 64 x 64 blocks live in LDS with row stride kTileLd (the padding spreads the
+// MFMA operand reads over the banks); rows / columns beyond the matrix edge
+// are zero (identity on a diagonal block).
+constexpr int kTileLd = 68;
+
+__device__ __forceinline__ void blk_load(double* dst, const double* src, int64_t ld, int nr, int nc) {
   for (int e = threadIdx.x; e < 4096; e += 256) {
     const int r = e >> 6, c = e & 63;
-    dst[r][c] = (r < nr && c < nc) ? src[(int64_t)r * ld + c] : T(0);
+    dst[r * kTileLd + c] = (r < nr && c < nc) ? src[(int64_t)r * ld + c] : 0.0;
   }
 }
-
-// acc (+)= sign * X Y for LDS operands X, Y (64 x 64)
-template <typename T>
-__device__ __forceinline__ void tile_mma(T (&acc)[4][4], T (*X)[65], T (*Y)[65], int rb, int cb, T sign) {
-#pragma unroll 4
-  for (int k = 0; k < 64; ++k) {
-    T l[4], u[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) l[i] = sign * X[4 * rb + i][k];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) u[j] = Y[k][4 * cb + j];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] += l[i] * u[j];
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_to_lds(const T (&acc)[4][4], T (*M)[65], int rb, int cb) {
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) M[4 * rb + i][4 * cb + j] = acc[i][j];
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_store(const T (&acc)[4][4], T* dst, int64_t ld, int nr, int nc, int rb, int cb) {
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = 4 * rb + i, c = 4 * cb + j;
-      if (r < nr && c < nc) dst[(int64_t)r * ld + c] = acc[i][j];
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_zero(T* dst, int64_t ld, int nr, int nc) {
+__device__ __forceinline__ void blk_store(const double* src, double* dst, int64_t ld, int nr, int nc) {
   for (int e = threadIdx.x; e < 4096; e += 256) {
     const int r = e >> 6, c = e & 63;
-    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = T(0);
+    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = src[r * kTileLd + c];
+  }
+}
+// Store of a factor64 inverse: the 16 x 16 tiles on the far side of the block
+// diagonal (above it for the lower factor, below it for the upper) hold no
+// data in LDS and are written as zeros.
+__device__ __forceinline__ void blk_store_tri(const double* src, double* dst, int64_t ld, int nr, int nc, bool lower) {
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const bool keep = lower ? ((c >> 4) <= (r >> 4)) : ((c >> 4) >= (r >> 4));
+    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = keep ? src[r * kTileLd + c] : 0.0;
+  }
+}
+__device__ __forceinline__ void blk_zero(double* dst, int64_t ld, int nr, int nc) {
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = 0.0;
   }
 }
 
-template <typename T>
 __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
+  using d64::mma64;
   const int n = it.n;
   const int nbk = (n + 63) / 64;
   const int bi = blockIdx.x / nbk, bj = blockIdx.x % nbk;
-  const int tid = threadIdx.x, rb = tid >> 4, cb = tid & 15;
+  const int tid = threadIdx.x;
   const int64_t ld = it.ld, ldi = it.ldi;
-  T* A = static_cast<T*>(it.a);
-  T* Li = static_cast<T*>(it.linv);
-  T* Ui = static_cast<T*>(it.uinv);
+  double* A = static_cast<double*>(it.a);
+  double* Li = static_cast<double*>(it.linv);
+  double* Ui = static_cast<double*>(it.uinv);
   int32_t* F = it.flags;
   int32_t* I = it.flags + nbk * nbk;
   const int32_t ep = it.epoch;
   const int64_t lim = it.spin_limit;
   auto rows = [&](int b) { return min(64, n - 64 * b); };
-  __shared__ T Ls[64][65];
-  __shared__ T Us[64][65];
-  __shared__ T M[64][65];
-  __shared__ T rowb[2][64];
-  __shared__ T colb[2][64];
-  __shared__ int bail_sh;
+  constexpr int LD = kTileLd;
+  __shared__ __align__(16) double M[64 * LD];
+  __shared__ __align__(16) double Ls[64 * LD];
+  __shared__ __align__(16) double Us[64 * LD];
+  __shared__ int bail_sh, zero_sh;
   if (tid == 0) bail_sh = 0;
-  __syncthreads();
   auto stamp = [&](int slot) {
-    if (it.trace && tid == 0) it.trace[8 * blockIdx.x + slot] = slot < 4 ? wall_clock64() : clock64();
+    if (it.trace && tid == 0) it.trace[4 * blockIdx.x + slot] = wall_clock64();
   };
   stamp(0);
   const int mi = rows(bi), nj = rows(bj);
-  T acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = 4 * rb + i, c = 4 * cb + j;
-      if (r < mi && c < nj)
-        acc[i][j] = A[(int64_t)(64 * bi + r) * ld + 64 * bj + c];
-      else
-        acc[i][j] = (bi == bj && r == c) ? T(1) : T(0);  // identity padding of the last diagonal block
-    }
+  double* Ablk = A + (int64_t)(64 * bi) * ld + 64 * bj;
+  blk_load(M, Ablk, ld, mi, nj);
+  if (bi == bj)
+    for (int r = mi + tid; r < 64; r += 256) M[r * LD + r] = 1.0;  // identity padding of the last diagonal block
+  __syncthreads();
   // 1. Schur updates from the earlier block columns / rows
   const int pmax = min(bi, bj);
   for (int p = 0; p < pmax; ++p) {
     if (!tile_wait(&F[bi * nbk + p], ep, lim, it.err, &bail_sh)) return;
     if (!tile_wait(&F[p * nbk + bj], ep, lim, it.err, &bail_sh)) return;
-    tile_stage(Ls, A + (int64_t)(64 * bi) * ld + 64 * p, ld, mi, 64);
-    tile_stage(Us, A + (int64_t)(64 * p) * ld + 64 * bj, ld, 64, nj);
+    blk_load(Ls, A + (int64_t)(64 * bi) * ld + 64 * p, ld, mi, 64);
+    blk_load(Us, A + (int64_t)(64 * p) * ld + 64 * bj, ld, 64, nj);
     __syncthreads();
-    tile_mma(acc, Ls, Us, rb, cb, T(-1));
+    mma64(M, LD, Ls, LD, Us, LD, 64, -1.0, true, false);
     __syncthreads();
   }
   stamp(1);
-  T* Ablk = A + (int64_t)(64 * bi) * ld + 64 * bj;
   if (bi == bj) {
-    // 2a. LU of the diagonal block in registers, one barrier per pivot.  The
-    // same rank-1 steps build both inverses: E <- M_k E (row operations, E ends
-    // as L^-1) and X <- X C_k (column operations that reduce U to I, X ends as
-    // U^-1); step k needs U's row k, A's column k, E's row k and X's column k.
-    T E[4][4], X[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) E[i][j] = X[i][j] = (4 * rb + i == 4 * cb + j) ? T(1) : T(0);
-    T (*erow)[64] = reinterpret_cast<T (*)[64]>(&M[0][0]);
-    T (*xcol)[64] = reinterpret_cast<T (*)[64]>(&M[2][0]);
-    const int wrow = 16 * __builtin_amdgcn_readfirstlane(tid >> 6);  // first row of this wave's blocks
-    stamp(4);
-    for (int k = 0; k < 64; ++k) {
-      const int buf = k & 1;
-      if (rb == (k >> 2)) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          if (4 * rb + i == k)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-              rowb[buf][4 * cb + j] = acc[i][j];
-              erow[buf][4 * cb + j] = E[i][j];
-            }
-      }
-      if (cb == (k >> 2)) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (4 * cb + j == k)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              colb[buf][4 * rb + i] = acc[i][j];
-              xcol[buf][4 * rb + i] = X[i][j];
-            }
-      }
-      __syncthreads();
-      const T piv = rowb[buf][k];
-      if (tid == 0 && piv == T(0) && k < mi) atomicCAS(it.info, 0, 64 * bi + k + 1);
-      const T rp = T(1) / piv;
-      // Masked broadcast vectors make the three rank-1 updates unconditional:
-      // l = 0 on rows <= k, u = 0 on columns <= k (E's row k and X's column k
-      // already vanish outside their triangles).  Column k is set afterwards.
-      T u[4], er[4], l[4], xs[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int c = 4 * cb + j;
-        const T v = rowb[buf][c];
-        u[j] = (c > k) ? v : T(0);
-        er[j] = erow[buf][c];
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = 4 * rb + i;
-        const T v = colb[buf][r] * rp;
-        l[i] = (r > k) ? v : T(0);
-        xs[i] = xcol[buf][r] * rp;
-      }
-      if (k < wrow + 15) {  // this wave still has rows below the pivot
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            acc[i][j] -= l[i] * u[j];
-            E[i][j] -= l[i] * er[j];
-          }
-      }
-      if (k >= wrow) {  // this wave has rows at or above the pivot
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) X[i][j] -= xs[i] * u[j];
-      }
-      if (cb == (k >> 2)) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          if (4 * cb + j == k)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              if (4 * rb + i > k) acc[i][j] = l[i];
-              X[i][j] = xs[i];
-            }
-      }
-    }
-    stamp(5);
-    tile_store(acc, Ablk, ld, mi, nj, rb, cb);
+    // 2a. LU of the diagonal block and the full inverses of both factors
+    const int z = d64::factor64(M, Ls, Us, LD, &zero_sh);
+    if (tid == 0 && z >= 0 && z < mi) atomicCAS(it.info, 0, 64 * bi + z + 1);
+    blk_store(M, Ablk, ld, mi, nj);
     tile_signal(&F[bi * nbk + bj], ep);
     stamp(2);
-    tile_store(E, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, rb, cb);
-    tile_store(X, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, rb, cb);
+    blk_store_tri(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, true);
+    blk_store_tri(Us, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, false);
     tile_signal(&I[bi * nbk + bj], ep);
     stamp(3);
     return;
   }
-  // 2c. panel blocks: L_ij = A_ij Uinv_jj (i > j), U_ij = Linv_ii A_ij (i < j)
+  // 2b. panel blocks: L_ij = A_ij Uinv_jj (i > j), U_ij = Linv_ii A_ij (i < j)
   const int dgb = (bi > bj) ? bj : bi;
   if (!tile_wait(&I[dgb * nbk + dgb], ep, lim, it.err, &bail_sh)) return;
-  tile_to_lds(acc, M, rb, cb);
-  if (bi > bj)
-    tile_stage(Us, Ui + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
-  else
-    tile_stage(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
+  if (bi > bj) {
+    blk_load(Us, Ui + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
+    __syncthreads();
+    mma64(M, LD, M, LD, Us, LD, 64, 1.0, false, false);
+  } else {
+    blk_load(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
+    __syncthreads();
+    mma64(M, LD, Ls, LD, M, LD, 64, 1.0, false, true);
+  }
   __syncthreads();
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-  if (bi > bj)
-    tile_mma(acc, M, Us, rb, cb, T(1));
-  else
-    tile_mma(acc, Ls, M, rb, cb, T(1));
-  tile_store(acc, Ablk, ld, mi, nj, rb, cb);
+  blk_store(M, Ablk, ld, mi, nj);
   tile_signal(&F[bi * nbk + bj], ep);
   stamp(2);
   // 3. block (bi, bj) of the full inverse of L (below the diagonal) or U (above)
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
   if (bi > bj) {
-    for (int k = bj; k < bi; ++k) {
+    // M <- L_ij Linv_jj + sum_{k=j+1}^{i-1} L_ik Linv_kj, then Linv_ij = -Linv_ii M
+    blk_load(Us, Li + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
+    __syncthreads();
+    mma64(M, LD, M, LD, Us, LD, 64, 1.0, false, false);
+    __syncthreads();
+    for (int k = bj + 1; k < bi; ++k) {
       if (!tile_wait(&F[bi * nbk + k], ep, lim, it.err, &bail_sh)) return;
       if (!tile_wait(&I[k * nbk + bj], ep, lim, it.err, &bail_sh)) return;
-      tile_stage(Ls, A + (int64_t)(64 * bi) * ld + 64 * k, ld, mi, rows(k));
-      tile_stage(Us, Li + (int64_t)(64 * k) * ldi + 64 * bj, ldi, rows(k), nj);
+      blk_load(Ls, A + (int64_t)(64 * bi) * ld + 64 * k, ld, mi, 64);
+      blk_load(Us, Li + (int64_t)(64 * k) * ldi + 64 * bj, ldi, 64, nj);
       __syncthreads();
-      tile_mma(acc, Ls, Us, rb, cb, T(1));
+      mma64(M, LD, Ls, LD, Us, LD, 64, 1.0, true, false);
       __syncthreads();
     }
     if (!tile_wait(&I[bi * nbk + bi], ep, lim, it.err, &bail_sh)) return;
-    tile_to_lds(acc, M, rb, cb);
-    tile_stage(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
+    blk_load(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-    tile_mma(acc, Ls, M, rb, cb, T(-1));
-    tile_store(acc, Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj, rb, cb);
-    tile_zero(Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+    mma64(M, LD, Ls, LD, M, LD, 64, -1.0, false, true);
+    __syncthreads();
+    blk_store(M, Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+    blk_zero(Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
   } else {
-    for (int k = bi; k < bj; ++k) {
+    // M <- Uinv_ii U_ij + sum_{k=i+1}^{j-1} Uinv_ik U_kj, then Uinv_ij = -M Uinv_jj
+    blk_load(Ls, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
+    __syncthreads();
+    mma64(M, LD, Ls, LD, M, LD, 64, 1.0, false, true);
+    __syncthreads();
+    for (int k = bi + 1; k < bj; ++k) {
       if (!tile_wait(&I[bi * nbk + k], ep, lim, it.err, &bail_sh)) return;
       if (!tile_wait(&F[k * nbk + bj], ep, lim, it.err, &bail_sh)) return;
-      tile_stage(Ls, Ui + (int64_t)(64 * bi) * ldi + 64 * k, ldi, mi, rows(k));
-      tile_stage(Us, A + (int64_t)(64 * k) * ld + 64 * bj, ld, rows(k), nj);
+      blk_load(Ls, Ui + (int64_t)(64 * bi) * ldi + 64 * k, ldi, mi, 64);
+      blk_load(Us, A + (int64_t)(64 * k) * ld + 64 * bj, ld, 64, nj);
       __syncthreads();
-      tile_mma(acc, Ls, Us, rb, cb, T(1));
+      mma64(M, LD, Ls, LD, Us, LD, 64, 1.0, true, false);
       __syncthreads();
     }
     if (!tile_wait(&I[bj * nbk + bj], ep, lim, it.err, &bail_sh)) return;
-    tile_to_lds(acc, M, rb, cb);
-    tile_stage(Us, Ui + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
+    blk_load(Us, Ui + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
-    tile_mma(acc, M, Us, rb, cb, T(-1));
-    tile_store(acc, Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj, rb, cb);
-    tile_zero(Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+    mma64(M, LD, M, LD, Us, LD, 64, -1.0, false, false);
+    __syncthreads();
+    blk_store(M, Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+    blk_zero(Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
   }
   tile_signal(&I[bi * nbk + bj], ep);
   stamp(3);
@@ -900,7 +786,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
 void launch_tile_lu(const TileLuItem& it, hipStream_t s) {
   if (it.n <= 0 || it.n > 512) throw std::runtime_error("tile LU: n must be in [1, 512]");
   const int nbk = (it.n + 63) / 64;
-  hipLaunchKernelGGL(tile_lu_kernel<double>, dim3(nbk * nbk), dim3(256), 0, s, it);
+  hipLaunchKernelGGL(tile_lu_kernel, dim3(nbk * nbk), dim3(256), 0, s, it);
   FRK_HIP_CHECK(hipGetLastError());
 }
 
