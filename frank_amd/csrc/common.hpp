@@ -211,4 +211,20 @@ struct TileLuItem {
                    // inverse published)
 };
 
+// ---------------------------------------------------------------------------
+// Block substitution update of one target block: b -= U (S (V x)) (low-rank,
+// r columns) or b -= D x (dense, D != nullptr).  All operands through strided
+// views, x (n x nrhs) and b (m x nrhs) row-major with row stride nrhs.
+// ---------------------------------------------------------------------------
+struct SolveUpdItem {
+  const void* U;
+  const void* S;
+  const void* V;
+  const void* D;
+  const void* x;
+  void* b;
+  int64_t u_rs, u_cs, s_rs, s_cs, v_rs, v_cs, d_rs, d_cs;
+  int32_t m, n, r, pad;
+};
+
 }  // namespace frk

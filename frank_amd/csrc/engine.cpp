@@ -1716,11 +1716,122 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
 // Block substitution (column-oriented: every batched update has a distinct
 // target block).  One plan (one descriptor upload) per sweep.
 // ============================================================================
+// Fast path: per block step one launch for the diagonal inverse and one for
+// every update of the remaining blocks (one workgroup per target block); the
+// sweep writes its result out of place, so no per-step copy is needed.
+template <typename T>
+static bool solve_fast_ok(const BlrMatrix<T>& F, int nrhs) {
+  if (nrhs < 1 || nrhs > 16) return false;
+  for (int i = 0; i < F.nb_r(); ++i)
+    if (F.rows(i) > 512 || F.cols(i) > 512 || F.rows(i) != F.cols(i)) return false;
+  for (size_t t = 0; t < F.rank.size(); ++t)
+    if (F.kind[t] == kLowRank && F.rank[t] > 64) return false;
+  return true;
+}
+
+template <typename T>
+static void blr_solve_fast(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx, int part) {
+  const int nb = F.nb_r();
+  const hipStream_t s = ctx.stream();
+  const int64_t N = F.roff[nb];
+  ctx.reserve(sizeof(T) * ((size_t)N * nrhs + 64) + 4096);
+  T* y = ctx.take<T>((size_t)N * nrhs + 1);
+  Plan pl(ctx);
+  auto sweep = [&](bool forward, bool lower_factor, bool transposed, T* src, T* dst) {
+    for (int step = 0; step < nb; ++step) {
+      const int i = forward ? step : nb - 1 - step;
+      const int ii = F.idx(i, i);
+      const int ni = (int)F.rows(i);
+      const int64_t ld = F.cols(i);
+      const T* tinv = lower_factor ? F.Linv[ii] : F.Uinv[ii];
+      T* sp = src + F.roff[i] * nrhs;
+      T* dp = dst + F.roff[i] * nrhs;
+      pl.launch([=](const char*) { launch_diag_apply<T>(tinv, ld, transposed, sp, dp, ni, nrhs, s); }, "solve_diag",
+                2.0 * ni * ni * nrhs, sizeof(T) * ((double)ni * ni + 2.0 * ni * nrhs));
+      std::vector<SolveUpdItem> items;
+      double fl = 0, by = 0;
+      for (int k = 0; k < nb; ++k) {
+        if (forward ? (k <= i) : (k >= i)) continue;
+        const int t = transposed ? F.idx(i, k) : F.idx(k, i);
+        if (F.kind[t] == kEmpty) continue;
+        SolveUpdItem u{};
+        u.x = dp;
+        u.b = src + F.roff[k] * nrhs;
+        u.m = (int)F.rows(k);
+        u.n = ni;
+        if (F.kind[t] == kDense) {
+          u.D = F.D[t];
+          if (!transposed) {
+            u.d_rs = F.cols(i);
+            u.d_cs = 1;
+          } else {
+            u.d_rs = 1;
+            u.d_cs = F.cols(k);
+          }
+          fl += 2.0 * u.m * ni * nrhs;
+          by += sizeof(T) * (double)u.m * ni;
+        } else {
+          const int r = F.rank[t];
+          u.r = r;
+          if (!transposed) {  // b_k -= U (S (V x)), tile (k, i)
+            u.U = F.U[t];
+            u.u_rs = 1;
+            u.u_cs = F.rows(k);
+            u.S = F.S[t];
+            u.s_rs = F.kcap;
+            u.s_cs = 1;
+            u.V = F.V[t];
+            u.v_rs = F.cols(i);
+            u.v_cs = 1;
+          } else {  // b_k -= V^T (S^T (U^T x)), tile (i, k)
+            u.U = F.V[t];
+            u.u_rs = 1;
+            u.u_cs = F.cols(k);
+            u.S = F.S[t];
+            u.s_rs = 1;
+            u.s_cs = F.kcap;
+            u.V = F.U[t];
+            u.v_rs = F.rows(i);
+            u.v_cs = 1;
+          }
+          fl += 2.0 * r * ((double)ni + r + u.m) * nrhs;
+          by += sizeof(T) * (double)r * (ni + u.m);
+        }
+        items.push_back(u);
+      }
+      if (items.empty()) continue;
+      const size_t off = pl.add(items);
+      const int cnt = (int)items.size();
+      pl.launch([=](const char* d) { launch_solve_update<T>(reinterpret_cast<const SolveUpdItem*>(d + off), cnt, nrhs, s); },
+                "solve_upd", fl, by);
+    }
+  };
+  const bool do_l = part != 2, do_u = part != 1;
+  const bool first_lower = !trans;  // forward sweep: L (plain) or U^T (adjoint)
+  const bool do_first = first_lower ? do_l : do_u, do_second = first_lower ? do_u : do_l;
+  T* cur = b;
+  T* other = y;
+  if (do_first) {
+    sweep(true, first_lower, trans, cur, other);
+    std::swap(cur, other);
+  }
+  if (do_second) {
+    sweep(false, !first_lower, trans, cur, other);
+    std::swap(cur, other);
+  }
+  if (cur != b) plan_copies<T>(pl, {gemm_item(cur, nrhs, 1, nullptr, 0, 0, b, nrhs, 1, (int)N, nrhs, 0, 0.0, 0.0)});
+  pl.run();
+}
+
 template <typename T>
 void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx, int part) {
   const int nb = F.nb_r();
   ctx.sync();
   F.pull_ranks(ctx.stream());
+  if (solve_fast_ok(F, nrhs)) {
+    blr_solve_fast(F, b, nrhs, trans, ctx, part);
+    return;
+  }
   int kmax = 1;
   for (int r : F.rank) kmax = std::max(kmax, r);
   ctx.reserve((size_t)nb * nb * 2 * sizeof(T) * ((size_t)kmax * nrhs + 64) +

@@ -1998,6 +1998,183 @@ void launch_convert(TD* dst, const TS* src, int64_t n, hipStream_t s) {
 // Row block of 256 per workgroup, columns split over gridDim.y partial sums
 // reduced in a fixed order by a second kernel (deterministic).
 // ===========================================================================
+
+// ===========================================================================
+// Block substitution kernels.  diag_apply: w = op(Tinv) b for one diagonal
+// block; a 32-lane group per output row (rows of Tinv read contiguously), or
+// for the transpose a lane per output row with the four waves splitting k.
+// solve_update: one workgroup per target block, b -= U (S (V x)) with x
+// staged in LDS, or b -= D x for a dense block.
+// ===========================================================================
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ Tinv, int64_t ld,
+                                                         const T* __restrict__ b, T* __restrict__ w, int n,
+                                                         int nrhs) {
+  const int tid = threadIdx.x, g = tid >> 5, l = tid & 31;
+  const int row = 8 * blockIdx.x + g;
+  T acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = T(0);
+  if (row < n) {
+    const T* tr = Tinv + (int64_t)row * ld;
+    for (int k = l; k < n; k += 32) {
+      const T a = tr[k];
+      const T* bk = b + (int64_t)k * nrhs;
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+        if (c < nrhs) acc[c] += a * bk[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < NR; ++c)
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 32);
+  if (row < n && l == 0)
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      if (c < nrhs) w[(int64_t)row * nrhs + c] = acc[c];
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void diag_apply_t_kernel(const T* __restrict__ Tinv, int64_t ld,
+                                                           const T* __restrict__ b, T* __restrict__ w, int n,
+                                                           int nrhs) {
+  __shared__ T red[4][64][NR];
+  const int tid = threadIdx.x, kk = tid >> 6, l = tid & 63;
+  const int row = 64 * blockIdx.x + l;
+  T acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = T(0);
+  if (row < n)
+    for (int k = kk; k < n; k += 4) {
+      const T a = Tinv[(int64_t)k * ld + row];
+      const T* bk = b + (int64_t)k * nrhs;
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+        if (c < nrhs) acc[c] += a * bk[c];
+    }
+#pragma unroll
+  for (int c = 0; c < NR; ++c) red[kk][l][c] = acc[c];
+  __syncthreads();
+  if (kk == 0 && row < n)
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      if (c < nrhs) w[(int64_t)row * nrhs + c] = red[0][l][c] + red[1][l][c] + red[2][l][c] + red[3][l][c];
+}
+
+template <typename T>
+void launch_diag_apply(const T* Tinv, int64_t ld, bool trans, const T* b, T* w, int n, int nrhs, hipStream_t s) {
+  if (n <= 0 || n > 512 || nrhs <= 0 || nrhs > 16) throw std::runtime_error("diag apply: n <= 512, nrhs <= 16");
+  const bool wide = nrhs > 8;
+  if (!trans) {
+    const dim3 grid((n + 7) / 8);
+    if (wide)
+      hipLaunchKernelGGL((diag_apply_kernel<T, 16>), grid, dim3(256), 0, s, Tinv, ld, b, w, n, nrhs);
+    else
+      hipLaunchKernelGGL((diag_apply_kernel<T, 8>), grid, dim3(256), 0, s, Tinv, ld, b, w, n, nrhs);
+  } else {
+    const dim3 grid((n + 63) / 64);
+    if (wide)
+      hipLaunchKernelGGL((diag_apply_t_kernel<T, 16>), grid, dim3(256), 0, s, Tinv, ld, b, w, n, nrhs);
+    else
+      hipLaunchKernelGGL((diag_apply_t_kernel<T, 8>), grid, dim3(256), 0, s, Tinv, ld, b, w, n, nrhs);
+  }
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void solve_update_kernel(const SolveUpdItem* __restrict__ items, int nrhs) {
+  __shared__ T xs[512][NR];
+  __shared__ T t1[64][NR];
+  __shared__ T t2[64][NR];
+  const SolveUpdItem it = items[blockIdx.x];
+  const int tid = threadIdx.x;
+  const T* x = static_cast<const T*>(it.x);
+  T* b = static_cast<T*>(it.b);
+  for (int e = tid; e < it.n * NR; e += 256) {
+    const int k = e / NR, c = e % NR;
+    xs[k][c] = (c < nrhs) ? x[(int64_t)k * nrhs + c] : T(0);
+  }
+  __syncthreads();
+  if (it.D) {
+    const T* D = static_cast<const T*>(it.D);
+    for (int row = tid; row < it.m; row += 256) {
+      T acc[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = T(0);
+      for (int k = 0; k < it.n; ++k) {
+        const T a = D[row * it.d_rs + k * it.d_cs];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] += a * xs[k][c];
+      }
+#pragma unroll
+      for (int c = 0; c < NR; ++c)
+        if (c < nrhs) b[(int64_t)row * nrhs + c] -= acc[c];
+    }
+    return;
+  }
+  const T* U = static_cast<const T*>(it.U);
+  const T* S = static_cast<const T*>(it.S);
+  const T* V = static_cast<const T*>(it.V);
+  const int r = it.r;
+  {  // t1 = V x: four lanes per row of V, then a quad reduction
+    const int rho = tid >> 2, part = tid & 3;
+    T acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = T(0);
+    if (rho < r)
+      for (int k = part; k < it.n; k += 4) {
+        const T a = V[rho * it.v_rs + k * it.v_cs];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] += a * xs[k][c];
+      }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) {
+      acc[c] += __shfl_xor(acc[c], 1, 4);
+      acc[c] += __shfl_xor(acc[c], 2, 4);
+    }
+    if (rho < r && part == 0)
+#pragma unroll
+      for (int c = 0; c < NR; ++c) t1[rho][c] = acc[c];
+  }
+  __syncthreads();
+  for (int e = tid; e < r * NR; e += 256) {  // t2 = S t1
+    const int rho = e / NR, c = e % NR;
+    T acc = T(0);
+    for (int q = 0; q < r; ++q) acc += S[rho * it.s_rs + q * it.s_cs] * t1[q][c];
+    t2[rho][c] = acc;
+  }
+  __syncthreads();
+  for (int row = tid; row < it.m; row += 256) {  // b -= U t2
+    T acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = T(0);
+    for (int q = 0; q < r; ++q) {
+      const T a = U[row * it.u_rs + q * it.u_cs];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] += a * t2[q][c];
+    }
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      if (c < nrhs) b[(int64_t)row * nrhs + c] -= acc[c];
+  }
+}
+
+template <typename T>
+void launch_solve_update(const SolveUpdItem* d_items, int n_items, int nrhs, hipStream_t s) {
+  if (n_items <= 0) return;
+  if (nrhs <= 0 || nrhs > 16) throw std::runtime_error("solve update: nrhs <= 16");
+  if (nrhs > 8)
+    hipLaunchKernelGGL((solve_update_kernel<T, 16>), dim3(n_items), dim3(256), 0, s, d_items, nrhs);
+  else
+    hipLaunchKernelGGL((solve_update_kernel<T, 8>), dim3(n_items), dim3(256), 0, s, d_items, nrhs);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_diag_apply<double>(const double*, int64_t, bool, const double*, double*, int, int, hipStream_t);
+template void launch_diag_apply<float>(const float*, int64_t, bool, const float*, float*, int, int, hipStream_t);
+template void launch_solve_update<double>(const SolveUpdItem*, int, int, hipStream_t);
+template void launch_solve_update<float>(const SolveUpdItem*, int, int, hipStream_t);
+
 constexpr int kResMaxRhs = 16;
 constexpr int kResSplits = 16;
 
@@ -2051,6 +2228,78 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(KernelSpec ks, co
     for (int c = 0; c < nrhs; ++c) part[((int64_t)split * n + i) * nrhs + c] = acc[c];
 }
 
+
+// Laplace kernel 1 / (|x_i - x_j| + 1e-3) (+ shift on the diagonal), with the
+// dimension and the (padded) right-hand-side count fixed at compile time and
+// the reciprocal from the hardware estimate refined by two Newton steps.
+template <int DIM, int NR>
+__global__ __launch_bounds__(256) void residual_laplace_kernel(KernelSpec ks, const double* __restrict__ X,
+                                                               double* __restrict__ part, int64_t n, int nrhs) {
+#pragma clang fp contract(off)
+  __shared__ double xs[DIM][256];
+  __shared__ __align__(16) double vs[256][NR];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int split = blockIdx.y;
+  const int64_t chunk = (n + kResSplits - 1) / kResSplits;
+  const int64_t j0 = split * chunk, j1 = std::min<int64_t>(n, j0 + chunk);
+  double acc[NR];
+#pragma unroll
+  for (int c = 0; c < NR; ++c) acc[c] = 0.0;
+  double xi[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) xi[d] = (i < n) ? ks.pts[d * ks.n_pts + i] : 0.0;
+  for (int64_t jb = j0; jb < j1; jb += 256) {
+    const int64_t j = jb + threadIdx.x;
+    __syncthreads();
+    if (j < j1) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) xs[d][threadIdx.x] = ks.pts[d * ks.n_pts + j];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) vs[threadIdx.x][c] = (c < nrhs) ? X[j * nrhs + c] : 0.0;
+    }
+    __syncthreads();
+    const int cnt = (int)std::min<int64_t>(256, j1 - jb);
+    const int64_t diag = i - jb;
+    for (int jj = 0; jj < cnt; ++jj) {
+      double r;
+      if (DIM == 1) {
+        r = fabs(xi[0] - xs[0][jj]);  // equals sqrt(d * d) exactly
+      } else {
+        double rij = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const double dd = xi[d] - xs[d][jj];
+          rij = rij + dd * dd;
+        }
+        r = sqrt(rij);
+      }
+      double a = d64::rcp2(r + 1e-3);
+      if (jj == diag) a = a + ks.shift;
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = fma(a, vs[jj][c], acc[c]);
+    }
+  }
+  if (i < n)
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      if (c < nrhs) part[((int64_t)split * n + i) * nrhs + c] = acc[c];
+}
+
+template <int DIM>
+static void launch_residual_laplace(dim3 grid, hipStream_t s, const KernelSpec& ks, const double* X, double* part,
+                                    int64_t n, int nrhs) {
+  if (nrhs <= 1)
+    hipLaunchKernelGGL((residual_laplace_kernel<DIM, 1>), grid, dim3(256), 0, s, ks, X, part, n, nrhs);
+  else if (nrhs <= 2)
+    hipLaunchKernelGGL((residual_laplace_kernel<DIM, 2>), grid, dim3(256), 0, s, ks, X, part, n, nrhs);
+  else if (nrhs <= 4)
+    hipLaunchKernelGGL((residual_laplace_kernel<DIM, 4>), grid, dim3(256), 0, s, ks, X, part, n, nrhs);
+  else if (nrhs <= 8)
+    hipLaunchKernelGGL((residual_laplace_kernel<DIM, 8>), grid, dim3(256), 0, s, ks, X, part, n, nrhs);
+  else
+    hipLaunchKernelGGL((residual_laplace_kernel<DIM, 16>), grid, dim3(256), 0, s, ks, X, part, n, nrhs);
+}
+
 __global__ void residual_reduce_kernel(const double* __restrict__ part, const double* __restrict__ B,
                                        double* __restrict__ R, int64_t n, int nrhs) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -2072,7 +2321,14 @@ void launch_kernel_residual(const KernelSpec& ks, const double* X, const double*
     part_cap = need;
   }
   dim3 grid((unsigned)((n + 255) / 256), kResSplits);
-  hipLaunchKernelGGL(residual_partial_kernel, grid, dim3(256), 0, s, ks, X, part, n, nrhs);
+  if (ks.kind == 0 && ks.dim == 1)
+    launch_residual_laplace<1>(grid, s, ks, X, part, n, nrhs);
+  else if (ks.kind == 0 && ks.dim == 2)
+    launch_residual_laplace<2>(grid, s, ks, X, part, n, nrhs);
+  else if (ks.kind == 0 && ks.dim == 3)
+    launch_residual_laplace<3>(grid, s, ks, X, part, n, nrhs);
+  else
+    hipLaunchKernelGGL(residual_partial_kernel, grid, dim3(256), 0, s, ks, X, part, n, nrhs);
   FRK_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(residual_reduce_kernel, dim3((unsigned)((n * nrhs + 255) / 256)), dim3(256), 0, s, part,
                      B, R, n, nrhs);

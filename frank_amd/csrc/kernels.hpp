@@ -100,6 +100,12 @@ void launch_set_int(const SetIntItem* d_items, int n, hipStream_t s);
 
 // Matrix-free residual R = B - A X for a kernel matrix (FP64), X/B/R row-major N x nrhs;
 // with B == nullptr, R = A X.
+// w = op(Tinv) b for one diagonal block (n <= 512 rows, nrhs <= 16), op = transpose when trans.
+template <typename T>
+void launch_diag_apply(const T* Tinv, int64_t ld, bool trans, const T* b, T* w, int n, int nrhs, hipStream_t s);
+// One workgroup per SolveUpdItem (n <= 512, r <= 64, nrhs <= 16).
+template <typename T>
+void launch_solve_update(const SolveUpdItem* d_items, int n_items, int nrhs, hipStream_t s);
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R,
                             int64_t n, int nrhs, hipStream_t s);
 
