@@ -75,6 +75,27 @@ def cpu_baseline(A64, nb: int, steps: int, nrhs: int):
                       f"factorization only, the {nrhs}-RHS solve adds under 1%"}
 
 
+def measured_traffic(cls: str):
+    """HBM bytes per launch of the roofline kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, calibrated against a
+    copy of known size), or None when no summary exists for the class."""
+    path = os.path.join(ROOT, "profiles", f"{cls}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    if "hbm_bytes_per_launch" not in vals:
+        return None
+    return vals["hbm_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +109,7 @@ def main():
     ap.add_argument("--mode", choices=["both", "f64", "mixed"], default="both")
     ap.add_argument("--eps-mixed", type=float, default=1e-6,
                     help="compression / recompression accuracy of the FP32-storage factorization")
-    ap.add_argument("--roofline-class", default="getrf_gemm",
+    ap.add_argument("--roofline-class", default="getrf_tile",
                     help="kernel class whose launches are event-timed for the roofline object")
     ap.add_argument("--cpu-steps", type=int, default=3, help="elimination steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -197,7 +218,8 @@ def main():
             ach = rc["flops"] / secs / 1e12
             roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
                     "frac": ach / PEAK_TFLOPS[prec]}
-        roof.update({"traffic": None, "kernel_class": args.roofline_class, "precision": prec,
+        traffic = measured_traffic(args.roofline_class)
+        roof.update({"traffic": traffic, "kernel_class": args.roofline_class, "precision": prec,
                      "launches_per_step": rc["launches"] / steps,
                      "avg_launch_us": rc["ms"] * 1e3 / rc["timed_launches"],
                      "alg_flops_per_launch": rc["flops"] / rc["launches"],

@@ -3,6 +3,7 @@
 #include "plan.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -118,6 +119,12 @@ void Context::account(const char* cls, double flops, double bytes) {
   st.launches += 1;
   st.flops += flops;
   st.bytes += bytes;
+}
+
+void Context::host_time(const char* cls, double ms) {
+  OpStat& st = owner_->stats_[cls];
+  st.ms += ms;
+  st.timed += 1;
 }
 
 void Context::event_begin() {
@@ -452,8 +459,9 @@ static void plan_tile_lu(Plan& pl, double* a, double* linv, double* uinv, int n,
   it.epoch = c.next_tile_epoch();
   it.spin_limit = int64_t(1) << 22;
   const hipStream_t s = c.stream();
+  // algorithmic counts: LU 2/3 n^3 + two triangular inverses n^3/3 each; read A, write LU and both inverses
   const double n3 = (double)n * n * n;
-  pl.launch([=](const char*) { launch_tile_lu(it, s); }, "getrf_tile", 2.0 * n3, 5.0 * sizeof(double) * n * n);
+  pl.launch([=](const char*) { launch_tile_lu(it, s); }, "getrf_tile", (4.0 / 3.0) * n3, 4.0 * sizeof(double) * n * n);
 }
 
 template <typename T>
@@ -1424,9 +1432,14 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   Context side(la.s);
   side.set_stats_owner(&ctx);
   bool diag_pending = false;
+  using clk = std::chrono::steady_clock;
+  auto ms_since = [](clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
   for (int i = 0; i < nb; ++i) {
+    auto th = clk::now();
     ctx.sync();
     A.pull_ranks(s);
+    ctx.host_time("host_sync", ms_since(th));
+    th = clk::now();
     const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
     const int ii = A.idx(i, i);
     const int rem = nb - 1 - i;
@@ -1471,10 +1484,15 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
       plan_diag_getrf<T>(pl, ctx, A, i, info + i * npan_max);
     }
     // 2. panel solves (src/operations/LAPACK/getrf.cpp:50-57) as products with the explicit
-    //    triangular inverses, written to workspace and copied back into the slots
-    std::vector<GemmItem> tg, tcp;
+    //    triangular inverses, written to workspace and copied back into the slots.
+    //    Index [1] holds the work the next diagonal tile depends on (the panels
+    //    (i+1, i), (i, i+1) and the Schur update of (i+1, i+1)); it is issued
+    //    first so the lookahead factorization starts as early as possible.
+    std::vector<GemmItem> tgv[2], tcpv[2];
     for (int ic = i + 1; ic < nb; ++ic) {
       const int t = A.idx(ic, i);
+      auto& tg = tgv[ic == i + 1];
+      auto& tcp = tcpv[ic == i + 1];
       if (A.kind[t] == kLowRank) {  // V <- V U^-1
         const int r = A.rank[t];
         T* w = ctx.take<T>((size_t)r * n_i + 1);
@@ -1489,6 +1507,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     }
     for (int j = i + 1; j < nb; ++j) {
       const int t = A.idx(i, j);
+      auto& tg = tgv[j == i + 1];
+      auto& tcp = tcpv[j == i + 1];
       if (A.kind[t] == kLowRank) {  // U <- L^-1 U
         const int r = A.rank[t];
         T* w = ctx.take<T>((size_t)m_i * r + 1);
@@ -1501,25 +1521,25 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         tcp.push_back(gemm_item(w, n, 1, nullptr, 0, 0, A.D[t], n, 1, m_i, n, 0, 0.0, 0.0));
       }
     }
-    plan_gemms<T>(pl, tg, "trsm_gemm");
-    plan_copies<T>(pl, tcp);
+    plan_gemms<T>(pl, tgv[1], "trsm_gemm");
+    plan_copies<T>(pl, tcpv[1]);
     if (rem > 0) {
       // 3. Schur updates A(ic,k) -= L(ic,i) A(i,k) (getrf.cpp:58-62, gemm.cpp:126-272)
       std::vector<T*> P(nb, nullptr), Q(nb, nullptr);
-      std::vector<GemmItem> gA, gB, gC, gD;
+      std::vector<GemmItem> gAv[2], gBv[2], gCv[2], gDv[2];
       for (int ic = i + 1; ic < nb; ++ic) {
         const int t = A.idx(ic, i);
         if (A.kind[t] != kLowRank) continue;
         const int kl = A.rank[t];
         P[ic] = ctx.take<T>((size_t)kl * n_i + 1);
-        gA.push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, 1.0, 0.0));
+        gAv[ic == i + 1].push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, 1.0, 0.0));
       }
       for (int k = i + 1; k < nb; ++k) {
         const int t = A.idx(i, k);
         if (A.kind[t] != kLowRank) continue;
         const int kb = A.rank[t];
         Q[k] = ctx.take<T>((size_t)m_i * kb + 1);
-        gA.push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
+        gAv[k == i + 1].push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
       }
       for (int ic = i + 1; ic < nb; ++ic)
         for (int k = i + 1; k < nb; ++k) {
@@ -1527,6 +1547,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           const int kl = A.kind[tl], kb = A.kind[tb], kc = A.kind[tc];
           const int m = (int)A.rows(ic), n = (int)A.cols(k);
           if (kl == kEmpty || kb == kEmpty || kc == kEmpty) continue;
+          const bool prio = ic == i + 1 && k == i + 1;
+          auto& gA = gAv[prio];
+          auto& gB = gBv[prio];
+          auto& gC = gCv[prio];
+          auto& gD = gDv[prio];
           if (kc == kDense) {
             T* C = A.D[tc];
             if (kl == kLowRank && kb == kLowRank) {
@@ -1648,10 +1673,12 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           }
           adds.push_back(ad);
         }
-      plan_gemms<T>(pl, gA, "schur_gemm");
-      plan_gemms<T>(pl, gB, "schur_gemm");
-      plan_gemms<T>(pl, gC, "schur_gemm");
-      plan_gemms<T>(pl, gD, "schur_gemm");
+      plan_gemms<T>(pl, gAv[1], "schur_gemm");
+      plan_gemms<T>(pl, gBv[1], "schur_gemm");
+      plan_gemms<T>(pl, gCv[1], "schur_gemm");
+      plan_gemms<T>(pl, gDv[1], "schur_gemm");
+      ctx.host_time("host_plan", ms_since(th));
+      th = clk::now();
       pl.run();
       // tile (i+1, i+1) has received its last update
       FRK_HIP_CHECK(hipEventRecord(la.upd, s));
@@ -1665,6 +1692,16 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         FRK_HIP_CHECK(hipEventRecord(la.diag, la.s));
         diag_pending = true;
       };
+      lookahead();
+      ctx.host_time("host_issue", ms_since(th));
+      th = clk::now();
+      // the rest of the step: remaining panel solves and Schur updates
+      plan_gemms<T>(pl, tgv[0], "trsm_gemm");
+      plan_copies<T>(pl, tcpv[0]);
+      plan_gemms<T>(pl, gAv[0], "schur_gemm");
+      plan_gemms<T>(pl, gBv[0], "schur_gemm");
+      plan_gemms<T>(pl, gCv[0], "schur_gemm");
+      plan_gemms<T>(pl, gDv[0], "schur_gemm");
       if (!comp.empty()) {
         // D.D -> LR needs the compressed ranks on the host: run what is planned,
         // compress on a side context (own workspace), then plan the additions.
@@ -1684,7 +1721,6 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           plan_lr_add<T>(p2, adds, A.lra);
           p2.run();
         }
-        lookahead();
         continue;
       }
       if (A.lra == kLraNaive) {
@@ -1692,10 +1728,15 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         lr_add_naive<T>(adds, ctx);
       } else {
         plan_lr_add<T>(pl, adds, A.lra);
+        ctx.host_time("host_plan_rest", ms_since(th));
+        th = clk::now();
         pl.run();
+        ctx.host_time("host_issue", ms_since(th));
       }
-      lookahead();
+      continue;
     }
+    plan_gemms<T>(pl, tgv[0], "trsm_gemm");
+    plan_copies<T>(pl, tcpv[0]);
     pl.run();
   }
   ctx.sync();

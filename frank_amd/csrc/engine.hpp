@@ -108,6 +108,8 @@ class Context {
   int32_t* counters() { return counters_.as<int32_t>(); }
   // op log
   void account(const char* cls, double flops, double bytes);
+  // host-side wall time attributed to a class of the op log (counted as timed launches)
+  void host_time(const char* cls, double ms);
   void time_class(const std::string& cls) { timed_cls_ = cls; }
   bool is_timed(const char* cls) const {
     return cls && !owner_->timed_cls_.empty() && owner_->timed_cls_ == cls;
