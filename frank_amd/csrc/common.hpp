@@ -227,4 +227,11 @@ struct SolveUpdItem {
   int32_t m, n, r, pad;
 };
 
+// One row of a stacked-row product: a is a contiguous K-vector (input row),
+// c where the finished row goes (contiguous N-vector).
+struct RowRef {
+  const void* a;
+  void* c;
+};
+
 }  // namespace frk

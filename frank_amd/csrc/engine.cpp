@@ -1454,6 +1454,14 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
       for (int r = 0; r < (int)A.rank.size(); ++r) kmax = std::max(kmax, A.rank[r]);
       const size_t bm = (size_t)std::max<int64_t>(A.rows(0), A.cols(0));
       need += (size_t)rem * 2 * sizeof(T) * (bm * kmax + 256);
+      {
+        size_t rp = 0, rq = 0;
+        for (int t2 = i + 1; t2 < nb; ++t2) {
+          if (A.kind[A.idx(t2, i)] == kLowRank) rp += A.rank[A.idx(t2, i)];
+          if (A.kind[A.idx(i, t2)] == kLowRank) rq += A.rank[A.idx(i, t2)];
+        }
+        need += sizeof(T) * (rp * rq + 2 * (rp + rq) * bm) + 8 * 256;
+      }
       for (int t2 = i + 1; t2 < nb; ++t2) {
         if (A.kind[A.idx(t2, i)] == kDense) need += sizeof(T) * (bm * bm + 64);
         if (A.kind[A.idx(i, t2)] == kDense) need += sizeof(T) * (bm * bm + 64);
@@ -1488,16 +1496,19 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     //    Index [1] holds the work the next diagonal tile depends on (the panels
     //    (i+1, i), (i, i+1) and the Schur update of (i+1, i+1)); it is issued
     //    first so the lookahead factorization starts as early as possible.
+    // Low-rank tiles go through stacked-row products (one launch per side and priority
+    // class): the rows of every V (column tiles) and every column of every U (row tiles).
     std::vector<GemmItem> tgv[2], tcpv[2];
+    std::vector<RowRef> vrows[2], urows[2];
     for (int ic = i + 1; ic < nb; ++ic) {
       const int t = A.idx(ic, i);
       auto& tg = tgv[ic == i + 1];
       auto& tcp = tcpv[ic == i + 1];
       if (A.kind[t] == kLowRank) {  // V <- V U^-1
-        const int r = A.rank[t];
-        T* w = ctx.take<T>((size_t)r * n_i + 1);
-        tg.push_back(gemm_item(A.V[t], A.cols(i), 1, A.Uinv[ii], n_i, 1, w, n_i, 1, r, n_i, n_i, 1.0, 0.0));
-        tcp.push_back(gemm_item(w, n_i, 1, nullptr, 0, 0, A.V[t], A.cols(i), 1, r, n_i, 0, 0.0, 0.0));
+        for (int rho = 0; rho < A.rank[t]; ++rho) {
+          T* row = A.V[t] + (size_t)rho * A.cols(i);
+          vrows[ic == i + 1].push_back(RowRef{row, row});
+        }
       } else if (A.kind[t] == kDense) {
         const int m = (int)A.rows(ic);
         T* w = ctx.take<T>((size_t)m * n_i + 1);
@@ -1509,11 +1520,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
       const int t = A.idx(i, j);
       auto& tg = tgv[j == i + 1];
       auto& tcp = tcpv[j == i + 1];
-      if (A.kind[t] == kLowRank) {  // U <- L^-1 U
-        const int r = A.rank[t];
-        T* w = ctx.take<T>((size_t)m_i * r + 1);
-        tg.push_back(gemm_item(A.Linv[ii], m_i, 1, A.U[t], 1, A.rows(i), w, 1, m_i, m_i, r, m_i, 1.0, 0.0));
-        tcp.push_back(gemm_item(w, 1, m_i, nullptr, 0, 0, A.U[t], 1, A.rows(i), m_i, r, 0, 0.0, 0.0));
+      if (A.kind[t] == kLowRank) {  // U <- L^-1 U, as U^T <- U^T L^-T on the columns of U
+        for (int rho = 0; rho < A.rank[t]; ++rho) {
+          T* col = A.U[t] + (size_t)rho * A.rows(i);
+          urows[j == i + 1].push_back(RowRef{col, col});
+        }
       } else if (A.kind[t] == kDense) {
         const int n = (int)A.cols(j);
         T* w = ctx.take<T>((size_t)m_i * n + 1);
@@ -1521,32 +1532,71 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         tcp.push_back(gemm_item(w, n, 1, nullptr, 0, 0, A.D[t], n, 1, m_i, n, 0, 0.0, 0.0));
       }
     }
+    auto plan_rows = [&](int pr) {
+      for (int side2 = 0; side2 < 2; ++side2) {
+        const std::vector<RowRef>& rows = side2 == 0 ? vrows[pr] : urows[pr];
+        if (rows.empty()) continue;
+        const int R = (int)rows.size();
+        const int N = side2 == 0 ? n_i : m_i;
+        T* W = ctx.take<T>((size_t)R * N + 1);
+        const size_t off = pl.add(rows);
+        const T* Bm = side2 == 0 ? A.Uinv[ii] : A.Linv[ii];
+        const int64_t brs = side2 == 0 ? n_i : 1, bcs = side2 == 0 ? 1 : m_i;
+        const hipStream_t st = s;
+        pl.launch(
+            [=](const char* d) {
+              const RowRef* rr = reinterpret_cast<const RowRef*>(d + off);
+              launch_rows_gemm<T>(rr, R, Bm, brs, bcs, W, N, N, st);
+              launch_rows_scatter<T>(rr, R, W, N, st);
+            },
+            "trsm_gemm", 2.0 * R * N * N, sizeof(T) * ((double)N * N + 2.0 * R * N));
+      }
+    };
+    plan_rows(1);
     plan_gemms<T>(pl, tgv[1], "trsm_gemm");
     plan_copies<T>(pl, tcpv[1]);
     if (rem > 0) {
       // 3. Schur updates A(ic,k) -= L(ic,i) A(i,k) (getrf.cpp:58-62, gemm.cpp:126-272)
+      // P(ic) = S V of the low-rank tiles (ic, i) and Q(k) = U S of (i, k), stacked so that all
+      // products P(ic) Q(k) of the step come from one GEMM (Sxa, row offsets poff, column offsets qoff)
       std::vector<T*> P(nb, nullptr), Q(nb, nullptr);
+      std::vector<int> poff(nb, -1), qoff(nb, -1);
+      int RP = 0, RQ = 0;
+      for (int ic = i + 1; ic < nb; ++ic)
+        if (A.kind[A.idx(ic, i)] == kLowRank) {
+          poff[ic] = RP;
+          RP += A.rank[A.idx(ic, i)];
+        }
+      for (int k = i + 1; k < nb; ++k)
+        if (A.kind[A.idx(i, k)] == kLowRank) {
+          qoff[k] = RQ;
+          RQ += A.rank[A.idx(i, k)];
+        }
+      T* Pst = ctx.take<T>((size_t)RP * n_i + 1);
+      T* Qst = ctx.take<T>((size_t)RQ * m_i + 1);
+      T* Sxa = ctx.take<T>((size_t)RP * RQ + 1);
       std::vector<GemmItem> gAv[2], gBv[2], gCv[2], gDv[2];
       for (int ic = i + 1; ic < nb; ++ic) {
         const int t = A.idx(ic, i);
         if (A.kind[t] != kLowRank) continue;
         const int kl = A.rank[t];
-        P[ic] = ctx.take<T>((size_t)kl * n_i + 1);
-        gAv[ic == i + 1].push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, 1.0, 0.0));
+        P[ic] = Pst + (size_t)poff[ic] * n_i;
+        // P(ic) = -S V: the minus sign of the Schur update lives in P
+        gAv[ic == i + 1].push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, -1.0, 0.0));
       }
       for (int k = i + 1; k < nb; ++k) {
         const int t = A.idx(i, k);
         if (A.kind[t] != kLowRank) continue;
         const int kb = A.rank[t];
-        Q[k] = ctx.take<T>((size_t)m_i * kb + 1);
+        Q[k] = Qst + (size_t)qoff[k] * m_i;
         gAv[k == i + 1].push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
       }
-      for (int ic = i + 1; ic < nb; ++ic)
-        for (int k = i + 1; k < nb; ++k) {
+      // Schur update of one target tile (ic, k): items into gXv[prio], additions into adds
+      auto pair_body = [&](int ic, int k) {
           const int tl = A.idx(ic, i), tb = A.idx(i, k), tc = A.idx(ic, k);
           const int kl = A.kind[tl], kb = A.kind[tb], kc = A.kind[tc];
           const int m = (int)A.rows(ic), n = (int)A.cols(k);
-          if (kl == kEmpty || kb == kEmpty || kc == kEmpty) continue;
+          if (kl == kEmpty || kb == kEmpty || kc == kEmpty) return;
           const bool prio = ic == i + 1 && k == i + 1;
           auto& gA = gAv[prio];
           auto& gB = gBv[prio];
@@ -1556,10 +1606,18 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
             T* C = A.D[tc];
             if (kl == kLowRank && kb == kLowRank) {
               const int rl = A.rank[tl], rb = A.rank[tb];
-              T* Sx = ctx.take<T>((size_t)rl * rb + 1);
+              T* Sx;
+              int64_t sxld;
+              if (prio) {
+                Sx = ctx.take<T>((size_t)rl * rb + 1);
+                sxld = rb;
+                gB.push_back(gemm_item(P[ic], n_i, 1, Q[k], 1, m_i, Sx, rb, 1, rl, rb, n_i, 1.0, 0.0));
+              } else {
+                Sx = Sxa + (size_t)poff[ic] * RQ + qoff[k];
+                sxld = RQ;
+              }
               T* z = ctx.take<T>((size_t)m * rb + 1);
-              gB.push_back(gemm_item(P[ic], n_i, 1, Q[k], 1, m_i, Sx, rb, 1, rl, rb, n_i, -1.0, 0.0));
-              gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, rb, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
+              gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, sxld, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
               gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
             } else if (kl == kLowRank && kb == kDense) {
               const int rl = A.rank[tl];
@@ -1578,7 +1636,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
             } else {
               gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, C, n, 1, m, n, n_i, -1.0, 1.0));
             }
-            continue;
+            return;
           }
           LrAddDesc<T> ad{};
           ad.cu = A.U[tc];
@@ -1597,14 +1655,12 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           ad.cap = A.kcap;
           if (kl == kLowRank && kb == kLowRank) {  // gemm.cpp:253-272
             const int rl = A.rank[tl], rb = A.rank[tb];
-            T* Sx = ctx.take<T>((size_t)rl * rb + 1);
-            gB.push_back(gemm_item(P[ic], n_i, 1, Q[k], 1, m_i, Sx, rb, 1, rl, rb, n_i, -1.0, 0.0));
             ad.xu = A.U[tl];
             ad.xu_rs = 1;
             ad.xu_cs = A.rows(ic);
             ad.kx1 = rl;
-            ad.xs = Sx;
-            ad.xs_ld = rb;
+            ad.xs = Sxa + (size_t)poff[ic] * RQ + qoff[k];  // P(ic) Q(k) (P carries the minus sign)
+            ad.xs_ld = RQ;
             ad.kx2 = rb;
             ad.xv = A.V[tb];
             ad.xv_rs = A.cols(k);
@@ -1672,7 +1728,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
             ad.kx1 = ad.kx2 = -1;  // known after compression
           }
           adds.push_back(ad);
-        }
+      };
+      pair_body(i + 1, i + 1);
       plan_gemms<T>(pl, gAv[1], "schur_gemm");
       plan_gemms<T>(pl, gBv[1], "schur_gemm");
       plan_gemms<T>(pl, gCv[1], "schur_gemm");
@@ -1682,59 +1739,88 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
       pl.run();
       // tile (i+1, i+1) has received its last update
       FRK_HIP_CHECK(hipEventRecord(la.upd, s));
-      auto lookahead = [&]() {
-        side.sync();
-        side.reserve(diag_getrf_workspace<T>((int)A.rows(i + 1)) + 4096);
-        FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.upd, 0));
+      side.sync();
+      side.reserve(diag_getrf_workspace<T>((int)A.rows(i + 1)) + 4096);
+      FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.upd, 0));
+      {
         Plan pd(side);
         plan_diag_getrf<T>(pd, side, A, i + 1, info + (i + 1) * npan_max);
         pd.run();
-        FRK_HIP_CHECK(hipEventRecord(la.diag, la.s));
-        diag_pending = true;
-      };
-      lookahead();
-      ctx.host_time("host_issue", ms_since(th));
-      th = clk::now();
-      // the rest of the step: remaining panel solves and Schur updates
+      }
+      FRK_HIP_CHECK(hipEventRecord(la.diag, la.s));
+      diag_pending = true;
+      // the rest of the step, first the remaining panel solves and the stacked products
+      plan_rows(0);
       plan_gemms<T>(pl, tgv[0], "trsm_gemm");
       plan_copies<T>(pl, tcpv[0]);
       plan_gemms<T>(pl, gAv[0], "schur_gemm");
-      plan_gemms<T>(pl, gBv[0], "schur_gemm");
-      plan_gemms<T>(pl, gCv[0], "schur_gemm");
-      plan_gemms<T>(pl, gDv[0], "schur_gemm");
-      if (!comp.empty()) {
-        // D.D -> LR needs the compressed ranks on the host: run what is planned,
-        // compress on a side context (own workspace), then plan the additions.
-        pl.run();
-        DevBuf crank(sizeof(int32_t) * comp.size());
-        for (size_t c = 0; c < comp.size(); ++c) comp[c].d_rank = crank.as<int32_t>() + c;
-        Context cside(s);
-        compress_batched<T>(comp, cside);
-        std::vector<int32_t> cr(comp.size());
-        FRK_HIP_CHECK(
-            hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
-        for (size_t c = 0; c < comp.size(); ++c) adds[comp_add[c]].kx1 = adds[comp_add[c]].kx2 = cr[c];
+      if (RP > 0 && RQ > 0) {
+        // Sxa = Pst Qst: rows of Pst against the (column-major) stacked Q
+        std::vector<RowRef> prow(RP);
+        for (int r2 = 0; r2 < RP; ++r2) prow[r2] = RowRef{Pst + (size_t)r2 * n_i, nullptr};
+        const size_t off = pl.add(prow);
+        const int rp_ = RP, rq_ = RQ, kk = n_i;
+        const int64_t qcs = m_i;
+        const hipStream_t st = s;
+        T* sxa = Sxa;
+        pl.launch([=](const char* d) {
+          launch_rows_gemm<T>(reinterpret_cast<const RowRef*>(d + off), rp_, Qst, 1, qcs, sxa, rq_, kk, st);
+        }, "schur_gemm", 2.0 * RP * RQ * n_i, sizeof(T) * ((double)RP * n_i + (double)RQ * m_i + (double)RP * RQ));
+      }
+      pl.run();
+      ctx.host_time("host_issue", ms_since(th));
+      // then the Schur updates of the other targets in row chunks: each chunk is planned
+      // while the previous one executes
+      const int nchunk = std::min(4, rem);
+      for (int c = 0; c < nchunk; ++c) {
+        th = clk::now();
+        const int ic0 = i + 1 + (rem * c) / nchunk, ic1 = i + 1 + (rem * (c + 1)) / nchunk;
+        adds.clear();
+        comp.clear();
+        comp_add.clear();
+        for (auto* g : {&gAv[0], &gBv[0], &gCv[0], &gDv[0]}) g->clear();
+        for (int ic = ic0; ic < ic1; ++ic)
+          for (int k = i + 1; k < nb; ++k)
+            if (!(ic == i + 1 && k == i + 1)) pair_body(ic, k);
+        plan_gemms<T>(pl, gAv[0], "schur_gemm");
+        plan_gemms<T>(pl, gBv[0], "schur_gemm");
+        plan_gemms<T>(pl, gCv[0], "schur_gemm");
+        plan_gemms<T>(pl, gDv[0], "schur_gemm");
+        if (!comp.empty()) {
+          // D.D -> LR needs the compressed ranks on the host: run what is planned,
+          // compress on a side context (own workspace), then plan the additions.
+          pl.run();
+          DevBuf crank(sizeof(int32_t) * comp.size());
+          for (size_t c2 = 0; c2 < comp.size(); ++c2) comp[c2].d_rank = crank.as<int32_t>() + c2;
+          Context cside(s);
+          compress_batched<T>(comp, cside);
+          std::vector<int32_t> cr(comp.size());
+          FRK_HIP_CHECK(
+              hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
+          for (size_t c2 = 0; c2 < comp.size(); ++c2) adds[comp_add[c2]].kx1 = adds[comp_add[c2]].kx2 = cr[c2];
+          if (A.lra == kLraNaive) {
+            lr_add_naive<T>(adds, ctx);
+          } else {
+            Plan p2(ctx);
+            plan_lr_add<T>(p2, adds, A.lra);
+            p2.run();
+          }
+          continue;
+        }
         if (A.lra == kLraNaive) {
+          pl.run();
           lr_add_naive<T>(adds, ctx);
         } else {
-          Plan p2(ctx);
-          plan_lr_add<T>(p2, adds, A.lra);
-          p2.run();
+          plan_lr_add<T>(pl, adds, A.lra);
+          ctx.host_time("host_plan_rest", ms_since(th));
+          th = clk::now();
+          pl.run();
+          ctx.host_time("host_issue", ms_since(th));
         }
-        continue;
-      }
-      if (A.lra == kLraNaive) {
-        pl.run();
-        lr_add_naive<T>(adds, ctx);
-      } else {
-        plan_lr_add<T>(pl, adds, A.lra);
-        ctx.host_time("host_plan_rest", ms_since(th));
-        th = clk::now();
-        pl.run();
-        ctx.host_time("host_issue", ms_since(th));
       }
       continue;
     }
+    plan_rows(0);
     plan_gemms<T>(pl, tgv[0], "trsm_gemm");
     plan_copies<T>(pl, tcpv[0]);
     pl.run();

@@ -2006,18 +2006,23 @@ void launch_convert(TD* dst, const TS* src, int64_t n, hipStream_t s) {
 // solve_update: one workgroup per target block, b -= U (S (V x)) with x
 // staged in LDS, or b -= D x for a dense block.
 // ===========================================================================
+// One wave per output row: 64 lanes over k, all loads of a lane issued before
+// the products, then a wave reduction per right-hand side.
 template <typename T, int NR>
 __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ Tinv, int64_t ld,
                                                          const T* __restrict__ b, T* __restrict__ w, int n,
                                                          int nrhs) {
-  const int tid = threadIdx.x, g = tid >> 5, l = tid & 31;
-  const int row = 8 * blockIdx.x + g;
+  const int lane = threadIdx.x & 63;
+  const int row = 4 * blockIdx.x + (threadIdx.x >> 6);
+  if (row >= n) return;
   T acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = T(0);
-  if (row < n) {
-    const T* tr = Tinv + (int64_t)row * ld;
-    for (int k = l; k < n; k += 32) {
+  const T* tr = Tinv + (int64_t)row * ld;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int k = lane + 64 * j;
+    if (k < n) {
       const T a = tr[k];
       const T* bk = b + (int64_t)k * nrhs;
 #pragma unroll
@@ -2026,40 +2031,51 @@ __global__ __launch_bounds__(256) void diag_apply_kernel(const T* __restrict__ T
     }
   }
 #pragma unroll
-  for (int c = 0; c < NR; ++c)
+  for (int c = 0; c < NR; ++c) acc[c] = wave_sum(acc[c]);
+  if (lane < nrhs) {
+    T v = acc[0];
 #pragma unroll
-    for (int o = 16; o > 0; o >>= 1) acc[c] += __shfl_xor(acc[c], o, 32);
-  if (row < n && l == 0)
-#pragma unroll
-    for (int c = 0; c < NR; ++c)
-      if (c < nrhs) w[(int64_t)row * nrhs + c] = acc[c];
+    for (int c = 1; c < NR; ++c)
+      if (lane == c) v = acc[c];
+    w[(int64_t)row * nrhs + lane] = v;
+  }
 }
 
+// Transpose: a workgroup per 16 output rows (columns of Tinv), 16 groups of
+// 16 lanes over k, partial sums combined in LDS.
 template <typename T, int NR>
 __global__ __launch_bounds__(256) void diag_apply_t_kernel(const T* __restrict__ Tinv, int64_t ld,
                                                            const T* __restrict__ b, T* __restrict__ w, int n,
                                                            int nrhs) {
-  __shared__ T red[4][64][NR];
-  const int tid = threadIdx.x, kk = tid >> 6, l = tid & 63;
-  const int row = 64 * blockIdx.x + l;
+  __shared__ T red[16][16][NR + 1];
+  const int tid = threadIdx.x, cc = tid & 15, kq = tid >> 4;
+  const int row = 16 * blockIdx.x + cc;
   T acc[NR];
 #pragma unroll
   for (int c = 0; c < NR; ++c) acc[c] = T(0);
-  if (row < n)
-    for (int k = kk; k < n; k += 4) {
+  if (row < n) {
+#pragma unroll 4
+    for (int k = kq; k < n; k += 16) {
       const T a = Tinv[(int64_t)k * ld + row];
       const T* bk = b + (int64_t)k * nrhs;
 #pragma unroll
       for (int c = 0; c < NR; ++c)
         if (c < nrhs) acc[c] += a * bk[c];
     }
+  }
 #pragma unroll
-  for (int c = 0; c < NR; ++c) red[kk][l][c] = acc[c];
+  for (int c = 0; c < NR; ++c) red[kq][cc][c] = acc[c];
   __syncthreads();
-  if (kk == 0 && row < n)
+  if (tid < 16 * NR) {
+    const int r16 = tid / NR, c = tid % NR;
+    const int orow = 16 * blockIdx.x + r16;
+    if (orow < n && c < nrhs) {
+      T sum = T(0);
 #pragma unroll
-    for (int c = 0; c < NR; ++c)
-      if (c < nrhs) w[(int64_t)row * nrhs + c] = red[0][l][c] + red[1][l][c] + red[2][l][c] + red[3][l][c];
+      for (int q = 0; q < 16; ++q) sum += red[q][r16][c];
+      w[(int64_t)orow * nrhs + c] = sum;
+    }
+  }
 }
 
 template <typename T>
@@ -2067,13 +2083,13 @@ void launch_diag_apply(const T* Tinv, int64_t ld, bool trans, const T* b, T* w, 
   if (n <= 0 || n > 512 || nrhs <= 0 || nrhs > 16) throw std::runtime_error("diag apply: n <= 512, nrhs <= 16");
   const bool wide = nrhs > 8;
   if (!trans) {
-    const dim3 grid((n + 7) / 8);
+    const dim3 grid((n + 3) / 4);
     if (wide)
       hipLaunchKernelGGL((diag_apply_kernel<T, 16>), grid, dim3(256), 0, s, Tinv, ld, b, w, n, nrhs);
     else
       hipLaunchKernelGGL((diag_apply_kernel<T, 8>), grid, dim3(256), 0, s, Tinv, ld, b, w, n, nrhs);
   } else {
-    const dim3 grid((n + 63) / 64);
+    const dim3 grid((n + 15) / 16);
     if (wide)
       hipLaunchKernelGGL((diag_apply_t_kernel<T, 16>), grid, dim3(256), 0, s, Tinv, ld, b, w, n, nrhs);
     else
@@ -2117,25 +2133,28 @@ __global__ __launch_bounds__(256) void solve_update_kernel(const SolveUpdItem* _
   const T* S = static_cast<const T*>(it.S);
   const T* V = static_cast<const T*>(it.V);
   const int r = it.r;
-  {  // t1 = V x: four lanes per row of V, then a quad reduction
-    const int rho = tid >> 2, part = tid & 3;
-    T acc[NR];
+  {  // t1 = V x: one wave per row of V (lanes over k), wave reductions
+    const int wv = tid >> 6, lane = tid & 63;
+    for (int rho = wv; rho < r; rho += 4) {
+      T acc[NR];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) acc[c] = T(0);
-    if (rho < r)
-      for (int k = part; k < it.n; k += 4) {
-        const T a = V[rho * it.v_rs + k * it.v_cs];
+      for (int c = 0; c < NR; ++c) acc[c] = T(0);
+      const T* vr = V + rho * it.v_rs;
 #pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] += a * xs[k][c];
+      for (int j = 0; j < 8; ++j) {
+        const int k = lane + 64 * j;
+        if (k < it.n) {
+          const T a = vr[k * it.v_cs];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] += a * xs[k][c];
+        }
       }
 #pragma unroll
-    for (int c = 0; c < NR; ++c) {
-      acc[c] += __shfl_xor(acc[c], 1, 4);
-      acc[c] += __shfl_xor(acc[c], 2, 4);
-    }
-    if (rho < r && part == 0)
+      for (int c = 0; c < NR; ++c) acc[c] = wave_sum(acc[c]);
+      if (lane == 0)
 #pragma unroll
-      for (int c = 0; c < NR; ++c) t1[rho][c] = acc[c];
+        for (int c = 0; c < NR; ++c) t1[rho][c] = acc[c];
+    }
   }
   __syncthreads();
   for (int e = tid; e < r * NR; e += 256) {  // t2 = S t1
@@ -2174,6 +2193,84 @@ template void launch_diag_apply<double>(const double*, int64_t, bool, const doub
 template void launch_diag_apply<float>(const float*, int64_t, bool, const float*, float*, int, int, hipStream_t);
 template void launch_solve_update<double>(const SolveUpdItem*, int, int, hipStream_t);
 template void launch_solve_update<float>(const SolveUpdItem*, int, int, hipStream_t);
+
+
+// ===========================================================================
+// Stacked-row products: the low-rank panel solves of one elimination step
+// (V <- V U^-1 for every tile of the block column, U^T <- U^T L^-T for every
+// tile of the block row) as one GEMM over the concatenated rows.  A 16 x 64
+// output tile per workgroup, K staged through LDS in chunks of 64.
+// ===========================================================================
+// The 16-row slab is staged in LDS once; the four waves split K, every lane
+// owns one output column (so the reads of B are contiguous in the row-major
+// case and cache-line reuse serves the transposed one), and the partial sums
+// of the waves meet in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
+                                                        const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
+                                                        T* __restrict__ W, int N, int K) {
+  __shared__ T As[16][512 + 1];
+  __shared__ T red[4][16][64];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int r0 = 16 * blockIdx.x, col = 64 * blockIdx.y + lane;
+  for (int r = 0; r < 16; ++r) {
+    const T* a = (r0 + r < R) ? static_cast<const T*>(rows[r0 + r].a) : nullptr;
+    for (int k = tid; k < K; k += 256) As[r][k] = a ? a[k] : T(0);
+  }
+  __syncthreads();
+  T acc[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = T(0);
+  const int kq = (K + 3) / 4, k0 = wv * kq, k1 = min(K, k0 + kq);
+  if (col < N) {
+    const T* bc = B + (int64_t)col * b_cs;
+#pragma unroll 4
+    for (int k = k0; k < k1; ++k) {
+      const T bv = bc[(int64_t)k * b_rs];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] += As[r][k] * bv;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) red[wv][r][lane] = acc[r];
+  __syncthreads();
+  for (int e = tid; e < 16 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const int oc = 64 * blockIdx.y + c;
+    if (r0 + r < R && oc < N) W[(int64_t)(r0 + r) * N + oc] = red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void rows_scatter_kernel(const RowRef* __restrict__ rows, int R,
+                                                           const T* __restrict__ W, int N) {
+  const int r = blockIdx.x;
+  if (r >= R) return;
+  T* c = static_cast<T*>(rows[r].c);
+  for (int j = threadIdx.x; j < N; j += 256) c[j] = W[(int64_t)r * N + j];
+}
+
+template <typename T>
+void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int64_t b_cs, T* W, int N, int K,
+                      hipStream_t s) {
+  if (R <= 0) return;
+  if (N <= 0 || K <= 0 || K > 512) throw std::runtime_error("rows gemm: N > 0, 0 < K <= 512");
+  hipLaunchKernelGGL(rows_gemm_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64), dim3(256), 0, s, d_rows, R, B, b_rs,
+                     b_cs, W, N, K);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s) {
+  if (R <= 0) return;
+  hipLaunchKernelGGL(rows_scatter_kernel<T>, dim3(R), dim3(256), 0, s, d_rows, R, W, N);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_rows_gemm<double>(const RowRef*, int, const double*, int64_t, int64_t, double*, int, int,
+                                       hipStream_t);
+template void launch_rows_gemm<float>(const RowRef*, int, const float*, int64_t, int64_t, float*, int, int,
+                                      hipStream_t);
+template void launch_rows_scatter<double>(const RowRef*, int, const double*, int, hipStream_t);
+template void launch_rows_scatter<float>(const RowRef*, int, const float*, int, hipStream_t);
 
 constexpr int kResMaxRhs = 16;
 constexpr int kResSplits = 16;

@@ -106,6 +106,13 @@ void launch_diag_apply(const T* Tinv, int64_t ld, bool trans, const T* b, T* w, 
 // One workgroup per SolveUpdItem (n <= 512, r <= 64, nrhs <= 16).
 template <typename T>
 void launch_solve_update(const SolveUpdItem* d_items, int n_items, int nrhs, hipStream_t s);
+// Stacked-row product W (R x N, row-major) = [rows[0].a; ...; rows[R-1].a] (R x K) B (K x N, strided),
+// then (scatter) each row of W copied to rows[r].c.  K <= 512.
+template <typename T>
+void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int64_t b_cs, T* W, int N, int K,
+                      hipStream_t s);
+template <typename T>
+void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s);
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R,
                             int64_t n, int nrhs, hipStream_t s);
 
