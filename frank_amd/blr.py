@@ -74,6 +74,11 @@ class Context:
                         "timed_launches": t.value}
         return out
 
+    def set_rank_cap(self, truncate: bool):
+        """Rank policy at the slot capacity for matrices built from now on:
+        False raises on an eps-rank above kcap, True keeps the leading kcap terms."""
+        L.check(L.lib().frk_ctx_set_rank_cap(self.h, 1 if truncate else 0))
+
     def reset_op_stats(self):
         L.check(L.lib().frk_ctx_reset_op_stats(self.h))
 

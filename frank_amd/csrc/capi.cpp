@@ -1,4 +1,5 @@
 // extern "C" boundary over the engine (declarations in include/frank_amd.h).
+#include <atomic>
 #include <algorithm>
 #include <cstring>
 #include <memory>
@@ -10,13 +11,18 @@
 
 using namespace frk;
 
+// Handles are reference counted: a matrix holds its context and a QR factor
+// holds its matrix, so destroying them in any order (e.g. a garbage collector
+// finalizing a cycle) never leaves a dangling owner.
 struct frk_ctx_s {
   hipStream_t stream = nullptr;
   std::unique_ptr<Context> ctx;
+  std::atomic<int> refs{1};
 };
 
 struct frk_blr_s {
   frk_ctx_s* owner = nullptr;
+  std::atomic<int> refs{1};
   int precision = FRK_PREC_F64;
   std::unique_ptr<BlrMatrix<double>> a64;
   std::unique_ptr<BlrMatrix<float>> a32;
@@ -35,6 +41,22 @@ struct frk_qrf_s {
 
 namespace {
 thread_local std::string g_err;
+
+void ctx_release(frk_ctx_s* c) {
+  if (--c->refs > 0) return;
+  (void)hipStreamSynchronize(c->stream);
+  c->ctx.reset();
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+void blr_release(frk_blr_s* a) {
+  if (--a->refs > 0) return;
+  frk_ctx_s* o = a->owner;
+  (void)hipStreamSynchronize(o->stream);
+  delete a;
+  ctx_release(o);
+}
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -192,9 +214,7 @@ int frk_ctx_destroy(frk_ctx ctx) {
   return guard([&] {
     if (!ctx) return FRK_OK;
     FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
-    ctx->ctx.reset();
-    FRK_HIP_CHECK(hipStreamDestroy(ctx->stream));
-    delete ctx;
+    ctx_release(ctx);  // freed once the last matrix built on it is destroyed
     return FRK_OK;
   });
 }
@@ -222,7 +242,10 @@ int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset) {
     FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     FRK_HIP_CHECK(hipMemcpy(c, ctx->ctx->counters(), sizeof(c), hipMemcpyDeviceToHost));
     for (int i = 0; i < n && i < kNumCounters; ++i) out[i] = c[i];
-    if (reset) FRK_HIP_CHECK(hipMemset(ctx->ctx->counters(), 0, sizeof(c)));
+    if (reset) {
+      FRK_HIP_CHECK(hipMemsetAsync(ctx->ctx->counters(), 0, sizeof(c), ctx->stream));
+      FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    }
     return FRK_OK;
   });
 }
@@ -232,6 +255,14 @@ int frk_ctx_time_class(frk_ctx ctx, const char* cls) {
     if (!ctx) throw std::invalid_argument("ctx is null");
     ctx->ctx->sync();
     ctx->ctx->time_class(cls ? std::string(cls) : std::string());
+    return FRK_OK;
+  });
+}
+
+int frk_ctx_set_rank_cap(frk_ctx ctx, int truncate) {
+  return guard([&] {
+    if (!ctx) throw std::invalid_argument("ctx is null");
+    ctx->ctx->set_cap_trunc(truncate != 0);
     return FRK_OK;
   });
 }
@@ -310,6 +341,7 @@ int frk_blr_create_kernel_rect(frk_ctx ctx, int kernel_kind, int dim, int64_t n_
     h->ks = upload_kernel(h->kd, h->pts, ctx->stream);
     h->has_kernel = n_rows == n_cols && n_rows == n_pts;
     FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
   });
@@ -344,6 +376,7 @@ int frk_blr_create(frk_ctx ctx, int nb_r, int nb_c, const int64_t* roff, const i
     else
       throw std::invalid_argument("unknown precision");
     FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
   });
@@ -372,6 +405,7 @@ int frk_blr_clone(frk_blr src, frk_blr* out) {
       h->anorm = src->anorm;
     }
     FRK_HIP_CHECK(hipStreamSynchronize(src->owner->stream));
+    ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
   });
@@ -398,6 +432,7 @@ int frk_blr_convert(frk_blr src, int precision, frk_blr* out) {
       h->anorm = src->anorm;
     }
     FRK_HIP_CHECK(hipStreamSynchronize(src->owner->stream));
+    ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
   });
@@ -435,7 +470,7 @@ int frk_blr_destroy(frk_blr a) {
   return guard([&] {
     if (!a) return FRK_OK;
     FRK_HIP_CHECK(hipStreamSynchronize(a->owner->stream));
-    delete a;
+    blr_release(a);
     return FRK_OK;
   });
 }
@@ -531,6 +566,7 @@ int frk_blr_qr(frk_blr a, frk_qrf* out) {
     f->a = a;
     f->f64 = std::make_unique<QrFactors<double>>(*a->a64);
     blr_tiled_qr<double>(*a->a64, *f->f64, *a->owner->ctx);
+    ++a->refs;  // the factors hold their matrix
     *out = f.release();
     return FRK_OK;
   });
@@ -540,7 +576,9 @@ int frk_qrf_destroy(frk_qrf f) {
   return guard([&] {
     if (!f) return FRK_OK;
     FRK_HIP_CHECK(hipStreamSynchronize(f->a->owner->stream));
+    frk_blr_s* a = f->a;
     delete f;
+    blr_release(a);
     return FRK_OK;
   });
 }

@@ -172,6 +172,8 @@ struct SvdItem {
   int32_t K, ldr, lds_in, kdim;  // kdim > 0: min-dimension used by the truncation rule
   int32_t ka, kx, fixed_rank, mode;
   int32_t lds, ldu, ldv, cap;
+  int32_t cap_trunc;  // a rank above cap keeps the leading cap terms (else the rank is reported negated)
+  int32_t pad2;
   double eps;
 };
 
@@ -185,6 +187,8 @@ struct Geqp3Item {
   int32_t* piv;
   int32_t* rank_out;
   int32_t m, n, lda, cap;
+  int32_t cap_trunc;  // stop after cap pivots instead of reporting the overflow
+  int32_t pad;
   double eps;
 };
 

@@ -25,6 +25,16 @@ void DevBuf::ensure(size_t bytes) {
   bytes_ = 0;
   FRK_HIP_CHECK(hipMalloc(&p_, bytes));
   bytes_ = bytes;
+  // FRK_POISON=1 fills every new allocation with NaN bit patterns, so a read of
+  // memory no kernel has written shows up in the results (a debugging aid).
+  static const bool poison = [] {
+    const char* e = std::getenv("FRK_POISON");
+    return e && e[0] == '1';
+  }();
+  if (poison) {
+    FRK_HIP_CHECK(hipMemset(p_, 0xFF, bytes));
+    FRK_HIP_CHECK(hipDeviceSynchronize());
+  }
 }
 
 Staging::~Staging() {
@@ -672,6 +682,7 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     sv.ldv = K;
     sv.rank_out = x.d_rank;
     sv.cap = x.cap;
+    sv.cap_trunc = x.cap_trunc;
     sv.stats = ctx.counters();
     sv.vwork = Vw ? Vw + (size_t)t * K * K : nullptr;
     si[t] = sv;
@@ -923,6 +934,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
       g.n = de[t].n;
       g.lda = (int)de[t].lda;
       g.cap = de[t].cap;
+      g.cap_trunc = de[t].cap_trunc;
       g.eps = de[t].eps;
       gi[t] = g;
     }
@@ -1257,6 +1269,7 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb_r, i
                                 : kDense;
   const int nb = nb_c;
   auto A = std::make_unique<BlrMatrix<T>>(roff, coff, kinds, kcap, eps, rank, ctx);
+  A->cap_trunc = ctx.cap_trunc();
   const auto& off = roff;
   DevBuf pts;
   KernelSpec ks = upload_kernel(kd, pts, s);
@@ -1327,6 +1340,7 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb_r, i
       c.lds = kcap;
       c.d_rank = A->d_rank(i, j);
       c.cap = kcap;
+      c.cap_trunc = ctx.cap_trunc();
       cd.push_back(c);
     }
     {
@@ -1653,6 +1667,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
           ad.n = n;
           ad.eps = A.eps;
           ad.cap = A.kcap;
+          ad.cap_trunc = A.cap_trunc;
           if (kl == kLowRank && kb == kLowRank) {  // gemm.cpp:253-272
             const int rl = A.rank[tl], rb = A.rank[tb];
             ad.xu = A.U[tl];
@@ -1715,6 +1730,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
             c.v_cs = 1;
             c.lds = A.kcap;
             c.cap = A.kcap;
+            c.cap_trunc = A.cap_trunc;
             comp.push_back(c);
             comp_add.push_back((int)adds.size());
             ad.xu = xu;

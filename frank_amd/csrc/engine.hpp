@@ -95,6 +95,9 @@ class Context {
   template <class X>
   const X* omega(int n, int p);
   int32_t* info() { return info_.as<int32_t>(); }
+  // Rank policy of the matrices built on this context (BlrMatrix::cap_trunc).
+  void set_cap_trunc(bool on) { cap_trunc_ = on; }
+  bool cap_trunc() const { return cap_trunc_; }
   // Per-block flags of the dataflow tile LU (zero at allocation, kTileFlags
   // ints; the last one is the timeout marker) and the launch epochs that
   // index them: every launch on this context takes the next epoch.
@@ -134,6 +137,7 @@ class Context {
   DevBuf counters_;
   DevBuf tile_flags_;
   int32_t tile_epoch_ = 0;
+  bool cap_trunc_ = false;
   std::map<std::string, OpStat> stats_;
   std::string timed_cls_;
   std::vector<hipEvent_t> ev_pool_;
@@ -212,6 +216,9 @@ class BlrMatrix {
   double eps;      // fixed accuracy when != 0
   int fixed_rank;  // used when eps == 0
   int lra = kLraFast;
+  // Rank policy at the slot capacity: false reports an overflow as an error,
+  // true keeps the best kcap terms (the capped-rank workloads).
+  bool cap_trunc = false;
   std::vector<T*> D, U, S, V;  // per tile slots (nullptr when unused)
   // inverses of the 64 x 64 diagonal blocks of the L and U factors of every
   // square dense diagonal tile (written by getrf, read by the blocked trsm)
@@ -240,6 +247,7 @@ struct LrAddDesc {
   int m, n;
   double eps;  // != 0: fixed accuracy (H2 rule); 0: keep kc
   int cap;
+  bool cap_trunc = false;  // keep the leading cap terms when the eps-rank exceeds cap
   // addend X = Ux Sx Vx, Ux m x kx1 (strided), Sx kx1 x kx2 (nullptr = identity),
   // Vx kx2 x n (strided)
   const T* xu;
@@ -278,6 +286,7 @@ struct CompressDesc {
   int lds;
   int32_t* d_rank;
   int cap;
+  bool cap_trunc = false;
 };
 
 // LowRank(Dense, eps) / LowRank(Dense, rank) for every descriptor.  Drains
@@ -384,6 +393,7 @@ struct Target {  // dense view, or low-rank slots in the BlrMatrix layout
   int32_t* d_rank;
   double eps;
   int fixed_rank;
+  bool cap_trunc = false;
 };
 
 template <typename T>

@@ -1200,6 +1200,8 @@ __global__ __launch_bounds__(W * 64) void tpqrt_kernel(const TpqrtItem* __restri
         const int rr = e / K, c = e % K, row = j0 + rr;
         Rst[rr][c] = (row < K && c >= row) ? R[row * it.r_rs + c * it.r_cs] : T(0);
       }
+      // the pivot Rst[jj][j] read below is staged by another wave
+      __syncthreads();
     }
     const int jj = j - j0;
     const int buf = j & 1;
@@ -1572,7 +1574,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
       } while (r < md && sqrt(err) > threshold);
     }
     if (r > it.cap) {
-      if (it.rank_out) *it.rank_out = -r;
+      if (it.rank_out) *it.rank_out = it.cap_trunc ? it.cap : -r;
       r = it.cap;
     } else if (it.rank_out) {
       *it.rank_out = r;
@@ -1760,7 +1762,7 @@ __global__ __launch_bounds__(512) void geqp3_kernel(const Geqp3Item* __restrict_
   }
   // eps < 0: exactly min(cap, min_dim) pivot steps (dgeqp3 prefix, used by the ID)
   const bool fixed = it.eps < 0.0;
-  const int steps = fixed ? min(it.cap, min_dim) : min_dim;
+  const int steps = (fixed || it.cap_trunc) ? min(it.cap, min_dim) : min_dim;
   int r = 0;
   while (r < steps && (fixed || maxv > threshold)) {
     // pivot: first maximum of cn[r:] (computed at the end of the previous step)
@@ -1854,7 +1856,7 @@ __global__ __launch_bounds__(512) void geqp3_kernel(const Geqp3Item* __restrict_
     }
   }
   for (int c = tid; c < n; c += 512) it.piv[c] = piv[c];
-  if (tid == 0) *it.rank_out = (!fixed && r > it.cap) ? -r : r;
+  if (tid == 0) *it.rank_out = (!fixed && !it.cap_trunc && r > it.cap) ? -r : r;
 }
 
 template <typename T>

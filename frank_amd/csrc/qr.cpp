@@ -96,6 +96,7 @@ Target<T> tile_target(BlrMatrix<T>& A, int i, int j) {
     c.s = A.S[t];
     c.v = A.V[t];
     c.cap = A.kcap;
+    c.cap_trunc = A.cap_trunc;
     c.rank = A.rank[t];
     c.d_rank = A.d_rank(i, j);
     c.eps = A.eps;
@@ -201,6 +202,7 @@ void typed_gemm(TypedBatch<T>& b, Context& ctx, const Op<T>& A, const Op<T>& B, 
   ad.n = n;
   ad.eps = C.eps;
   ad.cap = C.cap;
+  ad.cap_trunc = C.cap_trunc;
   // S of the addend must be row-major: materialise a transposed view
   auto s_rowmajor = [&](const Op<T>& o, int& ld) -> const T* {
     if (o.s_cs == 1) {
@@ -277,6 +279,7 @@ void typed_gemm(TypedBatch<T>& b, Context& ctx, const Op<T>& A, const Op<T>& B, 
     c.v_cs = 1;
     c.lds = cap;
     c.cap = cap;
+    c.cap_trunc = C.cap_trunc;
     b.comp.push_back(c);
     b.comp_add.push_back((int)b.adds.size());
     ad.xu = xu;
@@ -476,7 +479,9 @@ QrFactors<T>::QrFactors(const BlrMatrix<T>& A) : nb_r(A.nb_r()), nb_c(A.nb_c()) 
     for (int i = k; i < nb_r; ++i) total += al(b * b) + al(b);
   }
   buf_.ensure(std::max<size_t>(total, 1) * sizeof(T));
+  // the factors are written on a non-blocking stream: finish the fill first
   FRK_HIP_CHECK(hipMemset(buf_.as<T>(), 0, std::max<size_t>(total, 1) * sizeof(T)));
+  FRK_HIP_CHECK(hipDeviceSynchronize());
   Tt.assign(nb_r * nb_c, nullptr);
   tau.assign(nb_r * nb_c, nullptr);
   Yx.assign(nk, nullptr);
@@ -682,6 +687,7 @@ static void tpmqrt_row(const BlrMatrix<T>& Y, QrFactors<T>& F, int i, int k, Blr
         c.d_rank = crank.as<int32_t>() + q;
         c.eps = Cm.eps;
         c.fixed_rank = Cm.fixed_rank;
+        c.cap_trunc = Cm.cap_trunc;
         r0[q] = Aop.r;
         cp.push_back(gemm_item(Aop.u, Aop.u_rs, Aop.u_cs, nullptr, 0, 0, c.u, 1, b, b, Aop.r, 0, 0.0, 0.0));
         cp.push_back(gemm_item(Aop.s, Aop.s_rs, Aop.s_cs, nullptr, 0, 0, c.s, cap, 1, Aop.r, Aop.r, 0, 0.0, 0.0));

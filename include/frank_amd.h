@@ -81,6 +81,13 @@ int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset);
  * context stream.  frk_ctx_op_classes writes the class names, comma
  * separated, into buf (NUL-terminated, truncated to len). */
 int frk_ctx_time_class(frk_ctx ctx, const char* cls);
+
+/* Rank policy at the low-rank slot capacity for matrices built on ctx from
+ * now on (no reference counterpart: the reference's ranks are unbounded; the
+ * capped-rank workloads, e.g. tiled QR with ranks capped at 64, need it).
+ * truncate = 0 (default): an eps-rank above kcap fails with FRK_ERR_CAPACITY.
+ * truncate = 1: compressions and recompressions keep the leading kcap terms. */
+int frk_ctx_set_rank_cap(frk_ctx ctx, int truncate);
 int frk_ctx_op_classes(frk_ctx ctx, char* buf, int len);
 int frk_ctx_op_stats(frk_ctx ctx, const char* cls, int64_t* launches, double* flops, double* bytes, double* ms,
                      int64_t* timed_launches);
