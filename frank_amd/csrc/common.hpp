@@ -233,6 +233,38 @@ struct SolveUpdItem {
 
 // One row of a stacked-row product: a is a contiguous K-vector (input row),
 // c where the finished row goes (contiguous N-vector).
+// Low-rank x low-rank core with device ranks: core = alpha S_l (V_l U_b) S_b.
+// V_l: rl x k (strides v_rs, v_cs), U_b: k x rb, S_l / S_b square with leading
+// dimension lds; core rl x rb with leading dimension ldk; ranks clamped to kcap.
+struct LrCoreItem {
+  const void* vl;
+  int64_t v_rs, v_cs;
+  const void* ub;
+  int64_t u_rs, u_cs;
+  const void* sl;
+  const void* sb;
+  void* core;
+  const int32_t* rl;
+  const int32_t* rb;
+  int32_t lds, ldk, k, kcap;
+  double alpha;
+};
+
+// C (m x n, leading dimension ldc) += U (m x r1) core (r1 x r2) V (r2 x n),
+// r1 / r2 read on the device and clamped to kcap.
+struct LrUpdItem {
+  const void* u;
+  int64_t u_rs, u_cs;
+  const void* core;
+  const void* v;
+  int64_t v_rs, v_cs;
+  void* c;
+  int64_t ldc;
+  const int32_t* r1;
+  const int32_t* r2;
+  int32_t ldk, m, n, kcap;
+};
+
 struct RowRef {
   const void* a;
   void* c;

@@ -1406,22 +1406,184 @@ static size_t diag_getrf_workspace(int n) {
   return (sizeof(T) == 4 ? sizeof(double) * ((size_t)3 * n * n + 768) : 0) + getrf_workspace<double>(n, 1);
 }
 
-// RAII pair of a non-blocking stream and two events for the lookahead
+// The lookahead stream, its events and a pinned snapshot of the rank table.
+// lu (lookahead stream): the factorization of the latest diagonal tile is done;
+// prio (main stream): the priority block of the latest step is done; rest (main
+// stream): the rest of a step has run and its ranks are in snap.
 struct Lookahead {
   hipStream_t s = nullptr;
-  hipEvent_t upd = nullptr, diag = nullptr;
-  Lookahead() {
+  hipEvent_t lu = nullptr, prio = nullptr, rest = nullptr;
+  int32_t* snap = nullptr;
+  explicit Lookahead(size_t nrank) {
     FRK_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    FRK_HIP_CHECK(hipEventCreateWithFlags(&upd, hipEventDisableTiming));
-    FRK_HIP_CHECK(hipEventCreateWithFlags(&diag, hipEventDisableTiming));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&lu, hipEventDisableTiming));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&prio, hipEventDisableTiming));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&rest, hipEventDisableTiming));
+    FRK_HIP_CHECK(hipHostMalloc((void**)&snap, sizeof(int32_t) * std::max<size_t>(nrank, 1)));
   }
   ~Lookahead() {
     (void)hipStreamSynchronize(s);
-    (void)hipEventDestroy(upd);
-    (void)hipEventDestroy(diag);
+    (void)hipEventDestroy(lu);
+    (void)hipEventDestroy(prio);
+    (void)hipEventDestroy(rest);
+    (void)hipHostFree(snap);
     (void)hipStreamDestroy(s);
   }
 };
+
+// Whether the priority block of step k can be planned before the ranks of its
+// panel tiles are known on the host: both low rank (the kernels read the ranks
+// on the device) or both dense (no rank enters).
+template <typename T>
+static bool prio_rank_free(const BlrMatrix<T>& A, int k) {
+  const int kl = A.kind[A.idx(k + 1, k)], kb = A.kind[A.idx(k, k + 1)];
+  return (kl == kLowRank && kb == kLowRank && A.kcap <= 64) || (kl == kDense && kb == kDense);
+}
+
+template <typename T>
+static size_t prio_workspace(const BlrMatrix<T>& A, int64_t bm) {
+  return sizeof(T) * (6 * (size_t)bm * bm + 4 * (size_t)A.kcap * bm + (size_t)A.kcap * A.kcap) + 16 * 256;
+}
+
+// Priority block of step k (getrf.cpp:50-62 restricted to what the diagonal
+// tile k+1 depends on): the panel solves of tiles (k+1, k) and (k, k+1) as
+// products with the explicit inverses of tile k, then the Schur update of the
+// dense tile (k+1, k+1).  With device_ranks (prio_rank_free) the low-rank tiles
+// are processed at capacity and their ranks read on the device; otherwise rl,
+// rb are the ranks of the two panel tiles.
+template <typename T>
+static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int rb, bool device_ranks) {
+  const hipStream_t s = W.stream();
+  const int kk = A.idx(k, k), tl = A.idx(k + 1, k), tb = A.idx(k, k + 1), tc = A.idx(k + 1, k + 1);
+  const int m_k = (int)A.rows(k), n_k = (int)A.cols(k);
+  const int m = (int)A.rows(k + 1), n = (int)A.cols(k + 1);
+  const int kl = A.kind[tl], kb = A.kind[tb];
+  const int32_t* dl = device_ranks ? A.d_rank(k + 1, k) : nullptr;
+  const int32_t* db = device_ranks ? A.d_rank(k, k + 1) : nullptr;
+  if (device_ranks) rl = rb = A.kcap;
+  std::vector<GemmItem> tg, tcp;
+  auto rows_solve = [&](std::vector<RowRef> rows, const T* Bm, int64_t brs, int64_t bcs, int N, const int32_t* dc) {
+    const int R = (int)rows.size();
+    T* Wb = W.take<T>((size_t)R * N + 1);
+    const size_t off = pl.add(rows);
+    pl.launch(
+        [=](const char* d) {
+          const RowRef* rr = reinterpret_cast<const RowRef*>(d + off);
+          launch_rows_gemm<T>(rr, R, Bm, brs, bcs, Wb, N, N, s, dc);
+          launch_rows_scatter<T>(rr, R, Wb, N, s, dc);
+        },
+        "trsm_gemm", 2.0 * R * N * N, sizeof(T) * ((double)N * N + 2.0 * R * N));
+  };
+  if (kl == kLowRank && rl > 0) {  // V <- V U^-1
+    std::vector<RowRef> rows(rl);
+    for (int r = 0; r < rl; ++r) {
+      T* row = A.V[tl] + (size_t)r * A.cols(k);
+      rows[r] = RowRef{row, row};
+    }
+    rows_solve(rows, A.Uinv[kk], n_k, 1, n_k, dl);
+  } else if (kl == kDense) {
+    T* w = W.take<T>((size_t)m * n_k + 1);
+    tg.push_back(gemm_item(A.D[tl], A.cols(k), 1, A.Uinv[kk], n_k, 1, w, n_k, 1, m, n_k, n_k, 1.0, 0.0));
+    tcp.push_back(gemm_item(w, n_k, 1, nullptr, 0, 0, A.D[tl], A.cols(k), 1, m, n_k, 0, 0.0, 0.0));
+  }
+  if (kb == kLowRank && rb > 0) {  // U <- L^-1 U, as U^T <- U^T L^-T on the columns of U
+    std::vector<RowRef> cols(rb);
+    for (int r = 0; r < rb; ++r) {
+      T* col = A.U[tb] + (size_t)r * A.rows(k);
+      cols[r] = RowRef{col, col};
+    }
+    rows_solve(cols, A.Linv[kk], 1, m_k, m_k, db);
+  } else if (kb == kDense) {
+    T* w = W.take<T>((size_t)m_k * n + 1);
+    tg.push_back(gemm_item(A.Linv[kk], m_k, 1, A.D[tb], n, 1, w, n, 1, m_k, n, m_k, 1.0, 0.0));
+    tcp.push_back(gemm_item(w, n, 1, nullptr, 0, 0, A.D[tb], n, 1, m_k, n, 0, 0.0, 0.0));
+  }
+  plan_gemms<T>(pl, tg, "trsm_gemm");
+  plan_copies<T>(pl, tcp);
+  // C(k+1, k+1) -= L(k+1, k) U(k, k+1) by operand kinds (gemm.cpp:126-272)
+  std::vector<GemmItem> gA, gB, gC, gD;
+  T* C = A.D[tc];
+  if (kl == kLowRank && kb == kLowRank && device_ranks) {
+    LrCoreItem ci{};
+    T* core = W.take<T>((size_t)A.kcap * A.kcap + 1);
+    ci.vl = A.V[tl];
+    ci.v_rs = A.cols(k);
+    ci.v_cs = 1;
+    ci.ub = A.U[tb];
+    ci.u_rs = 1;
+    ci.u_cs = A.rows(k);
+    ci.sl = A.S[tl];
+    ci.sb = A.S[tb];
+    ci.core = core;
+    ci.rl = dl;
+    ci.rb = db;
+    ci.lds = A.kcap;
+    ci.ldk = A.kcap;
+    ci.k = n_k;
+    ci.kcap = A.kcap;
+    ci.alpha = -1.0;
+    LrUpdItem ui{};
+    ui.u = A.U[tl];
+    ui.u_rs = 1;
+    ui.u_cs = A.rows(k + 1);
+    ui.core = core;
+    ui.v = A.V[tb];
+    ui.v_rs = A.cols(k + 1);
+    ui.v_cs = 1;
+    ui.c = C;
+    ui.ldc = n;
+    ui.r1 = dl;
+    ui.r2 = db;
+    ui.ldk = A.kcap;
+    ui.m = m;
+    ui.n = n;
+    ui.kcap = A.kcap;
+    pl.launch([=](const char*) {
+      launch_lr_core<T>(ci, s);
+      launch_lr_update<T>(ui, s);
+    }, "schur_gemm", 2.0 * A.kcap * ((double)n_k * A.kcap + (double)m * n), sizeof(T) * (2.0 * m * n));
+  } else if (kl == kLowRank && kb == kLowRank) {
+    if (rl > 0 && rb > 0) {
+      T* P = W.take<T>((size_t)rl * n_k + 1);
+      T* Q = W.take<T>((size_t)m_k * rb + 1);
+      T* Sx = W.take<T>((size_t)rl * rb + 1);
+      T* z = W.take<T>((size_t)m * rb + 1);
+      gA.push_back(gemm_item(A.S[tl], A.kcap, 1, A.V[tl], A.cols(k), 1, P, n_k, 1, rl, n_k, rl, -1.0, 0.0));
+      gA.push_back(gemm_item(A.U[tb], 1, A.rows(k), A.S[tb], A.kcap, 1, Q, 1, m_k, m_k, rb, rb, 1.0, 0.0));
+      gB.push_back(gemm_item(P, n_k, 1, Q, 1, m_k, Sx, rb, 1, rl, rb, n_k, 1.0, 0.0));
+      gC.push_back(gemm_item(A.U[tl], 1, A.rows(k + 1), Sx, rb, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
+      gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k + 1), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+    }
+  } else if (kl == kLowRank && kb == kDense) {
+    if (rl > 0) {
+      T* w = W.take<T>((size_t)rl * n + 1);
+      T* w2 = W.take<T>((size_t)rl * n + 1);
+      gA.push_back(gemm_item(A.V[tl], A.cols(k), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_k, -1.0, 0.0));
+      gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
+      gC.push_back(gemm_item(A.U[tl], 1, A.rows(k + 1), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
+    }
+  } else if (kl == kDense && kb == kLowRank) {
+    if (rb > 0) {
+      T* a = W.take<T>((size_t)m * rb + 1);
+      T* a2 = W.take<T>((size_t)m * rb + 1);
+      gA.push_back(gemm_item(A.D[tl], n_k, 1, A.U[tb], 1, A.rows(k), a, 1, m, m, rb, n_k, -1.0, 0.0));
+      gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
+      gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k + 1), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+    }
+  } else if (kl == kDense && kb == kDense) {
+    gA.push_back(gemm_item(A.D[tl], n_k, 1, A.D[tb], n, 1, C, n, 1, m, n, n_k, -1.0, 1.0));
+  }
+  plan_gemms<T>(pl, gA, "schur_gemm");
+  plan_gemms<T>(pl, gB, "schur_gemm");
+  plan_gemms<T>(pl, gC, "schur_gemm");
+  plan_gemms<T>(pl, gD, "schur_gemm");
+}
+
+// Row chunks of the rest of a step, each planned while the previous one runs.
+// Every chunk ends in one batched recompression whose latency (QR, Jacobi SVD,
+// basis update) does not shrink with the batch, so few chunks win: two measured
+// best at N = 32768 (FP32 tiles), one to four chunks tried.
+constexpr int kRestChunks = 2;
 
 template <typename T>
 void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
@@ -1431,29 +1593,78 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   int32_t* info = ctx.info();
   const int npan_max = 8;
   if (nb * npan_max > 4096) throw std::invalid_argument("too many diagonal tiles for the info buffer");
+  int64_t bm = 1;
   for (int i = 0; i < nb; ++i) {
     const int ii = A.idx(i, i);
     if (A.kind[ii] != kDense || A.rows(i) != A.cols(i))
       throw std::runtime_error("getrf: diagonal tile is not square dense");
     if (A.rows(i) > 512) throw std::runtime_error("getrf: tile size above 512");
+    bm = std::max(bm, A.rows(i));
   }
   FRK_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int32_t) * 4096, s));
+  ctx.sync();
+  A.pull_ranks(s);
   const int32_t epoch0 = ctx.tile_epoch();
-  // Lookahead: the diagonal tile i+1 is final once step i's Schur GEMMs have
-  // run, so its factorization proceeds on a second stream while step i's
-  // low-rank recompressions still run on the first.
-  Lookahead la;
+  // Two streams.  The critical chain is: factorization of the diagonal tile k
+  // (lookahead stream), the priority block of step k (main stream: the panels
+  // (k+1, k), (k, k+1) and the Schur update of (k+1, k+1)), factorization of
+  // tile k+1, ...  The rest of step k runs on the main stream beside the
+  // factorization of tile k+1.  The priority block of step k+1 is queued right
+  // behind the rest of step k, planned before that rest has run: its low-rank
+  // kernels read the panel ranks on the device.  The host waits only for the
+  // ranks of a finished step, to plan the next rest, and is never on the chain.
+  Lookahead la(A.rank.size());
   Context side(la.s);
   side.set_stats_owner(&ctx);
-  bool diag_pending = false;
   using clk = std::chrono::steady_clock;
   auto ms_since = [](clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
-  for (int i = 0; i < nb; ++i) {
+  const size_t side_ws = diag_getrf_workspace<T>((int)bm) + 8192;
+  // factorization of the diagonal tile k on the lookahead stream, after the main stream's work so far
+  auto enqueue_lu = [&](int k, bool after_main) {
+    if (after_main) FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.prio, 0));
+    side.reserve(side_ws);
+    Plan pd(side);
+    plan_diag_getrf<T>(pd, side, A, k, info + k * npan_max);
+    pd.run();
+    FRK_HIP_CHECK(hipEventRecord(la.lu, la.s));
+  };
+  // priority block of step k on the main stream (after the factorization of tile k),
+  // then the factorization of tile k+1; the plan takes workspace from ctx after
+  // whatever the current step has taken
+  auto enqueue_prio = [&](int k, bool device_ranks) {
+    const auto tp = clk::now();
+    FRK_HIP_CHECK(hipStreamWaitEvent(s, la.lu, 0));
+    Plan pp(ctx);
+    plan_prio<T>(pp, ctx, A, k, A.rank[A.idx(k + 1, k)], A.rank[A.idx(k, k + 1)], device_ranks);
+    pp.run();
+    FRK_HIP_CHECK(hipEventRecord(la.prio, s));
+    enqueue_lu(k + 1, true);
+    ctx.host_time("host_prio", ms_since(tp));
+  };
+  enqueue_lu(0, false);
+  if (nb > 1) {
+    ctx.reserve(prio_workspace(A, bm) + 4096);
+    enqueue_prio(0, false);
+  }
+  bool prio_pending = false;  // the priority block of step i waits for the host ranks
+  for (int i = 0; i + 1 < nb; ++i) {
     auto th = clk::now();
-    ctx.sync();
-    A.pull_ranks(s);
+    if (i > 0) {
+      FRK_HIP_CHECK(hipEventSynchronize(la.rest));
+      for (size_t t = 0; t < A.rank.size(); ++t) {
+        A.rank[t] = la.snap[t];
+        if (A.kind[t] == kLowRank && A.rank[t] < 0)
+          throw std::runtime_error("low-rank capacity exceeded: rank " + std::to_string(-A.rank[t]) + " > kcap " +
+                                   std::to_string(A.kcap));
+      }
+    }
     ctx.host_time("host_sync", ms_since(th));
     th = clk::now();
+    if (prio_pending) {
+      ctx.reserve(prio_workspace(A, bm) + 4096);
+      enqueue_prio(i, false);
+      prio_pending = false;
+    }
     const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
     const int ii = A.idx(i, i);
     const int rem = nb - 1 - i;
@@ -1461,13 +1672,12 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     std::vector<LrAddDesc<T>> adds;
     std::vector<CompressDesc<T>> comp;
     std::vector<int> comp_add;
-    size_t need = 4096;
+    size_t need = 4096 + prio_workspace(A, bm);
     {
       // count workspace: panel products + per-update temporaries + addition
       int kmax = 1;
       for (int r = 0; r < (int)A.rank.size(); ++r) kmax = std::max(kmax, A.rank[r]);
-      const size_t bm = (size_t)std::max<int64_t>(A.rows(0), A.cols(0));
-      need += (size_t)rem * 2 * sizeof(T) * (bm * kmax + 256);
+      need += (size_t)rem * 2 * sizeof(T) * ((size_t)bm * kmax + 256);
       {
         size_t rp = 0, rq = 0;
         for (int t2 = i + 1; t2 < nb; ++t2) {
@@ -1477,51 +1687,39 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         need += sizeof(T) * (rp * rq + 2 * (rp + rq) * bm) + 8 * 256;
       }
       for (int t2 = i + 1; t2 < nb; ++t2) {
-        if (A.kind[A.idx(t2, i)] == kDense) need += sizeof(T) * (bm * bm + 64);
-        if (A.kind[A.idx(i, t2)] == kDense) need += sizeof(T) * (bm * bm + 64);
+        if (A.kind[A.idx(t2, i)] == kDense) need += sizeof(T) * ((size_t)bm * bm + 64);
+        if (A.kind[A.idx(i, t2)] == kDense) need += sizeof(T) * ((size_t)bm * bm + 64);
       }
       for (int ic = i + 1; ic < nb; ++ic)
         for (int k = i + 1; k < nb; ++k) {
           const int kl = A.kind[A.idx(ic, i)], kb = A.kind[A.idx(i, k)], kc = A.kind[A.idx(ic, k)];
           if (kc == kDense)
-            need += sizeof(T) * (2 * bm * kmax + (size_t)kmax * kmax + 512);
+            need += sizeof(T) * (2 * (size_t)bm * kmax + (size_t)kmax * kmax + 512);
           else if (kl == kDense && kb == kDense)
-            need += sizeof(T) * (bm * bm + bm * A.kcap * 2 + (size_t)A.kcap * A.kcap + 1024);
+            need += sizeof(T) * ((size_t)bm * bm + (size_t)bm * A.kcap * 2 + (size_t)A.kcap * A.kcap + 1024);
           else
-            need += sizeof(T) * (bm * kmax + (size_t)kmax * kmax + 512);
+            need += sizeof(T) * ((size_t)bm * kmax + (size_t)kmax * kmax + 512);
           if (kc == kLowRank) {
             const size_t K = (size_t)kmax + ((kl == kDense && kb == kDense) ? (size_t)A.kcap : (size_t)kmax);
             need += sizeof(T) * (2 * bm * K + 2 * K + 5 * K * K) + 8 * 256;
           }
         }
     }
-    if (!diag_pending) need += diag_getrf_workspace<T>(m_i);
-    ctx.reserve(need);
+  ctx.reserve(need);
     Plan pl(ctx);
-    // 1. diagonal tile: already factorized on the lookahead stream, or now
-    if (diag_pending) {
-      FRK_HIP_CHECK(hipStreamWaitEvent(s, la.diag, 0));
-      diag_pending = false;
-    } else {
-      plan_diag_getrf<T>(pl, ctx, A, i, info + i * npan_max);
-    }
-    // 2. panel solves (src/operations/LAPACK/getrf.cpp:50-57) as products with the explicit
-    //    triangular inverses, written to workspace and copied back into the slots.
-    //    Index [1] holds the work the next diagonal tile depends on (the panels
-    //    (i+1, i), (i, i+1) and the Schur update of (i+1, i+1)); it is issued
-    //    first so the lookahead factorization starts as early as possible.
-    // Low-rank tiles go through stacked-row products (one launch per side and priority
-    // class): the rows of every V (column tiles) and every column of every U (row tiles).
-    std::vector<GemmItem> tgv[2], tcpv[2];
-    std::vector<RowRef> vrows[2], urows[2];
-    for (int ic = i + 1; ic < nb; ++ic) {
+    // 1. the panel solves (src/operations/LAPACK/getrf.cpp:50-57) not in the priority block,
+    //    as products with the explicit triangular inverses.  Low-rank tiles go through
+    //    stacked-row products (one launch per side): the rows of every V (column tiles)
+    //    and every column of every U (row tiles); dense tiles through a product into
+    //    workspace and a copy back.
+    std::vector<GemmItem> tg, tcp;
+    std::vector<RowRef> vrows, urows;
+    for (int ic = i + 2; ic < nb; ++ic) {
       const int t = A.idx(ic, i);
-      auto& tg = tgv[ic == i + 1];
-      auto& tcp = tcpv[ic == i + 1];
       if (A.kind[t] == kLowRank) {  // V <- V U^-1
         for (int rho = 0; rho < A.rank[t]; ++rho) {
           T* row = A.V[t] + (size_t)rho * A.cols(i);
-          vrows[ic == i + 1].push_back(RowRef{row, row});
+          vrows.push_back(RowRef{row, row});
         }
       } else if (A.kind[t] == kDense) {
         const int m = (int)A.rows(ic);
@@ -1530,14 +1728,12 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         tcp.push_back(gemm_item(w, n_i, 1, nullptr, 0, 0, A.D[t], A.cols(i), 1, m, n_i, 0, 0.0, 0.0));
       }
     }
-    for (int j = i + 1; j < nb; ++j) {
+    for (int j = i + 2; j < nb; ++j) {
       const int t = A.idx(i, j);
-      auto& tg = tgv[j == i + 1];
-      auto& tcp = tcpv[j == i + 1];
       if (A.kind[t] == kLowRank) {  // U <- L^-1 U, as U^T <- U^T L^-T on the columns of U
         for (int rho = 0; rho < A.rank[t]; ++rho) {
           T* col = A.U[t] + (size_t)rho * A.rows(i);
-          urows[j == i + 1].push_back(RowRef{col, col});
+          urows.push_back(RowRef{col, col});
         }
       } else if (A.kind[t] == kDense) {
         const int n = (int)A.cols(j);
@@ -1546,300 +1742,275 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         tcp.push_back(gemm_item(w, n, 1, nullptr, 0, 0, A.D[t], n, 1, m_i, n, 0, 0.0, 0.0));
       }
     }
-    auto plan_rows = [&](int pr) {
-      for (int side2 = 0; side2 < 2; ++side2) {
-        const std::vector<RowRef>& rows = side2 == 0 ? vrows[pr] : urows[pr];
-        if (rows.empty()) continue;
-        const int R = (int)rows.size();
-        const int N = side2 == 0 ? n_i : m_i;
-        T* W = ctx.take<T>((size_t)R * N + 1);
-        const size_t off = pl.add(rows);
-        const T* Bm = side2 == 0 ? A.Uinv[ii] : A.Linv[ii];
-        const int64_t brs = side2 == 0 ? n_i : 1, bcs = side2 == 0 ? 1 : m_i;
-        const hipStream_t st = s;
-        pl.launch(
-            [=](const char* d) {
-              const RowRef* rr = reinterpret_cast<const RowRef*>(d + off);
-              launch_rows_gemm<T>(rr, R, Bm, brs, bcs, W, N, N, st);
-              launch_rows_scatter<T>(rr, R, W, N, st);
-            },
-            "trsm_gemm", 2.0 * R * N * N, sizeof(T) * ((double)N * N + 2.0 * R * N));
+    for (int side2 = 0; side2 < 2; ++side2) {
+      const std::vector<RowRef>& rows = side2 == 0 ? vrows : urows;
+      if (rows.empty()) continue;
+      const int R = (int)rows.size();
+      const int N = side2 == 0 ? n_i : m_i;
+      T* W = ctx.take<T>((size_t)R * N + 1);
+      const size_t off = pl.add(rows);
+      const T* Bm = side2 == 0 ? A.Uinv[ii] : A.Linv[ii];
+      const int64_t brs = side2 == 0 ? n_i : 1, bcs = side2 == 0 ? 1 : m_i;
+      pl.launch(
+          [=](const char* d) {
+            const RowRef* rr = reinterpret_cast<const RowRef*>(d + off);
+            launch_rows_gemm<T>(rr, R, Bm, brs, bcs, W, N, N, s);
+            launch_rows_scatter<T>(rr, R, W, N, s);
+          },
+          "trsm_gemm", 2.0 * R * N * N, sizeof(T) * ((double)N * N + 2.0 * R * N));
+    }
+    plan_gemms<T>(pl, tg, "trsm_gemm");
+    plan_copies<T>(pl, tcp);
+    // 2. Schur updates A(ic,k) -= L(ic,i) A(i,k) (getrf.cpp:58-62, gemm.cpp:126-272) of every
+    // target but (i+1, i+1).  P(ic) = S V of the low-rank tiles (ic, i) and Q(k) = U S of (i, k),
+    // stacked so that all products P(ic) Q(k) of the step come from one GEMM (Sxa, row offsets
+    // poff, column offsets qoff)
+    std::vector<T*> P(nb, nullptr), Q(nb, nullptr);
+    std::vector<int> poff(nb, -1), qoff(nb, -1);
+    int RP = 0, RQ = 0;
+    for (int ic = i + 1; ic < nb; ++ic)
+      if (A.kind[A.idx(ic, i)] == kLowRank) {
+        poff[ic] = RP;
+        RP += A.rank[A.idx(ic, i)];
+      }
+    for (int k = i + 1; k < nb; ++k)
+      if (A.kind[A.idx(i, k)] == kLowRank) {
+        qoff[k] = RQ;
+        RQ += A.rank[A.idx(i, k)];
+      }
+    T* Pst = ctx.take<T>((size_t)RP * n_i + 1);
+    T* Qst = ctx.take<T>((size_t)RQ * m_i + 1);
+    T* Sxa = ctx.take<T>((size_t)RP * RQ + 1);
+    std::vector<GemmItem> gA, gB, gC, gD;
+    for (int ic = i + 1; ic < nb; ++ic) {
+      const int t = A.idx(ic, i);
+      if (A.kind[t] != kLowRank) continue;
+      const int kl = A.rank[t];
+      P[ic] = Pst + (size_t)poff[ic] * n_i;
+      // P(ic) = -S V: the minus sign of the Schur update lives in P
+      gA.push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, -1.0, 0.0));
+    }
+    for (int k = i + 1; k < nb; ++k) {
+      const int t = A.idx(i, k);
+      if (A.kind[t] != kLowRank) continue;
+      const int kb = A.rank[t];
+      Q[k] = Qst + (size_t)qoff[k] * m_i;
+      gA.push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
+    }
+    plan_gemms<T>(pl, gA, "schur_gemm");
+    gA.clear();
+    if (RP > 0 && RQ > 0) {
+      // Sxa = Pst Qst: rows of Pst against the (column-major) stacked Q
+      std::vector<RowRef> prow(RP);
+      for (int r2 = 0; r2 < RP; ++r2) prow[r2] = RowRef{Pst + (size_t)r2 * n_i, nullptr};
+      const size_t off = pl.add(prow);
+      const int rp_ = RP, rq_ = RQ, kk = n_i;
+      const int64_t qcs = m_i;
+      T* sxa = Sxa;
+      pl.launch([=](const char* d) {
+        launch_rows_gemm<T>(reinterpret_cast<const RowRef*>(d + off), rp_, Qst, 1, qcs, sxa, rq_, kk, s);
+      }, "schur_gemm", 2.0 * RP * RQ * n_i, sizeof(T) * ((double)RP * n_i + (double)RQ * m_i + (double)RP * RQ));
+    }
+    // Schur update of one target tile (ic, k): items into gA..gD, additions into adds
+    auto pair_body = [&](int ic, int k) {
+      const int tl = A.idx(ic, i), tb = A.idx(i, k), tc = A.idx(ic, k);
+      const int kl = A.kind[tl], kb = A.kind[tb], kc = A.kind[tc];
+      const int m = (int)A.rows(ic), n = (int)A.cols(k);
+      if (kl == kEmpty || kb == kEmpty || kc == kEmpty) return;
+      if (kc == kDense) {
+        T* C = A.D[tc];
+        if (kl == kLowRank && kb == kLowRank) {
+          const int rl = A.rank[tl], rb = A.rank[tb];
+          T* Sx = Sxa + (size_t)poff[ic] * RQ + qoff[k];
+          T* z = ctx.take<T>((size_t)m * rb + 1);
+          gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, RQ, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
+          gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+        } else if (kl == kLowRank && kb == kDense) {
+          const int rl = A.rank[tl];
+          T* w = ctx.take<T>((size_t)rl * n + 1);
+          T* w2 = ctx.take<T>((size_t)rl * n + 1);
+          gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
+          gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
+          gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
+        } else if (kl == kDense && kb == kLowRank) {
+          const int rb = A.rank[tb];
+          T* a = ctx.take<T>((size_t)m * rb + 1);
+          T* a2 = ctx.take<T>((size_t)m * rb + 1);
+          gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
+          gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
+          gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+        } else {
+          gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, C, n, 1, m, n, n_i, -1.0, 1.0));
+        }
+        return;
+      }
+      LrAddDesc<T> ad{};
+      ad.cu = A.U[tc];
+      ad.cs = A.S[tc];
+      ad.cv = A.V[tc];
+      ad.d_rank = A.d_rank(ic, k);
+      ad.kc = A.rank[tc];
+      ad.cu_rs = 1;
+      ad.cu_cs = m;
+      ad.cv_rs = n;
+      ad.cv_cs = 1;
+      ad.lds = A.kcap;
+      ad.m = m;
+      ad.n = n;
+      ad.eps = A.eps;
+      ad.cap = A.kcap;
+      ad.cap_trunc = A.cap_trunc;
+      if (kl == kLowRank && kb == kLowRank) {  // gemm.cpp:253-272
+        const int rl = A.rank[tl], rb = A.rank[tb];
+        ad.xu = A.U[tl];
+        ad.xu_rs = 1;
+        ad.xu_cs = A.rows(ic);
+        ad.kx1 = rl;
+        ad.xs = Sxa + (size_t)poff[ic] * RQ + qoff[k];  // P(ic) Q(k) (P carries the minus sign)
+        ad.xs_ld = RQ;
+        ad.kx2 = rb;
+        ad.xv = A.V[tb];
+        ad.xv_rs = A.cols(k);
+        ad.xv_cs = 1;
+      } else if (kl == kLowRank && kb == kDense) {  // gemm.cpp:219-234
+        const int rl = A.rank[tl];
+        T* w = ctx.take<T>((size_t)rl * n + 1);
+        gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
+        ad.xu = A.U[tl];
+        ad.xu_rs = 1;
+        ad.xu_cs = A.rows(ic);
+        ad.kx1 = rl;
+        ad.xs = A.S[tl];
+        ad.xs_ld = A.kcap;
+        ad.kx2 = rl;
+        ad.xv = w;
+        ad.xv_rs = n;
+        ad.xv_cs = 1;
+      } else if (kl == kDense && kb == kLowRank) {  // gemm.cpp:236-251
+        const int rb = A.rank[tb];
+        T* a = ctx.take<T>((size_t)m * rb + 1);
+        gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
+        ad.xu = a;
+        ad.xu_rs = 1;
+        ad.xu_cs = m;
+        ad.kx1 = rb;
+        ad.xs = A.S[tb];
+        ad.xs_ld = A.kcap;
+        ad.kx2 = rb;
+        ad.xv = A.V[tb];
+        ad.xv_rs = A.cols(k);
+        ad.xv_cs = 1;
+      } else {  // D.D -> LR: compress the dense product first (gemm.cpp:202-217)
+        T* prod = ctx.take<T>((size_t)m * n + 1);
+        gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, prod, n, 1, m, n, n_i, -1.0, 0.0));
+        T* xu = ctx.take<T>((size_t)m * A.kcap + 1);
+        T* xs = ctx.take<T>((size_t)A.kcap * A.kcap + 1);
+        T* xv = ctx.take<T>((size_t)A.kcap * n + 1);
+        CompressDesc<T> c{};
+        c.a = prod;
+        c.m = m;
+        c.n = n;
+        c.lda = n;
+        c.eps = A.eps;
+        c.rank = A.rank[tc];
+        c.u = xu;
+        c.s = xs;
+        c.v = xv;
+        c.u_rs = 1;
+        c.u_cs = m;
+        c.v_rs = n;
+        c.v_cs = 1;
+        c.lds = A.kcap;
+        c.cap = A.kcap;
+        c.cap_trunc = A.cap_trunc;
+        comp.push_back(c);
+        comp_add.push_back((int)adds.size());
+        ad.xu = xu;
+        ad.xu_rs = 1;
+        ad.xu_cs = m;
+        ad.xs = xs;
+        ad.xs_ld = A.kcap;
+        ad.xv = xv;
+        ad.xv_rs = n;
+        ad.xv_cs = 1;
+        ad.kx1 = ad.kx2 = -1;  // known after compression
+      }
+      adds.push_back(ad);
+    };
+    // naive additions drain the stream and reset the workspace of the context they run on:
+    // give them their own, so Pst, Qst and Sxa survive for the later chunks
+    auto naive_adds = [&]() {
+      if (adds.empty()) return;
+      Context cn(s);
+      cn.set_stats_owner(&ctx);
+      lr_add_naive<T>(adds, cn);
+    };
+    // plan and issue the Schur updates of a list of targets (the plan may already hold launches)
+    auto issue_targets = [&](const std::vector<std::pair<int, int>>& targets, const char* plan_cls) {
+      adds.clear();
+      comp.clear();
+      comp_add.clear();
+      for (auto* g : {&gA, &gB, &gC, &gD}) g->clear();
+      for (const auto& t : targets) pair_body(t.first, t.second);
+      plan_gemms<T>(pl, gA, "schur_gemm");
+      plan_gemms<T>(pl, gB, "schur_gemm");
+      plan_gemms<T>(pl, gC, "schur_gemm");
+      plan_gemms<T>(pl, gD, "schur_gemm");
+      if (!comp.empty()) {
+        // D.D -> LR needs the compressed ranks on the host: run what is planned,
+        // compress on a side context (own workspace), then plan the additions.
+        pl.run();
+        DevBuf crank(sizeof(int32_t) * comp.size());
+        for (size_t c2 = 0; c2 < comp.size(); ++c2) comp[c2].d_rank = crank.as<int32_t>() + c2;
+        Context cside(s);
+        compress_batched<T>(comp, cside);
+        std::vector<int32_t> cr(comp.size());
+        FRK_HIP_CHECK(hipMemcpyAsync(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(),
+                                     hipMemcpyDeviceToHost, s));
+        FRK_HIP_CHECK(hipStreamSynchronize(s));
+        for (size_t c2 = 0; c2 < comp.size(); ++c2) adds[comp_add[c2]].kx1 = adds[comp_add[c2]].kx2 = cr[c2];
+        if (A.lra == kLraNaive) {
+          naive_adds();
+        } else {
+          Plan p2(ctx);
+          plan_lr_add<T>(p2, adds, A.lra);
+          p2.run();
+        }
+        return;
+      }
+      if (A.lra == kLraNaive) {
+        pl.run();
+        naive_adds();
+      } else {
+        plan_lr_add<T>(pl, adds, A.lra);
+        ctx.host_time(plan_cls, ms_since(th));
+        th = clk::now();
+        pl.run();
+        ctx.host_time("host_issue", ms_since(th));
       }
     };
-    plan_rows(1);
-    plan_gemms<T>(pl, tgv[1], "trsm_gemm");
-    plan_copies<T>(pl, tcpv[1]);
-    if (rem > 0) {
-      // 3. Schur updates A(ic,k) -= L(ic,i) A(i,k) (getrf.cpp:58-62, gemm.cpp:126-272)
-      // P(ic) = S V of the low-rank tiles (ic, i) and Q(k) = U S of (i, k), stacked so that all
-      // products P(ic) Q(k) of the step come from one GEMM (Sxa, row offsets poff, column offsets qoff)
-      std::vector<T*> P(nb, nullptr), Q(nb, nullptr);
-      std::vector<int> poff(nb, -1), qoff(nb, -1);
-      int RP = 0, RQ = 0;
-      for (int ic = i + 1; ic < nb; ++ic)
-        if (A.kind[A.idx(ic, i)] == kLowRank) {
-          poff[ic] = RP;
-          RP += A.rank[A.idx(ic, i)];
-        }
-      for (int k = i + 1; k < nb; ++k)
-        if (A.kind[A.idx(i, k)] == kLowRank) {
-          qoff[k] = RQ;
-          RQ += A.rank[A.idx(i, k)];
-        }
-      T* Pst = ctx.take<T>((size_t)RP * n_i + 1);
-      T* Qst = ctx.take<T>((size_t)RQ * m_i + 1);
-      T* Sxa = ctx.take<T>((size_t)RP * RQ + 1);
-      std::vector<GemmItem> gAv[2], gBv[2], gCv[2], gDv[2];
-      for (int ic = i + 1; ic < nb; ++ic) {
-        const int t = A.idx(ic, i);
-        if (A.kind[t] != kLowRank) continue;
-        const int kl = A.rank[t];
-        P[ic] = Pst + (size_t)poff[ic] * n_i;
-        // P(ic) = -S V: the minus sign of the Schur update lives in P
-        gAv[ic == i + 1].push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, -1.0, 0.0));
-      }
-      for (int k = i + 1; k < nb; ++k) {
-        const int t = A.idx(i, k);
-        if (A.kind[t] != kLowRank) continue;
-        const int kb = A.rank[t];
-        Q[k] = Qst + (size_t)qoff[k] * m_i;
-        gAv[k == i + 1].push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
-      }
-      // Schur update of one target tile (ic, k): items into gXv[prio], additions into adds
-      auto pair_body = [&](int ic, int k) {
-          const int tl = A.idx(ic, i), tb = A.idx(i, k), tc = A.idx(ic, k);
-          const int kl = A.kind[tl], kb = A.kind[tb], kc = A.kind[tc];
-          const int m = (int)A.rows(ic), n = (int)A.cols(k);
-          if (kl == kEmpty || kb == kEmpty || kc == kEmpty) return;
-          const bool prio = ic == i + 1 && k == i + 1;
-          auto& gA = gAv[prio];
-          auto& gB = gBv[prio];
-          auto& gC = gCv[prio];
-          auto& gD = gDv[prio];
-          if (kc == kDense) {
-            T* C = A.D[tc];
-            if (kl == kLowRank && kb == kLowRank) {
-              const int rl = A.rank[tl], rb = A.rank[tb];
-              T* Sx;
-              int64_t sxld;
-              if (prio) {
-                Sx = ctx.take<T>((size_t)rl * rb + 1);
-                sxld = rb;
-                gB.push_back(gemm_item(P[ic], n_i, 1, Q[k], 1, m_i, Sx, rb, 1, rl, rb, n_i, 1.0, 0.0));
-              } else {
-                Sx = Sxa + (size_t)poff[ic] * RQ + qoff[k];
-                sxld = RQ;
-              }
-              T* z = ctx.take<T>((size_t)m * rb + 1);
-              gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, sxld, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
-              gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
-            } else if (kl == kLowRank && kb == kDense) {
-              const int rl = A.rank[tl];
-              T* w = ctx.take<T>((size_t)rl * n + 1);
-              T* w2 = ctx.take<T>((size_t)rl * n + 1);
-              gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
-              gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
-              gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
-            } else if (kl == kDense && kb == kLowRank) {
-              const int rb = A.rank[tb];
-              T* a = ctx.take<T>((size_t)m * rb + 1);
-              T* a2 = ctx.take<T>((size_t)m * rb + 1);
-              gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
-              gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
-              gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
-            } else {
-              gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, C, n, 1, m, n, n_i, -1.0, 1.0));
-            }
-            return;
-          }
-          LrAddDesc<T> ad{};
-          ad.cu = A.U[tc];
-          ad.cs = A.S[tc];
-          ad.cv = A.V[tc];
-          ad.d_rank = A.d_rank(ic, k);
-          ad.kc = A.rank[tc];
-          ad.cu_rs = 1;
-          ad.cu_cs = m;
-          ad.cv_rs = n;
-          ad.cv_cs = 1;
-          ad.lds = A.kcap;
-          ad.m = m;
-          ad.n = n;
-          ad.eps = A.eps;
-          ad.cap = A.kcap;
-          ad.cap_trunc = A.cap_trunc;
-          if (kl == kLowRank && kb == kLowRank) {  // gemm.cpp:253-272
-            const int rl = A.rank[tl], rb = A.rank[tb];
-            ad.xu = A.U[tl];
-            ad.xu_rs = 1;
-            ad.xu_cs = A.rows(ic);
-            ad.kx1 = rl;
-            ad.xs = Sxa + (size_t)poff[ic] * RQ + qoff[k];  // P(ic) Q(k) (P carries the minus sign)
-            ad.xs_ld = RQ;
-            ad.kx2 = rb;
-            ad.xv = A.V[tb];
-            ad.xv_rs = A.cols(k);
-            ad.xv_cs = 1;
-          } else if (kl == kLowRank && kb == kDense) {  // gemm.cpp:219-234
-            const int rl = A.rank[tl];
-            T* w = ctx.take<T>((size_t)rl * n + 1);
-            gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
-            ad.xu = A.U[tl];
-            ad.xu_rs = 1;
-            ad.xu_cs = A.rows(ic);
-            ad.kx1 = rl;
-            ad.xs = A.S[tl];
-            ad.xs_ld = A.kcap;
-            ad.kx2 = rl;
-            ad.xv = w;
-            ad.xv_rs = n;
-            ad.xv_cs = 1;
-          } else if (kl == kDense && kb == kLowRank) {  // gemm.cpp:236-251
-            const int rb = A.rank[tb];
-            T* a = ctx.take<T>((size_t)m * rb + 1);
-            gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
-            ad.xu = a;
-            ad.xu_rs = 1;
-            ad.xu_cs = m;
-            ad.kx1 = rb;
-            ad.xs = A.S[tb];
-            ad.xs_ld = A.kcap;
-            ad.kx2 = rb;
-            ad.xv = A.V[tb];
-            ad.xv_rs = A.cols(k);
-            ad.xv_cs = 1;
-          } else {  // D.D -> LR: compress the dense product first (gemm.cpp:202-217)
-            T* prod = ctx.take<T>((size_t)m * n + 1);
-            gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, prod, n, 1, m, n, n_i, -1.0, 0.0));
-            T* xu = ctx.take<T>((size_t)m * A.kcap + 1);
-            T* xs = ctx.take<T>((size_t)A.kcap * A.kcap + 1);
-            T* xv = ctx.take<T>((size_t)A.kcap * n + 1);
-            CompressDesc<T> c{};
-            c.a = prod;
-            c.m = m;
-            c.n = n;
-            c.lda = n;
-            c.eps = A.eps;
-            c.rank = A.rank[tc];
-            c.u = xu;
-            c.s = xs;
-            c.v = xv;
-            c.u_rs = 1;
-            c.u_cs = m;
-            c.v_rs = n;
-            c.v_cs = 1;
-            c.lds = A.kcap;
-            c.cap = A.kcap;
-            c.cap_trunc = A.cap_trunc;
-            comp.push_back(c);
-            comp_add.push_back((int)adds.size());
-            ad.xu = xu;
-            ad.xu_rs = 1;
-            ad.xu_cs = m;
-            ad.xs = xs;
-            ad.xs_ld = A.kcap;
-            ad.xv = xv;
-            ad.xv_rs = n;
-            ad.xv_cs = 1;
-            ad.kx1 = ad.kx2 = -1;  // known after compression
-          }
-          adds.push_back(ad);
-      };
-      pair_body(i + 1, i + 1);
-      plan_gemms<T>(pl, gAv[1], "schur_gemm");
-      plan_gemms<T>(pl, gBv[1], "schur_gemm");
-      plan_gemms<T>(pl, gCv[1], "schur_gemm");
-      plan_gemms<T>(pl, gDv[1], "schur_gemm");
-      ctx.host_time("host_plan", ms_since(th));
-      th = clk::now();
-      pl.run();
-      // tile (i+1, i+1) has received its last update
-      FRK_HIP_CHECK(hipEventRecord(la.upd, s));
-      side.sync();
-      side.reserve(diag_getrf_workspace<T>((int)A.rows(i + 1)) + 4096);
-      FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.upd, 0));
-      {
-        Plan pd(side);
-        plan_diag_getrf<T>(pd, side, A, i + 1, info + (i + 1) * npan_max);
-        pd.run();
-      }
-      FRK_HIP_CHECK(hipEventRecord(la.diag, la.s));
-      diag_pending = true;
-      // the rest of the step, first the remaining panel solves and the stacked products
-      plan_rows(0);
-      plan_gemms<T>(pl, tgv[0], "trsm_gemm");
-      plan_copies<T>(pl, tcpv[0]);
-      plan_gemms<T>(pl, gAv[0], "schur_gemm");
-      if (RP > 0 && RQ > 0) {
-        // Sxa = Pst Qst: rows of Pst against the (column-major) stacked Q
-        std::vector<RowRef> prow(RP);
-        for (int r2 = 0; r2 < RP; ++r2) prow[r2] = RowRef{Pst + (size_t)r2 * n_i, nullptr};
-        const size_t off = pl.add(prow);
-        const int rp_ = RP, rq_ = RQ, kk = n_i;
-        const int64_t qcs = m_i;
-        const hipStream_t st = s;
-        T* sxa = Sxa;
-        pl.launch([=](const char* d) {
-          launch_rows_gemm<T>(reinterpret_cast<const RowRef*>(d + off), rp_, Qst, 1, qcs, sxa, rq_, kk, st);
-        }, "schur_gemm", 2.0 * RP * RQ * n_i, sizeof(T) * ((double)RP * n_i + (double)RQ * m_i + (double)RP * RQ));
-      }
-      pl.run();
-      ctx.host_time("host_issue", ms_since(th));
-      // then the Schur updates of the other targets in row chunks: each chunk is planned
-      // while the previous one executes
-      const int nchunk = std::min(4, rem);
-      for (int c = 0; c < nchunk; ++c) {
-        th = clk::now();
-        const int ic0 = i + 1 + (rem * c) / nchunk, ic1 = i + 1 + (rem * (c + 1)) / nchunk;
-        adds.clear();
-        comp.clear();
-        comp_add.clear();
-        for (auto* g : {&gAv[0], &gBv[0], &gCv[0], &gDv[0]}) g->clear();
-        for (int ic = ic0; ic < ic1; ++ic)
-          for (int k = i + 1; k < nb; ++k)
-            if (!(ic == i + 1 && k == i + 1)) pair_body(ic, k);
-        plan_gemms<T>(pl, gAv[0], "schur_gemm");
-        plan_gemms<T>(pl, gBv[0], "schur_gemm");
-        plan_gemms<T>(pl, gCv[0], "schur_gemm");
-        plan_gemms<T>(pl, gDv[0], "schur_gemm");
-        if (!comp.empty()) {
-          // D.D -> LR needs the compressed ranks on the host: run what is planned,
-          // compress on a side context (own workspace), then plan the additions.
-          pl.run();
-          DevBuf crank(sizeof(int32_t) * comp.size());
-          for (size_t c2 = 0; c2 < comp.size(); ++c2) comp[c2].d_rank = crank.as<int32_t>() + c2;
-          Context cside(s);
-          compress_batched<T>(comp, cside);
-          std::vector<int32_t> cr(comp.size());
-          FRK_HIP_CHECK(
-              hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
-          for (size_t c2 = 0; c2 < comp.size(); ++c2) adds[comp_add[c2]].kx1 = adds[comp_add[c2]].kx2 = cr[c2];
-          if (A.lra == kLraNaive) {
-            lr_add_naive<T>(adds, ctx);
-          } else {
-            Plan p2(ctx);
-            plan_lr_add<T>(p2, adds, A.lra);
-            p2.run();
-          }
-          continue;
-        }
-        if (A.lra == kLraNaive) {
-          pl.run();
-          lr_add_naive<T>(adds, ctx);
-        } else {
-          plan_lr_add<T>(pl, adds, A.lra);
-          ctx.host_time("host_plan_rest", ms_since(th));
-          th = clk::now();
-          pl.run();
-          ctx.host_time("host_issue", ms_since(th));
-        }
-      }
-      continue;
+    // 3. every target but (i+1, i+1), in row chunks; each chunk is planned while the
+    //    previous one executes
+    const int nchunk = std::min(kRestChunks, rem);
+    for (int c = 0; c < nchunk; ++c) {
+      if (c > 0) th = clk::now();
+      const int ic0 = i + 1 + (rem * c) / nchunk, ic1 = i + 1 + (rem * (c + 1)) / nchunk;
+      std::vector<std::pair<int, int>> targets;
+      for (int ic = ic0; ic < ic1; ++ic)
+        for (int k = i + 1; k < nb; ++k)
+          if (!(ic == i + 1 && k == i + 1)) targets.emplace_back(ic, k);
+      issue_targets(targets, c == 0 ? "host_plan" : "host_plan_rest");
     }
-    plan_rows(0);
-    plan_gemms<T>(pl, tgv[0], "trsm_gemm");
-    plan_copies<T>(pl, tcpv[0]);
     pl.run();
+    // 4. the ranks of the finished step for the host, then the next priority block
+    FRK_HIP_CHECK(hipMemcpyAsync(la.snap, A.d_rank(0, 0), sizeof(int32_t) * A.rank.size(), hipMemcpyDeviceToHost, s));
+    FRK_HIP_CHECK(hipEventRecord(la.rest, s));
+    if (i + 2 < nb) {
+      if (prio_rank_free(A, i + 1))
+        enqueue_prio(i + 1, true);
+      else
+        prio_pending = true;
+    }
   }
   ctx.sync();
   side.sync();

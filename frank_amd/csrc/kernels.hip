@@ -2207,14 +2207,19 @@ template void launch_solve_update<float>(const SolveUpdItem*, int, int, hipStrea
 // owns one output column (so the reads of B are contiguous in the row-major
 // case and cache-line reuse serves the transposed one), and the partial sums
 // of the waves meet in LDS.
+// d_count (optional): the number of rows actually present, read on the device
+// (the table then holds R >= count entries); workgroups past it exit.
 template <typename T>
 __global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
                                                         const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
-                                                        T* __restrict__ W, int N, int K) {
+                                                        T* __restrict__ W, int N, int K,
+                                                        const int32_t* __restrict__ d_count) {
   __shared__ T As[16][512 + 1];
   __shared__ T red[4][16][64];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int r0 = 16 * blockIdx.x, col = 64 * blockIdx.y + lane;
+  if (d_count) R = min(R, max(0, *d_count));
+  if (r0 >= R) return;
   for (int r = 0; r < 16; ++r) {
     const T* a = (r0 + r < R) ? static_cast<const T*>(rows[r0 + r].a) : nullptr;
     for (int k = tid; k < K; k += 256) As[r][k] = a ? a[k] : T(0);
@@ -2245,8 +2250,10 @@ __global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict
 
 template <typename T>
 __global__ __launch_bounds__(256) void rows_scatter_kernel(const RowRef* __restrict__ rows, int R,
-                                                           const T* __restrict__ W, int N) {
+                                                           const T* __restrict__ W, int N,
+                                                           const int32_t* __restrict__ d_count) {
   const int r = blockIdx.x;
+  if (d_count) R = min(R, max(0, *d_count));
   if (r >= R) return;
   T* c = static_cast<T*>(rows[r].c);
   for (int j = threadIdx.x; j < N; j += 256) c[j] = W[(int64_t)r * N + j];
@@ -2254,25 +2261,139 @@ __global__ __launch_bounds__(256) void rows_scatter_kernel(const RowRef* __restr
 
 template <typename T>
 void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int64_t b_cs, T* W, int N, int K,
-                      hipStream_t s) {
+                      hipStream_t s, const int32_t* d_count) {
   if (R <= 0) return;
   if (N <= 0 || K <= 0 || K > 512) throw std::runtime_error("rows gemm: N > 0, 0 < K <= 512");
   hipLaunchKernelGGL(rows_gemm_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64), dim3(256), 0, s, d_rows, R, B, b_rs,
-                     b_cs, W, N, K);
+                     b_cs, W, N, K, d_count);
   FRK_HIP_CHECK(hipGetLastError());
 }
 template <typename T>
-void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s) {
+void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s, const int32_t* d_count) {
   if (R <= 0) return;
-  hipLaunchKernelGGL(rows_scatter_kernel<T>, dim3(R), dim3(256), 0, s, d_rows, R, W, N);
+  hipLaunchKernelGGL(rows_scatter_kernel<T>, dim3(R), dim3(256), 0, s, d_rows, R, W, N, d_count);
   FRK_HIP_CHECK(hipGetLastError());
 }
 template void launch_rows_gemm<double>(const RowRef*, int, const double*, int64_t, int64_t, double*, int, int,
-                                       hipStream_t);
+                                       hipStream_t, const int32_t*);
 template void launch_rows_gemm<float>(const RowRef*, int, const float*, int64_t, int64_t, float*, int, int,
-                                      hipStream_t);
-template void launch_rows_scatter<double>(const RowRef*, int, const double*, int, hipStream_t);
-template void launch_rows_scatter<float>(const RowRef*, int, const float*, int, hipStream_t);
+                                      hipStream_t, const int32_t*);
+template void launch_rows_scatter<double>(const RowRef*, int, const double*, int, hipStream_t, const int32_t*);
+template void launch_rows_scatter<float>(const RowRef*, int, const float*, int, hipStream_t, const int32_t*);
+
+// ===========================================================================
+// Schur update of a dense tile by a product of two low-rank tiles whose ranks
+// are read on the device, so the launch can be queued before they are known:
+// C += alpha U_l (S_l (V_l U_b) S_b) V_b.  lr_core_kernel (one workgroup)
+// forms the rl x rb core; lr_update_kernel applies it to 64 x 64 tiles of C.
+// ===========================================================================
+template <typename T>
+__global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
+  __shared__ T M1[64][65];
+  __shared__ T T1[64][65];
+  __shared__ T Va[16][65];
+  __shared__ T Ub[64][17];
+  const int tid = threadIdx.x;
+  const int rl = min(it.kcap, max(0, *it.rl)), rb = min(it.kcap, max(0, *it.rb));
+  if (rl == 0 || rb == 0) return;
+  const T* V = static_cast<const T*>(it.vl);
+  const T* U = static_cast<const T*>(it.ub);
+  const T* Sl = static_cast<const T*>(it.sl);
+  const T* Sb = static_cast<const T*>(it.sb);
+  T* core = static_cast<T*>(it.core);
+  const int K = it.k;
+  const int ti = tid >> 4, tj = tid & 15;
+  // M1 = V_l U_b (rl x rb), 16 x 16 output blocks, K in chunks of 64
+  for (int bi = 0; bi < rl; bi += 16)
+    for (int bj = 0; bj < rb; bj += 16) {
+      T acc = T(0);
+      for (int k0 = 0; k0 < K; k0 += 64) {
+        __syncthreads();
+        for (int e = tid; e < 16 * 64; e += 256) {
+          const int r = e >> 6, kk = e & 63;
+          Va[r][kk] = (bi + r < rl && k0 + kk < K) ? V[(int64_t)(bi + r) * it.v_rs + (int64_t)(k0 + kk) * it.v_cs] : T(0);
+        }
+        for (int e = tid; e < 64 * 16; e += 256) {
+          const int kk = e >> 4, c = e & 15;
+          Ub[kk][c] = (bj + c < rb && k0 + kk < K) ? U[(int64_t)(k0 + kk) * it.u_rs + (int64_t)(bj + c) * it.u_cs] : T(0);
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < 64; ++kk) acc += Va[ti][kk] * Ub[kk][tj];
+      }
+      if (bi + ti < rl && bj + tj < rb) M1[bi + ti][bj + tj] = acc;
+    }
+  __syncthreads();
+  // T1 = M1 S_b, core = alpha S_l T1
+  for (int e = tid; e < rl * rb; e += 256) {
+    const int i = e / rb, j = e % rb;
+    T sacc = T(0);
+    for (int q = 0; q < rb; ++q) sacc += M1[i][q] * Sb[(int64_t)q * it.lds + j];
+    T1[i][j] = sacc;
+  }
+  __syncthreads();
+  for (int e = tid; e < rl * rb; e += 256) {
+    const int i = e / rb, j = e % rb;
+    T sacc = T(0);
+    for (int q = 0; q < rl; ++q) sacc += Sl[(int64_t)i * it.lds + q] * T1[q][j];
+    core[(int64_t)i * it.ldk + j] = T(it.alpha) * sacc;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void lr_update_kernel(LrUpdItem it) {
+  __shared__ T Ks[64][65];
+  __shared__ T Vt[64][65];
+  __shared__ T Wt[64][65];
+  __shared__ T Ut[64][65];
+  const int tid = threadIdx.x;
+  const int r1 = min(it.kcap, max(0, *it.r1)), r2 = min(it.kcap, max(0, *it.r2));
+  if (r1 == 0 || r2 == 0) return;
+  const int c0 = 64 * blockIdx.x, i0 = 64 * blockIdx.y;
+  const T* core = static_cast<const T*>(it.core);
+  const T* V = static_cast<const T*>(it.v);
+  const T* U = static_cast<const T*>(it.u);
+  T* C = static_cast<T*>(it.c);
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int a = e >> 6, b = e & 63;
+    Ks[a][b] = (a < r1 && b < r2) ? core[(int64_t)a * it.ldk + b] : T(0);
+    Vt[a][b] = (a < r2 && c0 + b < it.n) ? V[(int64_t)a * it.v_rs + (int64_t)(c0 + b) * it.v_cs] : T(0);
+    Ut[a][b] = (i0 + a < it.m && b < r1) ? U[(int64_t)(i0 + a) * it.u_rs + (int64_t)b * it.u_cs] : T(0);
+  }
+  __syncthreads();
+  // W = core V_b(:, c0:c0+64)  (r1 x 64)
+  for (int e = tid; e < r1 * 64; e += 256) {
+    const int a = e >> 6, b = e & 63;
+    T sacc = T(0);
+    for (int q = 0; q < r2; ++q) sacc += Ks[a][q] * Vt[q][b];
+    Wt[a][b] = sacc;
+  }
+  __syncthreads();
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int a = e >> 6, b = e & 63;
+    if (i0 + a >= it.m || c0 + b >= it.n) continue;
+    T sacc = T(0);
+    for (int q = 0; q < r1; ++q) sacc += Ut[a][q] * Wt[q][b];
+    C[(int64_t)(i0 + a) * it.ldc + c0 + b] += sacc;
+  }
+}
+
+template <typename T>
+void launch_lr_core(const LrCoreItem& it, hipStream_t s) {
+  if (it.kcap <= 0 || it.kcap > 64 || it.k <= 0) throw std::runtime_error("lr core: 0 < kcap <= 64, k > 0");
+  hipLaunchKernelGGL(lr_core_kernel<T>, dim3(1), dim3(256), 0, s, it);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template <typename T>
+void launch_lr_update(const LrUpdItem& it, hipStream_t s) {
+  if (it.kcap <= 0 || it.kcap > 64 || it.m <= 0 || it.n <= 0) throw std::runtime_error("lr update: 0 < kcap <= 64");
+  hipLaunchKernelGGL(lr_update_kernel<T>, dim3((it.n + 63) / 64, (it.m + 63) / 64), dim3(256), 0, s, it);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_lr_core<double>(const LrCoreItem&, hipStream_t);
+template void launch_lr_core<float>(const LrCoreItem&, hipStream_t);
+template void launch_lr_update<double>(const LrUpdItem&, hipStream_t);
+template void launch_lr_update<float>(const LrUpdItem&, hipStream_t);
 
 constexpr int kResMaxRhs = 16;
 constexpr int kResSplits = 16;

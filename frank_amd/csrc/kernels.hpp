@@ -108,11 +108,19 @@ template <typename T>
 void launch_solve_update(const SolveUpdItem* d_items, int n_items, int nrhs, hipStream_t s);
 // Stacked-row product W (R x N, row-major) = [rows[0].a; ...; rows[R-1].a] (R x K) B (K x N, strided),
 // then (scatter) each row of W copied to rows[r].c.  K <= 512.
+// d_count (optional): row count read on the device, at most R (the table's length).
 template <typename T>
 void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int64_t b_cs, T* W, int N, int K,
-                      hipStream_t s);
+                      hipStream_t s, const int32_t* d_count = nullptr);
 template <typename T>
-void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s);
+void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s,
+                         const int32_t* d_count = nullptr);
+// Low-rank x low-rank Schur update with device ranks (kcap <= 64):
+// core = alpha S_l (V_l U_b) S_b, then C += U_l core V_b on 64 x 64 tiles.
+template <typename T>
+void launch_lr_core(const LrCoreItem& it, hipStream_t s);
+template <typename T>
+void launch_lr_update(const LrUpdItem& it, hipStream_t s);
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R,
                             int64_t n, int nrhs, hipStream_t s);
 
