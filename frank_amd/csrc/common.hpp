@@ -213,6 +213,13 @@ struct TileLuItem {
   int64_t spin_limit;
   int64_t* trace;  // optional: per workgroup 4 wall-clock stamps (start, updates done, factor published,
                    // inverse published)
+  // optional FP32 input (leading dimension ld) and FP32 copies of the results
+  // (L\U: ld, inverses: ldi); the FP64 a / linv / uinv then serve as the
+  // workgroups' exchange buffers and need no initial contents
+  const float* a32_in;
+  float* a32_out;
+  float* linv32;
+  float* uinv32;
 };
 
 // ---------------------------------------------------------------------------
@@ -248,6 +255,28 @@ struct LrCoreItem {
   const int32_t* rb;
   int32_t lds, ldk, k, kcap;
   double alpha;
+  // optional: rows r < rl of V_l written to vl_dst (row stride vl_dst_ld, nv
+  // columns) and columns c < rb of U_b to ub_dst (column stride ub_dst_ld, nu
+  // rows) -- the panel results, when vl / ub point at workspace
+  void* vl_dst;
+  void* ub_dst;
+  int64_t vl_dst_ld, ub_dst_ld;
+  int32_t nv, nu;
+};
+
+// Two stacked-row products in one launch (blockIdx.z selects the job), rows
+// r < R of base + r * stride (R capped on the device by d_count when given).
+struct RowsJob {
+  const void* base;
+  int64_t stride;
+  const void* b;
+  int64_t b_rs, b_cs;
+  void* w;
+  const int32_t* d_count;
+  int32_t R, N, K, pad;
+};
+struct RowsPair {
+  RowsJob job[2];
 };
 
 // C (m x n, leading dimension ldc) += U (m x r1) core (r1 x r2) V (r2 x n),

@@ -454,12 +454,25 @@ static bool use_tile_lu() {
 
 // One-launch FP64 LU of an n x n tile (n <= 512) with full inverses of both
 // factors (row stride n, the opposite triangles zeroed).
-static void plan_tile_lu(Plan& pl, double* a, double* linv, double* uinv, int n, int64_t ld, int32_t* info) {
+// f32 (optional): the FP32 tile read as input and overwritten with L\U, and the
+// FP32 inverses written beside the FP64 ones (a, linv, uinv are then scratch).
+struct TileLu32 {
+  float* a = nullptr;
+  float* linv = nullptr;
+  float* uinv = nullptr;
+};
+
+static void plan_tile_lu(Plan& pl, double* a, double* linv, double* uinv, int n, int64_t ld, int32_t* info,
+                         TileLu32 f32 = {}) {
   Context& c = pl.ctx();
   TileLuItem it{};
   it.a = a;
   it.linv = linv;
   it.uinv = uinv;
+  it.a32_in = f32.a;
+  it.a32_out = f32.a;
+  it.linv32 = f32.linv;
+  it.uinv32 = f32.uinv;
   it.flags = c.tile_flags();
   it.err = it.flags + Context::kTileFlags - 1;
   it.info = info;
@@ -1382,17 +1395,21 @@ static void plan_diag_getrf(Plan& p, Context& c, BlrMatrix<T>& A, int i, int32_t
     T* dt = A.D[ii];
     T* lt = A.Linv[ii];
     T* ut = A.Uinv[ii];
+    if (tile) {
+      // the tile LU reads the FP32 tile and writes the FP32 results itself
+      TileLu32 f32;
+      f32.a = reinterpret_cast<float*>(dt);
+      f32.linv = reinterpret_cast<float*>(lt);
+      f32.uinv = reinterpret_cast<float*>(ut);
+      plan_tile_lu(p, d64, l64, u64, n_i, n_i, info, f32);
+      return;
+    }
     p.launch([=](const char*) {
       launch_convert<double, T>(d64, dt, nn, s);
-      if (!tile) {
-        launch_fill_value(l64, nn, 0.0, s);
-        launch_fill_value(u64, nn, 0.0, s);
-      }
+      launch_fill_value(l64, nn, 0.0, s);
+      launch_fill_value(u64, nn, 0.0, s);
     });
-    if (tile)
-      plan_tile_lu(p, d64, l64, u64, n_i, n_i, info);
-    else
-      plan_getrf<double>(p, {d64}, {l64}, {u64}, m_i, n_i, info);
+    plan_getrf<double>(p, {d64}, {l64}, {u64}, m_i, n_i, info);
     p.launch([=](const char*) {
       launch_convert<T, double>(dt, d64, nn, s);
       launch_convert<T, double>(lt, l64, nn, s);
@@ -1408,14 +1425,16 @@ static size_t diag_getrf_workspace(int n) {
 
 // The lookahead stream, its events and a pinned snapshot of the rank table.
 // lu (lookahead stream): the factorization of the latest diagonal tile is done;
-// prio (main stream): the priority block of the latest step is done; rest (main
-// stream): the rest of a step has run and its ranks are in snap.
+// prio (main stream): the priority block of the latest step is done; done (main
+// stream): the rest of a step has run; rest (copy stream): its ranks are in snap.
 struct Lookahead {
-  hipStream_t s = nullptr;
-  hipEvent_t lu = nullptr, prio = nullptr, rest = nullptr;
+  hipStream_t s = nullptr, cs = nullptr;  // lookahead stream, rank-snapshot copy stream
+  hipEvent_t lu = nullptr, prio = nullptr, rest = nullptr, done = nullptr;
   int32_t* snap = nullptr;
   explicit Lookahead(size_t nrank) {
     FRK_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    FRK_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
     FRK_HIP_CHECK(hipEventCreateWithFlags(&lu, hipEventDisableTiming));
     FRK_HIP_CHECK(hipEventCreateWithFlags(&prio, hipEventDisableTiming));
     FRK_HIP_CHECK(hipEventCreateWithFlags(&rest, hipEventDisableTiming));
@@ -1423,6 +1442,9 @@ struct Lookahead {
   }
   ~Lookahead() {
     (void)hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(cs);
+    (void)hipEventDestroy(done);
+    (void)hipStreamDestroy(cs);
     (void)hipEventDestroy(lu);
     (void)hipEventDestroy(prio);
     (void)hipEventDestroy(rest);
@@ -1461,6 +1483,63 @@ static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int 
   const int32_t* dl = device_ranks ? A.d_rank(k + 1, k) : nullptr;
   const int32_t* db = device_ranks ? A.d_rank(k, k + 1) : nullptr;
   if (device_ranks) rl = rb = A.kcap;
+  if (device_ranks && kl == kLowRank && kb == kLowRank) {
+    // three launches, no descriptor upload: both panel products (V U^-1 and
+    // (L^-1 U)^T, rows at capacity, counts read on the device), the core
+    // -S_l (V_l' U_b') S_b with the panel results written back, the update of C
+    T* wv = W.take<T>((size_t)A.kcap * n_k + 1);
+    T* wu = W.take<T>((size_t)A.kcap * m_k + 1);
+    T* core = W.take<T>((size_t)A.kcap * A.kcap + 1);
+    RowsPair rp{};
+    rp.job[0] = RowsJob{A.V[tl], A.cols(k), A.Uinv[kk], n_k, 1, wv, dl, A.kcap, n_k, n_k, 0};
+    rp.job[1] = RowsJob{A.U[tb], A.rows(k), A.Linv[kk], 1, m_k, wu, db, A.kcap, m_k, m_k, 0};
+    LrCoreItem ci{};
+    ci.vl = wv;
+    ci.v_rs = n_k;
+    ci.v_cs = 1;
+    ci.ub = wu;
+    ci.u_rs = 1;
+    ci.u_cs = m_k;
+    ci.sl = A.S[tl];
+    ci.sb = A.S[tb];
+    ci.core = core;
+    ci.rl = dl;
+    ci.rb = db;
+    ci.lds = A.kcap;
+    ci.ldk = A.kcap;
+    ci.k = n_k;
+    ci.kcap = A.kcap;
+    ci.alpha = -1.0;
+    ci.vl_dst = A.V[tl];
+    ci.vl_dst_ld = A.cols(k);
+    ci.nv = n_k;
+    ci.ub_dst = A.U[tb];
+    ci.ub_dst_ld = A.rows(k);
+    ci.nu = m_k;
+    LrUpdItem ui{};
+    ui.u = A.U[tl];
+    ui.u_rs = 1;
+    ui.u_cs = A.rows(k + 1);
+    ui.core = core;
+    ui.v = A.V[tb];
+    ui.v_rs = A.cols(k + 1);
+    ui.v_cs = 1;
+    ui.c = A.D[tc];
+    ui.ldc = n;
+    ui.r1 = dl;
+    ui.r2 = db;
+    ui.ldk = A.kcap;
+    ui.m = m;
+    ui.n = n;
+    ui.kcap = A.kcap;
+    pl.launch([=](const char*) { launch_rows_pair<T>(rp, s); }, "trsm_gemm",
+              2.0 * A.kcap * ((double)n_k * n_k + (double)m_k * m_k), sizeof(T) * ((double)n_k * n_k + m_k * m_k));
+    pl.launch([=](const char*) {
+      launch_lr_core<T>(ci, s);
+      launch_lr_update<T>(ui, s);
+    }, "schur_gemm", 2.0 * A.kcap * ((double)n_k * A.kcap + (double)m * n), sizeof(T) * (2.0 * m * n));
+    return;
+  }
   std::vector<GemmItem> tg, tcp;
   auto rows_solve = [&](std::vector<RowRef> rows, const T* Bm, int64_t brs, int64_t bcs, int N, const int32_t* dc) {
     const int R = (int)rows.size();
@@ -1503,46 +1582,7 @@ static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int 
   // C(k+1, k+1) -= L(k+1, k) U(k, k+1) by operand kinds (gemm.cpp:126-272)
   std::vector<GemmItem> gA, gB, gC, gD;
   T* C = A.D[tc];
-  if (kl == kLowRank && kb == kLowRank && device_ranks) {
-    LrCoreItem ci{};
-    T* core = W.take<T>((size_t)A.kcap * A.kcap + 1);
-    ci.vl = A.V[tl];
-    ci.v_rs = A.cols(k);
-    ci.v_cs = 1;
-    ci.ub = A.U[tb];
-    ci.u_rs = 1;
-    ci.u_cs = A.rows(k);
-    ci.sl = A.S[tl];
-    ci.sb = A.S[tb];
-    ci.core = core;
-    ci.rl = dl;
-    ci.rb = db;
-    ci.lds = A.kcap;
-    ci.ldk = A.kcap;
-    ci.k = n_k;
-    ci.kcap = A.kcap;
-    ci.alpha = -1.0;
-    LrUpdItem ui{};
-    ui.u = A.U[tl];
-    ui.u_rs = 1;
-    ui.u_cs = A.rows(k + 1);
-    ui.core = core;
-    ui.v = A.V[tb];
-    ui.v_rs = A.cols(k + 1);
-    ui.v_cs = 1;
-    ui.c = C;
-    ui.ldc = n;
-    ui.r1 = dl;
-    ui.r2 = db;
-    ui.ldk = A.kcap;
-    ui.m = m;
-    ui.n = n;
-    ui.kcap = A.kcap;
-    pl.launch([=](const char*) {
-      launch_lr_core<T>(ci, s);
-      launch_lr_update<T>(ui, s);
-    }, "schur_gemm", 2.0 * A.kcap * ((double)n_k * A.kcap + (double)m * n), sizeof(T) * (2.0 * m * n));
-  } else if (kl == kLowRank && kb == kLowRank) {
+  if (kl == kLowRank && kb == kLowRank) {
     if (rl > 0 && rb > 0) {
       T* P = W.take<T>((size_t)rl * n_k + 1);
       T* Q = W.take<T>((size_t)m_k * rb + 1);
@@ -1644,7 +1684,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   enqueue_lu(0, false);
   if (nb > 1) {
     ctx.reserve(prio_workspace(A, bm) + 4096);
-    enqueue_prio(0, false);
+    enqueue_prio(0, prio_rank_free(A, 0));
   }
   bool prio_pending = false;  // the priority block of step i waits for the host ranks
   for (int i = 0; i + 1 < nb; ++i) {
@@ -2003,8 +2043,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     }
     pl.run();
     // 4. the ranks of the finished step for the host, then the next priority block
-    FRK_HIP_CHECK(hipMemcpyAsync(la.snap, A.d_rank(0, 0), sizeof(int32_t) * A.rank.size(), hipMemcpyDeviceToHost, s));
-    FRK_HIP_CHECK(hipEventRecord(la.rest, s));
+    FRK_HIP_CHECK(hipEventRecord(la.done, s));
+    FRK_HIP_CHECK(hipStreamWaitEvent(la.cs, la.done, 0));
+    FRK_HIP_CHECK(
+        hipMemcpyAsync(la.snap, A.d_rank(0, 0), sizeof(int32_t) * A.rank.size(), hipMemcpyDeviceToHost, la.cs));
+    FRK_HIP_CHECK(hipEventRecord(la.rest, la.cs));
     if (i + 2 < nb) {
       if (prio_rank_free(A, i + 1))
         enqueue_prio(i + 1, true);

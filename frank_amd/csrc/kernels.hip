@@ -660,6 +660,36 @@ __device__ __forceinline__ void blk_zero(double* dst, int64_t ld, int nr, int nc
     if (r < nr && c < nc) dst[(int64_t)r * ld + c] = 0.0;
   }
 }
+// FP32 input / output variants (the mixed-precision path keeps its tiles in FP32)
+__device__ __forceinline__ void blk_load32(double* dst, const float* src, int64_t ld, int nr, int nc) {
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    dst[r * kTileLd + c] = (r < nr && c < nc) ? (double)src[(int64_t)r * ld + c] : 0.0;
+  }
+}
+__device__ __forceinline__ void blk_store32(const double* src, float* dst, int64_t ld, int nr, int nc) {
+  if (!dst) return;
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = (float)src[r * kTileLd + c];
+  }
+}
+__device__ __forceinline__ void blk_store_tri32(const double* src, float* dst, int64_t ld, int nr, int nc,
+                                                bool lower) {
+  if (!dst) return;
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const bool keep = lower ? ((c >> 4) <= (r >> 4)) : ((c >> 4) >= (r >> 4));
+    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = keep ? (float)src[r * kTileLd + c] : 0.f;
+  }
+}
+__device__ __forceinline__ void blk_zero32(float* dst, int64_t ld, int nr, int nc) {
+  if (!dst) return;
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (r < nr && c < nc) dst[(int64_t)r * ld + c] = 0.f;
+  }
+}
 
 __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
   using d64::mma64;
@@ -688,7 +718,11 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
   stamp(0);
   const int mi = rows(bi), nj = rows(bj);
   double* Ablk = A + (int64_t)(64 * bi) * ld + 64 * bj;
-  blk_load(M, Ablk, ld, mi, nj);
+  const int64_t off_a = (int64_t)(64 * bi) * ld + 64 * bj, off_i = (int64_t)(64 * bi) * ldi + 64 * bj;
+  if (it.a32_in)
+    blk_load32(M, it.a32_in + off_a, ld, mi, nj);
+  else
+    blk_load(M, Ablk, ld, mi, nj);
   if (bi == bj)
     for (int r = mi + tid; r < 64; r += 256) M[r * LD + r] = 1.0;  // identity padding of the last diagonal block
   __syncthreads();
@@ -715,6 +749,9 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     blk_store_tri(Us, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, false);
     tile_signal(&I[bi * nbk + bj], ep);
     stamp(3);
+    blk_store32(M, it.a32_out ? it.a32_out + off_a : nullptr, ld, mi, nj);
+    blk_store_tri32(Ls, it.linv32 ? it.linv32 + off_i : nullptr, ldi, mi, mi, true);
+    blk_store_tri32(Us, it.uinv32 ? it.uinv32 + off_i : nullptr, ldi, mi, mi, false);
     return;
   }
   // 2b. panel blocks: L_ij = A_ij Uinv_jj (i > j), U_ij = Linv_ii A_ij (i < j)
@@ -733,6 +770,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
   blk_store(M, Ablk, ld, mi, nj);
   tile_signal(&F[bi * nbk + bj], ep);
   stamp(2);
+  blk_store32(M, it.a32_out ? it.a32_out + off_a : nullptr, ld, mi, nj);
   // 3. block (bi, bj) of the full inverse of L (below the diagonal) or U (above)
   if (bi > bj) {
     // M <- L_ij Linv_jj + sum_{k=j+1}^{i-1} L_ik Linv_kj, then Linv_ij = -Linv_ii M
@@ -756,6 +794,8 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     __syncthreads();
     blk_store(M, Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
     blk_zero(Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+    blk_store32(M, it.linv32 ? it.linv32 + off_i : nullptr, ldi, mi, nj);
+    blk_zero32(it.uinv32 ? it.uinv32 + off_i : nullptr, ldi, mi, nj);
   } else {
     // M <- Uinv_ii U_ij + sum_{k=i+1}^{j-1} Uinv_ik U_kj, then Uinv_ij = -M Uinv_jj
     blk_load(Ls, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
@@ -778,6 +818,8 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     __syncthreads();
     blk_store(M, Ui + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
     blk_zero(Li + (int64_t)(64 * bi) * ldi + 64 * bj, ldi, mi, nj);
+    blk_store32(M, it.uinv32 ? it.uinv32 + off_i : nullptr, ldi, mi, nj);
+    blk_zero32(it.linv32 ? it.linv32 + off_i : nullptr, ldi, mi, nj);
   }
   tile_signal(&I[bi * nbk + bj], ep);
   stamp(3);
@@ -2207,21 +2249,16 @@ template void launch_solve_update<float>(const SolveUpdItem*, int, int, hipStrea
 // owns one output column (so the reads of B are contiguous in the row-major
 // case and cache-line reuse serves the transposed one), and the partial sums
 // of the waves meet in LDS.
-// d_count (optional): the number of rows actually present, read on the device
-// (the table then holds R >= count entries); workgroups past it exit.
-template <typename T>
-__global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
-                                                        const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
-                                                        T* __restrict__ W, int N, int K,
-                                                        const int32_t* __restrict__ d_count) {
+template <typename T, class RowFn>
+__device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const T* __restrict__ B, int64_t b_rs,
+                                               int64_t b_cs, T* __restrict__ W, int N, int K) {
   __shared__ T As[16][512 + 1];
   __shared__ T red[4][16][64];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int r0 = 16 * blockIdx.x, col = 64 * blockIdx.y + lane;
-  if (d_count) R = min(R, max(0, *d_count));
-  if (r0 >= R) return;
+  if (r0 >= R || 64 * (int)blockIdx.y >= N) return;
   for (int r = 0; r < 16; ++r) {
-    const T* a = (r0 + r < R) ? static_cast<const T*>(rows[r0 + r].a) : nullptr;
+    const T* a = (r0 + r < R) ? row_of(r0 + r) : nullptr;
     for (int k = tid; k < K; k += 256) As[r][k] = a ? a[k] : T(0);
   }
   __syncthreads();
@@ -2246,6 +2283,28 @@ __global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict
     const int oc = 64 * blockIdx.y + c;
     if (r0 + r < R && oc < N) W[(int64_t)(r0 + r) * N + oc] = red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
   }
+}
+
+// d_count (optional): the number of rows actually present, read on the device
+// (the table then holds R >= count entries); workgroups past it exit.
+template <typename T>
+__global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
+                                                        const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
+                                                        T* __restrict__ W, int N, int K,
+                                                        const int32_t* __restrict__ d_count) {
+  if (d_count) R = min(R, max(0, *d_count));
+  rows_gemm_body<T>([&](int r) { return static_cast<const T*>(rows[r].a); }, R, B, b_rs, b_cs, W, N, K);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void rows_pair_kernel(RowsPair p) {
+  const RowsJob j = blockIdx.z ? p.job[1] : p.job[0];
+  int R = j.R;
+  if (j.d_count) R = min(R, max(0, *j.d_count));
+  const T* base = static_cast<const T*>(j.base);
+  const int64_t st = j.stride;
+  rows_gemm_body<T>([&](int r) { return base + (int64_t)r * st; }, R, static_cast<const T*>(j.b), j.b_rs, j.b_cs,
+                    static_cast<T*>(j.w), j.N, j.K);
 }
 
 template <typename T>
@@ -2274,6 +2333,21 @@ void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStre
   hipLaunchKernelGGL(rows_scatter_kernel<T>, dim3(R), dim3(256), 0, s, d_rows, R, W, N, d_count);
   FRK_HIP_CHECK(hipGetLastError());
 }
+template <typename T>
+void launch_rows_pair(const RowsPair& p, hipStream_t s) {
+  int R = 0, N = 0;
+  for (const auto& j : p.job) {
+    if (j.R <= 0) continue;
+    if (j.N <= 0 || j.K <= 0 || j.K > 512) throw std::runtime_error("rows pair: N > 0, 0 < K <= 512");
+    R = std::max(R, (int)j.R);
+    N = std::max(N, (int)j.N);
+  }
+  if (R <= 0) return;
+  hipLaunchKernelGGL(rows_pair_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64, 2), dim3(256), 0, s, p);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_rows_pair<double>(const RowsPair&, hipStream_t);
+template void launch_rows_pair<float>(const RowsPair&, hipStream_t);
 template void launch_rows_gemm<double>(const RowRef*, int, const double*, int64_t, int64_t, double*, int, int,
                                        hipStream_t, const int32_t*);
 template void launch_rows_gemm<float>(const RowRef*, int, const float*, int64_t, int64_t, float*, int, int,
@@ -2289,10 +2363,11 @@ template void launch_rows_scatter<float>(const RowRef*, int, const float*, int, 
 // ===========================================================================
 template <typename T>
 __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
+  constexpr int KC = 256;
   __shared__ T M1[64][65];
   __shared__ T T1[64][65];
-  __shared__ T Va[16][65];
-  __shared__ T Ub[64][17];
+  __shared__ T Va[16][KC + 1];
+  __shared__ T Ub[KC][17];
   const int tid = threadIdx.x;
   const int rl = min(it.kcap, max(0, *it.rl)), rb = min(it.kcap, max(0, *it.rb));
   if (rl == 0 || rb == 0) return;
@@ -2303,23 +2378,37 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
   T* core = static_cast<T*>(it.core);
   const int K = it.k;
   const int ti = tid >> 4, tj = tid & 15;
-  // M1 = V_l U_b (rl x rb), 16 x 16 output blocks, K in chunks of 64
+  if (it.vl_dst) {
+    T* d = static_cast<T*>(it.vl_dst);
+    for (int e = tid; e < rl * it.nv; e += 256) {
+      const int r = e / it.nv, j = e % it.nv;
+      d[(int64_t)r * it.vl_dst_ld + j] = V[(int64_t)r * it.v_rs + (int64_t)j * it.v_cs];
+    }
+  }
+  if (it.ub_dst) {
+    T* d = static_cast<T*>(it.ub_dst);
+    for (int e = tid; e < rb * it.nu; e += 256) {
+      const int c = e / it.nu, j = e % it.nu;
+      d[(int64_t)c * it.ub_dst_ld + j] = U[(int64_t)j * it.u_rs + (int64_t)c * it.u_cs];
+    }
+  }
+  // M1 = V_l U_b (rl x rb), 16 x 16 output blocks, K in chunks of KC
   for (int bi = 0; bi < rl; bi += 16)
     for (int bj = 0; bj < rb; bj += 16) {
       T acc = T(0);
-      for (int k0 = 0; k0 < K; k0 += 64) {
+      for (int k0 = 0; k0 < K; k0 += KC) {
         __syncthreads();
-        for (int e = tid; e < 16 * 64; e += 256) {
-          const int r = e >> 6, kk = e & 63;
+        for (int e = tid; e < 16 * KC; e += 256) {
+          const int r = e / KC, kk = e % KC;
           Va[r][kk] = (bi + r < rl && k0 + kk < K) ? V[(int64_t)(bi + r) * it.v_rs + (int64_t)(k0 + kk) * it.v_cs] : T(0);
         }
-        for (int e = tid; e < 64 * 16; e += 256) {
+        for (int e = tid; e < KC * 16; e += 256) {
           const int kk = e >> 4, c = e & 15;
           Ub[kk][c] = (bj + c < rb && k0 + kk < K) ? U[(int64_t)(k0 + kk) * it.u_rs + (int64_t)(bj + c) * it.u_cs] : T(0);
         }
         __syncthreads();
 #pragma unroll 8
-        for (int kk = 0; kk < 64; ++kk) acc += Va[ti][kk] * Ub[kk][tj];
+        for (int kk = 0; kk < KC; ++kk) acc += Va[ti][kk] * Ub[kk][tj];
       }
       if (bi + ti < rl && bj + tj < rb) M1[bi + ti][bj + tj] = acc;
     }
@@ -2354,6 +2443,12 @@ __global__ __launch_bounds__(256) void lr_update_kernel(LrUpdItem it) {
   const T* V = static_cast<const T*>(it.v);
   const T* U = static_cast<const T*>(it.u);
   T* C = static_cast<T*>(it.c);
+  T cv[16];  // this thread's 16 entries of the C tile, loaded with the operands
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = tid + 256 * q, a = e >> 6, b = e & 63;
+    cv[q] = (i0 + a < it.m && c0 + b < it.n) ? C[(int64_t)(i0 + a) * it.ldc + c0 + b] : T(0);
+  }
   for (int e = tid; e < 64 * 64; e += 256) {
     const int a = e >> 6, b = e & 63;
     Ks[a][b] = (a < r1 && b < r2) ? core[(int64_t)a * it.ldk + b] : T(0);
@@ -2369,12 +2464,12 @@ __global__ __launch_bounds__(256) void lr_update_kernel(LrUpdItem it) {
     Wt[a][b] = sacc;
   }
   __syncthreads();
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int a = e >> 6, b = e & 63;
-    if (i0 + a >= it.m || c0 + b >= it.n) continue;
+#pragma unroll
+  for (int q2 = 0; q2 < 16; ++q2) {
+    const int e = tid + 256 * q2, a = e >> 6, b = e & 63;
     T sacc = T(0);
     for (int q = 0; q < r1; ++q) sacc += Ut[a][q] * Wt[q][b];
-    C[(int64_t)(i0 + a) * it.ldc + c0 + b] += sacc;
+    if (i0 + a < it.m && c0 + b < it.n) C[(int64_t)(i0 + a) * it.ldc + c0 + b] = cv[q2] + sacc;
   }
 }
 

@@ -115,6 +115,9 @@ void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int
 template <typename T>
 void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s,
                          const int32_t* d_count = nullptr);
+// Two strided stacked-row products in one launch (see RowsPair).
+template <typename T>
+void launch_rows_pair(const RowsPair& p, hipStream_t s);
 // Low-rank x low-rank Schur update with device ranks (kcap <= 64):
 // core = alpha S_l (V_l U_b) S_b, then C += U_l core V_b on 64 x 64 tiles.
 template <typename T>
