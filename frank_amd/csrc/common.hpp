@@ -178,6 +178,49 @@ struct SvdItem {
 };
 
 // ---------------------------------------------------------------------------
+// One low-rank addition C += X (rounded / fast rounded): target C = U S V in its
+// slots (rank kc known at launch), addend X = Ux Sx Vx with Ux m x kx1 (strided),
+// Sx kx1 x kx2 (nullptr = identity), Vx kx2 x n (strided).  Element (i, t) of U
+// at cu[i*cu_rs + t*cu_cs]; (t, j) of V likewise.  The QR, SVD and basis-update
+// kernels of a batch derive their items from these on the device (LrBatch).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LrAddDesc {
+  // target C = U S V (slots, rank kc known on the host at launch time)
+  T* cu;
+  T* cs;
+  T* cv;
+  int32_t* d_rank;
+  int kc;
+  int64_t cu_rs, cu_cs, cv_rs, cv_cs;  // element (i, t) of U at cu[i*cu_rs + t*cu_cs]; (t, j) of V likewise
+  int lds;
+  int m, n;
+  double eps;  // != 0: fixed accuracy (H2 rule); 0: keep kc
+  int cap;
+  bool cap_trunc = false;  // keep the leading cap terms when the eps-rank exceeds cap
+  // addend X = Ux Sx Vx, Ux m x kx1 (strided), Sx kx1 x kx2 (nullptr = identity),
+  // Vx kx2 x n (strided)
+  const T* xu;
+  int64_t xu_rs, xu_cs;
+  int kx1;
+  const T* xs;
+  int xs_ld;
+  int kx2;
+  const T* xv;
+  int64_t xv_rs, xv_cs;
+};
+
+// Per-batch workspace of a batched low-rank addition: addition t owns
+// Yu/Yv + t*mmax*K / t*nmax*K (reflectors), tu/tv + t*K, Ru/Rv/Um/Vm + t*K*K
+// (and Vw + t*K*K when set).  lra: 1 rounded, 2 fast (when eps == 0).
+template <typename T>
+struct LrBatch {
+  T *Yu, *Yv, *tu, *tv, *Ru, *Rv, *Um, *Vm, *Vw;
+  int32_t* stats;
+  int32_t mmax, nmax, K, lra;
+};
+
+// ---------------------------------------------------------------------------
 // Truncated column-pivoted QR item (construction).  A (m x n, row-major,
 // ld = lda) is overwritten.  Outputs rank, tau, pivot order.
 // ---------------------------------------------------------------------------

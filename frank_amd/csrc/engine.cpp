@@ -91,8 +91,19 @@ void Context::check_tile_lu(int32_t since) {
   if (e > since) throw std::runtime_error("tile LU: a block wait timed out (workgroups not co-resident?)");
 }
 
+hipStream_t Context::aux_stream(int k) {
+  if (k < 0 || k > 1) throw std::invalid_argument("aux_stream: index 0 or 1");
+  if (!aux_[k]) FRK_HIP_CHECK(hipStreamCreateWithFlags(&aux_[k], hipStreamNonBlocking));
+  return aux_[k];
+}
+
 Context::~Context() {
   (void)hipStreamSynchronize(s_);
+  for (hipStream_t a : aux_)
+    if (a) {
+      (void)hipStreamSynchronize(a);
+      (void)hipStreamDestroy(a);
+    }
   for (auto& pr : ev_pending_) {
     ev_pool_.push_back(pr.first);
     ev_pool_.push_back(pr.second);
@@ -349,7 +360,11 @@ void BlrMatrix<T>::push_ranks(hipStream_t s) {
 // ============================================================================
 void Plan::run() {
   if (launches_.empty()) return;
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
   const char* d = static_cast<const char*>(ctx_.staging().push_bytes(bytes_.data(), bytes_.size(), ctx_.stream()));
+  const auto t1 = clk::now();
+  ctx_.host_time("host_stage", std::chrono::duration<double, std::milli>(t1 - t0).count());
   for (size_t i = 0; i < launches_.size(); ++i) {
     const Meta& m = meta_[i];
     const bool timed = ctx_.is_timed(m.cls);
@@ -570,227 +585,64 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   Context& ctx = pl.ctx();
   const hipStream_t s = ctx.stream();
   const int nd = (int)d.size();
-  int kmax = 0, mmax = 0, nmax = 0;
+  int kmax = 0, mmax = 0, nmax = 0, qk = 0, capmax = 0;
+  double qf = 0, qb = 0, sf = 0, sb = 0, af = 0, ab = 0;
   for (const auto& x : d) {
-    kmax = std::max(kmax, x.kc + x.kx2);
+    const int Kt = x.kc + x.kx2;
+    kmax = std::max(kmax, Kt);
     mmax = std::max(mmax, x.m);
     nmax = std::max(nmax, x.n);
+    capmax = std::max(capmax, std::min(x.cap, Kt));
   }
-  const int K = kmax;
-  T* Yu = ctx.take<T>((size_t)nd * mmax * K);
-  T* Yv = ctx.take<T>((size_t)nd * nmax * K);
-  T* tu = ctx.take<T>((size_t)nd * K);
-  T* tv = ctx.take<T>((size_t)nd * K);
-  T* Ru = ctx.take<T>((size_t)nd * K * K);
-  T* Rv = ctx.take<T>((size_t)nd * K * K);
-  T* Um = ctx.take<T>((size_t)nd * K * K);
-  T* Vm = ctx.take<T>((size_t)nd * K * K);
-  // FP64 cores above 64 keep their right vectors in global scratch
-  T* Vw = (sizeof(T) == 8 && K > 64) ? ctx.take<T>((size_t)nd * K * K) : nullptr;
-  std::vector<QrItem> qi;
-  qi.reserve(2 * nd);
-  std::vector<SvdItem> si(nd);
-  std::vector<ApplyQItem> ai;
-  ai.reserve(2 * nd);
-  for (int t = 0; t < nd; ++t) {
-    const auto& x = d[t];
+  // op log: QR (geqrf + orgqr 4mK^2 - 4/3 K^3, panel formation, projection), SVD 24 K^3, reflector
+  // application 4 m K ncol with ncol <= min(K, cap) (the kept rank is known on the device only).
+  for (const auto& x : d) {
     const bool fast = (lra == kLraFast) && (x.eps == 0.0);
     const int Kt = x.kc + x.kx2;
-    T* yu = Yu + (size_t)t * mmax * K;
-    T* yv = Yv + (size_t)t * nmax * K;
-    T* ru = Ru + (size_t)t * K * K;
-    T* rv = Rv + (size_t)t * K * K;
-    T* um = Um + (size_t)t * K * K;
-    T* vm = Vm + (size_t)t * K * K;
-    // a null S stands for the identity (X = U V): the QR loads U directly
-    const T* xs = x.xs;
-    const int xsld = x.xs_ld;
-    QrItem q{};
-    q.Y = yu;
-    q.y_rs = 1;
-    q.y_cs = x.m;
-    q.tau = tu + (size_t)t * K;
-    q.R = ru;
-    q.ldr = K;
-    q.m = x.m;
-    QrItem v = q;
-    v.Y = yv;
-    v.y_rs = 1;
-    v.y_cs = x.n;
-    v.tau = tv + (size_t)t * K;
-    v.R = rv;
-    v.m = x.n;
-    if (!fast) {
-      // Uc = [Uc Sc | Ux Sx] (formed inside the QR kernel), Vc^T = [Vc^T | Vx^T]
-      q.A1 = x.cu;
-      q.a1_rs = x.cu_rs;
-      q.a1_cs = x.cu_cs;
-      q.S1 = x.cs;
-      q.s1_ld = x.lds;
-      q.k1a = q.k1 = x.kc;
-      q.A2 = x.xu;
-      q.a2_rs = x.xu_rs;
-      q.a2_cs = x.xu_cs;
-      q.S2 = xs;
-      q.s2_ld = xsld;
-      q.k2a = x.kx1;
-      q.k2 = x.kx2;
-      v.A1 = x.cv;
-      v.a1_rs = x.cv_cs;
-      v.a1_cs = x.cv_rs;
-      v.k1a = v.k1 = x.kc;
-      v.A2 = x.xv;
-      v.a2_rs = x.xv_cs;
-      v.a2_cs = x.xv_rs;
-      v.k2a = v.k2 = x.kx2;
-    } else {
-      // project X's bases on the (orthonormal) bases of C, QR the remainders
-      q.A1 = x.xu;
-      q.a1_rs = x.xu_rs;
-      q.a1_cs = x.xu_cs;
-      q.S1 = xs;
-      q.s1_ld = xsld;
-      q.k1a = x.kx1;
-      q.k1 = x.kx2;
-      q.k2 = 0;
-      q.P = x.cu;
-      q.p_rs = x.cu_rs;
-      q.p_cs = x.cu_cs;
-      q.kp = x.kc;
-      v.A1 = x.xv;
-      v.a1_rs = x.xv_cs;
-      v.a1_cs = x.xv_rs;
-      v.k1a = v.k1 = x.kx2;
-      v.k2 = 0;
-      v.P = x.cv;
-      v.p_rs = x.cv_cs;
-      v.p_cs = x.cv_rs;
-      v.kp = x.kc;
+    const double kq = fast ? x.kx2 : Kt, kp = fast ? x.kc : 0;
+    qk = std::max(qk, (int)kq);
+    for (const double m : {(double)x.m, (double)x.n}) {
+      qf += 4.0 * m * kq * kq - 4.0 / 3.0 * kq * kq * kq + 4.0 * m * kp * kq;
+      qb += sizeof(T) * (m * (kq + kp + kq) + kq * kq);
     }
-    qi.push_back(q);
-    qi.push_back(v);
-    SvdItem sv{};
-    sv.mode = 1;
-    sv.Ru = ru;
-    sv.Rv = rv;
-    sv.ldr = K;
-    sv.K = Kt;
-    if (!fast) {
-      sv.kx = Kt;
-      sv.ka = 0;
-    } else {
-      sv.kx = x.kx2;
-      sv.ka = x.kc;
-      sv.Sa = x.cs;
-      sv.lds_in = x.lds;
+    if (x.xs) qf += 2.0 * x.m * x.kx1 * x.kx2;
+    if (!fast) qf += 2.0 * x.m * x.kc * x.kc;
+    sf += 24.0 * (double)Kt * Kt * Kt;
+    sb += sizeof(T) * 5.0 * Kt * Kt;
+    const double nc = std::min(Kt, capmax);
+    for (const double m : {(double)x.m, (double)x.n}) {
+      const double nref = std::min(m, fast ? (double)x.kx2 : (double)Kt);
+      af += 4.0 * m * nref * nc;
+      ab += sizeof(T) * m * (nref + nc);
     }
-    sv.fixed_rank = (x.eps != 0.0) ? -1 : x.kc;
-    sv.kdim = std::min(std::min(x.m, Kt), std::min(x.n, Kt));
-    sv.eps = x.eps;
-    sv.S = x.cs;
-    sv.lds = x.lds;
-    sv.Um = um;
-    sv.ldu = K;
-    sv.Vm = vm;
-    sv.ldv = K;
-    sv.rank_out = x.d_rank;
-    sv.cap = x.cap;
-    sv.cap_trunc = x.cap_trunc;
-    sv.stats = ctx.counters();
-    sv.vwork = Vw ? Vw + (size_t)t * K * K : nullptr;
-    si[t] = sv;
-    ApplyQItem a{};
-    a.Y = yu;
-    a.y_rs = 1;
-    a.y_cs = x.m;
-    a.tau = tu + (size_t)t * K;
-    a.m = x.m;
-    a.ncol_ptr = x.d_rank;
-    a.ncol = Kt;
-    a.out = x.cu;
-    a.out_rs = x.cu_rs;
-    a.out_cs = x.cu_cs;
-    a.trans = 0;
-    ApplyQItem b = a;
-    b.Y = yv;
-    b.y_cs = x.n;
-    b.tau = tv + (size_t)t * K;
-    b.m = x.n;
-    b.out = x.cv;
-    b.out_rs = x.cv_cs;
-    b.out_cs = x.cv_rs;
-    if (!fast) {
-      a.nref = std::min(x.m, Kt);
-      a.X = um;
-      a.x_rs = K;
-      a.x_cs = 1;
-      a.kx = Kt;
-      b.nref = std::min(x.n, Kt);
-      b.X = vm;
-      b.x_rs = K;
-      b.x_cs = 1;
-      b.kx = Kt;
-    } else {
-      a.nref = std::min(x.m, x.kx2);
-      a.X = um + (size_t)x.kc * K;
-      a.x_rs = K;
-      a.x_cs = 1;
-      a.kx = x.kx2;
-      a.Base = x.cu;
-      a.base_rs = x.cu_rs;
-      a.base_cs = x.cu_cs;
-      a.Xb = um;
-      a.xb_rs = K;
-      a.xb_cs = 1;
-      a.kb = x.kc;
-      b.nref = std::min(x.n, x.kx2);
-      b.X = vm + (size_t)x.kc * K;
-      b.x_rs = K;
-      b.x_cs = 1;
-      b.kx = x.kx2;
-      b.Base = x.cv;
-      b.base_rs = x.cv_cs;
-      b.base_cs = x.cv_rs;
-      b.Xb = vm;
-      b.xb_rs = K;
-      b.xb_cs = 1;
-      b.kb = x.kc;
-    }
-    ai.push_back(a);
-    ai.push_back(b);
   }
-  int qk = 0;
-  for (const auto& q : qi) qk = std::max(qk, q.k1 + q.k2);
-  int capmax = 0;
-  for (const auto& x : d) capmax = std::max(capmax, std::min(x.cap, x.kc + x.kx2));
-  // op log: QR (geqrf + orgqr 4mK^2 - 4/3 K^3, panel formation, projection), SVD 22 K^3, reflector
-  // application 4 m K ncol with ncol <= min(K, cap) (the kept rank is known on the device only).
-  double qf = 0, qb = 0, sf = 0, sb = 0, af = 0, ab = 0;
-  for (const auto& q : qi) {
-    const double m = q.m, Kq = q.k1 + q.k2;
-    qf += 4.0 * m * Kq * Kq - 4.0 / 3.0 * Kq * Kq * Kq + 4.0 * m * q.kp * Kq;
-    if (q.S1) qf += 2.0 * m * q.k1a * q.k1;
-    if (q.S2) qf += 2.0 * m * q.k2a * q.k2;
-    qb += sizeof(T) * (m * ((q.S1 ? q.k1a : q.k1) + (q.S2 ? q.k2a : q.k2) + q.kp + Kq) + Kq * Kq);
-  }
-  for (const auto& v : si) {
-    const double Kv = v.K;
-    sf += 24.0 * Kv * Kv * Kv;
-    sb += sizeof(T) * 5.0 * Kv * Kv;
-  }
-  for (const auto& a : ai) {
-    const double m = a.m, nc = std::min(a.ncol, capmax);
-    af += 4.0 * m * a.nref * nc;
-    ab += sizeof(T) * m * (a.nref + nc);
-  }
-  const size_t oq = pl.add(qi), os = pl.add(si), oa = pl.add(ai);
-  const int nq = (int)qi.size(), na = (int)ai.size();
-  pl.launch([=](const char* dd) { launch_qr<T>(reinterpret_cast<const QrItem*>(dd + oq), nq, qk, s); }, "lr_qr", qf,
-            qb);
-  pl.launch([=](const char* dd) { launch_svd<T>(reinterpret_cast<const SvdItem*>(dd + os), nd, K, s); }, "lr_svd", sf,
-            sb);
-  pl.launch([=](const char* dd) { launch_apply_q<T>(reinterpret_cast<const ApplyQItem*>(dd + oa), na, capmax, s); },
-            "lr_apply_q", af, ab);
+  const int K = kmax;
+  LrBatch<T> b{};
+  b.Yu = ctx.take<T>((size_t)nd * mmax * K);
+  b.Yv = ctx.take<T>((size_t)nd * nmax * K);
+  b.tu = ctx.take<T>((size_t)nd * K);
+  b.tv = ctx.take<T>((size_t)nd * K);
+  b.Ru = ctx.take<T>((size_t)nd * K * K);
+  b.Rv = ctx.take<T>((size_t)nd * K * K);
+  b.Um = ctx.take<T>((size_t)nd * K * K);
+  b.Vm = ctx.take<T>((size_t)nd * K * K);
+  // FP64 cores above 64 keep their right vectors in global scratch
+  b.Vw = (sizeof(T) == 8 && K > 64) ? ctx.take<T>((size_t)nd * K * K) : nullptr;
+  b.stats = ctx.counters();
+  b.mmax = mmax;
+  b.nmax = nmax;
+  b.K = K;
+  b.lra = lra;
+  // only the additions' descriptors are uploaded: the QR, SVD and basis-update
+  // kernels derive their items from them
+  const size_t od = pl.add(d);
+  pl.launch([=](const char* dd) { launch_lr_add_qr<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, qk, s); },
+            "lr_qr", qf, qb);
+  pl.launch([=](const char* dd) { launch_lr_add_svd<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, s); },
+            "lr_svd", sf, sb);
+  pl.launch(
+      [=](const char* dd) { launch_lr_add_apply<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, capmax, s); },
+      "lr_apply_q", af, ab);
 }
 
 template <typename T>
@@ -1423,33 +1275,25 @@ static size_t diag_getrf_workspace(int n) {
   return (sizeof(T) == 4 ? sizeof(double) * ((size_t)3 * n * n + 768) : 0) + getrf_workspace<double>(n, 1);
 }
 
-// The lookahead stream, its events and a pinned snapshot of the rank table.
-// lu (lookahead stream): the factorization of the latest diagonal tile is done;
-// prio (main stream): the priority block of the latest step is done; done (main
-// stream): the rest of a step has run; rest (copy stream): its ranks are in snap.
+// The lookahead streams, their events and a pinned snapshot of the rank table.
+// prio (lookahead stream): the priority block of the latest step, and so the
+// factorization of its diagonal tile, is done; sxa (main): a step's panels and
+// stacked products are done; c0 / c1: its first / second chunk is done; rest
+// (second stream): the step's ranks are in snap.
 struct Lookahead {
-  hipStream_t s = nullptr, cs = nullptr;  // lookahead stream, rank-snapshot copy stream
-  hipEvent_t lu = nullptr, prio = nullptr, rest = nullptr, done = nullptr;
+  hipStream_t s = nullptr, s2 = nullptr;  // lookahead stream, second rest-chunk stream (the context's)
+  hipEvent_t prio = nullptr, rest = nullptr, sxa = nullptr, c0 = nullptr, c1 = nullptr;
   int32_t* snap = nullptr;
-  explicit Lookahead(size_t nrank) {
-    FRK_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    FRK_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-    FRK_HIP_CHECK(hipEventCreateWithFlags(&done, hipEventDisableTiming));
-    FRK_HIP_CHECK(hipEventCreateWithFlags(&lu, hipEventDisableTiming));
-    FRK_HIP_CHECK(hipEventCreateWithFlags(&prio, hipEventDisableTiming));
-    FRK_HIP_CHECK(hipEventCreateWithFlags(&rest, hipEventDisableTiming));
+  Lookahead(Context& ctx, size_t nrank) : s(ctx.aux_stream(0)), s2(ctx.aux_stream(1)) {
+    for (hipEvent_t* e : {&prio, &rest, &sxa, &c0, &c1})
+      FRK_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
     FRK_HIP_CHECK(hipHostMalloc((void**)&snap, sizeof(int32_t) * std::max<size_t>(nrank, 1)));
   }
   ~Lookahead() {
     (void)hipStreamSynchronize(s);
-    (void)hipStreamSynchronize(cs);
-    (void)hipEventDestroy(done);
-    (void)hipStreamDestroy(cs);
-    (void)hipEventDestroy(lu);
-    (void)hipEventDestroy(prio);
-    (void)hipEventDestroy(rest);
+    (void)hipStreamSynchronize(s2);
+    for (hipEvent_t e : {prio, rest, sxa, c0, c1}) (void)hipEventDestroy(e);
     (void)hipHostFree(snap);
-    (void)hipStreamDestroy(s);
   }
 };
 
@@ -1645,52 +1489,101 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   ctx.sync();
   A.pull_ranks(s);
   const int32_t epoch0 = ctx.tile_epoch();
-  // Two streams.  The critical chain is: factorization of the diagonal tile k
-  // (lookahead stream), the priority block of step k (main stream: the panels
-  // (k+1, k), (k, k+1) and the Schur update of (k+1, k+1)), factorization of
-  // tile k+1, ...  The rest of step k runs on the main stream beside the
-  // factorization of tile k+1.  The priority block of step k+1 is queued right
-  // behind the rest of step k, planned before that rest has run: its low-rank
-  // kernels read the panel ranks on the device.  The host waits only for the
-  // ranks of a finished step, to plan the next rest, and is never on the chain.
-  Lookahead la(A.rank.size());
+  // Three streams.  The lookahead stream carries the critical chain: factorization
+  // of the diagonal tile k, the priority block of step k (the panels (k+1, k),
+  // (k, k+1) and the Schur update of (k+1, k+1)), factorization of tile k+1, ...
+  // The rest of step k runs beside the factorization of tile k+1, in two row
+  // chunks on the main and the second stream.  The priority block of step k+1 is
+  // queued behind the factorization of tile k+1 before the rest of step k has
+  // run: its low-rank kernels read the panel ranks on the device, so its plan
+  // does not need them.  The host waits only for the rank snapshot of a finished
+  // step, to plan the next rest, and is never on the chain.
+  Lookahead la(ctx, A.rank.size());
   Context side(la.s);
   side.set_stats_owner(&ctx);
+  Context rest2(la.s2);
+  rest2.set_stats_owner(&ctx);
   using clk = std::chrono::steady_clock;
   auto ms_since = [](clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); };
-  const size_t side_ws = diag_getrf_workspace<T>((int)bm) + 8192;
-  // factorization of the diagonal tile k on the lookahead stream, after the main stream's work so far
-  auto enqueue_lu = [&](int k, bool after_main) {
-    if (after_main) FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.prio, 0));
-    side.reserve(side_ws);
+  const size_t side_ws = diag_getrf_workspace<T>((int)bm) + prio_workspace(A, bm) + 8192;
+  // factorization of the diagonal tile k on the lookahead stream (workspace taken
+  // after whatever the current job took: the jobs of this stream run in order)
+  auto enqueue_lu = [&](int k) {
     Plan pd(side);
     plan_diag_getrf<T>(pd, side, A, k, info + k * npan_max);
     pd.run();
-    FRK_HIP_CHECK(hipEventRecord(la.lu, la.s));
   };
-  // priority block of step k on the main stream (after the factorization of tile k),
-  // then the factorization of tile k+1; the plan takes workspace from ctx after
-  // whatever the current step has taken
-  auto enqueue_prio = [&](int k, bool device_ranks) {
+  // priority block of step k on the lookahead stream, after the factorization of
+  // tile k and (after_rest) the rest of step k-1, then the factorization of tile k+1
+  auto enqueue_prio = [&](int k, bool device_ranks, bool after_rest) {
     const auto tp = clk::now();
-    FRK_HIP_CHECK(hipStreamWaitEvent(s, la.lu, 0));
-    Plan pp(ctx);
-    plan_prio<T>(pp, ctx, A, k, A.rank[A.idx(k + 1, k)], A.rank[A.idx(k, k + 1)], device_ranks);
+    if (after_rest) FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.c1, 0));
+    side.reserve(side_ws);
+    Plan pp(side);
+    plan_prio<T>(pp, side, A, k, A.rank[A.idx(k + 1, k)], A.rank[A.idx(k, k + 1)], device_ranks);
     pp.run();
-    FRK_HIP_CHECK(hipEventRecord(la.prio, s));
-    enqueue_lu(k + 1, true);
+    FRK_HIP_CHECK(hipEventRecord(la.prio, la.s));
+    enqueue_lu(k + 1);
     ctx.host_time("host_prio", ms_since(tp));
   };
-  enqueue_lu(0, false);
-  if (nb > 1) {
-    ctx.reserve(prio_workspace(A, bm) + 4096);
-    enqueue_prio(0, prio_rank_free(A, 0));
+  // workspace of one step (panel products, per-update temporaries, additions) with
+  // ranks up to kmax
+  auto max_rank = [&]() {
+    int k = 1;
+    for (size_t r = 0; r < A.rank.size(); ++r) k = std::max(k, A.rank[r]);
+    return k;
+  };
+  auto step_need = [&](int i, int kmax) {
+    const int rem = nb - 1 - i;
+    size_t need = 4096;
+    need += (size_t)rem * 2 * sizeof(T) * ((size_t)bm * kmax + 256);
+    {
+      size_t rp = 0, rq = 0;
+      for (int t2 = i + 1; t2 < nb; ++t2) {
+        if (A.kind[A.idx(t2, i)] == kLowRank) rp += A.rank[A.idx(t2, i)];
+        if (A.kind[A.idx(i, t2)] == kLowRank) rq += A.rank[A.idx(i, t2)];
+      }
+      need += sizeof(T) * (rp * rq + 2 * (rp + rq) * bm) + 8 * 256;
+    }
+    for (int t2 = i + 1; t2 < nb; ++t2) {
+      if (A.kind[A.idx(t2, i)] == kDense) need += sizeof(T) * ((size_t)bm * bm + 64);
+      if (A.kind[A.idx(i, t2)] == kDense) need += sizeof(T) * ((size_t)bm * bm + 64);
+    }
+    for (int ic = i + 1; ic < nb; ++ic)
+      for (int k = i + 1; k < nb; ++k) {
+        const int kl = A.kind[A.idx(ic, i)], kb = A.kind[A.idx(i, k)], kc = A.kind[A.idx(ic, k)];
+        if (kc == kDense)
+          need += sizeof(T) * (2 * (size_t)bm * kmax + (size_t)kmax * kmax + 512);
+        else if (kl == kDense && kb == kDense)
+          need += sizeof(T) * ((size_t)bm * bm + (size_t)bm * A.kcap * 2 + (size_t)A.kcap * A.kcap + 1024);
+        else
+          need += sizeof(T) * ((size_t)bm * kmax + (size_t)kmax * kmax + 512);
+        if (kc == kLowRank) {
+          const size_t K = (size_t)kmax + ((kl == kDense && kb == kDense) ? (size_t)A.kcap : (size_t)kmax);
+          need += sizeof(T) * (2 * bm * K + 2 * K + 5 * K * K) + 8 * 256;
+        }
+      }
+    return need;
+  };
+  // Both rest workspaces are sized once, for the first step with room for the ranks
+  // to grow: a larger reservation inside the loop frees the old buffer, and a free
+  // synchronizes the whole device.
+  {
+    const size_t need0 = step_need(0, std::min(A.kcap, 2 * max_rank() + 4));
+    ctx.reserve(need0);
+    rest2.reserve(need0);
   }
+  side.reserve(side_ws);
+  enqueue_lu(0);
+  if (nb > 1) enqueue_prio(0, prio_rank_free(A, 0), false);
   bool prio_pending = false;  // the priority block of step i waits for the host ranks
   for (int i = 0; i + 1 < nb; ++i) {
     auto th = clk::now();
     if (i > 0) {
       FRK_HIP_CHECK(hipEventSynchronize(la.rest));
+      // every launch of the previous step's rest has run: its descriptors are consumed
+      ctx.staging().reset();
+      rest2.staging().reset();
       for (size_t t = 0; t < A.rank.size(); ++t) {
         A.rank[t] = la.snap[t];
         if (A.kind[t] == kLowRank && A.rank[t] < 0)
@@ -1701,10 +1594,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     ctx.host_time("host_sync", ms_since(th));
     th = clk::now();
     if (prio_pending) {
-      ctx.reserve(prio_workspace(A, bm) + 4096);
-      enqueue_prio(i, false);
+      enqueue_prio(i, false, true);
       prio_pending = false;
     }
+    // the factors of tile i and the priority panels of step i come from the lookahead stream
+    FRK_HIP_CHECK(hipStreamWaitEvent(s, la.prio, 0));
     const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
     const int ii = A.idx(i, i);
     const int rem = nb - 1 - i;
@@ -1712,40 +1606,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     std::vector<LrAddDesc<T>> adds;
     std::vector<CompressDesc<T>> comp;
     std::vector<int> comp_add;
-    size_t need = 4096 + prio_workspace(A, bm);
-    {
-      // count workspace: panel products + per-update temporaries + addition
-      int kmax = 1;
-      for (int r = 0; r < (int)A.rank.size(); ++r) kmax = std::max(kmax, A.rank[r]);
-      need += (size_t)rem * 2 * sizeof(T) * ((size_t)bm * kmax + 256);
-      {
-        size_t rp = 0, rq = 0;
-        for (int t2 = i + 1; t2 < nb; ++t2) {
-          if (A.kind[A.idx(t2, i)] == kLowRank) rp += A.rank[A.idx(t2, i)];
-          if (A.kind[A.idx(i, t2)] == kLowRank) rq += A.rank[A.idx(i, t2)];
-        }
-        need += sizeof(T) * (rp * rq + 2 * (rp + rq) * bm) + 8 * 256;
-      }
-      for (int t2 = i + 1; t2 < nb; ++t2) {
-        if (A.kind[A.idx(t2, i)] == kDense) need += sizeof(T) * ((size_t)bm * bm + 64);
-        if (A.kind[A.idx(i, t2)] == kDense) need += sizeof(T) * ((size_t)bm * bm + 64);
-      }
-      for (int ic = i + 1; ic < nb; ++ic)
-        for (int k = i + 1; k < nb; ++k) {
-          const int kl = A.kind[A.idx(ic, i)], kb = A.kind[A.idx(i, k)], kc = A.kind[A.idx(ic, k)];
-          if (kc == kDense)
-            need += sizeof(T) * (2 * (size_t)bm * kmax + (size_t)kmax * kmax + 512);
-          else if (kl == kDense && kb == kDense)
-            need += sizeof(T) * ((size_t)bm * bm + (size_t)bm * A.kcap * 2 + (size_t)A.kcap * A.kcap + 1024);
-          else
-            need += sizeof(T) * ((size_t)bm * kmax + (size_t)kmax * kmax + 512);
-          if (kc == kLowRank) {
-            const size_t K = (size_t)kmax + ((kl == kDense && kb == kDense) ? (size_t)A.kcap : (size_t)kmax);
-            need += sizeof(T) * (2 * bm * K + 2 * K + 5 * K * K) + 8 * 256;
-          }
-        }
-    }
-  ctx.reserve(need);
+    const size_t need = step_need(i, max_rank());
+    ctx.reserve(need);
     Plan pl(ctx);
     // 1. the panel solves (src/operations/LAPACK/getrf.cpp:50-57) not in the priority block,
     //    as products with the explicit triangular inverses.  Low-rank tiles go through
@@ -1851,6 +1713,9 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         launch_rows_gemm<T>(reinterpret_cast<const RowRef*>(d + off), rp_, Qst, 1, qcs, sxa, rq_, kk, s);
       }, "schur_gemm", 2.0 * RP * RQ * n_i, sizeof(T) * ((double)RP * n_i + (double)RQ * m_i + (double)RP * RQ));
     }
+    // the context (stream and workspace) and plan the next targets go to
+    Context* cur = &ctx;
+    Plan* curp = &pl;
     // Schur update of one target tile (ic, k): items into gA..gD, additions into adds
     auto pair_body = [&](int ic, int k) {
       const int tl = A.idx(ic, i), tb = A.idx(i, k), tc = A.idx(ic, k);
@@ -1862,20 +1727,20 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         if (kl == kLowRank && kb == kLowRank) {
           const int rl = A.rank[tl], rb = A.rank[tb];
           T* Sx = Sxa + (size_t)poff[ic] * RQ + qoff[k];
-          T* z = ctx.take<T>((size_t)m * rb + 1);
+          T* z = cur->take<T>((size_t)m * rb + 1);
           gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, RQ, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
           gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
         } else if (kl == kLowRank && kb == kDense) {
           const int rl = A.rank[tl];
-          T* w = ctx.take<T>((size_t)rl * n + 1);
-          T* w2 = ctx.take<T>((size_t)rl * n + 1);
+          T* w = cur->take<T>((size_t)rl * n + 1);
+          T* w2 = cur->take<T>((size_t)rl * n + 1);
           gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
           gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
           gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
         } else if (kl == kDense && kb == kLowRank) {
           const int rb = A.rank[tb];
-          T* a = ctx.take<T>((size_t)m * rb + 1);
-          T* a2 = ctx.take<T>((size_t)m * rb + 1);
+          T* a = cur->take<T>((size_t)m * rb + 1);
+          T* a2 = cur->take<T>((size_t)m * rb + 1);
           gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
           gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
           gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
@@ -1914,7 +1779,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         ad.xv_cs = 1;
       } else if (kl == kLowRank && kb == kDense) {  // gemm.cpp:219-234
         const int rl = A.rank[tl];
-        T* w = ctx.take<T>((size_t)rl * n + 1);
+        T* w = cur->take<T>((size_t)rl * n + 1);
         gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
         ad.xu = A.U[tl];
         ad.xu_rs = 1;
@@ -1928,7 +1793,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         ad.xv_cs = 1;
       } else if (kl == kDense && kb == kLowRank) {  // gemm.cpp:236-251
         const int rb = A.rank[tb];
-        T* a = ctx.take<T>((size_t)m * rb + 1);
+        T* a = cur->take<T>((size_t)m * rb + 1);
         gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
         ad.xu = a;
         ad.xu_rs = 1;
@@ -1941,11 +1806,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         ad.xv_rs = A.cols(k);
         ad.xv_cs = 1;
       } else {  // D.D -> LR: compress the dense product first (gemm.cpp:202-217)
-        T* prod = ctx.take<T>((size_t)m * n + 1);
+        T* prod = cur->take<T>((size_t)m * n + 1);
         gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, prod, n, 1, m, n, n_i, -1.0, 0.0));
-        T* xu = ctx.take<T>((size_t)m * A.kcap + 1);
-        T* xs = ctx.take<T>((size_t)A.kcap * A.kcap + 1);
-        T* xv = ctx.take<T>((size_t)A.kcap * n + 1);
+        T* xu = cur->take<T>((size_t)m * A.kcap + 1);
+        T* xs = cur->take<T>((size_t)A.kcap * A.kcap + 1);
+        T* xv = cur->take<T>((size_t)A.kcap * n + 1);
         CompressDesc<T> c{};
         c.a = prod;
         c.m = m;
@@ -1981,7 +1846,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     // give them their own, so Pst, Qst and Sxa survive for the later chunks
     auto naive_adds = [&]() {
       if (adds.empty()) return;
-      Context cn(s);
+      Context cn(cur->stream());
       cn.set_stats_owner(&ctx);
       lr_add_naive<T>(adds, cn);
     };
@@ -1991,47 +1856,58 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
       comp.clear();
       comp_add.clear();
       for (auto* g : {&gA, &gB, &gC, &gD}) g->clear();
-      for (const auto& t : targets) pair_body(t.first, t.second);
-      plan_gemms<T>(pl, gA, "schur_gemm");
-      plan_gemms<T>(pl, gB, "schur_gemm");
-      plan_gemms<T>(pl, gC, "schur_gemm");
-      plan_gemms<T>(pl, gD, "schur_gemm");
+      {
+        const auto tp0 = clk::now();
+        for (const auto& t : targets) pair_body(t.first, t.second);
+        ctx.host_time("host_pairs", ms_since(tp0));
+      }
+      plan_gemms<T>(*curp, gA, "schur_gemm");
+      plan_gemms<T>(*curp, gB, "schur_gemm");
+      plan_gemms<T>(*curp, gC, "schur_gemm");
+      plan_gemms<T>(*curp, gD, "schur_gemm");
       if (!comp.empty()) {
         // D.D -> LR needs the compressed ranks on the host: run what is planned,
         // compress on a side context (own workspace), then plan the additions.
-        pl.run();
+        curp->run();
         DevBuf crank(sizeof(int32_t) * comp.size());
         for (size_t c2 = 0; c2 < comp.size(); ++c2) comp[c2].d_rank = crank.as<int32_t>() + c2;
-        Context cside(s);
+        Context cside(cur->stream());
         compress_batched<T>(comp, cside);
         std::vector<int32_t> cr(comp.size());
         FRK_HIP_CHECK(hipMemcpyAsync(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(),
-                                     hipMemcpyDeviceToHost, s));
-        FRK_HIP_CHECK(hipStreamSynchronize(s));
+                                     hipMemcpyDeviceToHost, cur->stream()));
+        FRK_HIP_CHECK(hipStreamSynchronize(cur->stream()));
         for (size_t c2 = 0; c2 < comp.size(); ++c2) adds[comp_add[c2]].kx1 = adds[comp_add[c2]].kx2 = cr[c2];
         if (A.lra == kLraNaive) {
           naive_adds();
         } else {
-          Plan p2(ctx);
+          Plan p2(*cur);
           plan_lr_add<T>(p2, adds, A.lra);
           p2.run();
         }
         return;
       }
       if (A.lra == kLraNaive) {
-        pl.run();
+        curp->run();
         naive_adds();
       } else {
-        plan_lr_add<T>(pl, adds, A.lra);
+        const auto tl0 = clk::now();
+        plan_lr_add<T>(*curp, adds, A.lra);
+        ctx.host_time("host_lradd", ms_since(tl0));
         ctx.host_time(plan_cls, ms_since(th));
         th = clk::now();
-        pl.run();
+        curp->run();
         ctx.host_time("host_issue", ms_since(th));
       }
     };
-    // 3. every target but (i+1, i+1), in row chunks; each chunk is planned while the
-    //    previous one executes
+    // 3. every target but (i+1, i+1), in two row chunks that run concurrently: the first
+    //    on the main stream, the second on the third stream with its own workspace, once
+    //    the panels and stacked products above are done.  Each chunk ends in a batched
+    //    recompression whose latency does not shrink with the batch, so the two overlap.
+    pl.run();
+    FRK_HIP_CHECK(hipEventRecord(la.sxa, s));
     const int nchunk = std::min(kRestChunks, rem);
+    std::unique_ptr<Plan> pl2;
     for (int c = 0; c < nchunk; ++c) {
       if (c > 0) th = clk::now();
       const int ic0 = i + 1 + (rem * c) / nchunk, ic1 = i + 1 + (rem * (c + 1)) / nchunk;
@@ -2039,18 +1915,31 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
       for (int ic = ic0; ic < ic1; ++ic)
         for (int k = i + 1; k < nb; ++k)
           if (!(ic == i + 1 && k == i + 1)) targets.emplace_back(ic, k);
+      if (c == 1) {
+        FRK_HIP_CHECK(hipStreamWaitEvent(la.s2, la.sxa, 0));
+        rest2.reserve(need);
+        pl2 = std::make_unique<Plan>(rest2);
+        cur = &rest2;
+        curp = pl2.get();
+      }
       issue_targets(targets, c == 0 ? "host_plan" : "host_plan_rest");
+      curp->run();
     }
-    pl.run();
-    // 4. the ranks of the finished step for the host, then the next priority block
-    FRK_HIP_CHECK(hipEventRecord(la.done, s));
-    FRK_HIP_CHECK(hipStreamWaitEvent(la.cs, la.done, 0));
+    // both chunks done: the rank snapshot (second stream), and the main stream waits for
+    // the second chunk before the next priority block
+    FRK_HIP_CHECK(hipEventRecord(la.c0, s));
+    FRK_HIP_CHECK(hipStreamWaitEvent(la.s2, la.c0, 0));
+    FRK_HIP_CHECK(hipEventRecord(la.c1, la.s2));
+    FRK_HIP_CHECK(hipStreamWaitEvent(s, la.c1, 0));
     FRK_HIP_CHECK(
-        hipMemcpyAsync(la.snap, A.d_rank(0, 0), sizeof(int32_t) * A.rank.size(), hipMemcpyDeviceToHost, la.cs));
-    FRK_HIP_CHECK(hipEventRecord(la.rest, la.cs));
+        hipMemcpyAsync(la.snap, A.d_rank(0, 0), sizeof(int32_t) * A.rank.size(), hipMemcpyDeviceToHost, la.s2));
+    FRK_HIP_CHECK(hipEventRecord(la.rest, la.s2));
+    cur = &ctx;
+    curp = &pl;
+    // 4. the next priority block
     if (i + 2 < nb) {
       if (prio_rank_free(A, i + 1))
-        enqueue_prio(i + 1, true);
+        enqueue_prio(i + 1, true, true);
       else
         prio_pending = true;
     }

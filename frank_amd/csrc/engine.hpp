@@ -108,7 +108,11 @@ class Context {
   // Throw if a tile LU launched at an epoch > since timed out (drains the stream).
   void check_tile_lu(int32_t since);
   // Device event counters (kCounter* slots), zero at creation.
-  int32_t* counters() { return counters_.as<int32_t>(); }
+  int32_t* counters() { return owner_->counters_.as<int32_t>(); }
+  // Persistent auxiliary streams (created on first use, one hardware queue each
+  // as long as the process holds few streams): 0 = lookahead factorizations,
+  // 1 = second chunk of a BLR-LU step.
+  hipStream_t aux_stream(int k);
   // op log
   void account(const char* cls, double flops, double bytes);
   // host-side wall time attributed to a class of the op log (counted as timed launches)
@@ -144,6 +148,7 @@ class Context {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending_;
   hipEvent_t ev_open_ = nullptr;
   Context* owner_ = this;
+  hipStream_t aux_[2] = {nullptr, nullptr};
   void resolve_events();
 };
 
@@ -234,31 +239,6 @@ class BlrMatrix {
 // ----------------------------------------------------------------------------
 // Batched building blocks (all asynchronous on ctx.stream()).
 // ----------------------------------------------------------------------------
-template <typename T>
-struct LrAddDesc {
-  // target C = U S V (slots, rank kc known on the host at launch time)
-  T* cu;
-  T* cs;
-  T* cv;
-  int32_t* d_rank;
-  int kc;
-  int64_t cu_rs, cu_cs, cv_rs, cv_cs;  // element (i, t) of U at cu[i*cu_rs + t*cu_cs]; (t, j) of V likewise
-  int lds;
-  int m, n;
-  double eps;  // != 0: fixed accuracy (H2 rule); 0: keep kc
-  int cap;
-  bool cap_trunc = false;  // keep the leading cap terms when the eps-rank exceeds cap
-  // addend X = Ux Sx Vx, Ux m x kx1 (strided), Sx kx1 x kx2 (nullptr = identity),
-  // Vx kx2 x n (strided)
-  const T* xu;
-  int64_t xu_rs, xu_cs;
-  int kx1;
-  const T* xs;
-  int xs_ld;
-  int kx2;
-  const T* xv;
-  int64_t xv_rs, xv_cs;
-};
 
 // C += X for every descriptor (reference: rounded_addition / fast_rounded_addition).
 // Workspace comes from the plan's context (reserve lr_add_workspace() first).

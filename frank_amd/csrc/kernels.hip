@@ -869,13 +869,225 @@ void launch_copy(const GemmItem* d_items, int n_items, int64_t max_elems, hipStr
 }
 
 // ===========================================================================
+// Item sources of the batched QR / SVD / basis-update kernels: an uploaded item
+// array, or the items of a batch of low-rank additions derived on the device
+// from their descriptors (the same construction the host would do, so only the
+// additions' descriptors cross PCIe).  Addition t owns QR items 2t (U side) and
+// 2t+1 (V side), SVD item t, basis-update items 2t and 2t+1.
+// ===========================================================================
+template <class Item>
+struct ItemArray {
+  const Item* p;
+  __device__ Item get(int b) const { return p[b]; }
+};
+
+template <typename T>
+__device__ __forceinline__ bool lr_fast(const LrAddDesc<T>& x, const LrBatch<T>& b) {
+  return b.lra == 2 && x.eps == 0.0;
+}
+
+template <typename T>
+struct LrQrItems {
+  const LrAddDesc<T>* d;
+  LrBatch<T> b;
+  __device__ QrItem get(int w) const {
+    const int t = w >> 1;
+    const LrAddDesc<T> x = d[t];
+    const int K = b.K;
+    QrItem q{};
+    if ((w & 1) == 0) {
+      q.Y = b.Yu + (size_t)t * b.mmax * K;
+      q.y_cs = x.m;
+      q.tau = b.tu + (size_t)t * K;
+      q.R = b.Ru + (size_t)t * K * K;
+      q.m = x.m;
+    } else {
+      q.Y = b.Yv + (size_t)t * b.nmax * K;
+      q.y_cs = x.n;
+      q.tau = b.tv + (size_t)t * K;
+      q.R = b.Rv + (size_t)t * K * K;
+      q.m = x.n;
+    }
+    q.y_rs = 1;
+    q.ldr = K;
+    const bool fast = lr_fast(x, b);
+    if ((w & 1) == 0) {
+      if (!fast) {  // Uc = [Uc Sc | Ux Sx] (formed inside the QR kernel)
+        q.A1 = x.cu;
+        q.a1_rs = x.cu_rs;
+        q.a1_cs = x.cu_cs;
+        q.S1 = x.cs;
+        q.s1_ld = x.lds;
+        q.k1a = q.k1 = x.kc;
+        q.A2 = x.xu;
+        q.a2_rs = x.xu_rs;
+        q.a2_cs = x.xu_cs;
+        q.S2 = x.xs;
+        q.s2_ld = x.xs_ld;
+        q.k2a = x.kx1;
+        q.k2 = x.kx2;
+      } else {  // project X's U basis on C's, QR the remainder
+        q.A1 = x.xu;
+        q.a1_rs = x.xu_rs;
+        q.a1_cs = x.xu_cs;
+        q.S1 = x.xs;
+        q.s1_ld = x.xs_ld;
+        q.k1a = x.kx1;
+        q.k1 = x.kx2;
+        q.k2 = 0;
+        q.P = x.cu;
+        q.p_rs = x.cu_rs;
+        q.p_cs = x.cu_cs;
+        q.kp = x.kc;
+      }
+    } else {
+      if (!fast) {  // Vc^T = [Vc^T | Vx^T]
+        q.A1 = x.cv;
+        q.a1_rs = x.cv_cs;
+        q.a1_cs = x.cv_rs;
+        q.k1a = q.k1 = x.kc;
+        q.A2 = x.xv;
+        q.a2_rs = x.xv_cs;
+        q.a2_cs = x.xv_rs;
+        q.k2a = q.k2 = x.kx2;
+      } else {
+        q.A1 = x.xv;
+        q.a1_rs = x.xv_cs;
+        q.a1_cs = x.xv_rs;
+        q.k1a = q.k1 = x.kx2;
+        q.k2 = 0;
+        q.P = x.cv;
+        q.p_rs = x.cv_cs;
+        q.p_cs = x.cv_rs;
+        q.kp = x.kc;
+      }
+    }
+    return q;
+  }
+};
+
+template <typename T>
+struct LrSvdItems {
+  const LrAddDesc<T>* d;
+  LrBatch<T> b;
+  __device__ SvdItem get(int t) const {
+    const LrAddDesc<T> x = d[t];
+    const int K = b.K;
+    const int Kt = x.kc + x.kx2;
+    const bool fast = lr_fast(x, b);
+    SvdItem sv{};
+    sv.mode = 1;
+    sv.Ru = b.Ru + (size_t)t * K * K;
+    sv.Rv = b.Rv + (size_t)t * K * K;
+    sv.ldr = K;
+    sv.K = Kt;
+    if (!fast) {
+      sv.kx = Kt;
+      sv.ka = 0;
+    } else {
+      sv.kx = x.kx2;
+      sv.ka = x.kc;
+      sv.Sa = x.cs;
+      sv.lds_in = x.lds;
+    }
+    sv.fixed_rank = (x.eps != 0.0) ? -1 : x.kc;
+    sv.kdim = min(min(x.m, Kt), min(x.n, Kt));
+    sv.eps = x.eps;
+    sv.S = x.cs;
+    sv.lds = x.lds;
+    sv.Um = b.Um + (size_t)t * K * K;
+    sv.ldu = K;
+    sv.Vm = b.Vm + (size_t)t * K * K;
+    sv.ldv = K;
+    sv.rank_out = x.d_rank;
+    sv.cap = x.cap;
+    sv.cap_trunc = x.cap_trunc ? 1 : 0;
+    sv.stats = b.stats;
+    sv.vwork = b.Vw ? b.Vw + (size_t)t * K * K : nullptr;
+    return sv;
+  }
+};
+
+template <typename T>
+struct LrApplyItems {
+  const LrAddDesc<T>* d;
+  LrBatch<T> b;
+  __device__ ApplyQItem get(int w) const {
+    const int t = w >> 1;
+    const LrAddDesc<T> x = d[t];
+    const int K = b.K;
+    const int Kt = x.kc + x.kx2;
+    const bool fast = lr_fast(x, b);
+    T* um = b.Um + (size_t)t * K * K;
+    T* vm = b.Vm + (size_t)t * K * K;
+    ApplyQItem a{};
+    a.y_rs = 1;
+    a.ncol_ptr = x.d_rank;
+    a.ncol = Kt;
+    a.trans = 0;
+    a.x_rs = K;
+    a.x_cs = 1;
+    if ((w & 1) == 0) {
+      a.Y = b.Yu + (size_t)t * b.mmax * K;
+      a.y_cs = x.m;
+      a.tau = b.tu + (size_t)t * K;
+      a.m = x.m;
+      a.out = x.cu;
+      a.out_rs = x.cu_rs;
+      a.out_cs = x.cu_cs;
+      if (!fast) {
+        a.nref = min(x.m, Kt);
+        a.X = um;
+        a.kx = Kt;
+      } else {
+        a.nref = min(x.m, x.kx2);
+        a.X = um + (size_t)x.kc * K;
+        a.kx = x.kx2;
+        a.Base = x.cu;
+        a.base_rs = x.cu_rs;
+        a.base_cs = x.cu_cs;
+        a.Xb = um;
+        a.xb_rs = K;
+        a.xb_cs = 1;
+        a.kb = x.kc;
+      }
+    } else {
+      a.Y = b.Yv + (size_t)t * b.nmax * K;
+      a.y_cs = x.n;
+      a.tau = b.tv + (size_t)t * K;
+      a.m = x.n;
+      a.out = x.cv;
+      a.out_rs = x.cv_cs;
+      a.out_cs = x.cv_rs;
+      if (!fast) {
+        a.nref = min(x.n, Kt);
+        a.X = vm;
+        a.kx = Kt;
+      } else {
+        a.nref = min(x.n, x.kx2);
+        a.X = vm + (size_t)x.kc * K;
+        a.kx = x.kx2;
+        a.Base = x.cv;
+        a.base_rs = x.cv_cs;
+        a.base_cs = x.cv_rs;
+        a.Xb = vm;
+        a.xb_rs = K;
+        a.xb_cs = 1;
+        a.kb = x.kc;
+      }
+    }
+    return a;
+  }
+};
+
+// ===========================================================================
 // Householder QR of [A1 S1 | A2 S2] (m x K, m <= 64 * MR, K <= W * CPW).
 // Optional MGS projection against an orthonormal basis P first; its
 // coefficients occupy rows [0, kp) of the R buffer, R itself rows [kp, kp+K).
 // ===========================================================================
-template <typename T, int MR, int W, int CPW>
-__global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ items) {
-  const QrItem it = items[blockIdx.x];
+template <typename T, int MR, int W, int CPW, class Src>
+__global__ __launch_bounds__(W * 64) void qr_kernel(Src items) {
+  const QrItem it = items.get(blockIdx.x);
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int m = it.m;
@@ -1059,22 +1271,27 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(const QrItem* __restrict__ i
   }
 }
 
-template <typename T>
-void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s) {
+template <typename T, class Src>
+static void launch_qr_src(Src d_items, int n_items, int max_k, hipStream_t s) {
   if (n_items <= 0) return;
   if (max_k <= 16) {
-    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 4>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 4, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_k <= 32) {
-    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 8>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 8, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_k <= 64) {
     // 16 waves x 4 columns: 32 register values per lane stay within the 1024-thread budget
-    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 4>), dim3(n_items), dim3(1024), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 4, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else if (max_k <= 128) {
-    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 8>), dim3(n_items), dim3(1024), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 8, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else {
     throw std::runtime_error("qr: K exceeds 128");
   }
   FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s) {
+  launch_qr_src<T>(ItemArray<QrItem>{d_items}, n_items, max_k, s);
 }
 
 // ===========================================================================
@@ -1082,9 +1299,9 @@ void launch_qr(const QrItem* d_items, int n_items, int max_k, hipStream_t s) {
 // Columns are independent: no workgroup barrier except before the write when
 // a base term may alias the output.
 // ===========================================================================
-template <typename T, int MR, int W, int CPW>
-__global__ __launch_bounds__(W * 64) void apply_q_kernel(const ApplyQItem* __restrict__ items) {
-  ApplyQItem it = items[blockIdx.x];
+template <typename T, int MR, int W, int CPW, class Src>
+__global__ __launch_bounds__(W * 64) void apply_q_kernel(Src items) {
+  ApplyQItem it = items.get(blockIdx.x);
   if (it.ncol_ptr != nullptr) it.ncol = *it.ncol_ptr;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
@@ -1178,21 +1395,26 @@ __global__ __launch_bounds__(W * 64) void apply_q_kernel(const ApplyQItem* __res
   }
 }
 
-template <typename T>
-void launch_apply_q(const ApplyQItem* d_items, int n_items, int max_ncol, hipStream_t s) {
+template <typename T, class Src>
+static void launch_apply_q_src(Src d_items, int n_items, int max_ncol, hipStream_t s) {
   if (n_items <= 0) return;
   if (max_ncol <= 16) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 4>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 4, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_ncol <= 32) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 8>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 8, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_ncol <= 64) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 8, 8>), dim3(n_items), dim3(512), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 8, 8, Src>), dim3(n_items), dim3(512), 0, s, d_items);
   } else if (max_ncol <= 128) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 16, 8>), dim3(n_items), dim3(1024), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, 8, 16, 8, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else {
     throw std::runtime_error("apply_q: more than 128 columns");
   }
   FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_apply_q(const ApplyQItem* d_items, int n_items, int max_ncol, hipStream_t s) {
+  launch_apply_q_src<T>(ItemArray<ApplyQItem>{d_items}, n_items, max_ncol, s);
 }
 
 // ===========================================================================
@@ -1409,9 +1631,9 @@ __device__ __forceinline__ int rr_pos(int pos, int r, int k2) {
   return pos == 0 ? 0 : ((pos - 1 + r) % (k2 - 1)) + 1;
 }
 
-template <typename T, int KMAX, int NT, bool VG>
-__global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ items) {
-  const SvdItem it = items[blockIdx.x];
+template <typename T, int KMAX, int NT, bool VG, class Src>
+__global__ __launch_bounds__(NT) void svd_kernel(Src items) {
+  const SvdItem it = items.get(blockIdx.x);
   const int K = it.K;
   __shared__ T G[KMAX][KMAX + 1];
   // right vectors: LDS, or a K x K global scratch (column j at vg + j * K) when they do not fit
@@ -1691,26 +1913,44 @@ __global__ __launch_bounds__(NT) void svd_kernel(const SvdItem* __restrict__ ite
   }
 }
 
-template <typename T>
-void launch_svd(const SvdItem* d_items, int n_items, int max_k, hipStream_t s) {
+template <typename T, class Src>
+static void launch_svd_src(Src d_items, int n_items, int max_k, hipStream_t s) {
   if (n_items <= 0) return;
   if (max_k <= 16) {
-    hipLaunchKernelGGL((svd_kernel<T, 16, 64, false>), dim3(n_items), dim3(64), 0, s, d_items);
+    hipLaunchKernelGGL((svd_kernel<T, 16, 64, false, Src>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 32) {
-    hipLaunchKernelGGL((svd_kernel<T, 32, 64, false>), dim3(n_items), dim3(64), 0, s, d_items);
+    hipLaunchKernelGGL((svd_kernel<T, 32, 64, false, Src>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 64) {
-    hipLaunchKernelGGL((svd_kernel<T, 64, 256, false>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((svd_kernel<T, 64, 256, false, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_k <= 128) {
     if constexpr (sizeof(T) == 4) {
-      hipLaunchKernelGGL((svd_kernel<T, 128, 256, false>), dim3(n_items), dim3(256), 0, s, d_items);
+      hipLaunchKernelGGL((svd_kernel<T, 128, 256, false, Src>), dim3(n_items), dim3(256), 0, s, d_items);
     } else {
       // every item must carry a K x K scratch (SvdItem::vwork) for its right vectors
-      hipLaunchKernelGGL((svd_kernel<T, 128, 256, true>), dim3(n_items), dim3(256), 0, s, d_items);
+      hipLaunchKernelGGL((svd_kernel<T, 128, 256, true, Src>), dim3(n_items), dim3(256), 0, s, d_items);
     }
   } else {
     throw std::runtime_error("svd: K above 128");
   }
   FRK_HIP_CHECK(hipGetLastError());
+}
+
+template <typename T>
+void launch_svd(const SvdItem* d_items, int n_items, int max_k, hipStream_t s) {
+  launch_svd_src<T>(ItemArray<SvdItem>{d_items}, n_items, max_k, s);
+}
+
+template <typename T>
+void launch_lr_add_qr(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_k, hipStream_t s) {
+  launch_qr_src<T>(LrQrItems<T>{d, b}, 2 * nd, max_k, s);
+}
+template <typename T>
+void launch_lr_add_svd(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s) {
+  launch_svd_src<T>(LrSvdItems<T>{d, b}, nd, b.K, s);
+}
+template <typename T>
+void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_ncol, hipStream_t s) {
+  launch_apply_q_src<T>(LrApplyItems<T>{d, b}, 2 * nd, max_ncol, s);
 }
 
 // ===========================================================================
@@ -2716,6 +2956,9 @@ template void launch_axpy_convert<float>(double*, const float*, int64_t, hipStre
   template void launch_tri_extract<T>(const T*, int64_t, int64_t, T*, int64_t, int, int, hipStream_t); \
   template void launch_apply_q<T>(const ApplyQItem*, int, int, hipStream_t);                     \
   template void launch_svd<T>(const SvdItem*, int, int, hipStream_t);                            \
+  template void launch_lr_add_qr<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, int, hipStream_t);  \
+  template void launch_lr_add_svd<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, hipStream_t);      \
+  template void launch_lr_add_apply<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, int, hipStream_t); \
   template void launch_geqp3<T>(const Geqp3Item*, int, hipStream_t);                             \
   template void launch_permute_r<T>(const PermuteItem*, int, int, int, hipStream_t);             \
   template void launch_flip_r<T>(const FlipItem*, int, int, hipStream_t);                        \

@@ -115,6 +115,14 @@ void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int
 template <typename T>
 void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStream_t s,
                          const int32_t* d_count = nullptr);
+// The three launches of a batch of low-rank additions (QR of both bases, SVD of
+// the core, basis update), items derived on the device from the descriptors.
+template <typename T>
+void launch_lr_add_qr(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_k, hipStream_t s);
+template <typename T>
+void launch_lr_add_svd(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s);
+template <typename T>
+void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_ncol, hipStream_t s);
 // Two strided stacked-row products in one launch (see RowsPair).
 template <typename T>
 void launch_rows_pair(const RowsPair& p, hipStream_t s);
