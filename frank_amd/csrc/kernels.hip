@@ -2603,34 +2603,40 @@ template void launch_rows_scatter<float>(const RowRef*, int, const float*, int, 
 // ===========================================================================
 template <typename T>
 __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
-  constexpr int KC = 256;
+  constexpr int KC = sizeof(T) == 8 ? 64 : 256;  // K chunk staged in LDS (FP64 has less room)
   __shared__ T M1[64][65];
   __shared__ T T1[64][65];
+  __shared__ T Sls[64][65];
+  __shared__ T Sbs[64][65];
   __shared__ T Va[16][KC + 1];
   __shared__ T Ub[KC][17];
   const int tid = threadIdx.x;
   const int rl = min(it.kcap, max(0, *it.rl)), rb = min(it.kcap, max(0, *it.rb));
   if (rl == 0 || rb == 0) return;
-  const T* V = static_cast<const T*>(it.vl);
-  const T* U = static_cast<const T*>(it.ub);
-  const T* Sl = static_cast<const T*>(it.sl);
-  const T* Sb = static_cast<const T*>(it.sb);
-  T* core = static_cast<T*>(it.core);
+  const T* __restrict__ V = static_cast<const T*>(it.vl);
+  const T* __restrict__ U = static_cast<const T*>(it.ub);
+  const T* __restrict__ Sl = static_cast<const T*>(it.sl);
+  const T* __restrict__ Sb = static_cast<const T*>(it.sb);
+  T* __restrict__ core = static_cast<T*>(it.core);
   const int K = it.k;
   const int ti = tid >> 4, tj = tid & 15;
+  // the S cores into LDS (one round of loads), and the panel rows written back
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    Sls[r][c] = (r < rl && c < rl) ? Sl[(int64_t)r * it.lds + c] : T(0);
+    Sbs[r][c] = (r < rb && c < rb) ? Sb[(int64_t)r * it.lds + c] : T(0);
+  }
   if (it.vl_dst) {
-    T* d = static_cast<T*>(it.vl_dst);
-    for (int e = tid; e < rl * it.nv; e += 256) {
-      const int r = e / it.nv, j = e % it.nv;
-      d[(int64_t)r * it.vl_dst_ld + j] = V[(int64_t)r * it.v_rs + (int64_t)j * it.v_cs];
-    }
+    T* __restrict__ d = static_cast<T*>(it.vl_dst);
+    for (int r = 0; r < rl; ++r)
+      for (int j = tid; j < it.nv; j += 256)
+        d[(int64_t)r * it.vl_dst_ld + j] = V[(int64_t)r * it.v_rs + (int64_t)j * it.v_cs];
   }
   if (it.ub_dst) {
-    T* d = static_cast<T*>(it.ub_dst);
-    for (int e = tid; e < rb * it.nu; e += 256) {
-      const int c = e / it.nu, j = e % it.nu;
-      d[(int64_t)c * it.ub_dst_ld + j] = U[(int64_t)j * it.u_rs + (int64_t)c * it.u_cs];
-    }
+    T* __restrict__ d = static_cast<T*>(it.ub_dst);
+    for (int c = 0; c < rb; ++c)
+      for (int j = tid; j < it.nu; j += 256)
+        d[(int64_t)c * it.ub_dst_ld + j] = U[(int64_t)j * it.u_rs + (int64_t)c * it.u_cs];
   }
   // M1 = V_l U_b (rl x rb), 16 x 16 output blocks, K in chunks of KC
   for (int bi = 0; bi < rl; bi += 16)
@@ -2653,18 +2659,18 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
       if (bi + ti < rl && bj + tj < rb) M1[bi + ti][bj + tj] = acc;
     }
   __syncthreads();
-  // T1 = M1 S_b, core = alpha S_l T1
+  // T1 = M1 S_b, core = alpha S_l T1 (all operands in LDS)
   for (int e = tid; e < rl * rb; e += 256) {
     const int i = e / rb, j = e % rb;
     T sacc = T(0);
-    for (int q = 0; q < rb; ++q) sacc += M1[i][q] * Sb[(int64_t)q * it.lds + j];
+    for (int q = 0; q < rb; ++q) sacc += M1[i][q] * Sbs[q][j];
     T1[i][j] = sacc;
   }
   __syncthreads();
   for (int e = tid; e < rl * rb; e += 256) {
     const int i = e / rb, j = e % rb;
     T sacc = T(0);
-    for (int q = 0; q < rl; ++q) sacc += Sl[(int64_t)i * it.lds + q] * T1[q][j];
+    for (int q = 0; q < rl; ++q) sacc += Sls[i][q] * T1[q][j];
     core[(int64_t)i * it.ldk + j] = T(it.alpha) * sacc;
   }
 }
@@ -2689,6 +2695,7 @@ __global__ __launch_bounds__(256) void lr_update_kernel(LrUpdItem it) {
     const int e = tid + 256 * q, a = e >> 6, b = e & 63;
     cv[q] = (i0 + a < it.m && c0 + b < it.n) ? C[(int64_t)(i0 + a) * it.ldc + c0 + b] : T(0);
   }
+#pragma unroll
   for (int e = tid; e < 64 * 64; e += 256) {
     const int a = e >> 6, b = e & 63;
     Ks[a][b] = (a < r1 && b < r2) ? core[(int64_t)a * it.ldk + b] : T(0);
