@@ -281,6 +281,16 @@ struct SolveUpdItem {
   int32_t m, n, r, pad;
 };
 
+// Diagonal part of one fused substitution step: x (n x nrhs) = op(Tinv) b after
+// the update items[n_upd] (when has_last) is applied to a copy of b.
+struct StepDiag {
+  const void* tinv;
+  int64_t ld;
+  const void* b;
+  void* x;
+  int32_t n, trans, has_last, pad;
+};
+
 // One row of a stacked-row product: a is a contiguous K-vector (input row),
 // c where the finished row goes (contiguous N-vector).
 // Low-rank x low-rank core with device ranks: core = alpha S_l (V_l U_b) S_b.

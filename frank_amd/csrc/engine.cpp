@@ -1983,73 +1983,98 @@ static void blr_solve_fast(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context&
   ctx.reserve(sizeof(T) * ((size_t)N * nrhs + 64) + 4096);
   T* y = ctx.take<T>((size_t)N * nrhs + 1);
   Plan pl(ctx);
+  // one sweep: block p of the sweep order (p = 0 .. nb-1) is solved by the fused step
+  // launch of position p-1, which also applies the updates from block p-1 to the
+  // blocks at positions p+1 and beyond; the update into block p itself from block
+  // p-1 is applied inside the diagonal workgroups of block p
   auto sweep = [&](bool forward, bool lower_factor, bool transposed, T* src, T* dst) {
-    for (int step = 0; step < nb; ++step) {
-      const int i = forward ? step : nb - 1 - step;
-      const int ii = F.idx(i, i);
+    auto blk_of = [&](int p) { return forward ? p : nb - 1 - p; };
+    // the update of target block k from the solved block i
+    auto upd_item = [&](int k, int i, SolveUpdItem& u, double& fl, double& by) -> bool {
+      const int t = transposed ? F.idx(i, k) : F.idx(k, i);
+      if (F.kind[t] == kEmpty) return false;
       const int ni = (int)F.rows(i);
-      const int64_t ld = F.cols(i);
-      const T* tinv = lower_factor ? F.Linv[ii] : F.Uinv[ii];
-      T* sp = src + F.roff[i] * nrhs;
-      T* dp = dst + F.roff[i] * nrhs;
-      pl.launch([=](const char*) { launch_diag_apply<T>(tinv, ld, transposed, sp, dp, ni, nrhs, s); }, "solve_diag",
-                2.0 * ni * ni * nrhs, sizeof(T) * ((double)ni * ni + 2.0 * ni * nrhs));
+      u = SolveUpdItem{};
+      u.x = dst + F.roff[i] * nrhs;
+      u.b = src + F.roff[k] * nrhs;
+      u.m = (int)F.rows(k);
+      u.n = ni;
+      if (F.kind[t] == kDense) {
+        u.D = F.D[t];
+        if (!transposed) {
+          u.d_rs = F.cols(i);
+          u.d_cs = 1;
+        } else {
+          u.d_rs = 1;
+          u.d_cs = F.cols(k);
+        }
+        fl += 2.0 * u.m * ni * nrhs;
+        by += sizeof(T) * (double)u.m * ni;
+      } else {
+        const int r = F.rank[t];
+        u.r = r;
+        if (!transposed) {  // b_k -= U (S (V x)), tile (k, i)
+          u.U = F.U[t];
+          u.u_rs = 1;
+          u.u_cs = F.rows(k);
+          u.S = F.S[t];
+          u.s_rs = F.kcap;
+          u.s_cs = 1;
+          u.V = F.V[t];
+          u.v_rs = F.cols(i);
+          u.v_cs = 1;
+        } else {  // b_k -= V^T (S^T (U^T x)), tile (i, k)
+          u.U = F.V[t];
+          u.u_rs = 1;
+          u.u_cs = F.cols(k);
+          u.S = F.S[t];
+          u.s_rs = 1;
+          u.s_cs = F.kcap;
+          u.V = F.U[t];
+          u.v_rs = F.rows(i);
+          u.v_cs = 1;
+        }
+        fl += 2.0 * r * ((double)ni + r + u.m) * nrhs;
+        by += sizeof(T) * (double)r * (ni + u.m);
+      }
+      return true;
+    };
+    for (int p = -1; p + 1 < nb; ++p) {
+      // launch of position p: bulk updates from block p (p >= 0), then block p+1
+      const int kn = blk_of(p + 1);
       std::vector<SolveUpdItem> items;
       double fl = 0, by = 0;
-      for (int k = 0; k < nb; ++k) {
-        if (forward ? (k <= i) : (k >= i)) continue;
-        const int t = transposed ? F.idx(i, k) : F.idx(k, i);
-        if (F.kind[t] == kEmpty) continue;
-        SolveUpdItem u{};
-        u.x = dp;
-        u.b = src + F.roff[k] * nrhs;
-        u.m = (int)F.rows(k);
-        u.n = ni;
-        if (F.kind[t] == kDense) {
-          u.D = F.D[t];
-          if (!transposed) {
-            u.d_rs = F.cols(i);
-            u.d_cs = 1;
-          } else {
-            u.d_rs = 1;
-            u.d_cs = F.cols(k);
-          }
-          fl += 2.0 * u.m * ni * nrhs;
-          by += sizeof(T) * (double)u.m * ni;
-        } else {
-          const int r = F.rank[t];
-          u.r = r;
-          if (!transposed) {  // b_k -= U (S (V x)), tile (k, i)
-            u.U = F.U[t];
-            u.u_rs = 1;
-            u.u_cs = F.rows(k);
-            u.S = F.S[t];
-            u.s_rs = F.kcap;
-            u.s_cs = 1;
-            u.V = F.V[t];
-            u.v_rs = F.cols(i);
-            u.v_cs = 1;
-          } else {  // b_k -= V^T (S^T (U^T x)), tile (i, k)
-            u.U = F.V[t];
-            u.u_rs = 1;
-            u.u_cs = F.cols(k);
-            u.S = F.S[t];
-            u.s_rs = 1;
-            u.s_cs = F.kcap;
-            u.V = F.U[t];
-            u.v_rs = F.rows(i);
-            u.v_cs = 1;
-          }
-          fl += 2.0 * r * ((double)ni + r + u.m) * nrhs;
-          by += sizeof(T) * (double)r * (ni + u.m);
-        }
-        items.push_back(u);
+      if (p >= 0) {
+        const int i = blk_of(p);
+        SolveUpdItem u;
+        for (int q = p + 2; q < nb; ++q)
+          if (upd_item(blk_of(q), i, u, fl, by)) items.push_back(u);
       }
-      if (items.empty()) continue;
-      const size_t off = pl.add(items);
-      const int cnt = (int)items.size();
-      pl.launch([=](const char* d) { launch_solve_update<T>(reinterpret_cast<const SolveUpdItem*>(d + off), cnt, nrhs, s); },
-                "solve_upd", fl, by);
+      const int n_upd = (int)items.size();
+      StepDiag sd{};
+      if (p >= 0) {
+        SolveUpdItem u;
+        if (upd_item(kn, blk_of(p), u, fl, by)) {
+          items.push_back(u);
+          sd.has_last = 1;
+        }
+      }
+      const int kk = F.idx(kn, kn);
+      sd.tinv = lower_factor ? F.Linv[kk] : F.Uinv[kk];
+      sd.ld = F.cols(kn);
+      sd.b = src + F.roff[kn] * nrhs;
+      sd.x = dst + F.roff[kn] * nrhs;
+      sd.n = (int)F.rows(kn);
+      sd.trans = transposed ? 1 : 0;
+      fl += 2.0 * sd.n * sd.n * nrhs;
+      by += sizeof(T) * ((double)sd.n * sd.n + 2.0 * sd.n * nrhs);
+      const size_t off = items.empty() ? 0 : pl.add(items);
+      const bool any = !items.empty();
+      pl.launch(
+          [=](const char* d) {
+            launch_solve_step<T>(any ? reinterpret_cast<const SolveUpdItem*>(d + off) : nullptr, n_upd, sd, nrhs, s);
+          },
+          "solve_step", fl, by);
     }
   };
   const bool do_l = part != 2, do_u = part != 1;

@@ -2382,12 +2382,12 @@ void launch_diag_apply(const T* Tinv, int64_t ld, bool trans, const T* b, T* w, 
   FRK_HIP_CHECK(hipGetLastError());
 }
 
+// b -= U (S (V x)) (low-rank) or b -= D x (dense) for one target block; the
+// source x is staged in xs.  With bl != nullptr the target is the LDS copy bl
+// (rows x NR) instead of the global b.
 template <typename T, int NR>
-__global__ __launch_bounds__(256) void solve_update_kernel(const SolveUpdItem* __restrict__ items, int nrhs) {
-  __shared__ T xs[512][NR];
-  __shared__ T t1[64][NR];
-  __shared__ T t2[64][NR];
-  const SolveUpdItem it = items[blockIdx.x];
+__device__ __forceinline__ void solve_update_body(const SolveUpdItem& it, int nrhs, T (*xs)[NR], T (*t1)[NR],
+                                                  T (*t2)[NR], T (*bl)[NR]) {
   const int tid = threadIdx.x;
   const T* x = static_cast<const T*>(it.x);
   T* b = static_cast<T*>(it.b);
@@ -2402,14 +2402,24 @@ __global__ __launch_bounds__(256) void solve_update_kernel(const SolveUpdItem* _
       T acc[NR];
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = T(0);
-      for (int k = 0; k < it.n; ++k) {
-        const T a = D[row * it.d_rs + k * it.d_cs];
+      // loads issued eight at a time (one memory round trip per chunk)
+      for (int k0 = 0; k0 < it.n; k0 += 8) {
+        T a[8];
 #pragma unroll
-        for (int c = 0; c < NR; ++c) acc[c] += a * xs[k][c];
+        for (int j = 0; j < 8; ++j) a[j] = (k0 + j < it.n) ? D[row * it.d_rs + (k0 + j) * it.d_cs] : T(0);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (k0 + j < it.n)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[c] += a[j] * xs[k0 + j][c];
       }
 #pragma unroll
-      for (int c = 0; c < NR; ++c)
-        if (c < nrhs) b[(int64_t)row * nrhs + c] -= acc[c];
+      for (int c = 0; c < NR; ++c) {
+        if (bl)
+          bl[row][c] -= acc[c];
+        else if (c < nrhs)
+          b[(int64_t)row * nrhs + c] -= acc[c];
+      }
     }
     return;
   }
@@ -2444,7 +2454,14 @@ __global__ __launch_bounds__(256) void solve_update_kernel(const SolveUpdItem* _
   for (int e = tid; e < r * NR; e += 256) {  // t2 = S t1
     const int rho = e / NR, c = e % NR;
     T acc = T(0);
-    for (int q = 0; q < r; ++q) acc += S[rho * it.s_rs + q * it.s_cs] * t1[q][c];
+    for (int q0 = 0; q0 < r; q0 += 8) {
+      T sv[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sv[j] = (q0 + j < r) ? S[rho * it.s_rs + (q0 + j) * it.s_cs] : T(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (q0 + j < r) acc += sv[j] * t1[q0 + j][c];
+    }
     t2[rho][c] = acc;
   }
   __syncthreads();
@@ -2452,16 +2469,136 @@ __global__ __launch_bounds__(256) void solve_update_kernel(const SolveUpdItem* _
     T acc[NR];
 #pragma unroll
     for (int c = 0; c < NR; ++c) acc[c] = T(0);
-    for (int q = 0; q < r; ++q) {
-      const T a = U[row * it.u_rs + q * it.u_cs];
+    for (int q0 = 0; q0 < r; q0 += 8) {
+      T uv[8];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) acc[c] += a * t2[q][c];
+      for (int j = 0; j < 8; ++j) uv[j] = (q0 + j < r) ? U[row * it.u_rs + (q0 + j) * it.u_cs] : T(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (q0 + j < r)
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] += uv[j] * t2[q0 + j][c];
     }
 #pragma unroll
-    for (int c = 0; c < NR; ++c)
-      if (c < nrhs) b[(int64_t)row * nrhs + c] -= acc[c];
+    for (int c = 0; c < NR; ++c) {
+      if (bl)
+        bl[row][c] -= acc[c];
+      else if (c < nrhs)
+        b[(int64_t)row * nrhs + c] -= acc[c];
+    }
   }
 }
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void solve_update_kernel(const SolveUpdItem* __restrict__ items, int nrhs) {
+  __shared__ T xs[512][NR];
+  __shared__ T t1[64][NR];
+  __shared__ T t2[64][NR];
+  const SolveUpdItem it = items[blockIdx.x];
+  solve_update_body<T, NR>(it, nrhs, xs, t1, t2, nullptr);
+}
+
+// One block step of a substitution sweep in one launch: workgroups
+// [0, n_upd) apply the updates from the block solved last to the blocks two or
+// more positions ahead; the remaining ceil(n / kStepRows) workgroups each take
+// the next block's right-hand side, apply the one update it still lacks
+// (items[n_upd], when present) in LDS, and compute kStepRows rows of
+// x = op(Tinv) b.  The two roles
+// do not depend on each other within the launch.
+constexpr int kStepRows = 16;  // rows of x per diagonal workgroup of a fused step
+
+template <typename T, int NR>
+__global__ __launch_bounds__(256) void solve_step_kernel(const SolveUpdItem* __restrict__ items, int n_upd,
+                                                         StepDiag sd, int nrhs) {
+  __shared__ T xs[512][NR];
+  __shared__ T t1[64][NR];
+  __shared__ T t2[64][NR];
+  __shared__ T bs[512][NR];
+  const int tid = threadIdx.x;
+  if ((int)blockIdx.x < n_upd) {
+    const SolveUpdItem it = items[blockIdx.x];
+    solve_update_body<T, NR>(it, nrhs, xs, t1, t2, nullptr);
+    return;
+  }
+  const int g = blockIdx.x - n_upd, n = sd.n, r0 = kStepRows * g;
+  const T* b = static_cast<const T*>(sd.b);
+  for (int e = tid; e < n * NR; e += 256) {
+    const int k = e / NR, c = e % NR;
+    bs[k][c] = (c < nrhs) ? b[(int64_t)k * nrhs + c] : T(0);
+  }
+  __syncthreads();
+  if (sd.has_last) {
+    const SolveUpdItem it = items[n_upd];
+    solve_update_body<T, NR>(it, nrhs, xs, t1, t2, bs);
+  }
+  __syncthreads();
+  const T* Ti = static_cast<const T*>(sd.tinv);
+  T* x = static_cast<T*>(sd.x);
+  const int lane = tid & 63, wv = tid >> 6;
+  if (!sd.trans) {
+    // x[r] = sum_k Tinv[r][k] b[k]: a wave per row, lanes over k
+    for (int rr = wv; rr < kStepRows && r0 + rr < n; rr += 4) {
+      const T* tr = Ti + (int64_t)(r0 + rr) * sd.ld;
+      T acc[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = T(0);
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int k = lane + 64 * jj;
+        if (k < n) {
+          const T a = tr[k];
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] += a * bs[k][c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = wave_sum(acc[c]);
+      if (lane < nrhs) {
+        T v = acc[0];
+#pragma unroll
+        for (int c = 1; c < NR; ++c)
+          if (lane == c) v = acc[c];
+        x[(int64_t)(r0 + rr) * nrhs + lane] = v;
+      }
+    }
+  } else {
+    // x[r] = sum_k Tinv[k][r] b[k]: a lane per row, the four waves split k,
+    // partial sums meet in LDS (xs is free once the update is applied)
+    T (*red)[NR + 1] = reinterpret_cast<T (*)[NR + 1]>(&xs[0][0]);
+    const int r = r0 + lane;
+    T acc[NR];
+#pragma unroll
+    for (int c = 0; c < NR; ++c) acc[c] = T(0);
+    if (lane < kStepRows && r < n)
+      for (int k = wv; k < n; k += 4) {
+        const T a = Ti[(int64_t)k * sd.ld + r];
+#pragma unroll
+        for (int c = 0; c < NR; ++c) acc[c] += a * bs[k][c];
+      }
+#pragma unroll
+    for (int c = 0; c < NR; ++c) red[wv * 64 + lane][c] = acc[c];
+    __syncthreads();
+    for (int e = tid; e < kStepRows * NR; e += 256) {
+      const int rr = e / NR, c = e % NR;
+      if (r0 + rr < n && c < nrhs)
+        x[(int64_t)(r0 + rr) * nrhs + c] =
+            red[rr][c] + red[64 + rr][c] + red[128 + rr][c] + red[192 + rr][c];
+    }
+  }
+}
+
+template <typename T>
+void launch_solve_step(const SolveUpdItem* d_items, int n_upd, const StepDiag& sd, int nrhs, hipStream_t s) {
+  if (nrhs <= 0 || nrhs > 16 || sd.n <= 0 || sd.n > 512) throw std::runtime_error("solve step: nrhs <= 16, n <= 512");
+  const dim3 grid(n_upd + (sd.n + kStepRows - 1) / kStepRows);
+  if (nrhs > 8)
+    hipLaunchKernelGGL((solve_step_kernel<T, 16>), grid, dim3(256), 0, s, d_items, n_upd, sd, nrhs);
+  else
+    hipLaunchKernelGGL((solve_step_kernel<T, 8>), grid, dim3(256), 0, s, d_items, n_upd, sd, nrhs);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_solve_step<double>(const SolveUpdItem*, int, const StepDiag&, int, hipStream_t);
+template void launch_solve_step<float>(const SolveUpdItem*, int, const StepDiag&, int, hipStream_t);
 
 template <typename T>
 void launch_solve_update(const SolveUpdItem* d_items, int n_items, int nrhs, hipStream_t s) {

@@ -123,6 +123,9 @@ template <typename T>
 void launch_lr_add_svd(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s);
 template <typename T>
 void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_ncol, hipStream_t s);
+// One fused block step of a substitution sweep (see StepDiag).
+template <typename T>
+void launch_solve_step(const SolveUpdItem* d_items, int n_upd, const StepDiag& sd, int nrhs, hipStream_t s);
 // Two strided stacked-row products in one launch (see RowsPair).
 template <typename T>
 void launch_rows_pair(const RowsPair& p, hipStream_t s);
