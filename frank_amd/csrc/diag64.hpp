@@ -187,15 +187,19 @@ __device__ __forceinline__ void mma64(double* C, int ldc, const double* A, int l
 __device__ __forceinline__ int factor64(double* M, double* Li, double* Ui, int ld, int* zero_sh) {
   const int tid = threadIdx.x, w = tid >> 6;
   if (tid == 0) *zero_sh = -1;
-  for (int kb = 0; kb < 4; ++kb) {
+  // the LU of the diagonal tile kb + 1 runs in wave 0 during the trailing update of
+  // step kb (lookahead): wave 0 updates that tile first and factors it, waves 1-3
+  // update the other tiles, which never touch it
+  auto diag_lu = [&](int kb) {
     double* Mkk = M + 16 * kb * ld + 16 * kb;
-    double* Lkk = Li + 16 * kb * ld + 16 * kb;
-    double* Ukk = Ui + 16 * kb * ld + 16 * kb;
-    if (w == 0) {
-      const int z = lu16(Mkk, Lkk, Ukk, ld);
-      if (z >= 0 && (threadIdx.x & 63) == 0 && *zero_sh < 0) *zero_sh = 16 * kb + z;
-    }
-    __syncthreads();
+    const int z = lu16(Mkk, Li + 16 * kb * ld + 16 * kb, Ui + 16 * kb * ld + 16 * kb, ld);
+    if (z >= 0 && (threadIdx.x & 63) == 0 && *zero_sh < 0) *zero_sh = 16 * kb + z;
+  };
+  if (w == 0) diag_lu(0);
+  __syncthreads();
+  for (int kb = 0; kb < 4; ++kb) {
+    const double* Lkk = Li + 16 * kb * ld + 16 * kb;
+    const double* Ukk = Ui + 16 * kb * ld + 16 * kb;
     const int rest = 3 - kb;
     for (int p = w; p < 6; p += 4) {
       if (p < rest) {  // L(ib, kb)
@@ -213,8 +217,10 @@ __device__ __forceinline__ int factor64(double* M, double* Li, double* Ui, int l
       }
     }
     __syncthreads();
+    if (rest == 0) break;
     const int nm = rest * rest, ne = rest * (kb + 1);
-    for (int t = w; t < nm + 2 * ne; t += 4) {
+    // tile 0 of the trailing list is (kb + 1, kb + 1): wave 0's, then its LU
+    auto trailing = [&](int t) {
       if (t < nm) {
         const int ib = kb + 1 + t / rest, jb = kb + 1 + t % rest;
         double* C = M + 16 * ib * ld + 16 * jb;
@@ -231,6 +237,12 @@ __device__ __forceinline__ int factor64(double* M, double* Li, double* Ui, int l
         const d4 c0 = (i < kb) ? ld_tile(C, ld) : d4{0, 0, 0, 0};
         st_tile(mma16(Ui + 16 * i * ld + 16 * kb, ld, M + 16 * kb * ld + 16 * jb, ld, 16, -1.0, c0), C, ld);
       }
+    };
+    if (w == 0) {
+      trailing(0);
+      diag_lu(kb + 1);
+    } else {
+      for (int t = w; t < nm + 2 * ne; t += 3) trailing(t);
     }
     __syncthreads();
   }
