@@ -239,7 +239,7 @@ struct Geqp3Item {
 // Dataflow LU of one square tile (n <= 512) without pivoting: one workgroup
 // per 64 x 64 block, ordered by per-block flags in global memory.  Outputs
 // the packed unit-L \ U factors in place and the full inverses of both
-// factors (row stride ldi).  Flags (3 nbk^2 ints, zero at allocation) are
+// factors (row stride ldi).  Flags (2 nbk^2 ints, zero at allocation) are
 // compared against targets derived from epoch, so no reset is needed between
 // launches as long as epoch grows.  A wait that exceeds spin_limit polls sets
 // *err and abandons the factorization (every wave still exits).

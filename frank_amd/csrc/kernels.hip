@@ -742,13 +742,16 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     // 2a. LU of the diagonal block and the full inverses of both factors
     const int z = d64::factor64(M, Ls, Us, LD, &zero_sh);
     if (tid == 0 && z >= 0 && z < mi) atomicCAS(it.info, 0, 64 * bi + z + 1);
-    blk_store(M, Ablk, ld, mi, nj);
-    tile_signal(&F[bi * nbk + bj], ep);
-    stamp(2);
-    blk_store_tri(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, true);
+    // the inverses go first, each with its own flag (the L panels of this block
+    // column need only U^-1, the U panels of its row only L^-1): I(k, k) marks
+    // U^-1, F(k, k) marks L^-1; nobody waits for the packed L\U block
     blk_store_tri(Us, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, false);
     tile_signal(&I[bi * nbk + bj], ep);
+    stamp(2);
+    blk_store_tri(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi, true);
+    tile_signal(&F[bi * nbk + bj], ep);
     stamp(3);
+    blk_store(M, Ablk, ld, mi, nj);
     blk_store32(M, it.a32_out ? it.a32_out + off_a : nullptr, ld, mi, nj);
     blk_store_tri32(Ls, it.linv32 ? it.linv32 + off_i : nullptr, ldi, mi, mi, true);
     blk_store_tri32(Us, it.uinv32 ? it.uinv32 + off_i : nullptr, ldi, mi, mi, false);
@@ -756,7 +759,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
   }
   // 2b. panel blocks: L_ij = A_ij Uinv_jj (i > j), U_ij = Linv_ii A_ij (i < j)
   const int dgb = (bi > bj) ? bj : bi;
-  if (!tile_wait(&I[dgb * nbk + dgb], ep, lim, it.err, &bail_sh)) return;
+  if (!tile_wait(bi > bj ? &I[dgb * nbk + dgb] : &F[dgb * nbk + dgb], ep, lim, it.err, &bail_sh)) return;
   if (bi > bj) {
     blk_load(Us, Ui + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
     __syncthreads();
@@ -774,6 +777,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
   // 3. block (bi, bj) of the full inverse of L (below the diagonal) or U (above)
   if (bi > bj) {
     // M <- L_ij Linv_jj + sum_{k=j+1}^{i-1} L_ik Linv_kj, then Linv_ij = -Linv_ii M
+    if (!tile_wait(&F[bj * nbk + bj], ep, lim, it.err, &bail_sh)) return;
     blk_load(Us, Li + (int64_t)(64 * bj) * ldi + 64 * bj, ldi, nj, nj);
     __syncthreads();
     mma64(M, LD, M, LD, Us, LD, 64, 1.0, false, false);
@@ -787,7 +791,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
       mma64(M, LD, Ls, LD, Us, LD, 64, 1.0, true, false);
       __syncthreads();
     }
-    if (!tile_wait(&I[bi * nbk + bi], ep, lim, it.err, &bail_sh)) return;
+    if (!tile_wait(&F[bi * nbk + bi], ep, lim, it.err, &bail_sh)) return;
     blk_load(Ls, Li + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
     __syncthreads();
     mma64(M, LD, Ls, LD, M, LD, 64, -1.0, false, true);
@@ -798,6 +802,7 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     blk_zero32(it.uinv32 ? it.uinv32 + off_i : nullptr, ldi, mi, nj);
   } else {
     // M <- Uinv_ii U_ij + sum_{k=i+1}^{j-1} Uinv_ik U_kj, then Uinv_ij = -M Uinv_jj
+    if (!tile_wait(&I[bi * nbk + bi], ep, lim, it.err, &bail_sh)) return;
     blk_load(Ls, Ui + (int64_t)(64 * bi) * ldi + 64 * bi, ldi, mi, mi);
     __syncthreads();
     mma64(M, LD, Ls, LD, M, LD, 64, 1.0, false, true);
