@@ -632,10 +632,23 @@ __device__ __forceinline__ void tile_signal(int32_t* flag, int32_t value) {
 // are zero (identity on a diagonal block).
 constexpr int kTileLd = 68;
 
+// Block loads issue all 16 of a thread's loads before the first use: each load
+// is unconditional, at the element itself or (outside the block) at element
+// (0, 0), and zero is selected afterwards, so no branch separates the loads and
+// one memory round trip serves the whole block.
 __device__ __forceinline__ void blk_load(double* dst, const double* src, int64_t ld, int nr, int nc) {
-  for (int e = threadIdx.x; e < 4096; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    dst[r * kTileLd + c] = (r < nr && c < nc) ? src[(int64_t)r * ld + c] : 0.0;
+  double v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q, r = e >> 6, c = e & 63;
+    const bool in = r < nr && c < nc;
+    v[q] = src[in ? (int64_t)r * ld + c : 0];
+    v[q] = in ? v[q] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q, r = e >> 6, c = e & 63;
+    dst[r * kTileLd + c] = v[q];
   }
 }
 __device__ __forceinline__ void blk_store(const double* src, double* dst, int64_t ld, int nr, int nc) {
@@ -662,9 +675,18 @@ __device__ __forceinline__ void blk_zero(double* dst, int64_t ld, int nr, int nc
 }
 // FP32 input / output variants (the mixed-precision path keeps its tiles in FP32)
 __device__ __forceinline__ void blk_load32(double* dst, const float* src, int64_t ld, int nr, int nc) {
-  for (int e = threadIdx.x; e < 4096; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    dst[r * kTileLd + c] = (r < nr && c < nc) ? (double)src[(int64_t)r * ld + c] : 0.0;
+  float v[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q, r = e >> 6, c = e & 63;
+    const bool in = r < nr && c < nc;
+    v[q] = src[in ? (int64_t)r * ld + c : 0];
+    v[q] = in ? v[q] : 0.f;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q, r = e >> 6, c = e & 63;
+    dst[r * kTileLd + c] = (double)v[q];
   }
 }
 __device__ __forceinline__ void blk_store32(const double* src, float* dst, int64_t ld, int nr, int nc) {
