@@ -2412,30 +2412,56 @@ void launch_diag_apply(const T* Tinv, int64_t ld, bool trans, const T* b, T* w, 
 // b -= U (S (V x)) (low-rank) or b -= D x (dense) for one target block; the
 // source x is staged in xs.  With bl != nullptr the target is the LDS copy bl
 // (rows x NR) instead of the global b.
+// Rows [0, n) of a row-major n x nrhs block into LDS rows of NR entries
+// (entries past nrhs zero), n <= 512.  Every load of the thread is issued
+// before the first use: out-of-range entries load element 0 and are zeroed.
+template <typename T, int NR>
+__device__ __forceinline__ void stage_rows(T (*dst)[NR], const T* __restrict__ src, int n, int nrhs) {
+  constexpr int NQ = 512 * NR / 256;
+  T v[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = threadIdx.x + 256 * q, k = e / NR, c = e % NR;
+    const bool in = k < n && c < nrhs;
+    v[q] = src[in ? (int64_t)k * nrhs + c : 0];
+    v[q] = in ? v[q] : T(0);
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int e = threadIdx.x + 256 * q, k = e / NR, c = e % NR;
+    if (k < n) dst[k][c] = v[q];
+  }
+}
+
+// b -= U (S (V x)) (low-rank) or b -= D x (dense) for one target block; the
+// source x is staged in xs.  With bl != nullptr the target is the LDS copy bl
+// (rows x NR) instead of the global b.  Loads are issued unconditionally at a
+// clamped index (element 0 of every operand exists) and the out-of-range
+// values zeroed after, so the loads of one batch are in flight together.
 template <typename T, int NR>
 __device__ __forceinline__ void solve_update_body(const SolveUpdItem& it, int nrhs, T (*xs)[NR], T (*t1)[NR],
                                                   T (*t2)[NR], T (*bl)[NR]) {
   const int tid = threadIdx.x;
   const T* x = static_cast<const T*>(it.x);
   T* b = static_cast<T*>(it.b);
-  for (int e = tid; e < it.n * NR; e += 256) {
-    const int k = e / NR, c = e % NR;
-    xs[k][c] = (c < nrhs) ? x[(int64_t)k * nrhs + c] : T(0);
-  }
-  __syncthreads();
   if (it.D) {
+    stage_rows<T, NR>(xs, x, it.n, nrhs);
+    __syncthreads();
     const T* D = static_cast<const T*>(it.D);
     for (int row = tid; row < it.m; row += 256) {
       T acc[NR];
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = T(0);
-      // loads issued eight at a time (one memory round trip per chunk)
-      for (int k0 = 0; k0 < it.n; k0 += 8) {
-        T a[8];
+      for (int k0 = 0; k0 < it.n; k0 += 16) {
+        T a[16];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) a[j] = (k0 + j < it.n) ? D[row * it.d_rs + (k0 + j) * it.d_cs] : T(0);
+        for (int j = 0; j < 16; ++j) {
+          const bool in = k0 + j < it.n;
+          a[j] = D[row * it.d_rs + (in ? k0 + j : 0) * it.d_cs];
+          a[j] = in ? a[j] : T(0);
+        }
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
+        for (int j = 0; j < 16; ++j)
           if (k0 + j < it.n)
 #pragma unroll
             for (int c = 0; c < NR; ++c) acc[c] += a[j] * xs[k0 + j][c];
@@ -2453,26 +2479,47 @@ __device__ __forceinline__ void solve_update_body(const SolveUpdItem& it, int nr
   const T* U = static_cast<const T*>(it.U);
   const T* S = static_cast<const T*>(it.S);
   const T* V = static_cast<const T*>(it.V);
-  const int r = it.r;
-  {  // t1 = V x: one wave per row of V (lanes over k), wave reductions
-    const int wv = tid >> 6, lane = tid & 63;
-    for (int rho = wv; rho < r; rho += 4) {
-      T acc[NR];
+  const int r = it.r, n = it.n;
+  if (r <= 0) return;
+  const int wv = tid >> 6, lane = tid & 63;
+  // t1 = V x: one wave per row of V (lanes over k), two rows per round, wave
+  // reductions; the first round's loads go out together with the staging of x
+  T a[2][8];
+  auto load_v = [&](int rho0) {
 #pragma unroll
-      for (int c = 0; c < NR; ++c) acc[c] = T(0);
-      const T* vr = V + rho * it.v_rs;
+    for (int h = 0; h < 2; ++h) {
+      const int rho = rho0 + 4 * h;
+      const T* vr = V + (rho < r ? rho : 0) * it.v_rs;
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int k = lane + 64 * j;
-        if (k < it.n) {
-          const T a = vr[k * it.v_cs];
+        const bool in = rho < r && k < n;
+        a[h][j] = vr[(in ? k : 0) * it.v_cs];
+        a[h][j] = in ? a[h][j] : T(0);
+      }
+    }
+  };
+  load_v(wv);
+  stage_rows<T, NR>(xs, x, n, nrhs);
+  __syncthreads();
+  for (int rho0 = wv; rho0 < r; rho0 += 8) {
+    if (rho0 != wv) load_v(rho0);
 #pragma unroll
-          for (int c = 0; c < NR; ++c) acc[c] += a * xs[k][c];
-        }
+    for (int h = 0; h < 2; ++h) {
+      T acc[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = T(0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = lane + 64 * j;
+        if (k < n)
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] += a[h][j] * xs[k][c];
       }
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = wave_sum(acc[c]);
-      if (lane == 0)
+      const int rho = rho0 + 4 * h;
+      if (lane == 0 && rho < r)
 #pragma unroll
         for (int c = 0; c < NR; ++c) t1[rho][c] = acc[c];
     }
@@ -2481,37 +2528,59 @@ __device__ __forceinline__ void solve_update_body(const SolveUpdItem& it, int nr
   for (int e = tid; e < r * NR; e += 256) {  // t2 = S t1
     const int rho = e / NR, c = e % NR;
     T acc = T(0);
-    for (int q0 = 0; q0 < r; q0 += 8) {
-      T sv[8];
+    for (int q0 = 0; q0 < r; q0 += 16) {
+      T sv[16];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) sv[j] = (q0 + j < r) ? S[rho * it.s_rs + (q0 + j) * it.s_cs] : T(0);
+      for (int j = 0; j < 16; ++j) {
+        const bool in = q0 + j < r;
+        sv[j] = S[rho * it.s_rs + (in ? q0 + j : 0) * it.s_cs];
+        sv[j] = in ? sv[j] : T(0);
+      }
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int j = 0; j < 16; ++j)
         if (q0 + j < r) acc += sv[j] * t1[q0 + j][c];
     }
     t2[rho][c] = acc;
   }
   __syncthreads();
-  for (int row = tid; row < it.m; row += 256) {  // b -= U t2
-    T acc[NR];
+  // b -= U t2: two rows per thread (row, row + 256), sixteen columns of U per round
+  for (int row0 = tid; row0 < it.m; row0 += 512) {
+    T acc[2][NR];
 #pragma unroll
-    for (int c = 0; c < NR; ++c) acc[c] = T(0);
-    for (int q0 = 0; q0 < r; q0 += 8) {
-      T uv[8];
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) uv[j] = (q0 + j < r) ? U[row * it.u_rs + (q0 + j) * it.u_cs] : T(0);
+      for (int c = 0; c < NR; ++c) acc[h][c] = T(0);
+    for (int q0 = 0; q0 < r; q0 += 16) {
+      T uv[2][16];
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int h = 0; h < 2; ++h) {
+        const int row = row0 + 256 * h;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const bool in = row < it.m && q0 + j < r;
+          uv[h][j] = U[in ? row * it.u_rs + (q0 + j) * it.u_cs : 0];
+          uv[h][j] = in ? uv[h][j] : T(0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j)
         if (q0 + j < r)
 #pragma unroll
-          for (int c = 0; c < NR; ++c) acc[c] += uv[j] * t2[q0 + j][c];
+          for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int c = 0; c < NR; ++c) acc[h][c] += uv[h][j] * t2[q0 + j][c];
     }
 #pragma unroll
-    for (int c = 0; c < NR; ++c) {
-      if (bl)
-        bl[row][c] -= acc[c];
-      else if (c < nrhs)
-        b[(int64_t)row * nrhs + c] -= acc[c];
+    for (int h = 0; h < 2; ++h) {
+      const int row = row0 + 256 * h;
+      if (row < it.m)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+          if (bl)
+            bl[row][c] -= acc[h][c];
+          else if (c < nrhs)
+            b[(int64_t)row * nrhs + c] -= acc[h][c];
+        }
     }
   }
 }
@@ -2548,11 +2617,7 @@ __global__ __launch_bounds__(256) void solve_step_kernel(const SolveUpdItem* __r
     return;
   }
   const int g = blockIdx.x - n_upd, n = sd.n, r0 = kStepRows * g;
-  const T* b = static_cast<const T*>(sd.b);
-  for (int e = tid; e < n * NR; e += 256) {
-    const int k = e / NR, c = e % NR;
-    bs[k][c] = (c < nrhs) ? b[(int64_t)k * nrhs + c] : T(0);
-  }
+  stage_rows<T, NR>(bs, static_cast<const T*>(sd.b), n, nrhs);
   __syncthreads();
   if (sd.has_last) {
     const SolveUpdItem it = items[n_upd];
@@ -2569,14 +2634,18 @@ __global__ __launch_bounds__(256) void solve_step_kernel(const SolveUpdItem* __r
       T acc[NR];
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = T(0);
+      T a[8];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
         const int k = lane + 64 * jj;
-        if (k < n) {
-          const T a = tr[k];
+        a[jj] = tr[k < n ? k : 0];
+      }
 #pragma unroll
-          for (int c = 0; c < NR; ++c) acc[c] += a * bs[k][c];
-        }
+      for (int jj = 0; jj < 8; ++jj) {
+        const int k = lane + 64 * jj;
+        if (k < n)
+#pragma unroll
+          for (int c = 0; c < NR; ++c) acc[c] += a[jj] * bs[k][c];
       }
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = wave_sum(acc[c]);
@@ -2661,9 +2730,27 @@ __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const T* __r
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int r0 = 16 * blockIdx.x, col = 64 * blockIdx.y + lane;
   if (r0 >= R || 64 * (int)blockIdx.y >= N) return;
-  for (int r = 0; r < 16; ++r) {
-    const T* a = (r0 + r < R) ? row_of(r0 + r) : nullptr;
-    for (int k = tid; k < K; k += 256) As[r][k] = a ? a[k] : T(0);
+  {
+    // stage the 16-row slab with every load issued up front: rows past R and
+    // columns past K read row r0, column 0 (always present) and are zeroed after
+    const T* rp[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) rp[r] = row_of(r0 + r < R ? r0 + r : r0);
+    T v[16][2];
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = tid + 256 * h;
+        v[r][h] = rp[r][k < K ? k : 0];
+      }
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int k = tid + 256 * h;
+        if (k < K) As[r][k] = (r0 + r < R) ? v[r][h] : T(0);
+      }
   }
   __syncthreads();
   T acc[16];
@@ -2784,37 +2871,84 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
   T* __restrict__ core = static_cast<T*>(it.core);
   const int K = it.k;
   const int ti = tid >> 4, tj = tid & 15;
-  // the S cores into LDS (one round of loads), and the panel rows written back
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    Sls[r][c] = (r < rl && c < rl) ? Sl[(int64_t)r * it.lds + c] : T(0);
-    Sbs[r][c] = (r < rb && c < rb) ? Sb[(int64_t)r * it.lds + c] : T(0);
+  // Loads are issued unconditionally at a clamped index (element 0 of every
+  // operand exists: rl, rb, K >= 1) and out-of-range entries zeroed after, so
+  // a thread's loads of one round are in flight together.
+  // the S cores into LDS (one round of loads)
+  {
+    T sl[16], sb[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      const bool il = r < rl && c < rl, ib = r < rb && c < rb;
+      sl[q] = Sl[il ? (int64_t)r * it.lds + c : 0];
+      sb[q] = Sb[ib ? (int64_t)r * it.lds + c : 0];
+      sl[q] = il ? sl[q] : T(0);
+      sb[q] = ib ? sb[q] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      Sls[r][c] = sl[q];
+      Sbs[r][c] = sb[q];
+    }
   }
+  // the panel rows written back: rows of V_l (columns of U_b), eight per round
   if (it.vl_dst) {
     T* __restrict__ d = static_cast<T*>(it.vl_dst);
-    for (int r = 0; r < rl; ++r)
-      for (int j = tid; j < it.nv; j += 256)
-        d[(int64_t)r * it.vl_dst_ld + j] = V[(int64_t)r * it.v_rs + (int64_t)j * it.v_cs];
+    for (int j0 = 0; j0 < it.nv; j0 += 256)
+      for (int r0 = 0; r0 < rl; r0 += 8) {
+        const int j = j0 + tid;
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const bool in = r0 + q < rl && j < it.nv;
+          v[q] = V[in ? (int64_t)(r0 + q) * it.v_rs + (int64_t)j * it.v_cs : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (r0 + q < rl && j < it.nv) d[(int64_t)(r0 + q) * it.vl_dst_ld + j] = v[q];
+      }
   }
   if (it.ub_dst) {
     T* __restrict__ d = static_cast<T*>(it.ub_dst);
-    for (int c = 0; c < rb; ++c)
-      for (int j = tid; j < it.nu; j += 256)
-        d[(int64_t)c * it.ub_dst_ld + j] = U[(int64_t)j * it.u_rs + (int64_t)c * it.u_cs];
+    for (int j0 = 0; j0 < it.nu; j0 += 256)
+      for (int c0 = 0; c0 < rb; c0 += 8) {
+        const int j = j0 + tid;
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const bool in = c0 + q < rb && j < it.nu;
+          v[q] = U[in ? (int64_t)j * it.u_rs + (int64_t)(c0 + q) * it.u_cs : 0];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (c0 + q < rb && j < it.nu) d[(int64_t)(c0 + q) * it.ub_dst_ld + j] = v[q];
+      }
   }
   // M1 = V_l U_b (rl x rb), 16 x 16 output blocks, K in chunks of KC
+  constexpr int NQ = 16 * KC / 256;
   for (int bi = 0; bi < rl; bi += 16)
     for (int bj = 0; bj < rb; bj += 16) {
       T acc = T(0);
       for (int k0 = 0; k0 < K; k0 += KC) {
-        __syncthreads();
-        for (int e = tid; e < 16 * KC; e += 256) {
-          const int r = e / KC, kk = e % KC;
-          Va[r][kk] = (bi + r < rl && k0 + kk < K) ? V[(int64_t)(bi + r) * it.v_rs + (int64_t)(k0 + kk) * it.v_cs] : T(0);
+        T va[NQ], ub[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int e = tid + 256 * q;
+          const int r = e / KC, kk = e % KC, kb = e >> 4, c = e & 15;
+          const bool iv = bi + r < rl && k0 + kk < K, iu = bj + c < rb && k0 + kb < K;
+          va[q] = V[iv ? (int64_t)(bi + r) * it.v_rs + (int64_t)(k0 + kk) * it.v_cs : 0];
+          ub[q] = U[iu ? (int64_t)(k0 + kb) * it.u_rs + (int64_t)(bj + c) * it.u_cs : 0];
+          va[q] = iv ? va[q] : T(0);
+          ub[q] = iu ? ub[q] : T(0);
         }
-        for (int e = tid; e < KC * 16; e += 256) {
-          const int kk = e >> 4, c = e & 15;
-          Ub[kk][c] = (bj + c < rb && k0 + kk < K) ? U[(int64_t)(k0 + kk) * it.u_rs + (int64_t)(bj + c) * it.u_cs] : T(0);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+          const int e = tid + 256 * q;
+          Va[e / KC][e % KC] = va[q];
+          Ub[e >> 4][e & 15] = ub[q];
         }
         __syncthreads();
 #pragma unroll 8
@@ -2853,18 +2987,29 @@ __global__ __launch_bounds__(256) void lr_update_kernel(LrUpdItem it) {
   const T* V = static_cast<const T*>(it.v);
   const T* U = static_cast<const T*>(it.u);
   T* C = static_cast<T*>(it.c);
-  T cv[16];  // this thread's 16 entries of the C tile, loaded with the operands
+  // every load of the thread issued before the first use (unconditional, at a
+  // clamped index; entries outside the operands are zeroed afterwards)
+  T cv[16], kv[16], vv[16], uv[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int e = tid + 256 * q, a = e >> 6, b = e & 63;
-    cv[q] = (i0 + a < it.m && c0 + b < it.n) ? C[(int64_t)(i0 + a) * it.ldc + c0 + b] : T(0);
+    const bool ic = i0 + a < it.m && c0 + b < it.n, ik = a < r1 && b < r2, iv = a < r2 && c0 + b < it.n,
+               iu = i0 + a < it.m && b < r1;
+    cv[q] = C[ic ? (int64_t)(i0 + a) * it.ldc + c0 + b : 0];
+    kv[q] = core[ik ? (int64_t)a * it.ldk + b : 0];
+    vv[q] = V[iv ? (int64_t)a * it.v_rs + (int64_t)(c0 + b) * it.v_cs : 0];
+    uv[q] = U[iu ? (int64_t)(i0 + a) * it.u_rs + (int64_t)b * it.u_cs : 0];
+    cv[q] = ic ? cv[q] : T(0);
+    kv[q] = ik ? kv[q] : T(0);
+    vv[q] = iv ? vv[q] : T(0);
+    uv[q] = iu ? uv[q] : T(0);
   }
 #pragma unroll
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int a = e >> 6, b = e & 63;
-    Ks[a][b] = (a < r1 && b < r2) ? core[(int64_t)a * it.ldk + b] : T(0);
-    Vt[a][b] = (a < r2 && c0 + b < it.n) ? V[(int64_t)a * it.v_rs + (int64_t)(c0 + b) * it.v_cs] : T(0);
-    Ut[a][b] = (i0 + a < it.m && b < r1) ? U[(int64_t)(i0 + a) * it.u_rs + (int64_t)b * it.u_cs] : T(0);
+  for (int q = 0; q < 16; ++q) {
+    const int e = tid + 256 * q, a = e >> 6, b = e & 63;
+    Ks[a][b] = kv[q];
+    Vt[a][b] = vv[q];
+    Ut[a][b] = uv[q];
   }
   __syncthreads();
   // W = core V_b(:, c0:c0+64)  (r1 x 64)
