@@ -148,11 +148,13 @@ void Context::host_time(const char* cls, double ms) {
   st.timed += 1;
 }
 
+// Launch-timing events are read only after a full synchronize, so they carry
+// no system-scope fence (which would write back the caches at every record).
 void Context::event_begin() {
   Context& o = *owner_;
   if (o.ev_pool_.empty()) {
     hipEvent_t e;
-    FRK_HIP_CHECK(hipEventCreate(&e));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     o.ev_pool_.push_back(e);
   }
   ev_open_ = o.ev_pool_.back();
@@ -164,7 +166,7 @@ void Context::event_end() {
   Context& o = *owner_;
   if (o.ev_pool_.empty()) {
     hipEvent_t e;
-    FRK_HIP_CHECK(hipEventCreate(&e));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     o.ev_pool_.push_back(e);
   }
   hipEvent_t e1 = o.ev_pool_.back();
@@ -1285,8 +1287,11 @@ struct Lookahead {
   hipEvent_t prio = nullptr, rest = nullptr, sxa = nullptr, c0 = nullptr, c1 = nullptr;
   int32_t* snap = nullptr;
   Lookahead(Context& ctx, size_t nrank) : s(ctx.aux_stream(0)), s2(ctx.aux_stream(1)) {
-    for (hipEvent_t* e : {&prio, &rest, &sxa, &c0, &c1})
-      FRK_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    // device-side ordering only: no system-scope fence (a cache write-back per
+    // record); rest keeps it, since the host reads the snapshot after it
+    for (hipEvent_t* e : {&prio, &sxa, &c0, &c1})
+      FRK_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
+    FRK_HIP_CHECK(hipEventCreateWithFlags(&rest, hipEventDisableTiming));
     FRK_HIP_CHECK(hipHostMalloc((void**)&snap, sizeof(int32_t) * std::max<size_t>(nrank, 1)));
   }
   ~Lookahead() {
