@@ -3130,7 +3130,6 @@ __global__ __launch_bounds__(256) void residual_laplace_kernel(KernelSpec ks, co
     }
     __syncthreads();
     const int cnt = (int)std::min<int64_t>(256, j1 - jb);
-    const int64_t diag = i - jb;
     for (int jj = 0; jj < cnt; ++jj) {
       double r;
       if (DIM == 1) {
@@ -3144,12 +3143,17 @@ __global__ __launch_bounds__(256) void residual_laplace_kernel(KernelSpec ks, co
         }
         r = sqrt(rij);
       }
-      double a = d64::rcp2(r + 1e-3);
-      if (jj == diag) a = a + ks.shift;
+      const double a = d64::rcp2(r + 1e-3);
 #pragma unroll
       for (int c = 0; c < NR; ++c) acc[c] = fma(a, vs[jj][c], acc[c]);
     }
   }
+  // the shift on the diagonal, once, by the split whose column range holds i
+  // (kept out of the pair loop: no compare and select per entry)
+  if (i < n && i >= j0 && i < j1 && ks.shift != 0.0)
+#pragma unroll
+    for (int c = 0; c < NR; ++c)
+      if (c < nrhs) acc[c] = fma(ks.shift, X[i * nrhs + c], acc[c]);
   if (i < n)
 #pragma unroll
     for (int c = 0; c < NR; ++c)
