@@ -2409,28 +2409,37 @@ void launch_diag_apply(const T* Tinv, int64_t ld, bool trans, const T* b, T* w, 
   FRK_HIP_CHECK(hipGetLastError());
 }
 
-// b -= U (S (V x)) (low-rank) or b -= D x (dense) for one target block; the
-// source x is staged in xs.  With bl != nullptr the target is the LDS copy bl
-// (rows x NR) instead of the global b.
 // Rows [0, n) of a row-major n x nrhs block into LDS rows of NR entries
-// (entries past nrhs zero), n <= 512.  Every load of the thread is issued
-// before the first use: out-of-range entries load element 0 and are zeroed.
+// (entries past nrhs zero), n <= 512, in two halves: load_rows issues every
+// load of the thread (out-of-range entries load element 0 and are zeroed),
+// store_rows writes the registers to LDS, so other loads can go out between.
 template <typename T, int NR>
-__device__ __forceinline__ void stage_rows(T (*dst)[NR], const T* __restrict__ src, int n, int nrhs) {
-  constexpr int NQ = 512 * NR / 256;
-  T v[NQ];
+struct RowsReg {
+  T v[512 * NR / 256];
+};
+template <typename T, int NR>
+__device__ __forceinline__ void load_rows(RowsReg<T, NR>& g, const T* __restrict__ src, int n, int nrhs) {
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
+  for (int q = 0; q < 2 * NR; ++q) {
     const int e = threadIdx.x + 256 * q, k = e / NR, c = e % NR;
     const bool in = k < n && c < nrhs;
-    v[q] = src[in ? (int64_t)k * nrhs + c : 0];
-    v[q] = in ? v[q] : T(0);
+    g.v[q] = src[in ? (int64_t)k * nrhs + c : 0];
+    g.v[q] = in ? g.v[q] : T(0);
   }
+}
+template <typename T, int NR>
+__device__ __forceinline__ void store_rows(T (*dst)[NR], const RowsReg<T, NR>& g, int n) {
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) {
+  for (int q = 0; q < 2 * NR; ++q) {
     const int e = threadIdx.x + 256 * q, k = e / NR, c = e % NR;
-    if (k < n) dst[k][c] = v[q];
+    if (k < n) dst[k][c] = g.v[q];
   }
+}
+template <typename T, int NR>
+__device__ __forceinline__ void stage_rows(T (*dst)[NR], const T* __restrict__ src, int n, int nrhs) {
+  RowsReg<T, NR> g;
+  load_rows<T, NR>(g, src, n, nrhs);
+  store_rows<T, NR>(dst, g, n);
 }
 
 // b -= U (S (V x)) (low-rank) or b -= D x (dense) for one target block; the
@@ -2440,12 +2449,14 @@ __device__ __forceinline__ void stage_rows(T (*dst)[NR], const T* __restrict__ s
 // values zeroed after, so the loads of one batch are in flight together.
 template <typename T, int NR>
 __device__ __forceinline__ void solve_update_body(const SolveUpdItem& it, int nrhs, T (*xs)[NR], T (*t1)[NR],
-                                                  T (*t2)[NR], T (*bl)[NR]) {
+                                                  T (*t2)[NR], T (*bl)[NR], const RowsReg<T, NR>* bpend = nullptr,
+                                                  int bn = 0) {
   const int tid = threadIdx.x;
   const T* x = static_cast<const T*>(it.x);
   T* b = static_cast<T*>(it.b);
   if (it.D) {
     stage_rows<T, NR>(xs, x, it.n, nrhs);
+    if (bpend) store_rows<T, NR>(bl, *bpend, bn);
     __syncthreads();
     const T* D = static_cast<const T*>(it.D);
     for (int row = tid; row < it.m; row += 256) {
@@ -2480,7 +2491,10 @@ __device__ __forceinline__ void solve_update_body(const SolveUpdItem& it, int nr
   const T* S = static_cast<const T*>(it.S);
   const T* V = static_cast<const T*>(it.V);
   const int r = it.r, n = it.n;
-  if (r <= 0) return;
+  if (r <= 0) {
+    if (bpend) store_rows<T, NR>(bl, *bpend, bn);
+    return;
+  }
   const int wv = tid >> 6, lane = tid & 63;
   // t1 = V x: one wave per row of V (lanes over k), two rows per round, wave
   // reductions; the first round's loads go out together with the staging of x
@@ -2500,7 +2514,12 @@ __device__ __forceinline__ void solve_update_body(const SolveUpdItem& it, int nr
     }
   };
   load_v(wv);
-  stage_rows<T, NR>(xs, x, n, nrhs);
+  {
+    RowsReg<T, NR> xr;
+    load_rows<T, NR>(xr, x, n, nrhs);
+    if (bpend) store_rows<T, NR>(bl, *bpend, bn);
+    store_rows<T, NR>(xs, xr, n);
+  }
   __syncthreads();
   for (int rho0 = wv; rho0 < r; rho0 += 8) {
     if (rho0 != wv) load_v(rho0);
@@ -2617,29 +2636,45 @@ __global__ __launch_bounds__(256) void solve_step_kernel(const SolveUpdItem* __r
     return;
   }
   const int g = blockIdx.x - n_upd, n = sd.n, r0 = kStepRows * g;
-  stage_rows<T, NR>(bs, static_cast<const T*>(sd.b), n, nrhs);
-  __syncthreads();
-  if (sd.has_last) {
-    const SolveUpdItem it = items[n_upd];
-    solve_update_body<T, NR>(it, nrhs, xs, t1, t2, bs);
-  }
-  __syncthreads();
   const T* Ti = static_cast<const T*>(sd.tinv);
-  T* x = static_cast<T*>(sd.x);
   const int lane = tid & 63, wv = tid >> 6;
+  // the rows of Tinv this workgroup applies depend on nothing computed here:
+  // their loads go out first, beside those of b (and of the last update)
+  constexpr int RPW = kStepRows / 4;  // rows per wave
+  T ta[RPW][8];
   if (!sd.trans) {
-    // x[r] = sum_k Tinv[r][k] b[k]: a wave per row, lanes over k
-    for (int rr = wv; rr < kStepRows && r0 + rr < n; rr += 4) {
-      const T* tr = Ti + (int64_t)(r0 + rr) * sd.ld;
-      T acc[NR];
 #pragma unroll
-      for (int c = 0; c < NR; ++c) acc[c] = T(0);
-      T a[8];
+    for (int i = 0; i < RPW; ++i) {
+      const int row = r0 + wv + 4 * i;
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
         const int k = lane + 64 * jj;
-        a[jj] = tr[k < n ? k : 0];
+        ta[i][jj] = Ti[(row < n && k < n) ? (int64_t)row * sd.ld + k : 0];
       }
+    }
+  }
+  {
+    RowsReg<T, NR> br;
+    load_rows<T, NR>(br, static_cast<const T*>(sd.b), n, nrhs);
+    if (sd.has_last) {  // the update stores b to LDS once its own loads are out
+      const SolveUpdItem it = items[n_upd];
+      solve_update_body<T, NR>(it, nrhs, xs, t1, t2, bs, &br, n);
+    } else {
+      store_rows<T, NR>(bs, br, n);
+    }
+  }
+  __syncthreads();
+  T* x = static_cast<T*>(sd.x);
+  if (!sd.trans) {
+    // x[r] = sum_k Tinv[r][k] b[k]: a wave per row, lanes over k
+#pragma unroll
+    for (int i = 0; i < RPW; ++i) {
+      const int rr = wv + 4 * i;
+      if (r0 + rr >= n) break;
+      T acc[NR];
+#pragma unroll
+      for (int c = 0; c < NR; ++c) acc[c] = T(0);
+      const T* a = ta[i];
 #pragma unroll
       for (int jj = 0; jj < 8; ++jj) {
         const int k = lane + 64 * jj;
@@ -2723,8 +2758,9 @@ template void launch_solve_update<float>(const SolveUpdItem*, int, int, hipStrea
 // case and cache-line reuse serves the transposed one), and the partial sums
 // of the waves meet in LDS.
 template <typename T, class RowFn>
-__device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const T* __restrict__ B, int64_t b_rs,
-                                               int64_t b_cs, T* __restrict__ W, int N, int K) {
+__device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_t* __restrict__ d_count,
+                                               const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
+                                               T* __restrict__ W, int N, int K) {
   __shared__ T As[16][512 + 1];
   __shared__ T red[4][16][64];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -2744,6 +2780,10 @@ __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const T* __r
         const int k = tid + 256 * h;
         v[r][h] = rp[r][k < K ? k : 0];
       }
+    // the device count (rows actually present, <= R) is read only now, so its
+    // load overlaps the slab's; rows of the table past it are in bounds
+    if (d_count) R = min(R, max(0, *d_count));
+    if (r0 >= R) return;
 #pragma unroll
     for (int r = 0; r < 16; ++r)
 #pragma unroll
@@ -2783,19 +2823,16 @@ __global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict
                                                         const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
                                                         T* __restrict__ W, int N, int K,
                                                         const int32_t* __restrict__ d_count) {
-  if (d_count) R = min(R, max(0, *d_count));
-  rows_gemm_body<T>([&](int r) { return static_cast<const T*>(rows[r].a); }, R, B, b_rs, b_cs, W, N, K);
+  rows_gemm_body<T>([&](int r) { return static_cast<const T*>(rows[r].a); }, R, d_count, B, b_rs, b_cs, W, N, K);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void rows_pair_kernel(RowsPair p) {
   const RowsJob j = blockIdx.z ? p.job[1] : p.job[0];
-  int R = j.R;
-  if (j.d_count) R = min(R, max(0, *j.d_count));
   const T* base = static_cast<const T*>(j.base);
   const int64_t st = j.stride;
-  rows_gemm_body<T>([&](int r) { return base + (int64_t)r * st; }, R, static_cast<const T*>(j.b), j.b_rs, j.b_cs,
-                    static_cast<T*>(j.w), j.N, j.K);
+  rows_gemm_body<T>([&](int r) { return base + (int64_t)r * st; }, j.R, j.d_count, static_cast<const T*>(j.b),
+                    j.b_rs, j.b_cs, static_cast<T*>(j.w), j.N, j.K);
 }
 
 template <typename T>
@@ -2862,8 +2899,7 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
   __shared__ T Va[16][KC + 1];
   __shared__ T Ub[KC][17];
   const int tid = threadIdx.x;
-  const int rl = min(it.kcap, max(0, *it.rl)), rb = min(it.kcap, max(0, *it.rb));
-  if (rl == 0 || rb == 0) return;
+  const int kc = it.kcap;
   const T* __restrict__ V = static_cast<const T*>(it.vl);
   const T* __restrict__ U = static_cast<const T*>(it.ub);
   const T* __restrict__ Sl = static_cast<const T*>(it.sl);
@@ -2874,17 +2910,24 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
   // Loads are issued unconditionally at a clamped index (element 0 of every
   // operand exists: rl, rb, K >= 1) and out-of-range entries zeroed after, so
   // a thread's loads of one round are in flight together.
-  // the S cores into LDS (one round of loads)
+  // the S cores into LDS (one round of loads, at the capacity kc, so that it
+  // overlaps the load of the device ranks)
   {
     T sl[16], sb[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int e = tid + 256 * q, r = e >> 6, c = e & 63;
-      const bool il = r < rl && c < rl, ib = r < rb && c < rb;
-      sl[q] = Sl[il ? (int64_t)r * it.lds + c : 0];
-      sb[q] = Sb[ib ? (int64_t)r * it.lds + c : 0];
-      sl[q] = il ? sl[q] : T(0);
-      sb[q] = ib ? sb[q] : T(0);
+      const bool in = r < kc && c < kc;
+      sl[q] = Sl[in ? (int64_t)r * it.lds + c : 0];
+      sb[q] = Sb[in ? (int64_t)r * it.lds + c : 0];
+    }
+    const int rl0 = min(kc, max(0, *it.rl)), rb0 = min(kc, max(0, *it.rb));
+    if (rl0 == 0 || rb0 == 0) return;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q, r = e >> 6, c = e & 63;
+      sl[q] = (r < rl0 && c < rl0) ? sl[q] : T(0);
+      sb[q] = (r < rb0 && c < rb0) ? sb[q] : T(0);
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
@@ -2893,6 +2936,7 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
       Sbs[r][c] = sb[q];
     }
   }
+  const int rl = min(kc, max(0, *it.rl)), rb = min(kc, max(0, *it.rb));
   // the panel rows written back: rows of V_l (columns of U_b), eight per round
   if (it.vl_dst) {
     T* __restrict__ d = static_cast<T*>(it.vl_dst);
@@ -2980,29 +3024,36 @@ __global__ __launch_bounds__(256) void lr_update_kernel(LrUpdItem it) {
   __shared__ T Wt[64][65];
   __shared__ T Ut[64][65];
   const int tid = threadIdx.x;
-  const int r1 = min(it.kcap, max(0, *it.r1)), r2 = min(it.kcap, max(0, *it.r2));
-  if (r1 == 0 || r2 == 0) return;
+  const int kc = it.kcap;
   const int c0 = 64 * blockIdx.x, i0 = 64 * blockIdx.y;
   const T* core = static_cast<const T*>(it.core);
   const T* V = static_cast<const T*>(it.v);
   const T* U = static_cast<const T*>(it.u);
   T* C = static_cast<T*>(it.c);
   // every load of the thread issued before the first use (unconditional, at a
-  // clamped index; entries outside the operands are zeroed afterwards)
+  // clamped index), the low-rank operands at their capacity kc (always
+  // allocated), so that none waits for the device ranks; entries outside the
+  // operands or past the ranks are zeroed afterwards
   T cv[16], kv[16], vv[16], uv[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int e = tid + 256 * q, a = e >> 6, b = e & 63;
-    const bool ic = i0 + a < it.m && c0 + b < it.n, ik = a < r1 && b < r2, iv = a < r2 && c0 + b < it.n,
-               iu = i0 + a < it.m && b < r1;
+    const bool ic = i0 + a < it.m && c0 + b < it.n, ik = a < kc && b < kc, iv = a < kc && c0 + b < it.n,
+               iu = i0 + a < it.m && b < kc;
     cv[q] = C[ic ? (int64_t)(i0 + a) * it.ldc + c0 + b : 0];
     kv[q] = core[ik ? (int64_t)a * it.ldk + b : 0];
     vv[q] = V[iv ? (int64_t)a * it.v_rs + (int64_t)(c0 + b) * it.v_cs : 0];
     uv[q] = U[iu ? (int64_t)(i0 + a) * it.u_rs + (int64_t)b * it.u_cs : 0];
-    cv[q] = ic ? cv[q] : T(0);
-    kv[q] = ik ? kv[q] : T(0);
-    vv[q] = iv ? vv[q] : T(0);
-    uv[q] = iu ? uv[q] : T(0);
+  }
+  const int r1 = min(kc, max(0, *it.r1)), r2 = min(kc, max(0, *it.r2));
+  if (r1 == 0 || r2 == 0) return;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = tid + 256 * q, a = e >> 6, b = e & 63;
+    cv[q] = (i0 + a < it.m && c0 + b < it.n) ? cv[q] : T(0);
+    kv[q] = (a < r1 && b < r2) ? kv[q] : T(0);
+    vv[q] = (a < r2 && c0 + b < it.n) ? vv[q] : T(0);
+    uv[q] = (i0 + a < it.m && b < r1) ? uv[q] : T(0);
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
