@@ -1519,11 +1519,10 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     pd.run();
   };
   // priority block of step k on the lookahead stream, after the factorization of
-  // tile k and (after != nullptr) the part of the rest of step k-1 recorded by
-  // that event, then the factorization of tile k+1
-  auto enqueue_prio = [&](int k, bool device_ranks, hipEvent_t after) {
+  // tile k and (after_rest) the rest of step k-1, then the factorization of tile k+1
+  auto enqueue_prio = [&](int k, bool device_ranks, bool after_rest) {
     const auto tp = clk::now();
-    if (after) FRK_HIP_CHECK(hipStreamWaitEvent(la.s, after, 0));
+    if (after_rest) FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.c1, 0));
     side.reserve(side_ws);
     Plan pp(side);
     plan_prio<T>(pp, side, A, k, A.rank[A.idx(k + 1, k)], A.rank[A.idx(k, k + 1)], device_ranks);
@@ -1581,7 +1580,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
   }
   side.reserve(side_ws);
   enqueue_lu(0);
-  if (nb > 1) enqueue_prio(0, prio_rank_free(A, 0), nullptr);
+  if (nb > 1) enqueue_prio(0, prio_rank_free(A, 0), false);
   bool prio_pending = false;  // the priority block of step i waits for the host ranks
   for (int i = 0; i + 1 < nb; ++i) {
     auto th = clk::now();
@@ -1600,7 +1599,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     ctx.host_time("host_sync", ms_since(th));
     th = clk::now();
     if (prio_pending) {
-      enqueue_prio(i, false, la.c1);
+      enqueue_prio(i, false, true);
       prio_pending = false;
     }
     // the factors of tile i and the priority panels of step i come from the lookahead stream
@@ -1942,15 +1941,10 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     FRK_HIP_CHECK(hipEventRecord(la.rest, la.s2));
     cur = &ctx;
     curp = &pl;
-    // 4. the next priority block.  It reads and writes only tiles of rows i+1 and
-    //    i+2 (the panels (i+2, i+1), (i+1, i+2) and the Schur target (i+2, i+2)),
-    //    which the first chunk updates when its rows include i+2; the second
-    //    chunk neither reads nor writes them, so the block then waits for the
-    //    first chunk only and runs beside the second.
+    // 4. the next priority block
     if (i + 2 < nb) {
-      const bool rows_in_first = rem / nchunk >= 2;
       if (prio_rank_free(A, i + 1))
-        enqueue_prio(i + 1, true, rows_in_first ? la.c0 : la.c1);
+        enqueue_prio(i + 1, true, true);
       else
         prio_pending = true;
     }
