@@ -78,6 +78,7 @@ _SIGS = {
     "frk_blr_get_lowrank": ([vp, C.c_int, C.c_int, vp, vp, vp], C.c_int),
     "frk_blr_set_lowrank": ([vp, C.c_int, C.c_int, C.c_int, vp, vp, vp], C.c_int),
     "frk_blr_getrf": ([vp], C.c_int),
+    "frk_blr_getrf_steps": ([vp, C.c_int], C.c_int),
     "frk_blr_solve": ([vp, vp, C.c_int, C.c_int], C.c_int),
     "frk_blr_trsm": ([vp, vp, C.c_int, C.c_int, C.c_int], C.c_int),
     "frk_blr_matvec": ([vp, vp, vp, C.c_int], C.c_int),

@@ -284,9 +284,16 @@ class BlrMatrix:
     def set_lra(self, lra: str):
         L.check(L.lib().frk_blr_set_lra(self.h, _LRA[lra]))
 
-    def getrf(self) -> "BlrMatrix":
-        """In-place tile LU: lower tiles -> L, upper tiles -> U (getrf.cpp:45-65)."""
-        L.check(L.lib().frk_blr_getrf(self.h))
+    def getrf(self, max_steps: Optional[int] = None) -> "BlrMatrix":
+        """In-place tile LU: lower tiles -> L, upper tiles -> U (getrf.cpp:45-65).
+
+        ``max_steps`` stops after that many elimination steps (a partial
+        factorization for parity checks; the diagonal tile of the next step is
+        factorized as well)."""
+        if max_steps is None:
+            L.check(L.lib().frk_blr_getrf(self.h))
+        else:
+            L.check(L.lib().frk_blr_getrf_steps(self.h, int(max_steps)))
         return self
 
     def qr(self) -> QrFactors:

@@ -108,11 +108,11 @@ template <class T>
 void to_dev(T* dst, const double* src, size_t n, hipStream_t s) {
   if constexpr (sizeof(T) == 8) {
     FRK_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, s));
-    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    host_wait_stream(s, __func__);
   } else {
     std::vector<float> tmp(src, src + n);
     FRK_HIP_CHECK(hipMemcpyAsync(dst, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, s));
-    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    host_wait_stream(s, __func__);
   }
 }
 
@@ -120,11 +120,11 @@ template <class T>
 void from_dev(double* dst, const T* src, size_t n, hipStream_t s) {
   if constexpr (sizeof(T) == 8) {
     FRK_HIP_CHECK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
-    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    host_wait_stream(s, __func__);
   } else {
     std::vector<float> tmp(n);
     FRK_HIP_CHECK(hipMemcpyAsync(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, s));
-    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    host_wait_stream(s, __func__);
     for (size_t k = 0; k < n; ++k) dst[k] = tmp[k];
   }
 }
@@ -213,7 +213,7 @@ int frk_ctx_create(frk_ctx* out) {
 int frk_ctx_destroy(frk_ctx ctx) {
   return guard([&] {
     if (!ctx) return FRK_OK;
-    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    host_wait_stream(ctx->stream, __func__);
     ctx_release(ctx);  // freed once the last matrix built on it is destroyed
     return FRK_OK;
   });
@@ -222,7 +222,7 @@ int frk_ctx_destroy(frk_ctx ctx) {
 int frk_ctx_synchronize(frk_ctx ctx) {
   return guard([&] {
     if (!ctx) throw std::invalid_argument("ctx is null");
-    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    host_wait_stream(ctx->stream, __func__);
     return FRK_OK;
   });
 }
@@ -239,12 +239,12 @@ int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset) {
   return guard([&] {
     if (!ctx || (!out && n > 0) || n < 0) throw std::invalid_argument("null or negative argument");
     int32_t c[kNumCounters];
-    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    host_wait_stream(ctx->stream, __func__);
     FRK_HIP_CHECK(hipMemcpy(c, ctx->ctx->counters(), sizeof(c), hipMemcpyDeviceToHost));
     for (int i = 0; i < n && i < kNumCounters; ++i) out[i] = c[i];
     if (reset) {
       FRK_HIP_CHECK(hipMemsetAsync(ctx->ctx->counters(), 0, sizeof(c), ctx->stream));
-      FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+      host_wait_stream(ctx->stream, __func__);
     }
     return FRK_OK;
   });
@@ -340,7 +340,7 @@ int frk_blr_create_kernel_rect(frk_ctx ctx, int kernel_kind, int dim, int64_t n_
     h->kd.host_pts = h->host_pts.data();
     h->ks = upload_kernel(h->kd, h->pts, ctx->stream);
     h->has_kernel = n_rows == n_cols && n_rows == n_pts;
-    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    host_wait_stream(ctx->stream, __func__);
     ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
@@ -375,7 +375,7 @@ int frk_blr_create(frk_ctx ctx, int nb_r, int nb_c, const int64_t* roff, const i
       h->a32 = std::make_unique<BlrMatrix<float>>(ro, co, kd, kcap, eps, rank, *ctx->ctx);
     else
       throw std::invalid_argument("unknown precision");
-    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    host_wait_stream(ctx->stream, __func__);
     ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
@@ -404,7 +404,7 @@ int frk_blr_clone(frk_blr src, frk_blr* out) {
       h->has_kernel = true;
       h->anorm = src->anorm;
     }
-    FRK_HIP_CHECK(hipStreamSynchronize(src->owner->stream));
+    host_wait_stream(src->owner->stream, __func__);
     ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
@@ -431,7 +431,7 @@ int frk_blr_convert(frk_blr src, int precision, frk_blr* out) {
       h->has_kernel = true;
       h->anorm = src->anorm;
     }
-    FRK_HIP_CHECK(hipStreamSynchronize(src->owner->stream));
+    host_wait_stream(src->owner->stream, __func__);
     ++h->owner->refs;  // the matrix holds its context
     *out = h.release();
     return FRK_OK;
@@ -469,7 +469,7 @@ int frk_blr_refine(frk_blr f, const double* b_dev, double* x_dev, int nrhs, doub
 int frk_blr_destroy(frk_blr a) {
   return guard([&] {
     if (!a) return FRK_OK;
-    FRK_HIP_CHECK(hipStreamSynchronize(a->owner->stream));
+    host_wait_stream(a->owner->stream, __func__);
     blr_release(a);
     return FRK_OK;
   });
@@ -547,13 +547,15 @@ int frk_blr_set_lowrank(frk_blr a, int i, int j, int rank, const double* U, cons
   });
 }
 
-int frk_blr_getrf(frk_blr a) {
+int frk_blr_getrf(frk_blr a) { return frk_blr_getrf_steps(a, -1); }
+
+int frk_blr_getrf_steps(frk_blr a, int steps) {
   return guard([&] {
     if (!a) throw std::invalid_argument("null matrix");
     if (a->a64)
-      blr_getrf<double>(*a->a64, *a->owner->ctx);
+      blr_getrf<double>(*a->a64, *a->owner->ctx, steps);
     else
-      blr_getrf<float>(*a->a32, *a->owner->ctx);
+      blr_getrf<float>(*a->a32, *a->owner->ctx, steps);
     return FRK_OK;
   });
 }
@@ -575,7 +577,7 @@ int frk_blr_qr(frk_blr a, frk_qrf* out) {
 int frk_qrf_destroy(frk_qrf f) {
   return guard([&] {
     if (!f) return FRK_OK;
-    FRK_HIP_CHECK(hipStreamSynchronize(f->a->owner->stream));
+    host_wait_stream(f->a->owner->stream, __func__);
     frk_blr_s* a = f->a;
     delete f;
     blr_release(a);
@@ -590,7 +592,7 @@ int frk_qrf_get_t(frk_qrf f, int i, int k, double* t_out, double* tau_out) {
     if (i < 0 || k < 0 || i >= F.nb_r || k >= F.nb_c || i < k || k >= (int)F.bsize.size())
       throw std::invalid_argument("T(i, k) exists for i >= k only");
     const size_t b = F.bsize[k];
-    FRK_HIP_CHECK(hipStreamSynchronize(f->a->owner->stream));
+    host_wait_stream(f->a->owner->stream, __func__);
     if (t_out) FRK_HIP_CHECK(hipMemcpy(t_out, F.tmat(i, k), sizeof(double) * b * b, hipMemcpyDeviceToHost));
     if (tau_out) FRK_HIP_CHECK(hipMemcpy(tau_out, F.tau_of(i, k), sizeof(double) * b, hipMemcpyDeviceToHost));
     return FRK_OK;
@@ -612,7 +614,7 @@ int frk_blr_trsm(frk_blr f, double* b_dev, int nrhs, int upper, int trans) {
     if (!f || !b_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
     if (!f->a64) throw std::runtime_error("trsm(FP32 Hierarchical, FP64 Dense) undefined!");
     blr_solve<double>(*f->a64, b_dev, nrhs, trans != 0, *f->owner->ctx, upper ? 2 : 1);
-    FRK_HIP_CHECK(hipStreamSynchronize(f->owner->stream));
+    host_wait_stream(f->owner->stream, __func__);
     return FRK_OK;
   });
 }
@@ -622,7 +624,7 @@ int frk_blr_solve(frk_blr f, double* b_dev, int nrhs, int trans) {
     if (!f || !b_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
     if (!f->a64) throw std::runtime_error("frk_blr_solve(FP32 factors, FP64 vector) undefined!");
     blr_solve<double>(*f->a64, b_dev, nrhs, trans != 0, *f->owner->ctx);
-    FRK_HIP_CHECK(hipStreamSynchronize(f->owner->stream));
+    host_wait_stream(f->owner->stream, __func__);
     return FRK_OK;
   });
 }
@@ -632,7 +634,7 @@ int frk_blr_matvec(frk_blr a, const double* x_dev, double* y_dev, int nrhs) {
     if (!a || !x_dev || !y_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
     if (!a->a64) throw std::runtime_error("frk_blr_matvec(FP32 tiles, FP64 vector) undefined!");
     blr_matvec<double>(*a->a64, x_dev, y_dev, nrhs, *a->owner->ctx);
-    FRK_HIP_CHECK(hipStreamSynchronize(a->owner->stream));
+    host_wait_stream(a->owner->stream, __func__);
     return FRK_OK;
   });
 }
@@ -643,7 +645,7 @@ int frk_blr_kernel_residual(frk_blr a, const double* x_dev, const double* b_dev,
     if (!a->has_kernel) throw std::invalid_argument("matrix was not built from a kernel");
     const int64_t n = a->a64 ? a->a64->roff.back() : a->a32->roff.back();
     launch_kernel_residual(a->ks, x_dev, b_dev, r_dev, n, nrhs, a->owner->stream);
-    FRK_HIP_CHECK(hipStreamSynchronize(a->owner->stream));
+    host_wait_stream(a->owner->stream, __func__);
     return FRK_OK;
   });
 }
@@ -658,7 +660,7 @@ int frk_kernel_apply(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const dou
     DevBuf dpts;
     const KernelSpec ks = upload_kernel(kd, dpts, ctx->stream);
     launch_kernel_residual(ks, x_dev, nullptr, y_dev, n, nrhs, ctx->stream);
-    FRK_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    host_wait_stream(ctx->stream, __func__);
     return FRK_OK;
   });
 }

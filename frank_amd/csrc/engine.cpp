@@ -8,8 +8,55 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <thread>
 
 namespace frk {
+
+// ============================================================================
+// Host waits with a deadline
+// ============================================================================
+double wait_timeout_s() {
+  static const double t = [] {
+    const char* e = std::getenv("FRK_WAIT_TIMEOUT_S");
+    const double v = e ? std::atof(e) : 0.0;
+    return v > 0.0 ? v : 60.0;
+  }();
+  return t;
+}
+
+void host_wait_event(hipEvent_t e, const char* what) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  const double limit = wait_timeout_s();
+  for (int64_t spin = 0;; ++spin) {
+    const hipError_t q = hipEventQuery(e);
+    if (q == hipSuccess) return;
+    if (q != hipErrorNotReady) FRK_HIP_CHECK(q);
+    // poll hard for the first few thousand rounds (the per-step waits are short
+    // and on the host's critical path), then check the clock and yield
+    if (spin < 4096) continue;
+    const double s = std::chrono::duration<double>(clk::now() - t0).count();
+    if (s > limit)
+      throw std::runtime_error(std::string("device work did not complete within ") + std::to_string(limit) +
+                               " s (" + what + "; FRK_WAIT_TIMEOUT_S sets the limit)");
+    std::this_thread::yield();
+  }
+}
+
+void host_wait_stream(hipStream_t s, const char* what) {
+  hipEvent_t e;
+  FRK_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  try {
+    FRK_HIP_CHECK(hipEventRecord(e, s));
+    host_wait_event(e, what);
+  } catch (...) {
+    (void)hipEventDestroy(e);
+    throw;
+  }
+  FRK_HIP_CHECK(hipEventDestroy(e));
+  // the stream's own error state (a failed launch reports here)
+  FRK_HIP_CHECK(hipStreamSynchronize(s));
+}
 
 // ============================================================================
 // Buffers, staging, context
@@ -47,7 +94,7 @@ const void* Staging::push_bytes(const void* src, size_t bytes, hipStream_t s) {
   if (off_ + aligned > cap_) {
     if (off_ != 0) {
       // the ring is in use by queued launches: drain before growing
-      FRK_HIP_CHECK(hipStreamSynchronize(s));
+      host_wait_stream(s, __func__);
       off_ = 0;
     }
     if (aligned > cap_) {
@@ -87,7 +134,7 @@ void Context::check_tile_lu(int32_t since) {
   if (tile_flags_.bytes() == 0 || tile_epoch_ <= since) return;
   int32_t e = 0;
   FRK_HIP_CHECK(hipMemcpyAsync(&e, tile_flags_.as<int32_t>() + kTileFlags - 1, sizeof(e), hipMemcpyDeviceToHost, s_));
-  FRK_HIP_CHECK(hipStreamSynchronize(s_));
+  host_wait_stream(s_, __func__);
   if (e > since) throw std::runtime_error("tile LU: a block wait timed out (workgroups not co-resident?)");
 }
 
@@ -113,7 +160,7 @@ Context::~Context() {
 
 void Context::reserve(size_t bytes) {
   if (bytes > ws_.bytes()) {
-    FRK_HIP_CHECK(hipStreamSynchronize(s_));
+    host_wait_stream(s_, __func__);
     ws_.ensure(bytes + (bytes >> 2));
   }
   ws_off_ = 0;
@@ -129,7 +176,7 @@ void* Context::take_bytes(size_t bytes) {
 }
 
 void Context::sync() {
-  FRK_HIP_CHECK(hipStreamSynchronize(s_));
+  host_wait_stream(s_, "context stream");
   stage_.reset();
   ws_off_ = 0;
   resolve_events();
@@ -205,7 +252,7 @@ template <class X>
 const X* Context::identity(int n) {
   if (n > ident_n_) {
     const int nn = std::max(n, 256);
-    FRK_HIP_CHECK(hipStreamSynchronize(s_));
+    host_wait_stream(s_, __func__);
     ident64_.ensure(sizeof(double) * nn * nn);
     ident32_.ensure(sizeof(float) * nn * nn);
     launch_fill_identity<double>(ident64_.as<double>(), nn, nn, nn, s_);
@@ -229,7 +276,7 @@ const X* Context::omega(int n, int p) {
     std::vector<double> h((size_t)n * p);
     for (auto& x : h) x = dist(gen);
     std::vector<float> hf(h.begin(), h.end());
-    FRK_HIP_CHECK(hipStreamSynchronize(s_));
+    host_wait_stream(s_, __func__);
     omega64_.ensure(sizeof(double) * h.size());
     omega32_.ensure(sizeof(float) * h.size());
     FRK_HIP_CHECK(hipMemcpy(omega64_.as<double>(), h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
@@ -318,7 +365,7 @@ template <typename T>
 void BlrMatrix<T>::pull_ranks(hipStream_t s) {
   FRK_HIP_CHECK(hipMemcpyAsync(rank.data(), d_ranks_.as<int32_t>(), sizeof(int32_t) * rank.size(),
                                hipMemcpyDeviceToHost, s));
-  FRK_HIP_CHECK(hipStreamSynchronize(s));
+  host_wait_stream(s, __func__);
   for (size_t t = 0; t < rank.size(); ++t)
     if (kind[t] == kLowRank && rank[t] < 0)
       throw std::runtime_error("low-rank capacity exceeded: rank " + std::to_string(-rank[t]) +
@@ -354,7 +401,7 @@ template <typename T>
 void BlrMatrix<T>::push_ranks(hipStream_t s) {
   FRK_HIP_CHECK(hipMemcpyAsync(d_ranks_.as<int32_t>(), rank.data(), sizeof(int32_t) * rank.size(),
                                hipMemcpyHostToDevice, s));
-  FRK_HIP_CHECK(hipStreamSynchronize(s));
+  host_wait_stream(s, __func__);
 }
 
 // ============================================================================
@@ -535,7 +582,7 @@ void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_
   }
   pl.run();
   ctx.check_tile_lu(e0);
-  FRK_HIP_CHECK(hipStreamSynchronize(ctx.stream()));
+  host_wait_stream(ctx.stream(), __func__);
 }
 
 // Generic trsm without precomputed inverses: invert the diagonal blocks first.
@@ -562,7 +609,7 @@ void trsm_generic(const T* A, int64_t a_rs, int64_t a_cs, bool lower, bool unit,
   pl.launch([=](const char* d) { launch_tri_inv<T>(reinterpret_cast<const TriInvItem*>(d + off), nbk, s); });
   plan_trsms<T>(pl, {trsm_item(A, a_rs, a_cs, inv.as<T>(), 64, 1, B, b_rs, b_cs, n, nrhs, lower)});
   pl.run();
-  FRK_HIP_CHECK(hipStreamSynchronize(s));
+  host_wait_stream(s, __func__);
 }
 
 // ============================================================================
@@ -814,7 +861,7 @@ void compress_batched(const std::vector<CompressDesc<T>>& d, Context& ctx) {
     std::vector<int32_t> rk(nd);
     for (int t = 0; t < nd; ++t)
       FRK_HIP_CHECK(hipMemcpyAsync(&rk[t], de[t].d_rank, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    host_wait_stream(s, __func__);
     int rmax = 1;
     for (int t = 0; t < nd; ++t) {
       if (rk[t] < 0)
@@ -1292,7 +1339,9 @@ struct Lookahead {
     for (hipEvent_t* e : {&prio, &sxa, &c0, &c1})
       FRK_HIP_CHECK(hipEventCreateWithFlags(e, hipEventDisableTiming | hipEventDisableSystemFence));
     FRK_HIP_CHECK(hipEventCreateWithFlags(&rest, hipEventDisableTiming));
-    FRK_HIP_CHECK(hipHostMalloc((void**)&snap, sizeof(int32_t) * std::max<size_t>(nrank, 1)));
+    // nrank rank slots, then the lookahead context's tile-LU timeout marker
+    FRK_HIP_CHECK(hipHostMalloc((void**)&snap, sizeof(int32_t) * (nrank + 1)));
+    snap[nrank] = 0;
   }
   ~Lookahead() {
     (void)hipStreamSynchronize(s);
@@ -1475,9 +1524,13 @@ static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int 
 constexpr int kRestChunks = 2;
 
 template <typename T>
-void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
+void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
   const hipStream_t s = ctx.stream();
   const int nb = A.nb_r();
+  // a partial factorization stops after max_steps elimination steps (the
+  // diagonal tile of step max_steps is factorized as well: its LU is queued
+  // on the chain ahead of the step)
+  const int nsteps = (max_steps < 0 || max_steps >= nb) ? nb : max_steps;
   if (A.nb_c() != nb) throw std::invalid_argument("getrf needs a square block grid");
   int32_t* info = ctx.info();
   const int npan_max = 8;
@@ -1579,13 +1632,22 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     rest2.reserve(need0);
   }
   side.reserve(side_ws);
+  // allocate (and zero, on the lookahead stream) the lookahead context's tile-LU
+  // flags before the first factorization: every step copies their timeout marker
+  (void)side.tile_flags();
   enqueue_lu(0);
-  if (nb > 1) enqueue_prio(0, prio_rank_free(A, 0), false);
+  if (nb > 1 && nsteps > 0) enqueue_prio(0, prio_rank_free(A, 0), false);
   bool prio_pending = false;  // the priority block of step i waits for the host ranks
-  for (int i = 0; i + 1 < nb; ++i) {
+  for (int i = 0; i + 1 < nb && i < nsteps; ++i) {
     auto th = clk::now();
     if (i > 0) {
-      FRK_HIP_CHECK(hipEventSynchronize(la.rest));
+      host_wait_event(la.rest, "getrf: rank snapshot of the previous step");
+      // the tile LU's timeout marker, copied behind the snapshot: a factorization
+      // of a diagonal tile whose block wait gave up ends the call here, before
+      // any later step runs on its output
+      if (la.snap[A.rank.size()] > 0)
+        throw std::runtime_error("tile LU: a block wait timed out in step " + std::to_string(i - 1) +
+                                 " (workgroups not co-resident?)");
       // every launch of the previous step's rest has run: its descriptors are consumed
       ctx.staging().reset();
       rest2.staging().reset();
@@ -1881,7 +1943,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
         std::vector<int32_t> cr(comp.size());
         FRK_HIP_CHECK(hipMemcpyAsync(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(),
                                      hipMemcpyDeviceToHost, cur->stream()));
-        FRK_HIP_CHECK(hipStreamSynchronize(cur->stream()));
+        host_wait_stream(cur->stream(), __func__);
         for (size_t c2 = 0; c2 < comp.size(); ++c2) adds[comp_add[c2]].kx1 = adds[comp_add[c2]].kx2 = cr[c2];
         if (A.lra == kLraNaive) {
           naive_adds();
@@ -1938,11 +2000,15 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx) {
     FRK_HIP_CHECK(hipStreamWaitEvent(s, la.c1, 0));
     FRK_HIP_CHECK(
         hipMemcpyAsync(la.snap, A.d_rank(0, 0), sizeof(int32_t) * A.rank.size(), hipMemcpyDeviceToHost, la.s2));
+    // the rest of step i ran after the priority block of step i, hence after the
+    // factorization of tile i: its timeout marker is final by now
+    FRK_HIP_CHECK(hipMemcpyAsync(la.snap + A.rank.size(), side.tile_flags() + Context::kTileFlags - 1,
+                                 sizeof(int32_t), hipMemcpyDeviceToHost, la.s2));
     FRK_HIP_CHECK(hipEventRecord(la.rest, la.s2));
     cur = &ctx;
     curp = &pl;
     // 4. the next priority block
-    if (i + 2 < nb) {
+    if (i + 2 < nb && i + 1 < nsteps) {
       if (prio_rank_free(A, i + 1))
         enqueue_prio(i + 1, true, true);
       else
@@ -2221,7 +2287,7 @@ std::unique_ptr<BlrMatrix<TD>> convert_blr(const BlrMatrix<TS>& src, Context& ct
   if (src.dinv_count() && dst->dinv_count() == src.dinv_count())
     launch_convert<TD, TS>(dst->dinv_data(), src.dinv_data(), (int64_t)src.dinv_count(), s);
   dst->set_ranks_from_device(src.d_rank(0, 0), s);
-  FRK_HIP_CHECK(hipStreamSynchronize(s));
+  host_wait_stream(s, __func__);
   dst->rank = src.rank;
   return dst;
 }
@@ -2236,7 +2302,7 @@ static std::vector<double> colmax(const double* a, int64_t n, int nrhs, Context&
   launch_colmax_abs(a, n, nrhs, part, ctx.stream());
   std::vector<double> h((size_t)kColmaxBlocks * nrhs);
   FRK_HIP_CHECK(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx.stream()));
-  FRK_HIP_CHECK(hipStreamSynchronize(ctx.stream()));
+  host_wait_stream(ctx.stream(), __func__);
   std::vector<double> out(nrhs, 0.0);
   for (int b = 0; b < kColmaxBlocks; ++b)
     for (int c = 0; c < nrhs; ++c) out[c] = std::max(out[c], h[(size_t)b * nrhs + c]);
@@ -2304,7 +2370,7 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
                                 Context&);                                                                  \
   template std::unique_ptr<BlrMatrix<T>> build_blr_kernel<T>(const KernelDesc&, int, int, double, int,    \
                                                              double, int, int, Context&);                   \
-  template void blr_getrf<T>(BlrMatrix<T>&, Context&);                                                      \
+  template void blr_getrf<T>(BlrMatrix<T>&, Context&, int);                                                      \
   template void blr_solve<T>(BlrMatrix<T>&, T*, int, bool, Context&, int);                                  \
   template void blr_matvec<T>(BlrMatrix<T>&, const T*, T*, int, Context&);                                  \
   template RefineResult blr_refine<T>(BlrMatrix<T>&, const KernelSpec&, double, const double*, double*, int,   \

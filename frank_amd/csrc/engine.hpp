@@ -21,6 +21,14 @@
 
 namespace frk {
 
+// Host waits with a deadline: poll the event (or an event recorded on the
+// stream) until it completes, and throw if it has not after wait_timeout_s()
+// seconds (FRK_WAIT_TIMEOUT_S, default 60), so that a device-side stall
+// surfaces as an error of the call instead of a hang.
+double wait_timeout_s();
+void host_wait_event(hipEvent_t e, const char* what);
+void host_wait_stream(hipStream_t s, const char* what);
+
 class DevBuf {
  public:
   DevBuf() = default;
@@ -309,7 +317,7 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb_r, i
 // getrf(Hierarchical&): right-looking tile LU without pivoting, in place
 // (lower tiles become L, upper tiles U, diagonal tiles packed L\U).
 template <typename T>
-void blr_getrf(BlrMatrix<T>& A, Context& ctx);
+void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps = -1);
 
 // Forward/backward block substitution on device vectors b (N x nrhs row-major).
 // trans = false: L then U (solves A x = b); trans = true: U^T then L^T.

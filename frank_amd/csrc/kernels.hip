@@ -2921,8 +2921,11 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
       sl[q] = Sl[in ? (int64_t)r * it.lds + c : 0];
       sb[q] = Sb[in ? (int64_t)r * it.lds + c : 0];
     }
+    // lr_core is the only writer of the solved panel rows (vl_dst / ub_dst), so
+    // it returns here only when both panels are empty; with one rank zero the
+    // other panel is still written back below and only the core is skipped
     const int rl0 = min(kc, max(0, *it.rl)), rb0 = min(kc, max(0, *it.rb));
-    if (rl0 == 0 || rb0 == 0) return;
+    if (rl0 == 0 && rb0 == 0) return;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int e = tid + 256 * q, r = e >> 6, c = e & 63;
@@ -2970,6 +2973,8 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
           if (c0 + q < rb && j < it.nu) d[(int64_t)(c0 + q) * it.ub_dst_ld + j] = v[q];
       }
   }
+  // an empty product leaves the core unused (lr_update_kernel skips the target)
+  if (rl == 0 || rb == 0) return;
   // M1 = V_l U_b (rl x rb), 16 x 16 output blocks, K in chunks of KC
   constexpr int NQ = 16 * KC / 256;
   for (int bi = 0; bi < rl; bi += 16)

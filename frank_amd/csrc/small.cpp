@@ -277,7 +277,7 @@ void geqp3_small_batched(const std::vector<Geqp3Job<T>>& jobs, Context& ctx) {
   }
   std::vector<int32_t> rk(nj);
   FRK_HIP_CHECK(hipMemcpyAsync(rk.data(), dr.as<int32_t>(), sizeof(int32_t) * nj, hipMemcpyDeviceToHost, s));
-  FRK_HIP_CHECK(hipStreamSynchronize(s));
+  host_wait_stream(s, __func__);
   int rmax = 1;
   for (int t = 0; t < nj; ++t) {
     if (rk[t] < 0) throw std::runtime_error("truncated_geqp3: rank above the capacity");

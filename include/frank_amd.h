@@ -138,6 +138,11 @@ int frk_blr_convert(frk_blr src, int precision, frk_blr* out);
  * hold U, diagonal tiles hold the packed unit-L \ U factors.  Returns > 0
  * (1-based global row) when a zero pivot is met. */
 int frk_blr_getrf(frk_blr a);
+/* The first `steps` elimination steps of getrf.cpp:45-65 only (steps < 0 or
+ * >= the block count: all of them).  A partial factorization for parity checks
+ * against a CPU restatement stopped at the same step; the diagonal tile of
+ * step `steps` is factorized as well.  No reference counterpart. */
+int frk_blr_getrf_steps(frk_blr a, int steps);
 
 /* tiled_householder_blr_qr(A, T) (include/FRANK/operations/LAPACK.h:450,
  * src/operations/LAPACK/qr.cpp:213-228): FP64, in place.  Afterwards the

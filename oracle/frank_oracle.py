@@ -140,12 +140,16 @@ def norm(a) -> float:
     return seq_sum(a * a)
 
 
-def find_svd_truncation_rank(s_diag: np.ndarray, eps: float) -> int:
+def find_svd_truncation_rank(s_diag: np.ndarray, eps: float, cap: Optional[int] = None) -> int:
     """src/operations/misc/misc.cpp:133-146 (H2).
 
     threshold = eps * sqrt(sum sigma^2); r = smallest r >= 1 with
     sqrt((min_dim - r) * sigma_r^2) <= threshold (0-based sigma), else min_dim.
     The reference's inner loop adds sigma_r^2 (min_dim - r) times.
+
+    cap (no reference counterpart: the reference's ranks are unbounded) keeps
+    at most the cap leading terms, the rule of the GPU path's capped-rank
+    mode (frk_ctx_set_rank_cap(ctx, 1)).
     """
     s = np.asarray(s_diag, dtype=np.float64)
     threshold = eps * math.sqrt(seq_sum(s * s))
@@ -157,7 +161,7 @@ def find_svd_truncation_rank(s_diag: np.ndarray, eps: float) -> int:
         for _ in range(rank, min_dim):
             err += s[rank] * s[rank]
         if not (rank < min_dim and math.sqrt(err) > threshold):
-            return rank
+            return rank if cap is None else min(rank, cap)
 
 
 def resize(a: np.ndarray, rows: int, cols: int) -> np.ndarray:
@@ -238,12 +242,14 @@ def dlarfg(n: int, alpha: float, x: np.ndarray) -> Tuple[float, np.ndarray, floa
     return float(beta), v, float(tau)
 
 
-def truncated_geqp3(a_in: np.ndarray, eps: float) -> Tuple[np.ndarray, np.ndarray]:
+def truncated_geqp3(a_in: np.ndarray, eps: float, cap: Optional[int] = None) -> Tuple[np.ndarray, np.ndarray]:
     """src/operations/LAPACK/geqp3.cpp:71-191 (H5).
 
     Column-pivoted Householder QR with LAWN-176 norm downdating, stopped when
     the largest partial column norm is <= eps * ||A||_F.  Returns the
     truncated Q (m x r) and R (r x n) with columns back in original order.
+    cap (no reference counterpart) also stops after cap pivots, the rule of
+    the GPU path's capped-rank mode.
     """
     a = np.array(a_in, dtype=np.float64, copy=True)
     m, n = a.shape
@@ -261,7 +267,7 @@ def truncated_geqp3(a_in: np.ndarray, eps: float) -> Tuple[np.ndarray, np.ndarra
         q = np.zeros((m, 1))
         q[0, 0] = 1.0
         return q, np.zeros((1, n))
-    while r < min_dim and max_cnorm > threshold:
+    while r < min_dim and (cap is None or r < cap) and max_cnorm > threshold:
         k = r + int(np.argmax(cnorm[r:]))
         a[:, [r, k]] = a[:, [k, r]]
         cnorm[[r, k]] = cnorm[[k, r]]
@@ -354,6 +360,7 @@ class LowRank:
     S: np.ndarray
     V: np.ndarray
     eps: float = 0.0
+    cap: Optional[int] = None  # rank cap of the capped-rank mode (no reference counterpart)
 
     @property
     def rank(self) -> int:
@@ -368,7 +375,7 @@ class LowRank:
         return (self.U @ self.S) @ self.V
 
     def copy(self) -> "LowRank":
-        return LowRank(self.U.copy(), self.S.copy(), self.V.copy(), self.eps)
+        return LowRank(self.U.copy(), self.S.copy(), self.V.copy(), self.eps, self.cap)
 
     @staticmethod
     def from_rank(a: np.ndarray, rank: int) -> "LowRank":
@@ -378,15 +385,15 @@ class LowRank:
         return LowRank(resize(u, m, rank), resize(s, rank, rank), resize(v, rank, n), 0.0)
 
     @staticmethod
-    def from_eps(a: np.ndarray, eps: float) -> "LowRank":
+    def from_eps(a: np.ndarray, eps: float, cap: Optional[int] = None) -> "LowRank":
         """src/classes/low_rank.cpp:50-59: truncated_geqp3 then RQ of R (S upper triangular)."""
-        u, r = truncated_geqp3(a, eps)
+        u, r = truncated_geqp3(a, eps, cap)
         s, v = rq(r)
-        return LowRank(u, s, v, eps)
+        return LowRank(u, s, v, eps, cap)
 
     def transpose(self) -> "LowRank":
         """src/operations/misc/transpose.cpp: (U S V)^T = V^T S^T U^T."""
-        return LowRank(self.V.T.copy(), self.S.T.copy(), self.U.T.copy(), self.eps)
+        return LowRank(self.V.T.copy(), self.S.T.copy(), self.U.T.copy(), self.eps, self.cap)
 
 
 def rsvd(a: np.ndarray, p: int) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
@@ -408,11 +415,11 @@ def rounded_addition(A: LowRank, B: LowRank) -> LowRank:
     rvt, qvt = rq(vct)
     m = ru @ rvt
     rru, rrs, rrv = svd(m)
-    rank = find_svd_truncation_rank(np.diag(rrs), A.eps) if A.eps != 0 else A.rank
+    rank = find_svd_truncation_rank(np.diag(rrs), A.eps, A.cap) if A.eps != 0 else A.rank
     S = resize(rrs, rank, rank)
     U = qu @ resize(rru, rru.shape[0], rank)
     V = resize(rrv, rank, rrv.shape[1]) @ qvt
-    return LowRank(U, S, V, A.eps)
+    return LowRank(U, S, V, A.eps, A.cap)
 
 
 def fast_rounded_addition(A: LowRank, B: LowRank) -> LowRank:
@@ -438,7 +445,7 @@ def fast_rounded_addition(A: LowRank, B: LowRank) -> LowRank:
     um, sm, vmt = svd(M)
     r = A.rank
     return LowRank(uc @ resize(um, um.shape[0], r), resize(sm, r, r),
-                   resize(vmt, r, vmt.shape[1]) @ vct, A.eps)
+                   resize(vmt, r, vmt.shape[1]) @ vct, A.eps, A.cap)
 
 
 def naive_addition(A: LowRank, B: LowRank) -> LowRank:
@@ -450,7 +457,7 @@ def naive_addition(A: LowRank, B: LowRank) -> LowRank:
     S = np.zeros((ka + kb, ka + kb))
     S[:ka, :ka] = A.S
     S[ka:, ka:] = B.S
-    return LowRank(np.hstack([A.U, B.U]), S, np.vstack([A.V, B.V]), A.eps)
+    return LowRank(np.hstack([A.U, B.U]), S, np.vstack([A.V, B.V]), A.eps, A.cap)
 
 
 LRA_DEFAULT = "fast_rounded_addition"
@@ -509,9 +516,9 @@ def gemm(A, B, C, alpha: float = 1.0, beta: float = 1.0, ta: bool = False, tb: b
         return C
     # LowRank target
     if isinstance(A, np.ndarray) and isinstance(B, np.ndarray):  # D D LR :202-217
-        C = LowRank(C.U, C.S * beta, C.V, C.eps)
+        C = LowRank(C.U, C.S * beta, C.V, C.eps, C.cap)
         prod = alpha * (_op(A, ta) @ _op(B, tb))
-        X = LowRank.from_eps(prod, C.eps) if C.eps != 0.0 else LowRank.from_rank(prod, C.rank)
+        X = LowRank.from_eps(prod, C.eps, C.cap) if C.eps != 0.0 else LowRank.from_rank(prod, C.rank)
         return lr_add(C, X, lra)
     if isinstance(A, LowRank) and isinstance(B, np.ndarray):  # LR D LR :219-234
         X = LowRank(A.V.T if ta else A.U, A.S.T if ta else A.S,
@@ -523,7 +530,7 @@ def gemm(A, B, C, alpha: float = 1.0, beta: float = 1.0, ta: bool = False, tb: b
         left = _op(A.S, ta) @ _op(A.U if ta else A.V, ta)
         right = _op(B.V if tb else B.U, tb) @ _op(B.S, tb)
         X = LowRank(A.V.T if ta else A.U, alpha * (left @ right), B.U.T if tb else B.V)
-    C = LowRank(C.U, C.S * beta, C.V, C.eps)
+    C = LowRank(C.U, C.S * beta, C.V, C.eps, C.cap)
     return lr_add(C, X, lra)
 
 
@@ -531,8 +538,8 @@ def trsm(A: np.ndarray, B, uplo: str, side: str):
     """src/operations/BLAS/trsm.cpp:92-115: Dense target, or LowRank (Left -> U, Right -> V)."""
     if isinstance(B, LowRank):
         if side == "left":
-            return LowRank(trsm_dense(A, B.U, uplo, side), B.S, B.V, B.eps)
-        return LowRank(B.U, B.S, trsm_dense(A, B.V, uplo, side), B.eps)
+            return LowRank(trsm_dense(A, B.U, uplo, side), B.S, B.V, B.eps, B.cap)
+        return LowRank(B.U, B.S, trsm_dense(A, B.V, uplo, side), B.eps, B.cap)
     B[...] = trsm_dense(A, B, uplo, side)
     return B
 
@@ -671,11 +678,12 @@ class Hier:
 
 def build_blr(kernel, n_rows: int, n_cols: int, nb_r: int, nb_c: int, *, eps: float = 0.0,
               rank: int = 0, admis: float = 0.0, admis_type: str = "position",
-              points=None) -> Hier:
+              points=None, cap: Optional[int] = None) -> Hier:
     """src/classes/hierarchical.cpp:49-109 for a one-level tree (every child a leaf).
 
     ``kernel(r0, nr, c0, nc)`` fills a block.  Fixed accuracy when ``eps`` != 0
-    (LowRank(D, eps)), else fixed rank (LowRank(D, rank)).
+    (LowRank(D, eps)), else fixed rank (LowRank(D, rank)).  ``cap`` bounds every
+    eps-rank (the capped-rank mode; no reference counterpart).
     """
     rr = index_split(0, n_rows, nb_r)
     cr = index_split(0, n_cols, nb_c)
@@ -684,7 +692,7 @@ def build_blr(kernel, n_rows: int, n_cols: int, nb_r: int, nb_c: int, *, eps: fl
         for j, c in enumerate(cr):
             d = kernel(r[0], r[1], c[0], c[1])
             if is_admissible(r, c, i, j, admis, admis_type, points or []):
-                blocks.append(LowRank.from_eps(d, eps) if eps != 0.0 else LowRank.from_rank(d, rank))
+                blocks.append(LowRank.from_eps(d, eps, cap) if eps != 0.0 else LowRank.from_rank(d, rank))
             else:
                 blocks.append(d)
     return Hier(nb_r, nb_c, blocks, rr, cr)
@@ -820,8 +828,8 @@ def trmm(A: np.ndarray, B, side: str, uplo: str, trans: str = "n", diag: str = "
     op = t.T if trans == "t" else t
     if isinstance(B, LowRank):
         if side == "left":
-            return LowRank(alpha * (op @ B.U), B.S, B.V, B.eps)
-        return LowRank(B.U, B.S, alpha * (B.V @ op), B.eps)
+            return LowRank(alpha * (op @ B.U), B.S, B.V, B.eps, B.cap)
+        return LowRank(B.U, B.S, alpha * (B.V @ op), B.eps, B.cap)
     B[...] = alpha * (op @ B) if side == "left" else alpha * (B @ op)
     return B
 
@@ -836,7 +844,7 @@ def geqrt(A: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
 def larfb(V: np.ndarray, T: np.ndarray, C, trans: bool):
     """src/operations/LAPACK/larfb.cpp:37-56: C <- (I - V T V^T)^{T|N} C ('L', 'F', 'C'); LowRank -> on C.U."""
     if isinstance(C, LowRank):
-        return LowRank(larfb(V, T, C.U.copy(), trans), C.S, C.V, C.eps)
+        return LowRank(larfb(V, T, C.U.copy(), trans), C.S, C.V, C.eps, C.cap)
     out, info = lapack.dgemqrt(V, T, np.array(C, dtype=np.float64, copy=True), side="L",
                                trans="T" if trans else "N")
     C[...] = out
@@ -852,7 +860,7 @@ def tpqrt(A: np.ndarray, B, T_out=None):
     if isinstance(B, LowRank):
         bv = B.S @ B.V
         A2, bv2, T = tpqrt(A, bv)
-        return A2, LowRank(B.U, np.eye(B.rank), bv2, B.eps), T
+        return A2, LowRank(B.U, np.eye(B.rank), bv2, B.eps, B.cap), T
     n = A.shape[1]
     a, b, t, info = lapack.dtpqrt(0, n, np.array(A, dtype=np.float64, copy=True),
                                   np.array(B, dtype=np.float64, copy=True))

@@ -119,3 +119,40 @@ def test_blr_solve_multi_rhs_and_transpose(torch_gpu, fa, trans):
     b = torch.from_numpy(rhs.copy()).cuda()
     G.solve(b, trans=trans)
     assert fo.l2_error(xs, b.cpu().numpy()) <= 1e-8
+
+
+@pytest.mark.parametrize("zero_tile", ["lower", "upper"])
+def test_blr_lu_rank_zero_priority_panel(torch_gpu, fa, zero_tile):
+    """One of the two panels of a priority block has rank 0 (the other not).
+
+    Nonsymmetric BLR matrix from host tiles, 4 x 4 tiles of 128, every
+    off-diagonal tile low rank; tile (2, 1) (or its mirror (1, 2)) has rank 0.
+    getrf + solve must match the dense FP64 solve of the same matrix.
+    """
+    torch = torch_gpu
+    print(f"rank-zero priority panel: {zero_tile}", flush=True)
+    nb, b = 4, 128
+    N = nb * b
+    rng = np.random.default_rng(7)
+    off = [i * b for i in range(nb + 1)]
+    tiles = []
+    for i in range(nb):
+        for j in range(nb):
+            if i == j:
+                tiles.append(rng.standard_normal((b, b)) + 4.0 * b * np.eye(b))
+                continue
+            r = 0 if (zero_tile == "lower" and (i, j) == (2, 1)) or (zero_tile == "upper" and (i, j) == (1, 2)) else 3
+            U = np.linalg.qr(rng.standard_normal((b, max(r, 1))))[0][:, :r]
+            V = np.linalg.qr(rng.standard_normal((b, max(r, 1))))[0][:, :r].T
+            S = np.triu(rng.standard_normal((r, r))) + 5.0 * np.eye(r)
+            tiles.append((U, S, V))
+    G = fa.BlrMatrix.from_tiles(off, off, tiles, eps=1e-12, kcap=32)
+    A0 = G.dense()
+    G.set_lra("rounded_addition")
+    G.getrf()
+    xs = rng.standard_normal((N, 2))
+    rhs = A0 @ xs
+    bt = torch.from_numpy(rhs.copy()).cuda()
+    G.solve(bt)
+    err = np.linalg.norm(bt.cpu().numpy() - xs) / np.linalg.norm(xs)
+    assert err <= 1e-9, err

@@ -55,62 +55,119 @@ def test_c0_fixed_rank_blr_lu_full_size(torch_gpu, fa):
     assert err_gpu <= 10 * max(err_ref, 1e-14), (err_gpu, err_ref)
 
 
-def test_c1_tiled_qr_full_size(torch_gpu, fa):
+def _c1_points(N):
     from frank_amd.inputs import morton_sort, uniform_points
+    pts, _ = morton_sort(uniform_points(N, 2), 6)
+    return [np.ascontiguousarray(p) for p in pts]
+
+
+def test_c1_capped_ranks_match_oracle(torch_gpu, fa):
+    """C1's rank cap where it binds, against the oracle with the same cap rule:
+    2-D Laplace on Morton-ordered points, 8 x 8 tiles of 256 (N = 2048),
+    eps 1e-8, ranks capped at 64.  Construction ranks, ranks after the tiled
+    QR and the first diagonal R tile must equal the oracle's."""
+    N, nb, eps, cap = 2048, 8, 1e-8, 64
+    b = N // nb
+    print(f"C1 capped parity N={N} nb={nb} eps={eps} cap={cap}", flush=True)
+    pts = _c1_points(N)
+    ctx = fa.Context()
+    ctx.set_rank_cap(True)
+    A = fa.BlrMatrix.from_kernel_rect(pts, N, N, nb, nb, eps=eps, kcap=cap, ctx=ctx)
+    A.set_lra("rounded_addition")
+    ker = lambda r0, nr, c0, nc: fo.laplacend(pts, r0, nr, c0, nc)
+    H = fo.build_blr(ker, N, N, nb, nb, eps=eps, points=pts, cap=cap)
+    r0 = H.ranks()
+    assert r0.max() == cap, "the cap does not bind on this input"
+    assert np.array_equal(A.ranks(), r0)
+    F = A.qr()
+    H, Tt = fo.tiled_householder_blr_qr(H)
+    rg, ro = A.ranks(), H.ranks()
+    print(f"C1 capped parity: {int((rg != ro).sum())} of {rg.size} ranks differ after QR; "
+          f"max rank {rg.max()}", flush=True)
+    assert np.array_equal(rg, ro)
+    R00 = np.triu(A.tile(0, 0))
+    R00r = np.triu(H[0, 0])
+    assert np.abs(R00 - R00r).max() <= 1e-10 * np.abs(R00r).max()
+    T00 = F.t(0, 0)[0]
+    assert np.abs(T00 - Tt[0, 0]).max() <= 1e-10 * max(1.0, np.abs(Tt[0, 0]).max())
+
+
+def test_c1_tiled_qr_full_size(torch_gpu, fa):
+    """C1 at full size: ||A - QR|| / ||A|| and ||Q^T Q - I|| over the whole
+    matrix, as test/blr_fixed_acc_test.cpp:148-190 computes them (Q formed by
+    applying the reflectors to the identity, Q^T Q by applying them again)."""
     N, nb, eps = 8192, 32, 1e-8
     b = N // nb
     print(f"C1 N={N} nb={nb} eps={eps}", flush=True)
-    pts, _ = morton_sort(uniform_points(N, 2), 6)
-    pts = [np.ascontiguousarray(p) for p in pts]
+    pts = _c1_points(N)
     ctx = fa.Context()
     ctx.set_rank_cap(True)  # ranks adaptive, capped at 64 (the 2-D eps-ranks of touching tiles exceed 64)
     A = fa.BlrMatrix.from_kernel_rect(pts, N, N, nb, nb, eps=eps, kcap=64, ctx=ctx)
     A.set_lra("rounded_addition")
-    # first block column of the compressed matrix, before the factorization
-    col = []
-    for i in range(nb):
-        t = A.tile(i, 0)
-        col.append(t if isinstance(t, np.ndarray) else (t[0] @ t[1]) @ t[2])
-    A_blr = np.vstack(col)
+    A_blr = A.dense()  # the compressed matrix before the factorization
     F = A.qr()
-    C = fa.BlrMatrix.from_kernel_rect(pts, N, b, nb, 1, eps=eps, kcap=64, kernel="identity", ctx=ctx)
-    C.set_lra("rounded_addition")
-    A.apply_q(F, C, trans=False)
-    Q0 = C.dense()                        # Q[:, :256]
-    R00 = np.triu(A.tile(0, 0))
-    A_col = fo.laplacend(pts, 0, N, 0, b)  # exact kernel matrix, first block column
-    res = fo.l2_error(A_blr, Q0 @ R00)
-    res_exact = fo.l2_error(A_col, Q0 @ R00)
-    A.apply_q(F, C, trans=True)
-    orth = fo.l2_error(np.eye(N, b), C.dense())
+    # Q as dense tiles (every block inadmissible), so Q itself carries no compression
+    Q = fa.BlrMatrix.from_kernel_rect(pts, N, N, nb, nb, eps=eps, kcap=64, kernel="identity",
+                                      admis=float(nb), ctx=ctx)
+    Q.set_lra("rounded_addition")
+    A.apply_q(F, Q, trans=False)
+    Qd = Q.dense()
+    Ad = A.dense()
+    R = np.zeros((N, N))
+    for i in range(nb):
+        for j in range(i, nb):
+            blk = Ad[i * b:(i + 1) * b, j * b:(j + 1) * b]
+            R[i * b:(i + 1) * b, j * b:(j + 1) * b] = np.triu(blk) if i == j else blk
+    QR = (torch_gpu.from_numpy(Qd).cuda() @ torch_gpu.from_numpy(R).cuda()).cpu().numpy()  # FP64
+    res = fo.l2_error(A_blr, QR)
+    del QR
+    A.apply_q(F, Q, trans=True)
+    orth = fo.l2_error(np.eye(N), Q.dense())
     r = A.ranks()
-    print(f"C1 first block column: residual {res:.3e} (compressed A), {res_exact:.3e} (exact kernel), "
-          f"orthogonality {orth:.3e}, max rank {r.max()}", flush=True)
-    assert res <= 10 * eps
-    assert orth <= 10 * eps
+    print(f"C1 full matrix: ||A - QR||/||A|| {res:.3e}, ||Q^T Q - I||/||I|| {orth:.3e}, max rank {r.max()}",
+          flush=True)
+    assert res <= eps
+    assert orth <= eps
     assert r.max() <= 64
 
 
-def test_c2_fp32_batched_rounded_addition_full_size(torch_gpu, fa):
-    torch = torch_gpu
-    batch, eps, cap = 4096, 1e-6, 128
-    print(f"C2 batch={batch} eps={eps}", flush=True)
-    rng = np.random.default_rng(2024)
+def c2_pairs(batch=4096, seed=2024):
+    """C2's inputs: (m, n) from {64, 128, 256, 512}, (k1, k2) from {8, 16, 32, 64},
+    U and V with N(0, 1) entries (not orthonormalised), S = diag(10^(-i/4))."""
+    rng = np.random.default_rng(seed)
     sizes = [64, 128, 256, 512]
     ranks = [8, 16, 32, 64]
 
     def factor(m, n, k):
-        U = np.linalg.qr(rng.standard_normal((m, k)))[0]
-        V = np.linalg.qr(rng.standard_normal((n, k)))[0].T
+        U = rng.standard_normal((m, k))
+        V = rng.standard_normal((k, n))
         S = np.diag(10.0 ** (-np.arange(k) / 4.0))
         return U.astype(np.float32), S.astype(np.float32), V.astype(np.float32)
 
-    pairs, As, Bs = [], [], []
+    pairs = []
     for _ in range(batch):
         m, n = int(rng.choice(sizes)), int(rng.choice(sizes))
         k1, k2 = int(rng.choice(ranks)), int(rng.choice(ranks))
-        a, bb = factor(m, n, k1), factor(m, n, k2)
-        pairs.append((a, bb))
+        pairs.append((factor(m, n, k1), factor(m, n, k2)))
+    return pairs
+
+
+def test_c2_fp32_batched_rounded_addition_full_size(torch_gpu, fa):
+    """C2 as specified: 4096 FP32 rounded additions at eps 1e-6 in one batched
+    call.  Every sum is reproduced to FP32 accuracy (FP64 probe check) and the
+    kept ranks are compared with the FP64 oracle's rounded_addition
+    (addition.cpp:93-127) on every 16th pair: the exact-match rate is
+    printed, at least 90 % must match and none may differ by more than 3
+    (FP32 singular values near the eps threshold carry relative errors of
+    order 1e-7 / 1e-6, which can move a borderline value across it)."""
+    torch = torch_gpu
+    batch, eps, cap = 4096, 1e-6, 128
+    print(f"C2 batch={batch} eps={eps}", flush=True)
+    pairs = c2_pairs(batch)
+    rng = np.random.default_rng(7)
+    As, Bs = [], []
+    for a, bb in pairs:
+        m, n, k1 = a[0].shape[0], a[2].shape[1], a[0].shape[1]
         U = torch.zeros((m, cap), dtype=torch.float32, device="cuda")
         S = torch.zeros((cap, cap), dtype=torch.float32, device="cuda")
         V = torch.zeros((cap, n), dtype=torch.float32, device="cuda")
@@ -134,15 +191,73 @@ def test_c2_fp32_batched_rounded_addition_full_size(torch_gpu, fa):
         worst = max(worst, np.linalg.norm(got - ref) / np.linalg.norm(ref))
     print(f"C2 worst relative probe error {worst:.3e}", flush=True)
     assert worst <= 1e-4
-    # kept ranks against the FP64 oracle on a sample
-    for t in range(0, batch, 128):
+    diffs = []
+    for t in range(0, batch, 16):
         (a, bb) = pairs[t]
         A64 = fo.LowRank(*[x.astype(np.float64) for x in a], eps)
         B64 = fo.LowRank(*[x.astype(np.float64) for x in bb], eps)
-        r_ref = fo.rounded_addition(A64, B64).rank
-        # FP32 singular values near the threshold carry relative errors of order
-        # 1e-7 / 1e-6: the kept rank may move by a few (the accuracy check above holds)
-        assert abs(out[t] - r_ref) <= 3, (t, out[t], r_ref)
+        diffs.append(out[t] - fo.rounded_addition(A64, B64).rank)
+    diffs = np.array(diffs)
+    rate = float(np.mean(diffs == 0))
+    print(f"C2 rank match rate vs FP64 oracle: {rate:.4f} over {diffs.size} pairs, "
+          f"max |diff| {int(np.abs(diffs).max())}", flush=True)
+    assert rate >= 0.9
+    assert np.abs(diffs).max() <= 3
+
+
+def _to_oracle(A, eps):
+    """The GPU BLR matrix's tiles as an oracle Hier (dense tiles as arrays,
+    low-rank tiles as LowRank at accuracy eps)."""
+    nb_r, nb_c = A.shape
+    rr, cr, r0, c0 = [], [], 0, 0
+    for i in range(nb_r):
+        n = A.tile_info(i, 0)[2]
+        rr.append((r0, n))
+        r0 += n
+    for j in range(nb_c):
+        n = A.tile_info(0, j)[3]
+        cr.append((c0, n))
+        c0 += n
+    blocks = []
+    for i in range(nb_r):
+        for j in range(nb_c):
+            t = A.tile(i, j)
+            blocks.append(fo.LowRank(t[0], t[1], t[2], eps) if isinstance(t, tuple) else t)
+    return fo.Hier(nb_r, nb_c, blocks, rr, cr)
+
+
+def _tile_dense(t):
+    return t.dense() if isinstance(t, fo.LowRank) else ((t[0] @ t[1]) @ t[2] if isinstance(t, tuple) else t)
+
+
+def test_c3_fp64_steps_match_oracle(torch_gpu, fa):
+    """C3's FP64 path at full size (N = 32768, 64 x 64 tiles of 512, eps 1e-10,
+    1-D Laplace + 1e-2 I, rounded addition) against the oracle on the same
+    compressed matrix (downloaded from the GPU), both stopped after the first
+    three elimination steps of getrf.cpp:45-65: every rank equal, and the
+    trailing tiles (3, 4) and (4, 4) equal to 1e-8 relative."""
+    from frank_amd.inputs import sorted_uniform_points
+    N, nb, eps, steps = 32768, 64, 1e-10, 3
+    print(f"C3 FP64 parity N={N} nb={nb} eps={eps} steps={steps}", flush=True)
+    pts = [sorted_uniform_points(N)]
+    A = fa.BlrMatrix.from_kernel(pts, nb, eps=eps, shift=1e-2, kcap=64)
+    A.set_lra("rounded_addition")
+    H = _to_oracle(A, eps)
+    A.getrf(max_steps=steps)
+    Lr, Ur = fo.blr_getrf(H, lra="rounded_addition", max_steps=steps)
+    ro = np.full((nb, nb), -1, dtype=np.int64)
+    for i in range(nb):
+        for j in range(nb):
+            blk = Lr[i, j] if (j < steps and i > j) else Ur[i, j]
+            if isinstance(blk, fo.LowRank):
+                ro[i, j] = blk.rank
+    rg = A.ranks()
+    print(f"C3 FP64 parity: {int((rg != ro).sum())} of {rg.size} ranks differ, max rank {rg.max()}", flush=True)
+    assert np.array_equal(rg, ro)
+    for (i, j) in [(steps, steps + 1), (steps + 1, steps), (steps + 1, steps + 1), (nb - 1, steps)]:
+        g = _tile_dense(A.tile(i, j))
+        r = _tile_dense(Ur[i, j])
+        assert np.linalg.norm(g - r) <= 1e-8 * np.linalg.norm(r), (i, j)
 
 
 def _gauss_loss_dense(pts, ell, b, shift=1e-4):
