@@ -1,30 +1,44 @@
 #!/usr/bin/env python3
-"""Benchmark: BLR-LU factor + refined solve at N=32768 on one MI355X.
+"""Benchmark: BLR-LU factor + refined solve at N=32768 on one MI355X, plus the
+other four workloads of the hot path.
 
-Workload (the metric's configuration): 1-D Laplace kernel
+Headline workload (the metric's configuration): 1-D Laplace kernel
 A_ij = 1/(|x_i - x_j| + 1e-3) + 1e-2 delta_ij, x = sorted U[0,1) points from
 std::mt19937(0) (src/util/experiment_setup.cpp:12-23), BLR 64 x 64 grid of
 512 x 512 tiles, weak admissibility, eps = 1e-10, 8 right-hand sides ~ N(0,1),
 FP64 iterative refinement with the exact matrix to backward error <= 1e-10.
-
 A step = restore the compressed matrix (one device copy) + getrf + refined
 solve.  Construction is timed separately (as examples/h_lu.cpp:24-43 does).
-
 Two paths are timed: plain FP64 (eps 1e-10) and mixed precision (FP32 tile
 storage and recompression at eps 1e-6, FP64 diagonal getrf, FP64 refinement
 against the uncompressed kernel).  `value` is the mixed-precision time.
 
-The `roofline` object covers one kernel class of the headline path (its
-launches are bracketed by HIP events on the library's stream during the timed
-steps; flops and bytes come from the library's op log).  `cpu_baseline` times
-the CPU restatement of the reference algorithm (oracle/) on the first
-elimination steps of the same FP64 matrix and scales by the tile-op count.
+The other workloads (`--workload`, and embedded under "workloads" in the
+default line unless --no-secondary):
+  c0  BLR-LU FP64, N=4096, 16 x 16 tiles of 256, fixed rank 16, fast rounded
+      addition; step = restore + getrf + solve (1 RHS)
+  c1  tiled Householder BLR-QR FP64, N=8192, 32 x 32 tiles of 256, 2-D Laplace
+      on Morton-ordered points, eps 1e-8, ranks capped at 64; step = restore + QR
+  c2  4096 FP32 rounded additions (U1 S1 V1^T) + (U2 S2 V2^T), m, n in
+      {64..512}, k1, k2 in {8..64}, N(0,1) factors, sigma_i = 10^(-i/4),
+      eps 1e-6, one batched call; step = restore the A factors + the call
+  c4  differentiable gaussian-kernel solves (a subset of the 64-system batch:
+      uneven N, ell ~ U[0.05, 0.3]); step = build + getrf + solve + adjoint
+
+`roofline` covers one kernel class of the headline path (its launches are
+bracketed by HIP events on the stream they run on during the timed steps;
+flops and bytes per launch come from the library's op log, getrf counted as
+2/3 n^3).  `path_roofline` is the op log's sum over all launches of
+max(flops / peak, bytes / 8 TB/s) over the measured step time.
+`cpu_baseline` times the CPU restatement of the reference algorithm (oracle/)
+on this host, median of three runs, with every host core and with one.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 
@@ -36,43 +50,84 @@ PEAK_HBM_GBS = 8000.0
 RIDGE = {"f64": 78.6e12 / 8e12, "f32": 157.3e12 / 8e12}
 
 
-def cpu_baseline(A64, nb: int, steps: int, nrhs: int):
-    """Oracle BLR-LU (rounded addition, FP64, one thread) on the first `steps`
-    elimination steps of the downloaded FP64 matrix, scaled to all steps by the
-    number of tile operations."""
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads() -> int:
+    """Host threads of this run: the OMP_NUM_THREADS share when set, else every core."""
+    v = os.environ.get("OMP_NUM_THREADS")
+    try:
+        return max(1, int(v)) if v else (os.cpu_count() or 1)
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_timed(fn, reps: int = 3):
+    """Median wall time of fn() over reps runs, at every host thread and at one
+    BLAS thread.  Returns (median_all_threads, median_one_thread, threads)."""
     from threadpoolctl import threadpool_limits
+    nt = host_threads()
+    out = []
+    for th in (nt, 1):
+        ts = []
+        with threadpool_limits(th):
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                fn()
+                ts.append(time.perf_counter() - t0)
+        out.append(statistics.median(ts))
+    return out[0], out[1], nt
 
-    from oracle import frank_oracle as O
 
-    t_dl = time.perf_counter()
-    blocks, rr, cr = [], [], []
-    r0 = 0
-    for i in range(nb):
-        rr.append((r0, A64.tile_info(i, 0)[2]))
-        r0 += rr[-1][1]
-    c0 = 0
-    for j in range(nb):
-        cr.append((c0, A64.tile_info(0, j)[3]))
-        c0 += cr[-1][1]
-    for i in range(nb):
-        for j in range(nb):
-            t = A64.tile(i, j)
-            blocks.append(O.LowRank(t[0], t[1], t[2], 1e-10) if isinstance(t, tuple) else t)
-    H = O.Hier(nb, nb, blocks, rr, cr)
-    t_dl = time.perf_counter() - t_dl
-    with threadpool_limits(1):
-        t0 = time.perf_counter()
-        O.blr_getrf(H, lra="rounded_addition", max_steps=steps)
-        dt = time.perf_counter() - t0
-    upd = [(nb - 1 - k) ** 2 + 2 * (nb - 1 - k) + 1 for k in range(nb)]  # updates + trsm + getrf per step
-    frac = sum(upd[:steps]) / sum(upd)
-    print(f"[bench] cpu baseline: {steps} steps in {dt:.1f}s (download {t_dl:.1f}s), fraction of work {frac:.3f}",
-          file=sys.stderr, flush=True)
-    return {"value": dt / frac, "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"oracle/frank_oracle.py blr_getrf (FP64, rounded addition, 1 BLAS thread) on the first "
-                      f"{steps} of {nb} elimination steps of the same FP64 BLR matrix (downloaded from the GPU), "
-                      f"{dt:.1f}s measured, scaled by the per-step tile-op count ({frac:.3f} of the total); "
-                      f"factorization only, the {nrhs}-RHS solve adds under 1%"}
+def cpu_record(t_all: float, t_one: float, nt: int, unit_scale: float, unit: str, sample: str) -> dict:
+    best = min(t_all, t_one)
+    return {"value": best * unit_scale, "unit": unit, "cores": nt if t_all <= t_one else 1, "kind": "port",
+            "value_all_threads": t_all * unit_scale, "value_1_thread": t_one * unit_scale,
+            "threads_available": nt, "cpu_model": cpu_model(), "timing": "median of 3 runs", "sample": sample}
+
+
+def roofline_of(ops: dict, cls: str, steps: int, head_prec: str):
+    rc = ops.get(cls)
+    if not rc or rc["timed_launches"] <= 0 or rc["ms"] <= 0:
+        return None
+    prec = "f64" if (head_prec == "f64" or cls.startswith("getrf")) else "f32"
+    secs = rc["ms"] / 1e3
+    intensity = rc["flops"] / max(rc["bytes"], 1.0)
+    if intensity < RIDGE[prec]:
+        ach = rc["bytes"] / secs / 1e9
+        roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS}
+    else:
+        ach = rc["flops"] / secs / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                "frac": ach / PEAK_TFLOPS[prec]}
+    roof.update({"traffic": measured_traffic(cls), "kernel_class": cls, "precision": prec,
+                 "launches_per_step": rc["launches"] / steps,
+                 "avg_launch_us": rc["ms"] * 1e3 / rc["timed_launches"],
+                 "alg_flops_per_launch": rc["flops"] / rc["launches"],
+                 "alg_bytes_per_launch": rc["bytes"] / rc["launches"]})
+    return roof
+
+
+def path_roofline(ops: dict, steps: int, step_s: float) -> dict:
+    """Sum over the op log of max(F / P_prec, B / BW), over the measured step."""
+    roof_ms = sum(v.get("roofline_ms", 0.0) for v in ops.values()) / steps
+    fl = sum(v["flops"] for k, v in ops.items() if not k.endswith("_inverses")) / steps
+    return {"roofline_ms_per_step": roof_ms, "measured_ms_per_step": step_s * 1e3,
+            "frac": roof_ms / (step_s * 1e3), "alg_tflops": fl / step_s / 1e12,
+            "by_class_ms": {k: v.get("roofline_ms", 0.0) / steps for k, v in sorted(ops.items())
+                            if v.get("roofline_ms", 0.0) > 0}}
 
 
 def measured_traffic(cls: str):
@@ -91,48 +146,91 @@ def measured_traffic(cls: str):
                     vals[k.strip()] = float(v.split()[0])
                 except (ValueError, IndexError):
                     pass
-    if "hbm_bytes_per_launch" not in vals:
-        return None
-    return vals["hbm_bytes_per_launch"]
+    return vals.get("hbm_bytes_per_launch")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=32768)
-    ap.add_argument("--nb", type=int, default=64)
-    ap.add_argument("--eps", type=float, default=1e-10)
-    ap.add_argument("--nrhs", type=int, default=8)
-    ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--mode", choices=["both", "f64", "mixed"], default="both")
-    ap.add_argument("--eps-mixed", type=float, default=1e-6,
-                    help="compression / recompression accuracy of the FP32-storage factorization")
-    ap.add_argument("--roofline-class", default="getrf_tile",
-                    help="kernel class whose launches are event-timed for the roofline object")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="elimination steps in the CPU baseline sample (0: skip)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="run one 256 MiB device copy first (calibrates HBM byte counters under a profiler)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(the sharding design is described in the documentation)", file=sys.stderr)
-        sys.exit(2)
+def print_ops(name: str, ops: dict, steps: int):
+    for k, v in sorted(ops.items()):
+        extra = f", {v['ms'] / steps:.2f} ms/step event-timed" if v["timed_launches"] else ""
+        log(f"  {name} {k:20s} {v['launches'] / steps:8.0f} launches/step {v['flops'] / steps / 1e9:9.2f} GFLOP/step "
+            f"{v['bytes'] / steps / 1e9:8.3f} GB/step roofline {v.get('roofline_ms', 0) / steps:7.3f} ms{extra}")
 
+
+def oracle_hier(A, eps: float):
+    """The GPU BLR matrix's tiles as an oracle Hier (downloaded, FP64)."""
+    from oracle import frank_oracle as O
+    nb_r, nb_c = A.shape
+    rr, cr, r0, c0 = [], [], 0, 0
+    for i in range(nb_r):
+        n = A.tile_info(i, 0)[2]
+        rr.append((r0, n))
+        r0 += n
+    for j in range(nb_c):
+        n = A.tile_info(0, j)[3]
+        cr.append((c0, n))
+        c0 += n
+    blocks = []
+    for i in range(nb_r):
+        for j in range(nb_c):
+            t = A.tile(i, j)
+            blocks.append(O.LowRank(t[0], t[1], t[2], eps) if isinstance(t, tuple) else t)
+    return O.Hier(nb_r, nb_c, blocks, rr, cr)
+
+
+def timed_steps(ctx, step, steps: int, warmup: int, name: str, timed_cls=None):
+    import torch
+    for w in range(warmup):
+        r = step()
+        log(f"{name} warmup {w}: {r}")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ctx.time_class(timed_cls)
+    ctx.reset_op_stats()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = []
+    for s_ in range(steps):
+        res.append(step())
+        log(f"{name} step {s_}: {res[-1]}")
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ops = ctx.op_stats()
+    ctx.time_class(None)
+    log(f"{name}: {dt * 1e3:.2f} ms/step")
+    print_ops(name, ops, steps)
+    return dt, ops, res
+
+
+# ---------------------------------------------------------------------------
+# C3: mixed-precision vs FP64 BLR-LU with refinement (the headline)
+# ---------------------------------------------------------------------------
+def cpu_baseline_c3(A64, nb: int, steps: int, nrhs: int, reps: int):
+    """Oracle BLR-LU (rounded addition, FP64) on the first `steps` elimination
+    steps of the downloaded FP64 matrix, scaled to all steps by the number of
+    tile operations."""
+    from oracle import frank_oracle as O
+    t_dl = time.perf_counter()
+    H = oracle_hier(A64, 1e-10)
+    t_dl = time.perf_counter() - t_dl
+    t_all, t_one, nt = cpu_timed(lambda: O.blr_getrf(H.copy(), lra="rounded_addition", max_steps=steps), reps)
+    upd = [(nb - 1 - k) ** 2 + 2 * (nb - 1 - k) + 1 for k in range(nb)]  # updates + trsm + getrf per step
+    frac = sum(upd[:steps]) / sum(upd)
+    log(f"cpu baseline C3: {steps} steps {t_all:.2f}s ({nt} threads) / {t_one:.2f}s (1 thread), "
+        f"download {t_dl:.1f}s, fraction of work {frac:.3f}")
+    return cpu_record(t_all / frac, t_one / frac, nt, 1.0, "s",
+                      f"oracle/frank_oracle.py blr_getrf (FP64, rounded addition) on the first {steps} of {nb} "
+                      f"elimination steps of the same FP64 BLR matrix (downloaded from the GPU), scaled by the "
+                      f"per-step tile-op count ({frac:.3f} of the total); factorization only, the {nrhs}-RHS "
+                      f"solve adds under 1%")
+
+
+def run_c3(args, ctx):
     import torch
 
     import frank_amd as fa
     from frank_amd.inputs import normal_rhs, sorted_uniform_points
-
-    print(f"[bench] N={args.n} nb={args.nb} eps={args.eps} nrhs={args.nrhs}", file=sys.stderr, flush=True)
-    if args.copy_probe:
-        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-    ctx = fa.default_context()
+    log(f"C3 N={args.n} nb={args.nb} eps={args.eps} nrhs={args.nrhs}")
     pts = [sorted_uniform_points(args.n)]
     b = torch.from_numpy(normal_rhs(args.n, args.nrhs)).cuda()
     head = "mixed" if args.mode in ("both", "mixed") else "f64"
@@ -144,8 +242,7 @@ def main():
         dt = time.perf_counter() - t0
         A.set_lra("rounded_addition")
         r = A.ranks()
-        print(f"[bench] construction eps={eps}: {dt:.2f}s, max rank {r.max()}, mean {r[r >= 0].mean():.2f}",
-              file=sys.stderr, flush=True)
+        log(f"C3 construction eps={eps}: {dt:.2f}s, max rank {r.max()}, mean {r[r >= 0].mean():.2f}")
         return A, dt
 
     def run(name, A):
@@ -155,37 +252,16 @@ def main():
         def step():
             F.copy_from(A)
             F.getrf()
-            return F.refine(b, x, tol=args.tol, max_iter=60)
+            it, be = F.refine(b, x, tol=args.tol, max_iter=60)
+            return f"iters {it} berr {be:.3e}", it, be
 
-        for w in range(args.warmup):
-            it, be = step()
-            print(f"[bench] {name} warmup {w}: iters {it} berr {be:.3e}", file=sys.stderr, flush=True)
-        ctx.synchronize()
-        torch.cuda.synchronize()
         ctx.counters(reset=True)
-        ctx.time_class(args.roofline_class if name == head else None)
-        ctx.reset_op_stats()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        res = []
-        for s_ in range(args.steps):
-            res.append(step())
-            print(f"[bench] {name} step {s_}: iters {res[-1][0]} berr {res[-1][1]:.3e}", file=sys.stderr, flush=True)
-        ctx.synchronize()
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
-        ops = ctx.op_stats()
-        ctx.time_class(None)
+        dt, ops, res = timed_steps(ctx, step, args.steps, args.warmup, f"C3 {name}",
+                                   args.roofline_class if name == head else None)
         cnt = ctx.counters(reset=True)
-        print(f"[bench] {name}: {dt * 1e3:.1f} ms/step; per step: {cnt['svd_problems'] / args.steps:.0f} small SVDs, "
-              f"{cnt['svd_sweeps'] / max(cnt['svd_problems'], 1):.2f} Jacobi sweeps each (max {cnt['svd_max_sweeps']})",
-              file=sys.stderr, flush=True)
-        for k, v in sorted(ops.items()):
-            extra = f", {v['ms'] / args.steps:.2f} ms/step event-timed" if v["timed_launches"] else ""
-            print(f"[bench]   {k:12s} {v['launches'] / args.steps:8.0f} launches/step "
-                  f"{v['flops'] / args.steps / 1e9:9.2f} GFLOP/step {v['bytes'] / args.steps / 1e9:8.3f} GB/step{extra}",
-                  file=sys.stderr, flush=True)
-        return {"s": dt, "berr": max(r[1] for r in res), "iters": res[-1][0], "ops": ops}
+        log(f"C3 {name}: per step {cnt['svd_problems'] / args.steps:.0f} small SVDs, "
+            f"{cnt['svd_sweeps'] / max(cnt['svd_problems'], 1):.2f} Jacobi sweeps each (max {cnt['svd_max_sweeps']})")
+        return {"s": dt, "berr": max(r[2] for r in res), "iters": res[-1][1], "ops": ops}
 
     out_modes, cpu = {}, None
     if args.mode in ("both", "f64"):
@@ -193,7 +269,7 @@ def main():
         out_modes["f64"] = run("f64", A64)
         out_modes["f64"]["build_s"] = tb64
         if args.cpu_steps > 0:
-            cpu = cpu_baseline(A64, args.nb, args.cpu_steps, args.nrhs)
+            cpu = cpu_baseline_c3(A64, args.nb, args.cpu_steps, args.nrhs, args.cpu_reps)
         del A64
     if args.mode in ("both", "mixed"):
         A64m, tbm = build(args.eps_mixed)
@@ -202,29 +278,8 @@ def main():
         out_modes["mixed"] = run("mixed", A32)
         out_modes["mixed"]["build_s"] = tbm
     h = out_modes[head]
-    dt = h["s"]
-    steps = args.steps
-    alg_flops = sum(v["flops"] for v in h["ops"].values()) / steps
-    roof = None
-    rc = h["ops"].get(args.roofline_class)
-    if rc and rc["timed_launches"] > 0 and rc["ms"] > 0:
-        prec = "f64" if (head == "f64" or args.roofline_class.startswith("getrf")) else "f32"
-        secs = rc["ms"] / 1e3
-        intensity = rc["flops"] / max(rc["bytes"], 1.0)
-        if intensity < RIDGE[prec]:
-            ach = rc["bytes"] / secs / 1e9
-            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": ach / PEAK_HBM_GBS}
-        else:
-            ach = rc["flops"] / secs / 1e12
-            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
-                    "frac": ach / PEAK_TFLOPS[prec]}
-        traffic = measured_traffic(args.roofline_class)
-        roof.update({"traffic": traffic, "kernel_class": args.roofline_class, "precision": prec,
-                     "launches_per_step": rc["launches"] / steps,
-                     "avg_launch_us": rc["ms"] * 1e3 / rc["timed_launches"],
-                     "alg_flops_per_launch": rc["flops"] / rc["launches"],
-                     "alg_bytes_per_launch": rc["bytes"] / rc["launches"]})
-    n = args.n
+    dt, steps, n = h["s"], args.steps, args.n
+    alg = sum(v["flops"] for k, v in h["ops"].items() if not k.endswith("_inverses")) / steps
     out = {
         "metric": "BLR-LU solve time & TFLOP/s @N=32768, err<=1e-10: mixed-prec vs FP64 GPU",
         "value": dt,
@@ -245,17 +300,353 @@ def main():
         "berr": h["berr"],
         "refine_iters": h["iters"],
         "construction_s": h["build_s"],
-        "alg_tflops": alg_flops / dt / 1e12,
+        "alg_tflops": alg / dt / 1e12,
         "effective_tflops": (2.0 / 3.0) * n ** 3 / dt / 1e12,
-        "roofline": roof,
+        "roofline": roofline_of(h["ops"], args.roofline_class, steps, head),
+        "path_roofline": path_roofline(h["ops"], steps, dt),
         "cpu_baseline": cpu,
     }
     if "f64" in out_modes:
-        out["f64_s"] = out_modes["f64"]["s"]
-        out["f64_berr"] = out_modes["f64"]["berr"]
-        out["f64_alg_tflops"] = sum(v["flops"] for v in out_modes["f64"]["ops"].values()) / steps / out["f64_s"] / 1e12
+        f = out_modes["f64"]
+        out["f64_s"] = f["s"]
+        out["f64_berr"] = f["berr"]
+        out["f64_alg_tflops"] = sum(v["flops"] for k, v in f["ops"].items()
+                                    if not k.endswith("_inverses")) / steps / f["s"] / 1e12
+        out["f64_path_roofline"] = path_roofline(f["ops"], steps, f["s"])
     if "f64" in out_modes and "mixed" in out_modes:
         out["speedup_vs_f64"] = out_modes["f64"]["s"] / out_modes["mixed"]["s"]
+    return out
+
+
+# ---------------------------------------------------------------------------
+# C0: fixed-rank FP64 BLR-LU, N=4096
+# ---------------------------------------------------------------------------
+def run_c0(args, ctx):
+    import numpy as np
+    import torch
+
+    import frank_amd as fa
+    from frank_amd.inputs import sorted_uniform_points
+    from oracle import frank_oracle as O
+    N, nb, k = 4096, 16, 16
+    log(f"C0 N={N} nb={nb} rank={k}")
+    pts = [sorted_uniform_points(N)]
+    t0 = time.perf_counter()
+    A = fa.BlrMatrix.from_kernel(pts, nb, eps=0.0, rank=k, kcap=k, ctx=ctx)
+    ctx.synchronize()
+    tb = time.perf_counter() - t0
+    A.set_lra("fast_rounded_addition")
+    xs = torch.from_numpy(O.random_uniform(N, 1)).cuda()
+    rhs = torch.zeros_like(xs)
+    A.matvec(xs, rhs)  # b = A x with the compressed A, as test/blr_fixed_acc_test.cpp builds it
+    F = A.clone()
+    sol = torch.empty_like(rhs)
+
+    def step():
+        F.copy_from(A)
+        F.getrf()
+        sol.copy_(rhs)
+        F.solve(sol)
+        return "ok"
+
+    dt, ops, _ = timed_steps(ctx, step, max(args.steps, 5), 1, "C0")
+    err = float(torch.linalg.norm(sol - xs) / torch.linalg.norm(xs))
+    r = F.ranks()
+    H = oracle_hier(A, 0.0)
+    bh = rhs.cpu().numpy()
+
+    def cpu_run():
+        Lr, Ur = O.blr_getrf(H.copy(), lra="fast_rounded_addition")
+        return O.blr_trsm_upper(Ur, O.blr_trsm_lower(Lr, bh.copy()))
+
+    t_all, t_one, nt = cpu_timed(cpu_run)
+    err_cpu = O.l2_error(xs.cpu().numpy(), cpu_run())
+    steps = max(args.steps, 5)
+    return {"workload": f"BLR-LU FP64 factor + solve, 1-D Laplace, N={N}, {nb}x{nb} tiles of {N // nb}, fixed rank "
+                        f"{k}, fast rounded addition, 1 RHS", "metric": "time per factor + solve",
+            "value": dt, "unit": "s", "dtype": "f64", "steps": steps, "construction_s": tb,
+            "solve_l2_error": err, "oracle_solve_l2_error": err_cpu, "ranks_all_k": bool(np.all(r[r >= 0] == k)),
+            "path_roofline": path_roofline(ops, steps, dt),
+            "cpu_baseline": cpu_record(t_all, t_one, nt, 1.0, "s",
+                                       "oracle/frank_oracle.py blr_getrf + block substitution, the whole C0 "
+                                       "factor + solve on the same compressed matrix (downloaded from the GPU)")}
+
+
+# ---------------------------------------------------------------------------
+# C1: tiled Householder BLR-QR, N=8192
+# ---------------------------------------------------------------------------
+def run_c1(args, ctx_unused):
+    import numpy as np
+
+    import frank_amd as fa
+    from frank_amd.inputs import morton_sort, uniform_points
+    from oracle import frank_oracle as O
+    N, nb, eps, cap = 8192, 32, 1e-8, 64
+    log(f"C1 N={N} nb={nb} eps={eps} cap={cap}")
+    p, _ = morton_sort(uniform_points(N, 2), 6)
+    pts = [np.ascontiguousarray(q) for q in p]
+    ctx = fa.Context()
+    ctx.set_rank_cap(True)
+    t0 = time.perf_counter()
+    A = fa.BlrMatrix.from_kernel_rect(pts, N, N, nb, nb, eps=eps, kcap=cap, ctx=ctx)
+    ctx.synchronize()
+    tb = time.perf_counter() - t0
+    A.set_lra("rounded_addition")
+    r0 = A.ranks()
+    F = A.clone()
+    holder = []
+
+    def step():
+        F.copy_from(A)
+        holder[:] = [F.qr()]
+        return "ok"
+
+    steps = max(1, args.steps // 3)
+    dt, ops, _ = timed_steps(ctx, step, steps, 1, "C1")
+    r = F.ranks()
+    H = oracle_hier(A, eps)
+    rows = 6
+
+    def cpu_run():
+        Hc = H.copy()
+        k, Tt = 0, {}
+        Hc[k, k], Tt[k, k] = O.geqrt(Hc[k, k])
+        for j in range(1, nb):
+            Hc[k, j] = O.larfb(Hc[k, k], Tt[k, k], Hc[k, j], True)
+        for i in range(1, 1 + rows):
+            Hc[k, k], Hc[i, k], Tt[i, k] = O.tpqrt(Hc[k, k], Hc[i, k])
+            for j in range(1, nb):
+                Hc[k, j], Hc[i, j] = O.tpmqrt(Hc[i, k], Tt[i, k], Hc[k, j], Hc[i, j], True, "rounded_addition")
+
+    t_all, t_one, nt = cpu_timed(cpu_run)
+    # scale by tpmqrt calls (the dominant tile op): rows * (nb - 1) sampled of sum_k (nb-1-k)^2
+    frac = rows * (nb - 1) / sum((nb - 1 - k) ** 2 for k in range(nb))
+    return {"workload": f"tiled Householder BLR-QR FP64, 2-D Laplace on Morton-ordered points, N={N}, {nb}x{nb} "
+                        f"tiles of {N // nb}, eps={eps}, ranks capped at {cap}", "metric": "time per QR",
+            "value": dt, "unit": "s", "dtype": "f64", "steps": steps, "construction_s": tb,
+            "max_rank_before": int(r0.max()), "max_rank_after": int(r.max()), "mean_rank_after": float(r[r >= 0].mean()),
+            "path_roofline": path_roofline(ops, steps, dt),
+            "cpu_baseline": cpu_record(t_all / frac, t_one / frac, nt, 1.0, "s",
+                                       f"oracle/frank_oracle.py tiled QR: geqrt + larfb row and the first {rows} "
+                                       f"tpqrt/tpmqrt rows of step 0 on the same compressed matrix, scaled by the "
+                                       f"tpmqrt count ({frac:.4f} of the total)")}
+
+
+# ---------------------------------------------------------------------------
+# C2: 4096 batched FP32 rounded additions in one call
+# ---------------------------------------------------------------------------
+def c2_pairs(batch: int = 4096, seed: int = 2024):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes, ranks = [64, 128, 256, 512], [8, 16, 32, 64]
+
+    def factor(m, n, k):
+        return (rng.standard_normal((m, k)), np.diag(10.0 ** (-np.arange(k) / 4.0)), rng.standard_normal((k, n)))
+
+    out = []
+    for _ in range(batch):
+        m, n = int(rng.choice(sizes)), int(rng.choice(sizes))
+        k1, k2 = int(rng.choice(ranks)), int(rng.choice(ranks))
+        out.append((factor(m, n, k1), factor(m, n, k2)))
+    return out
+
+
+def run_c2(args, ctx):
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from frank_amd import _lib as L
+    from oracle import frank_oracle as O
+    batch, eps, cap = 4096, 1e-6, 128
+    log(f"C2 batch={batch} eps={eps}")
+    pairs = c2_pairs(batch)
+    # every A factor in one slab (U m x cap, S cap x cap, V cap x n, row-major), B factors in another
+    offs, tot, boffs, btot = [], 0, [], 0
+    for (a, bb) in pairs:
+        m, n = a[0].shape[0], a[2].shape[1]
+        offs.append((tot, tot + m * cap, tot + m * cap + cap * cap))
+        tot += m * cap + cap * cap + cap * n
+        k2 = bb[0].shape[1]
+        boffs.append((btot, btot + m * k2, btot + m * k2 + k2 * k2))
+        btot += m * k2 + k2 * k2 + k2 * n
+    hA = np.zeros(tot, dtype=np.float32)
+    hB = np.zeros(btot, dtype=np.float32)
+    for (a, bb), (ou, os_, ov), (pu, ps, pv) in zip(pairs, offs, boffs):
+        m, k1 = a[0].shape
+        n = a[2].shape[1]
+        k2 = bb[0].shape[1]
+        U = np.zeros((m, cap), np.float32)
+        U[:, :k1] = a[0]
+        S = np.zeros((cap, cap), np.float32)
+        S[:k1, :k1] = a[1]
+        V = np.zeros((cap, n), np.float32)
+        V[:k1] = a[2]
+        hA[ou:ou + m * cap] = U.ravel()
+        hA[os_:os_ + cap * cap] = S.ravel()
+        hA[ov:ov + cap * n] = V.ravel()
+        hB[pu:pu + m * k2] = bb[0].astype(np.float32).ravel()
+        hB[ps:ps + k2 * k2] = bb[1].astype(np.float32).ravel()
+        hB[pv:pv + k2 * n] = bb[2].astype(np.float32).ravel()
+    A0 = torch.from_numpy(hA).cuda()
+    Aw = A0.clone()
+    Bd = torch.from_numpy(hB).cuda()
+    torch.cuda.synchronize()
+    pa, pb = Aw.data_ptr(), Bd.data_ptr()
+    arrp = lambda base, xs: (C.c_void_p * batch)(*[base + 4 * x for x in xs])
+    Ua, Sa, Va = arrp(pa, [o[0] for o in offs]), arrp(pa, [o[1] for o in offs]), arrp(pa, [o[2] for o in offs])
+    Ub, Sb, Vb = arrp(pb, [o[0] for o in boffs]), arrp(pb, [o[1] for o in boffs]), arrp(pb, [o[2] for o in boffs])
+    m = np.array([a[0].shape[0] for a, _ in pairs], np.int32)
+    n = np.array([a[2].shape[1] for a, _ in pairs], np.int32)
+    k1 = np.array([a[0].shape[1] for a, _ in pairs], np.int32)
+    kb = np.array([bb[0].shape[1] for _, bb in pairs], np.int32)
+    ranks = k1.copy()
+    lib = L.lib()
+
+    def step():
+        Aw.copy_(A0)
+        torch.cuda.current_stream().synchronize()
+        ranks[:] = k1
+        L.check(lib.frk_slr_add_batched(ctx.h, L.LRA_ROUNDED, batch, m.ctypes.data, n.ctypes.data, eps, cap,
+                                        Ua, Sa, Va, ranks.ctypes.data, Ub, Sb, Vb, kb.ctypes.data))
+        return f"mean kept rank {ranks.mean():.2f}"
+
+    steps = max(args.steps, 5)
+    dt, ops, _ = timed_steps(ctx, step, steps, 1, "C2")
+    # rank agreement with the FP64 oracle on every 16th pair
+    sample = list(range(0, batch, 16))
+    diffs = np.array([ranks[t] - O.rounded_addition(O.LowRank(*pairs[t][0], eps), O.LowRank(*pairs[t][1], eps)).rank
+                      for t in sample])
+    cpu_pairs = [pairs[t] for t in range(0, batch, 64)]
+
+    def cpu_run():
+        for a, bb in cpu_pairs:
+            O.rounded_addition(O.LowRank(*a, eps), O.LowRank(*bb, eps))
+
+    t_all, t_one, nt = cpu_timed(cpu_run)
+    scale = batch / len(cpu_pairs)
+    alg = sum(v["flops"] for v in ops.values()) / steps
+    return {"workload": f"{batch} rounded additions (U1 S1 V1^T)+(U2 S2 V2^T), m,n in {{64,128,256,512}}, k1,k2 in "
+                        f"{{8,16,32,64}}, N(0,1) factors, sigma_i=10^(-i/4), eps={eps}, FP32, one batched call",
+            "metric": "time per batch", "value": dt, "unit": "s", "dtype": "f32", "steps": steps,
+            "additions_per_s": batch / dt, "alg_gflops": alg / dt / 1e9,
+            "rank_match_rate_vs_fp64_oracle": float(np.mean(diffs == 0)), "rank_max_abs_diff": int(np.abs(diffs).max()),
+            "rank_sample": len(sample), "path_roofline": path_roofline(ops, steps, dt),
+            "cpu_baseline": cpu_record(t_all * scale, t_one * scale, nt, 1.0, "s",
+                                       f"oracle/frank_oracle.py rounded_addition (FP64) on every 64th pair "
+                                       f"({len(cpu_pairs)} pairs), scaled to {batch}")}
+
+
+# ---------------------------------------------------------------------------
+# C4: differentiable gaussian-kernel solves (a subset of the batch)
+# ---------------------------------------------------------------------------
+def run_c4(args, ctx):
+    import numpy as np
+    import torch
+
+    from frank_amd.autograd import blr_solve_batched
+    from frank_amd.inputs import morton_sort, uniform_points
+    rng = np.random.default_rng(64)
+    Ns = [int(x) for x in args.c4_sizes.split(",")]
+    ells = rng.uniform(0.05, 0.3, size=len(Ns))
+    log(f"C4 N={Ns} ell={np.round(ells, 4).tolist()}")
+    pts = []
+    for s, N in enumerate(Ns):
+        p, _ = morton_sort(uniform_points(N, 3, seed=1000 + s), 4)
+        pts.append([np.ascontiguousarray(q) for q in p])
+    b_np = [rng.standard_normal((N, 1)) for N in Ns]
+
+    def step():
+        bs = [torch.from_numpy(b).cuda().requires_grad_(True) for b in b_np]
+        ell_t = torch.tensor(ells, dtype=torch.float64, requires_grad=True)
+        us = blr_solve_batched(bs, ell_t, pts, [N // 64 for N in Ns], eps=1e-12, kcap=64, workers=2)
+        loss = sum(0.5 * (u * u).sum() for u in us)
+        loss.backward()
+        return ell_t.grad.detach().cpu().numpy()
+
+    steps = 1
+    dt, ops, res = timed_steps(ctx, step, steps, 0, "C4")
+    grads = res[-1]
+    # dense FP64 gradient of the smallest system on the host (the CPU baseline's own computation)
+    s0 = int(np.argmin(Ns))
+
+    def dense_grad():
+        p = pts[s0]
+        d2 = sum((q[:, None] - q[None, :]) ** 2 for q in p)
+        G = np.exp(-d2 / (2.0 * ells[s0] ** 2))
+        A = G + 1e-4 * np.eye(Ns[s0])
+        u = np.linalg.solve(A, b_np[s0])
+        lam = np.linalg.solve(A.T, u)
+        return -float((lam * ((G * d2 / ells[s0] ** 3) @ u)).sum())
+
+    g_ref = dense_grad()
+    t_all, t_one, nt = cpu_timed(dense_grad)
+    cpu_total = sum((N / Ns[s0]) ** 3 for N in Ns)  # dense LU work scales as N^3
+    return {"workload": f"differentiable BLR solves, gaussian kernel + 1e-4 I, x ~ U[0,1]^3, N={Ns}, "
+                        f"ell ~ U[0.05,0.3], 64-point tiles, eps=1e-12, loss = sum 1/2||A^-1 b||^2, forward + adjoint",
+            "metric": "time per batch (build + factor + solve + adjoint)", "value": dt, "unit": "s", "dtype": "f64",
+            "systems": len(Ns), "dL_dell": grads.tolist(),
+            "dL_dell_rel_err_vs_dense_fp64_smallest": abs(grads[s0] - g_ref) / abs(g_ref),
+            "cpu_baseline": cpu_record(t_all * cpu_total, t_one * cpu_total, nt, 1.0, "s",
+                                       f"dense FP64 LAPACK gradient (numpy solve, two solves + dA/dell contraction) "
+                                       f"of the N={Ns[s0]} system, scaled by N^3 to the batch")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["c3", "c0", "c1", "c2", "c4"], default="c3")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="with --workload c3: skip the other workloads' entries under 'workloads'")
+    ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--nb", type=int, default=64)
+    ap.add_argument("--eps", type=float, default=1e-10)
+    ap.add_argument("--nrhs", type=int, default=8)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--mode", choices=["both", "f64", "mixed"], default="both")
+    ap.add_argument("--eps-mixed", type=float, default=1e-6,
+                    help="compression / recompression accuracy of the FP32-storage factorization")
+    ap.add_argument("--roofline-class", default="getrf_tile",
+                    help="kernel class whose launches are event-timed for the roofline object")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="elimination steps in the C3 CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--c4-sizes", default="2048,4096,2048,8192",
+                    help="system sizes of the C4 subset (comma separated)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 256 MiB device copy first (calibrates HBM byte counters under a profiler)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(the sharding design is described in the documentation)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import frank_amd as fa
+    if args.copy_probe:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    ctx = fa.default_context()
+    runners = {"c0": run_c0, "c1": run_c1, "c2": run_c2, "c4": run_c4}
+    if args.workload != "c3":
+        r = runners[args.workload](args, ctx)
+        r.update({"n_gpus": 1, "higher_is_better": False, "data": "synthetic"})
+        print(json.dumps(r), flush=True)
+        return
+    out = run_c3(args, ctx)
+    if not args.no_secondary:
+        sec = {}
+        for k, fn in runners.items():
+            t0 = time.perf_counter()
+            sec[k] = fn(args, ctx)
+            log(f"{k} done in {time.perf_counter() - t0:.1f}s: {sec[k]['value'] * 1e3:.2f} ms per step")
+        out["workloads"] = sec
     print(json.dumps(out), flush=True)
 
 

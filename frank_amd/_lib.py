@@ -64,6 +64,7 @@ _SIGS = {
     "frk_ctx_time_class": ([vp, C.c_char_p], C.c_int),
     "frk_ctx_set_rank_cap": ([vp, C.c_int], C.c_int),
     "frk_ctx_op_classes": ([vp, C.c_char_p, C.c_int], C.c_int),
+    "frk_ctx_op_roofline": ([vp, C.c_char_p, vp], C.c_int),
     "frk_ctx_op_stats": ([vp, C.c_char_p, pi64, pdbl, pdbl, pdbl, pi64], C.c_int),
     "frk_ctx_reset_op_stats": ([vp], C.c_int),
     "frk_blr_clone": ([vp, C.POINTER(vp)], C.c_int),

@@ -20,12 +20,18 @@ _PREC = {"f64": L.PREC_F64, "f32": L.PREC_F32}
 _LRA = {"naive": L.LRA_NAIVE, "rounded_addition": L.LRA_ROUNDED, "fast_rounded_addition": L.LRA_FAST}
 
 
-def _dptr(t) -> int:
-    """Device pointer of a contiguous torch tensor on the GPU."""
+def _dptr(t, sync: bool = True) -> int:
+    """Device pointer of a contiguous torch tensor on the GPU.  The library
+    works on its own non-blocking streams, so the work torch has queued on the
+    tensor's current stream is drained first (every entry point drains the
+    library's stream before it returns, so results are ready for torch)."""
     if not t.is_cuda:
         raise ValueError("expected a GPU tensor")
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
+    if sync:
+        import torch
+        torch.cuda.current_stream(t.device).synchronize()
     return t.data_ptr()
 
 
@@ -61,7 +67,8 @@ class Context:
         L.check(L.lib().frk_ctx_time_class(self.h, cls.encode() if cls else None))
 
     def op_stats(self) -> dict:
-        """Op log per kernel class: launches, algorithmic flops/bytes, event-timed ms."""
+        """Op log per kernel class: launches, algorithmic flops/bytes, event-timed ms,
+        roofline ms (sum over launches of max(flops / peak, bytes / HBM bandwidth))."""
         buf = C.create_string_buffer(4096)
         L.check(L.lib().frk_ctx_op_classes(self.h, buf, 4096))
         out = {}
@@ -70,8 +77,10 @@ class Context:
             f, b, ms = C.c_double(), C.c_double(), C.c_double()
             L.check(L.lib().frk_ctx_op_stats(self.h, cls.encode(), C.byref(n), C.byref(f), C.byref(b), C.byref(ms),
                                              C.byref(t)))
+            roof = C.c_double()
+            L.check(L.lib().frk_ctx_op_roofline(self.h, cls.encode(), C.byref(roof)))
             out[cls] = {"launches": n.value, "flops": f.value, "bytes": b.value, "ms": ms.value,
-                        "timed_launches": t.value}
+                        "timed_launches": t.value, "roofline_ms": roof.value}
         return out
 
     def set_rank_cap(self, truncate: bool):
@@ -380,8 +389,10 @@ def lr_add_batched(As: list, Bs: list, *, eps: float = 0.0, lra: str = "fast_rou
     kb = np.array([b[1].shape[0] for b in Bs], dtype=np.int32)
     cap = cap or As[0][0].shape[1]
 
+    torch.cuda.current_stream().synchronize()
+
     def arr(xs):
-        return (C.c_void_p * nb)(*[_dptr(x) for x in xs])
+        return (C.c_void_p * nb)(*[_dptr(x, sync=False) for x in xs])
 
     fn = L.lib().frk_slr_add_batched if As[0][0].dtype == torch.float32 else L.lib().frk_dlr_add_batched
     L.check(fn(ctx.h, _LRA[lra], nb, _hptr(m), _hptr(n), eps, cap,

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/frank_amd.h"
@@ -299,6 +300,16 @@ int frk_ctx_op_stats(frk_ctx ctx, const char* cls, int64_t* launches, double* fl
   });
 }
 
+int frk_ctx_op_roofline(frk_ctx ctx, const char* cls, double* roofline_ms) {
+  return guard([&] {
+    if (!ctx || !cls || !roofline_ms) throw std::invalid_argument("null argument");
+    const auto& st = ctx->ctx->op_stats();
+    const auto it = st.find(cls);
+    *roofline_ms = (it == st.end()) ? 0.0 : 1e3 * it->second.roof_s;
+    return FRK_OK;
+  });
+}
+
 int frk_ctx_reset_op_stats(frk_ctx ctx) {
   return guard([&] {
     if (!ctx) throw std::invalid_argument("ctx is null");
@@ -442,10 +453,17 @@ int frk_blr_copy(frk_blr dst, frk_blr src) {
   return guard([&] {
     if (!dst || !src) throw std::invalid_argument("null argument");
     if ((dst->a64 != nullptr) != (src->a64 != nullptr)) throw std::invalid_argument("precisions differ");
+    Context& c = *dst->owner->ctx;
+    auto run = [&](auto& d, auto& s) {
+      d.copy_from(s, dst->owner->stream);
+      // op log: one read and one write of the tile arena and the diagonal inverses
+      using E = std::remove_reference_t<decltype(*d.dinv_data())>;
+      c.account("restore_copy", 0.0, 2.0 * ((double)d.arena_bytes() + sizeof(E) * (double)d.dinv_count()));
+    };
     if (dst->a64)
-      dst->a64->copy_from(*src->a64, dst->owner->stream);
+      run(*dst->a64, *src->a64);
     else
-      dst->a32->copy_from(*src->a32, dst->owner->stream);
+      run(*dst->a32, *src->a32);
     return FRK_OK;
   });
 }

@@ -182,11 +182,12 @@ void Context::sync() {
   resolve_events();
 }
 
-void Context::account(const char* cls, double flops, double bytes) {
+void Context::account(const char* cls, double flops, double bytes, double peak) {
   OpStat& st = owner_->stats_[cls];
   st.launches += 1;
   st.flops += flops;
   st.bytes += bytes;
+  st.roof_s += std::max(flops / (peak > 0 ? peak : owner_->peak_), bytes / kHbmBytesPerS);
 }
 
 void Context::host_time(const char* cls, double ms) {
@@ -420,7 +421,7 @@ void Plan::run() {
     if (timed) ctx_.event_begin();
     launches_[i](d);
     if (timed) ctx_.event_end();
-    if (m.cls) ctx_.account(m.cls, m.flops, m.bytes);
+    if (m.cls) ctx_.account(m.cls, m.flops, m.bytes, m.peak);
   }
   launches_.clear();
   meta_.clear();
@@ -546,9 +547,14 @@ static void plan_tile_lu(Plan& pl, double* a, double* linv, double* uinv, int n,
   it.epoch = c.next_tile_epoch();
   it.spin_limit = int64_t(1) << 22;
   const hipStream_t s = c.stream();
-  // algorithmic counts: LU 2/3 n^3 + two triangular inverses n^3/3 each; read A, write LU and both inverses
+  // algorithmic counts of getrf (LAWN 41): 2/3 n^3 flops, read A and write L\U
+  // at the storage precision.  The two triangular inverses the kernel also forms
+  // (n^3/3 flops each, two more n x n results) are logged apart, under
+  // "getrf_tile_inverses", with no launch of their own.
   const double n3 = (double)n * n * n;
-  pl.launch([=](const char*) { launch_tile_lu(it, s); }, "getrf_tile", (4.0 / 3.0) * n3, 4.0 * sizeof(double) * n * n);
+  const double sb = f32.a ? sizeof(float) : sizeof(double);
+  pl.launch([=](const char*) { launch_tile_lu(it, s); }, "getrf_tile", (2.0 / 3.0) * n3, 2.0 * sb * n * n, kPeakF64);
+  c.account("getrf_tile_inverses", (2.0 / 3.0) * n3, 2.0 * sb * n * n, kPeakF64);
 }
 
 template <typename T>
@@ -1526,6 +1532,7 @@ constexpr int kRestChunks = 2;
 template <typename T>
 void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
   const hipStream_t s = ctx.stream();
+  ctx.set_default_peak(mfma_peak<T>());
   const int nb = A.nb_r();
   // a partial factorization stops after max_steps elimination steps (the
   // diagonal tile of step max_steps is factorized as well: its LU is queued
@@ -2168,6 +2175,7 @@ static void blr_solve_fast(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context&
 template <typename T>
 void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx, int part) {
   const int nb = F.nb_r();
+  ctx.set_default_peak(mfma_peak<T>());
   ctx.sync();
   F.pull_ranks(ctx.stream());
   if (solve_fast_ok(F, nrhs)) {
@@ -2326,9 +2334,10 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
                         double tol, int max_iter, Context& ctx) {
   const int64_t n = F.roff.back();
   const hipStream_t s = ctx.stream();
-  DevBuf rbuf(sizeof(double) * n * nrhs), dbuf(sizeof(T) * n * nrhs);
-  double* r = rbuf.as<double>();
-  T* d = dbuf.as<T>();
+  ctx.set_default_peak(mfma_peak<T>());
+  // residual and correction vectors persist in the context between calls
+  double* r = static_cast<double*>(ctx.persistent(0, sizeof(double) * n * nrhs));
+  T* d = static_cast<T*>(ctx.persistent(1, sizeof(T) * n * nrhs));
   ctx.sync();
   ctx.reserve(sizeof(double) * kColmaxBlocks * nrhs * 4 + 4096);
   const std::vector<double> bn = colmax(b, n, nrhs, ctx);
@@ -2339,6 +2348,10 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
   RefineResult res{0, 0.0};
   for (int it = 0;; ++it) {
     launch_kernel_residual(ks, x, b, r, n, nrhs, s);
+    // op log: the product with the exact matrix (2 n^2 nrhs) plus the kernel
+    // evaluation (difference, absolute value, shift, quotient: 4 n^2), FP64; the
+    // matrix is generated on the fly, so only x, b and r cross HBM
+    ctx.account("residual", 2.0 * n * n * nrhs + 4.0 * n * n, 3.0 * sizeof(double) * n * nrhs, kPeakF64);
     ctx.sync();
     ctx.reserve(sizeof(double) * kColmaxBlocks * nrhs * 4 + 4096);
     const std::vector<double> rn = colmax(r, n, nrhs, ctx);

@@ -78,7 +78,16 @@ struct OpStat {
   int64_t launches = 0;
   double flops = 0, bytes = 0, ms = 0;
   int64_t timed = 0;
+  double roof_s = 0;  // sum over launches of max(flops / peak, bytes / HBM bandwidth)
 };
+
+// MI355X published peaks used by the op log's roofline time (dense MFMA; the
+// FP64 and FP32 vector rates equal the matrix rates on this part).
+constexpr double kPeakF64 = 78.6e12, kPeakF32 = 157.3e12, kPeakBf16 = 2.5e15, kHbmBytesPerS = 8.0e12;
+template <typename T>
+constexpr double mfma_peak() {
+  return sizeof(T) == 8 ? kPeakF64 : (sizeof(T) == 4 ? kPeakF32 : kPeakBf16);
+}
 
 class Context {
  public:
@@ -122,7 +131,20 @@ class Context {
   // 1 = second chunk of a BLR-LU step.
   hipStream_t aux_stream(int k);
   // op log
-  void account(const char* cls, double flops, double bytes);
+  void account(const char* cls, double flops, double bytes, double peak = 0);
+  // peak arithmetic rate of launches that name none (the precision the caller computes in)
+  void set_default_peak(double p) { owner_->peak_ = p; }
+  // Device buffers kept between calls (slot < 4), grown on demand: a call that
+  // reuses a size allocates nothing.
+  void* persistent(int slot, size_t bytes) {
+    DevBuf& b = persist_[slot];
+    if (b.bytes() < bytes) {
+      host_wait_stream(s_, "persistent buffer growth");
+      b.ensure(bytes);
+    }
+    return b.as<void>();
+  }
+  double default_peak() const { return owner_->peak_; }
   // host-side wall time attributed to a class of the op log (counted as timed launches)
   void host_time(const char* cls, double ms);
   void time_class(const std::string& cls) { timed_cls_ = cls; }
@@ -156,6 +178,8 @@ class Context {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending_;
   hipEvent_t ev_open_ = nullptr;
   Context* owner_ = this;
+  double peak_ = kPeakF64;
+  DevBuf persist_[4];
   hipStream_t aux_[2] = {nullptr, nullptr};
   void resolve_events();
 };
@@ -177,9 +201,11 @@ class Plan {
     return off;
   }
   // cls (a string literal) names the kernel class for the op log
-  void launch(std::function<void(const char*)> f, const char* cls = nullptr, double flops = 0, double bytes = 0) {
+  // peak: arithmetic rate the launch's flops run at (0: the context's default)
+  void launch(std::function<void(const char*)> f, const char* cls = nullptr, double flops = 0, double bytes = 0,
+              double peak = 0) {
     launches_.push_back(std::move(f));
-    meta_.push_back({cls, flops, bytes});
+    meta_.push_back({cls, flops, bytes, peak});
   }
   void run();
   bool empty() const { return launches_.empty(); }
@@ -191,7 +217,7 @@ class Plan {
   std::vector<std::function<void(const char*)>> launches_;
   struct Meta {
     const char* cls;
-    double flops, bytes;
+    double flops, bytes, peak;
   };
   std::vector<Meta> meta_;
 };

@@ -1764,8 +1764,13 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   if (tid == 0) {
     T fro2 = T(0);
     for (int j = 0; j < K; ++j) fro2 += sig[j];
-    // columns below eps * ||M||_F are numerically zero: rotating them only churns rounding noise
-    tiny_sh = Eps<T>::v * Eps<T>::v * fro2;
+    // columns below u ||M||_F (u the unit roundoff) are numerically zero: rotating
+    // them only churns rounding noise.  The floor also stays a factor 16 under the
+    // truncation threshold eps ||sigma|| / sqrt(K) of an eps-rank decision, so in
+    // FP32 at eps near 1e-6 the columns the rank is decided on are still rotated.
+    T fl = Eps<T>::v;
+    if (it.fixed_rank < 0 && it.eps > 0.0) fl = fmin(fl, static_cast<T>(it.eps / (16.0 * sqrt((double)K))));
+    tiny_sh = fl * fl * fro2;
   }
   __syncthreads();
   const T tiny = tiny_sh;

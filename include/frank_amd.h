@@ -91,6 +91,11 @@ int frk_ctx_set_rank_cap(frk_ctx ctx, int truncate);
 int frk_ctx_op_classes(frk_ctx ctx, char* buf, int len);
 int frk_ctx_op_stats(frk_ctx ctx, const char* cls, int64_t* launches, double* flops, double* bytes, double* ms,
                      int64_t* timed_launches);
+/* Roofline time of one op-log class (no reference counterpart): the sum over
+ * its launches of max(flops / peak, bytes / 8 TB/s), with MI355X's published
+ * peak of the precision each launch computes in (FP64 78.6, FP32 157.3 TFLOP/s),
+ * in milliseconds. */
+int frk_ctx_op_roofline(frk_ctx ctx, const char* cls, double* roofline_ms);
 int frk_ctx_reset_op_stats(frk_ctx ctx);
 
 /* ---------------------------------------------------------------------------
