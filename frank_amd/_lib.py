@@ -85,6 +85,7 @@ _SIGS = {
     "frk_blr_matvec": ([vp, vp, vp, C.c_int], C.c_int),
     "frk_blr_kernel_residual": ([vp, vp, vp, vp, C.c_int], C.c_int),
     "frk_dgetrf_nopiv_batched": ([vp, vp, C.c_int, C.c_int, i64, vp], C.c_int),
+    "frk_dgetrf_nopiv_batched_noalloc": ([vp, vp, C.c_int, C.c_int, i64, vp], C.c_int),
     "frk_dtrsm": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, vp, i64, vp, i64], C.c_int),
     "frk_dgemm": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dbl, vp, i64, vp, i64, dbl, vp, i64], C.c_int),
     "frk_dlr_add_batched": ([vp, C.c_int, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp],

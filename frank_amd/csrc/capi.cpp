@@ -711,6 +711,19 @@ int frk_dgetrf_nopiv_batched(frk_ctx ctx, double* const* tiles_host, int batch, 
   });
 }
 
+int frk_dgetrf_nopiv_batched_noalloc(frk_ctx ctx, double* const* tiles_host, int batch, int n, int64_t ld,
+                                     int32_t* info_host) {
+  return guard([&] {
+    if (!ctx || !tiles_host || batch <= 0 || n <= 0 || ld < n) throw std::invalid_argument("bad argument");
+    for (int b = 0; b < batch; ++b)
+      if (!tiles_host[b]) throw std::invalid_argument("bad argument: null tile pointer");
+    std::string msg;
+    const int st = getrf_nopiv_noalloc(ctx->stream, tiles_host, batch, n, ld, info_host, msg);
+    if (st != FRK_OK) g_err = msg;
+    return st;
+  });
+}
+
 int frk_dtrsm(frk_ctx ctx, int side, int lower, int trans, int m, int n, const double* A, int64_t lda, double* B,
               int64_t ldb) {
   return guard([&] {

@@ -317,6 +317,10 @@ void plan_getrf(Plan& pl, const std::vector<T*>& tiles, const std::vector<T*>& l
                 int n, int64_t ld, int32_t* info_dev);
 template <typename T>
 void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_dev, Context& ctx);
+// Allocation-free batched tile LU (noalloc.hip): scratch in a file-scope device
+// array; status as frk_dgetrf_nopiv_batched, message in msg.
+int getrf_nopiv_noalloc(hipStream_t s, double* const* tiles, int batch, int n, int64_t ld, int32_t* info_host,
+                        std::string& msg);
 // Triangular solve T X = B (left, in place) for a triangle without precomputed
 // block inverses (computes them first).  Drains the stream.
 template <typename T>

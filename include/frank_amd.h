@@ -203,6 +203,17 @@ int frk_blr_refine(frk_blr f, const double* b_dev, double* x_dev, int nrhs, doub
 int frk_dgetrf_nopiv_batched(frk_ctx ctx, double* const* tiles_host, int batch, int n, int64_t ld,
                              int32_t* info_host);
 
+/* The batched no-pivot tile LU without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops).  Same operation, status and info as
+ * frk_dgetrf_nopiv_batched; the tiles are factorized one after another with
+ * the library's fixed per-GPU scratch (the inverses of both factors, the block
+ * flags and the info words: about 4 MiB), so n <= 512 and batch <= 1024
+ * (FRK_ERR_CAPACITY beyond).  The scratch keeps the last call's contents and
+ * is shared by every call on one GPU: calls that overlap on one GPU race. */
+int frk_dgetrf_nopiv_batched_noalloc(frk_ctx ctx, double* const* tiles_host, int batch, int n, int64_t ld,
+                                     int32_t* info_host);
+
 /* trsm_omm(Dense, Dense) (src/operations/BLAS/trsm.cpp:92-104): uplo lower =>
  * unit diagonal, upper => non-unit.  side 0 = left (A X = B), 1 = right (X A = B).
  * trans != 0 uses op(A) = A^T (adjoint sweeps; no reference counterpart). */
