@@ -613,7 +613,7 @@ def main():
                     help="kernel class whose launches are event-timed for the roofline object")
     ap.add_argument("--cpu-steps", type=int, default=3, help="elimination steps in the C3 CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-reps", type=int, default=3)
-    ap.add_argument("--c4-sizes", default="2048,4096,2048,8192",
+    ap.add_argument("--c4-sizes", default="2048,4096,2048,4096",
                     help="system sizes of the C4 subset (comma separated)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibrates HBM byte counters under a profiler)")

@@ -92,6 +92,11 @@ _SIGS = {
                             C.c_int),
     "frk_slr_add_batched": ([vp, C.c_int, C.c_int, vp, vp, dbl, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp],
                             C.c_int),
+    "frk_dgemm_tiles_batched": ([vp, C.c_int, vp, vp, vp, dbl, dbl, C.c_int, C.c_int, C.c_int], C.c_int),
+    "frk_dgeqrt_batched": ([vp, C.c_int, C.c_int, vp, i64, vp, i64], C.c_int),
+    "frk_dlarfb_batched": ([vp, C.c_int, C.c_int, vp, i64, vp, i64, vp, C.c_int], C.c_int),
+    "frk_dtpqrt_batched": ([vp, C.c_int, C.c_int, vp, i64, vp, vp, i64], C.c_int),
+    "frk_dtpmqrt_batched": ([vp, C.c_int, C.c_int, vp, vp, i64, vp, vp, C.c_int, C.c_int], C.c_int),
     "frk_dqr_batched": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
     "frk_drq_batched": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),
     "frk_dsvd_batched": ([vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], C.c_int),

@@ -858,6 +858,117 @@ int frk_slr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const
   });
 }
 
+namespace {
+RawTile<double> raw_of(const frk_dtile& t) {
+  RawTile<double> r;
+  if (t.kind != FRK_TILE_DENSE && t.kind != FRK_TILE_LOWRANK) throw std::invalid_argument("tile kind must be dense or low-rank");
+  if (t.rows <= 0 || t.cols <= 0) throw std::invalid_argument("tile dimensions must be positive");
+  r.kind = t.kind == FRK_TILE_DENSE ? kDense : kLowRank;
+  r.rows = t.rows;
+  r.cols = t.cols;
+  if (r.kind == kDense) {
+    if (!t.d || t.ld < t.cols) throw std::invalid_argument("dense tile: null pointer or ld < cols");
+    r.d = t.d;
+    r.ld = t.ld;
+  } else {
+    if (!t.u || !t.s || !t.v) throw std::invalid_argument("low-rank tile: null factor pointer");
+    if (t.cap <= 0 || t.rank < 0 || t.rank > t.cap) throw std::invalid_argument("low-rank tile: need 0 <= rank <= cap");
+    r.u = t.u;
+    r.u_rs = t.cap;  // U rows x cap row-major, as frk_dlr_add_batched takes it
+    r.u_cs = 1;
+    r.s = t.s;
+    r.v = t.v;
+    r.rank = t.rank;
+    r.cap = t.cap;
+    r.eps = t.eps;
+  }
+  return r;
+}
+std::vector<RawTile<double>> raws(const frk_dtile* t, int n) {
+  if (!t) throw std::invalid_argument("null tile array");
+  std::vector<RawTile<double>> v;
+  for (int q = 0; q < n; ++q) v.push_back(raw_of(t[q]));
+  return v;
+}
+void ranks_back(frk_dtile* t, const std::vector<RawTile<double>>& r) {
+  for (size_t q = 0; q < r.size(); ++q)
+    if (r[q].kind == kLowRank) t[q].rank = r[q].rank;
+}
+int check_lra(int lra) {
+  if (lra != FRK_LRA_ROUNDED && lra != FRK_LRA_FAST) throw std::invalid_argument("lra must be rounded or fast");
+  return lra == FRK_LRA_ROUNDED ? kLraRounded : kLraFast;
+}
+}  // namespace
+
+int frk_dgemm_tiles_batched(frk_ctx ctx, int batch, const frk_dtile* A, const frk_dtile* B, frk_dtile* C,
+                            double alpha, double beta, int trans_a, int trans_b, int lra) {
+  return guard([&] {
+    if (!ctx || batch <= 0) throw std::invalid_argument("bad argument");
+    const int l = check_lra(lra);
+    auto a = raws(A, batch), b = raws(B, batch), c = raws(C, batch);
+    for (int q = 0; q < batch; ++q) {
+      const int m = trans_a ? a[q].cols : a[q].rows, k = trans_a ? a[q].rows : a[q].cols;
+      const int kb = trans_b ? b[q].cols : b[q].rows, n = trans_b ? b[q].rows : b[q].cols;
+      if (k != kb || m != c[q].rows || n != c[q].cols) throw std::invalid_argument("gemm: shape mismatch in item " + std::to_string(q));
+    }
+    tiles_gemm<double>(a, b, c, alpha, beta, trans_a != 0, trans_b != 0, l, *ctx->ctx);
+    ranks_back(C, c);
+    return FRK_OK;
+  });
+}
+
+int frk_dgeqrt_batched(frk_ctx ctx, int batch, int n, double* const* A, int64_t lda, double* const* T, int64_t ldt) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || n <= 0 || n > 512 || !A || !T || lda < n || ldt < n) throw std::invalid_argument("bad argument");
+    std::vector<double*> a(A, A + batch), t(T, T + batch);
+    tiles_geqrt<double>(a, lda, t, ldt, n, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_dlarfb_batched(frk_ctx ctx, int batch, int n, const double* const* V, int64_t ldv, const double* const* T,
+                       int64_t ldt, frk_dtile* C, int trans) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || n <= 0 || n > 512 || !V || !T || ldv < n || ldt < n) throw std::invalid_argument("bad argument");
+    auto c = raws(C, batch);
+    for (int q = 0; q < batch; ++q)
+      if (c[q].rows != n) throw std::invalid_argument("larfb: C must have n rows");
+    std::vector<const double*> v(V, V + batch), t(T, T + batch);
+    tiles_larfb<double>(v, ldv, t, ldt, c, n, trans != 0, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_dtpqrt_batched(frk_ctx ctx, int batch, int n, double* const* A, int64_t lda, frk_dtile* B,
+                       double* const* T, int64_t ldt) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || n <= 0 || n > 512 || !A || !T || lda < n || ldt < n) throw std::invalid_argument("bad argument");
+    auto b = raws(B, batch);
+    for (int q = 0; q < batch; ++q)
+      if (b[q].cols != n) throw std::invalid_argument("tpqrt: B must have n columns");
+    std::vector<double*> a(A, A + batch), t(T, T + batch);
+    tiles_tpqrt<double>(a, lda, b, t, ldt, n, *ctx->ctx);
+    return FRK_OK;
+  });
+}
+
+int frk_dtpmqrt_batched(frk_ctx ctx, int batch, int n, const frk_dtile* V, const double* const* T, int64_t ldt,
+                        frk_dtile* A, frk_dtile* B, int trans, int lra) {
+  return guard([&] {
+    if (!ctx || batch <= 0 || n <= 0 || n > 512 || !T || ldt < n) throw std::invalid_argument("bad argument");
+    const int l = check_lra(lra);
+    auto v = raws(V, batch), a = raws(A, batch), b = raws(B, batch);
+    for (int q = 0; q < batch; ++q)
+      if (v[q].cols != n || a[q].rows != n || v[q].rows != b[q].rows || a[q].cols != b[q].cols)
+        throw std::invalid_argument("tpmqrt: shape mismatch in item " + std::to_string(q));
+    std::vector<const double*> t(T, T + batch);
+    tiles_tpmqrt<double>(v, t, ldt, a, b, n, trans != 0, l, *ctx->ctx);
+    ranks_back(A, a);
+    ranks_back(B, b);
+    return FRK_OK;
+  });
+}
+
 int frk_dqr_batched(frk_ctx ctx, int batch, const int32_t* m, const int32_t* n, const double* const* A,
                     const int64_t* lda, double* const* Q, const int64_t* ldq, double* const* R, const int64_t* ldr) {
   return guard([&] {

@@ -377,6 +377,7 @@ template <typename T>
 std::unique_ptr<BlrMatrix<T>> BlrMatrix<T>::clone_structure(Context& ctx) const {
   auto c = std::make_unique<BlrMatrix<T>>(roff, coff, kind, kcap, eps, fixed_rank, ctx);
   c->lra = lra;
+  c->cap_trunc = cap_trunc;
   return c;
 }
 
@@ -2290,6 +2291,7 @@ template <typename TD, typename TS>
 std::unique_ptr<BlrMatrix<TD>> convert_blr(const BlrMatrix<TS>& src, Context& ctx) {
   auto dst = std::make_unique<BlrMatrix<TD>>(src.roff, src.coff, src.kind, src.kcap, src.eps, src.fixed_rank, ctx);
   dst->lra = src.lra;
+  dst->cap_trunc = src.cap_trunc;
   const hipStream_t s = ctx.stream();
   launch_convert<TD, TS>(dst->arena_data(), src.arena_data(), (int64_t)(src.arena_bytes() / sizeof(TS)), s);
   if (src.dinv_count() && dst->dinv_count() == src.dinv_count())

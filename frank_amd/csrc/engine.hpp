@@ -404,10 +404,11 @@ struct Target {  // dense view, or low-rank slots in the BlrMatrix layout
   int m, n;
   T* d;
   int64_t d_rs, d_cs;
-  T* u;  // m x cap column-major
+  T* u;  // m x cap, column-major unless u_rs / u_cs say otherwise
   T* s;  // cap x cap row-major
   T* v;  // cap x n row-major
   int cap, rank;
+  int64_t u_rs = 1, u_cs = 0;  // row / column strides of U (u_cs = 0: m, column-major)
   int32_t* d_rank;
   double eps;
   int fixed_rank;
@@ -525,6 +526,37 @@ struct QrFactors {
 
 // tiled_householder_blr_qr(A, T): in place, R in the upper tiles, reflectors
 // in the diagonal tiles' lower parts and in the tiles below the diagonal.
+// A caller-owned tile of the tile-level operators: Dense (rows x cols, row
+// stride ld) or LowRank (U rows x cap with strides u_rs / u_cs, S cap x cap and
+// V cap x cols row-major), with its rank and accuracy (0: fixed rank).
+template <typename T>
+struct RawTile {
+  int kind = kDense;
+  int rows = 0, cols = 0;
+  int rank = 0, cap = 0;
+  double eps = 0.0;
+  T* d = nullptr;
+  int64_t ld = 0;
+  T* u = nullptr;
+  T* s = nullptr;
+  T* v = nullptr;
+  int64_t u_rs = 1, u_cs = 0;  // U strides (u_cs = 0: rows, column-major)
+};
+template <typename T>
+void tiles_geqrt(const std::vector<T*>& A, int64_t lda, const std::vector<T*>& Tout, int64_t ldt, int n, Context& ctx);
+template <typename T>
+void tiles_larfb(const std::vector<const T*>& V, int64_t ldv, const std::vector<const T*>& Tin, int64_t ldt,
+                 std::vector<RawTile<T>>& C, int n, bool trans, Context& ctx);
+template <typename T>
+void tiles_tpqrt(const std::vector<T*>& R, int64_t ldr, std::vector<RawTile<T>>& B, const std::vector<T*>& Tout,
+                 int64_t ldt, int n, Context& ctx);
+template <typename T>
+void tiles_tpmqrt(const std::vector<RawTile<T>>& V, const std::vector<const T*>& Tin, int64_t ldt,
+                  std::vector<RawTile<T>>& A, std::vector<RawTile<T>>& B, int n, bool trans, int lra, Context& ctx);
+template <typename T>
+void tiles_gemm(const std::vector<RawTile<T>>& A, const std::vector<RawTile<T>>& B, std::vector<RawTile<T>>& C,
+                double alpha, double beta, bool ta, bool tb, int lra, Context& ctx);
+
 template <typename T>
 void blr_tiled_qr(BlrMatrix<T>& A, QrFactors<T>& F, Context& ctx);
 // left_multiply_tiled_reflector(Y, T, C, trans): C <- Q C (trans = false) or Q^T C.

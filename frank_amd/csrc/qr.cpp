@@ -118,8 +118,8 @@ Op<T> target_op(const Target<T>& c) {
   } else {
     o.r = c.rank;
     o.u = c.u;
-    o.u_rs = 1;
-    o.u_cs = c.m;
+    o.u_rs = c.u_rs;
+    o.u_cs = c.u_cs ? c.u_cs : c.m;
     o.s = c.s;
     o.s_rs = c.cap;
     o.s_cs = 1;
@@ -193,8 +193,8 @@ void typed_gemm(TypedBatch<T>& b, Context& ctx, const Op<T>& A, const Op<T>& B, 
   ad.cv = C.v;
   ad.d_rank = C.d_rank;
   ad.kc = C.rank;
-  ad.cu_rs = 1;
-  ad.cu_cs = m;
+  ad.cu_rs = C.u_rs;
+  ad.cu_cs = C.u_cs ? C.u_cs : m;
   ad.cv_rs = n;
   ad.cv_cs = 1;
   ad.lds = C.cap;
@@ -505,25 +505,33 @@ QrFactors<T>::QrFactors(const BlrMatrix<T>& A) : nb_r(A.nb_r()), nb_c(A.nb_c()) 
 // ============================================================================
 // tpqrt / tpmqrt on BLR tiles
 // ============================================================================
-// tpqrt(A_kk, A_ik, T_ik) (tpqrt.cpp:35-53): R in A_kk's upper triangle is
-// updated; A_ik's reflector parts replace the dense tile, or V (with S <- I)
-// of a low-rank tile.
+// tpqrt(A, B, T) (tpqrt.cpp:35-53) on raw operands: R (b x b, upper
+// triangle, row stride ldr) is updated; B is dense (m x b, row stride ld,
+// overwritten by its reflector parts) or low rank (S with leading dimension
+// lds, V r x b row-major: V <- S V, S <- I, then V is overwritten by the
+// reflector parts, tpqrt.cpp:35-53).  tau (b) and T (b x b, row stride b)
+// receive the reflector scalars and the triangular factor.
 template <typename T>
-static void tpqrt_tile(BlrMatrix<T>& A, QrFactors<T>& F, int i, int k, Context& ctx) {
+struct TpqrtB {
+  int kind = kDense;
+  T* d = nullptr;
+  int64_t ld = 0;
+  int m = 0;
+  T* s = nullptr;
+  T* v = nullptr;
+  int r = 0, lds = 0;
+};
+
+template <typename T>
+static void tpqrt_raw(T* R, int64_t ldr, int b, const TpqrtB<T>& B, T* tau, T* Tm, Context& ctx) {
   const hipStream_t s = ctx.stream();
-  const int kk = A.idx(k, k), ik = A.idx(i, k);
-  const int b = (int)A.cols(k);
-  T* R = A.D[kk];
-  const int64_t ldr = A.cols(k);
-  T* Tm = F.tmat(i, k);
-  T* tau = F.tau_of(i, k);
-  if (A.kind[ik] == kLowRank) {
-    const int r = A.rank[ik];
+  if (B.kind == kLowRank) {
+    const int r = B.r;
     if (r > 64) throw std::runtime_error("tpqrt: low-rank block above rank 64");
     ctx.reserve(sizeof(T) * ((size_t)r * b + (size_t)b * b + 256) + larft_workspace<T>(b) + 8192);
     Plan pl(ctx);
     T* Bv = take_ws<T>(ctx, (size_t)r * b);
-    plan_gemms<T>(pl, {gemm_item(A.S[ik], A.kcap, 1, A.V[ik], b, 1, Bv, b, 1, r, b, r, 1.0, 0.0)}, "qr_gemm");
+    plan_gemms<T>(pl, {gemm_item(B.s, B.lds, 1, B.v, b, 1, Bv, b, 1, r, b, r, 1.0, 0.0)}, "qr_gemm");
     TpqrtItem t{};
     t.R = R;
     t.r_rs = ldr;
@@ -537,11 +545,11 @@ static void tpqrt_tile(BlrMatrix<T>& A, QrFactors<T>& F, int i, int k, Context& 
     const size_t ot = pl.add(std::vector<TpqrtItem>{t});
     pl.launch([=](const char* d) { launch_tpqrt<T>(reinterpret_cast<const TpqrtItem*>(d + ot), 1, r, b, s); },
               "tpqrt", 2.0 * r * b * b, sizeof(T) * (2.0 * r * b + b * b));
-    T* V = A.V[ik];
-    T* S = A.S[ik];
-    const int kcap = A.kcap;
+    T* V = B.v;
+    T* S = B.s;
+    const int lds = B.lds;
     plan_copies<T>(pl, {gemm_item(Bv, b, 1, nullptr, 0, 0, V, b, 1, r, b, 0, 0.0, 0.0)});
-    pl.launch([=](const char*) { launch_fill_identity<T>(S, r, r, kcap, s); });
+    pl.launch([=](const char*) { launch_fill_identity<T>(S, r, r, lds, s); });
     T* G = take_ws<T>(ctx, (size_t)b * b);
     plan_gemms<T>(pl, {gemm_item(Bv, 1, b, Bv, b, 1, G, b, 1, b, b, r, 1.0, 0.0)}, "qr_gemm");
     plan_larft_blocked<T>(pl, G, b, tau, Tm, b, b);
@@ -549,10 +557,10 @@ static void tpqrt_tile(BlrMatrix<T>& A, QrFactors<T>& F, int i, int k, Context& 
     ctx.sync();
     return;
   }
-  if (A.kind[ik] != kDense) throw std::runtime_error("tpqrt: empty block");
-  const int m = (int)A.rows(i);
-  T* Bd = A.D[ik];
-  const int64_t ldb = A.cols(k);
+  if (B.kind != kDense) throw std::runtime_error("tpqrt: empty block");
+  const int m = B.m;
+  T* Bd = B.d;
+  const int64_t ldb = B.ld;
   if (m > 512) throw std::runtime_error("tpqrt: dense block above 512 rows");
   ctx.reserve(sizeof(T) * (4 * (size_t)b * b + (size_t)64 * 64 * 2 + 1024) + larft_workspace<T>(b) + 8192);
   Plan pl(ctx);
@@ -597,6 +605,26 @@ static void tpqrt_tile(BlrMatrix<T>& A, QrFactors<T>& F, int i, int k, Context& 
   plan_larft_blocked<T>(pl, G, b, tau, Tm, b, b);
   pl.run();
   ctx.sync();
+}
+
+// tpqrt(A_kk, A_ik, T_ik) on BLR tiles: R in A_kk's upper triangle; A_ik's
+// reflector parts replace the dense tile, or V (with S <- I) of a low-rank tile.
+template <typename T>
+static void tpqrt_tile(BlrMatrix<T>& A, QrFactors<T>& F, int i, int k, Context& ctx) {
+  const int kk = A.idx(k, k), ik = A.idx(i, k);
+  TpqrtB<T> B;
+  B.kind = A.kind[ik];
+  if (B.kind == kLowRank) {
+    B.s = A.S[ik];
+    B.v = A.V[ik];
+    B.r = A.rank[ik];
+    B.lds = A.kcap;
+  } else if (B.kind == kDense) {
+    B.d = A.D[ik];
+    B.ld = A.cols(k);
+    B.m = (int)A.rows(i);
+  }
+  tpqrt_raw<T>(A.D[kk], A.cols(k), (int)A.cols(k), B, F.tau_of(i, k), F.tmat(i, k), ctx);
 }
 
 // The reflector block of tile (i, k) as an operand: dense tile, or U (I) V.
@@ -886,6 +914,390 @@ void blr_apply_q(const BlrMatrix<T>& Y, QrFactors<T>& F, BlrMatrix<T>& C, bool t
   ctx.sync();
   C.pull_ranks(ctx.stream());
 }
+
+// ============================================================================
+// Tile-level operators on caller-owned tiles (the C ABI's batched geqrt,
+// larfb, tpqrt, tpmqrt and typed gemm).  Low-rank tiles use the BLR slot
+// layout: U rows x cap column-major, S cap x cap and V cap x cols row-major.
+// ============================================================================
+template <typename T>
+static Op<T> raw_op(const RawTile<T>& t) {
+  Op<T> o{};
+  o.kind = t.kind;
+  o.m = t.rows;
+  o.n = t.cols;
+  if (t.kind == kDense) {
+    o.d = t.d;
+    o.d_rs = t.ld;
+    o.d_cs = 1;
+  } else {
+    o.r = t.rank;
+    o.u = t.u;
+    o.u_rs = t.u_rs;
+    o.u_cs = t.u_cs ? t.u_cs : t.rows;
+    o.s = t.s;
+    o.s_rs = t.cap;
+    o.s_cs = 1;
+    o.v = t.v;
+    o.v_rs = t.cols;
+    o.v_cs = 1;
+  }
+  return o;
+}
+
+template <typename T>
+static Target<T> raw_target(const RawTile<T>& t, int32_t* d_rank) {
+  Target<T> c{};
+  c.kind = t.kind;
+  c.m = t.rows;
+  c.n = t.cols;
+  if (t.kind == kDense) {
+    c.d = t.d;
+    c.d_rs = t.ld;
+    c.d_cs = 1;
+  } else {
+    c.u = t.u;
+    c.u_rs = t.u_rs;
+    c.u_cs = t.u_cs;
+    c.s = t.s;
+    c.v = t.v;
+    c.cap = t.cap;
+    c.rank = t.rank;
+    c.d_rank = d_rank;
+    c.eps = t.eps;
+    c.fixed_rank = t.eps == 0.0 ? t.rank : -1;
+    c.cap_trunc = false;
+  }
+  return c;
+}
+
+// Device ranks of the low-rank tiles of a batch (one slot per tile), for the
+// recompressions to write, and their return to the host descriptors.
+template <typename T>
+struct RankSlots {
+  DevBuf buf;
+  std::vector<RawTile<T>*> tiles;
+  int32_t* push(std::vector<RawTile<T>*> ts, hipStream_t s) {
+    tiles = std::move(ts);
+    std::vector<int32_t> h(tiles.size() + 1, 0);
+    for (size_t q = 0; q < tiles.size(); ++q) h[q] = tiles[q]->rank;
+    buf.ensure(sizeof(int32_t) * h.size());
+    FRK_HIP_CHECK(hipMemcpyAsync(buf.as<int32_t>(), h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, s));
+    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    return buf.as<int32_t>();
+  }
+  void pull(hipStream_t s, const char* what) {
+    std::vector<int32_t> h(tiles.size() + 1, 0);
+    FRK_HIP_CHECK(hipMemcpyAsync(h.data(), buf.as<int32_t>(), sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost, s));
+    FRK_HIP_CHECK(hipStreamSynchronize(s));
+    for (size_t q = 0; q < tiles.size(); ++q) {
+      if (tiles[q]->kind != kLowRank) continue;
+      if (h[q] < 0) throw std::runtime_error(std::string("low-rank capacity exceeded in ") + what);
+      tiles[q]->rank = h[q];
+    }
+  }
+};
+
+template <typename T>
+void tiles_geqrt(const std::vector<T*>& A, int64_t lda, const std::vector<T*>& Tout, int64_t ldt, int n,
+                 Context& ctx) {
+  const hipStream_t s = ctx.stream();
+  const size_t nt = A.size();
+  ctx.sync();
+  ctx.reserve(nt * (sizeof(T) * (3 * (size_t)n * n + n + 4 * 64) + larft_workspace<T>(n)) + 8192);
+  Plan pl(ctx);
+  std::vector<GemmItem> cp;
+  for (size_t t = 0; t < nt; ++t) {
+    T* tau = take_ws<T>(ctx, n);
+    T* Tm = take_ws<T>(ctx, (size_t)n * n);
+    T* Yx = take_ws<T>(ctx, (size_t)n * n);
+    FRK_HIP_CHECK(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)n * n, s));
+    plan_geqrt<T>(pl, A[t], lda, n, tau, Tm, Yx);
+    cp.push_back(gemm_item(Tm, n, 1, nullptr, 0, 0, Tout[t], ldt, 1, n, n, 0, 0.0, 0.0));
+  }
+  plan_copies<T>(pl, cp);
+  pl.run();
+  ctx.sync();
+}
+
+template <typename T>
+void tiles_larfb(const std::vector<const T*>& V, int64_t ldv, const std::vector<const T*>& Tin, int64_t ldt,
+                 std::vector<RawTile<T>>& C, int n, bool trans, Context& ctx) {
+  const hipStream_t s = ctx.stream();
+  ctx.sync();
+  size_t need = 8192;
+  for (const auto& c : C) need += sizeof(T) * (2 * (size_t)n * n + 2 * (size_t)n * std::max(c.cols, c.rank) + 4 * 64);
+  ctx.reserve(need);
+  Plan pl(ctx);
+  std::vector<GemmItem> cp, g1, g2, g3;
+  for (size_t q = 0; q < C.size(); ++q) {
+    T* Yx = take_ws<T>(ctx, (size_t)n * n);
+    T* Tm = take_ws<T>(ctx, (size_t)n * n);
+    const T* v = V[q];
+    pl.launch([=](const char*) { launch_tri_extract<T>(v, ldv, 1, Yx, n, n, 0, s); });
+    cp.push_back(gemm_item(Tin[q], ldt, 1, nullptr, 0, 0, Tm, n, 1, n, n, 0, 0.0, 0.0));
+    // larfb.cpp: a low-rank C is updated through its U
+    const bool lr = C[q].kind == kLowRank;
+    T* X = lr ? C[q].u : C[q].d;
+    const int64_t xrs = lr ? C[q].u_rs : C[q].ld, xcs = lr ? (C[q].u_cs ? C[q].u_cs : C[q].rows) : 1;
+    const int nx = lr ? C[q].rank : C[q].cols;
+    if (nx <= 0) continue;
+    T* w1 = take_ws<T>(ctx, (size_t)n * nx);
+    T* w2 = take_ws<T>(ctx, (size_t)n * nx);
+    g1.push_back(gemm_item(Yx, 1, n, X, xrs, xcs, w1, nx, 1, n, nx, n, 1.0, 0.0));
+    g2.push_back(trans ? gemm_item(Tm, 1, n, w1, nx, 1, w2, nx, 1, n, nx, n, 1.0, 0.0)
+                       : gemm_item(Tm, n, 1, w1, nx, 1, w2, nx, 1, n, nx, n, 1.0, 0.0));
+    g3.push_back(gemm_item(Yx, n, 1, w2, nx, 1, X, xrs, xcs, n, nx, n, -1.0, 1.0));
+  }
+  plan_copies<T>(pl, cp);
+  plan_gemms<T>(pl, g1, "qr_gemm");
+  plan_gemms<T>(pl, g2, "qr_gemm");
+  plan_gemms<T>(pl, g3, "qr_gemm");
+  pl.run();
+  ctx.sync();
+}
+
+template <typename T>
+void tiles_tpqrt(const std::vector<T*>& R, int64_t ldr, std::vector<RawTile<T>>& B, const std::vector<T*>& Tout,
+                 int64_t ldt, int n, Context& ctx) {
+  const hipStream_t s = ctx.stream();
+  ctx.sync();
+  DevBuf scratch(sizeof(T) * ((size_t)n * n + n + 64));
+  for (size_t q = 0; q < B.size(); ++q) {
+    TpqrtB<T> b;
+    b.kind = B[q].kind;
+    if (b.kind == kLowRank) {
+      b.s = B[q].s;
+      b.v = B[q].v;
+      b.r = B[q].rank;
+      b.lds = B[q].cap;
+    } else {
+      b.d = B[q].d;
+      b.ld = B[q].ld;
+      b.m = B[q].rows;
+    }
+    T* Tm = scratch.as<T>();
+    T* tau = Tm + (size_t)n * n;
+    FRK_HIP_CHECK(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)n * n, s));
+    tpqrt_raw<T>(R[q], ldr, n, b, tau, Tm, ctx);
+    FRK_HIP_CHECK(hipMemcpy2DAsync(Tout[q], sizeof(T) * ldt, Tm, sizeof(T) * n, sizeof(T) * n, n,
+                                   hipMemcpyDeviceToDevice, s));
+    FRK_HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+// tpmqrt(V, T, A, B, trans) (tpmqrt.cpp:45-170) for a batch of independent
+// items, every operand combination through the typed gemm:
+//   C = A;  C += V^T B;  C = op(T) C;  A -= C;  B -= V C
+// (C low rank when A is low rank and V or B is, else dense).
+template <typename T>
+void tiles_tpmqrt(const std::vector<RawTile<T>>& V, const std::vector<const T*>& Tin, int64_t ldt,
+                  std::vector<RawTile<T>>& A, std::vector<RawTile<T>>& B, int n, bool trans, int lra, Context& ctx) {
+  const hipStream_t s = ctx.stream();
+  const int nj = (int)V.size();
+  ctx.sync();
+  auto al = [](size_t x) { return (x + 63) & ~size_t(63); };
+  size_t slot = 0;
+  for (int q = 0; q < nj; ++q) {
+    const size_t cap = std::max(A[q].cap, 1), nc = A[q].cols;
+    slot = std::max(slot, al((size_t)n * cap) + al(cap * cap) + al(cap * nc) + al((size_t)n * nc) + al((size_t)n * n));
+  }
+  DevBuf cbuf(sizeof(T) * slot * nj);
+  std::vector<RawTile<T>> Cr(nj);
+  std::vector<const T*> Tm(nj);
+  {
+    size_t need = 8192;
+    for (int q = 0; q < nj; ++q) need += sizeof(T) * ((size_t)n * std::max(A[q].rank, 1) + 64);
+    ctx.reserve(need);
+    Plan pl(ctx);
+    std::vector<GemmItem> cp, g1, g2;
+    for (int q = 0; q < nj; ++q) {
+      const Op<T> Aop = raw_op(A[q]);
+      const int nc = A[q].cols;
+      T* base = cbuf.as<T>() + slot * q;
+      const size_t cap = std::max(A[q].cap, 1);
+      T* tcopy = base + al((size_t)n * cap) + al(cap * cap) + al(cap * nc) + al((size_t)n * nc);
+      cp.push_back(gemm_item(Tin[q], ldt, 1, nullptr, 0, 0, tcopy, n, 1, n, n, 0, 0.0, 0.0));
+      Tm[q] = tcopy;
+      RawTile<T>& c = Cr[q];
+      c.rows = n;
+      c.cols = nc;
+      if (A[q].kind == kLowRank && (V[q].kind == kLowRank || B[q].kind == kLowRank)) {
+        c.kind = kLowRank;
+        c.cap = (int)cap;
+        c.u = base;
+        c.s = base + al((size_t)n * cap);
+        c.v = c.s + al(cap * cap);
+        c.rank = Aop.r;
+        c.eps = A[q].eps;
+        cp.push_back(gemm_item(Aop.u, Aop.u_rs, Aop.u_cs, nullptr, 0, 0, c.u, 1, n, n, Aop.r, 0, 0.0, 0.0));
+        cp.push_back(gemm_item(Aop.s, Aop.s_rs, Aop.s_cs, nullptr, 0, 0, c.s, c.cap, 1, Aop.r, Aop.r, 0, 0.0, 0.0));
+        cp.push_back(gemm_item(Aop.v, Aop.v_rs, Aop.v_cs, nullptr, 0, 0, c.v, nc, 1, Aop.r, nc, 0, 0.0, 0.0));
+      } else {
+        c.kind = kDense;
+        c.d = base + al((size_t)n * cap) + al(cap * cap) + al(cap * nc);
+        c.ld = nc;
+        if (Aop.kind == kDense) {
+          cp.push_back(gemm_item(Aop.d, Aop.d_rs, Aop.d_cs, nullptr, 0, 0, c.d, nc, 1, n, nc, 0, 0.0, 0.0));
+        } else {  // dense(A) = (U S) V
+          T* us = take_ws<T>(ctx, (size_t)n * std::max(Aop.r, 1));
+          g1.push_back(gemm_item(Aop.u, Aop.u_rs, Aop.u_cs, Aop.s, Aop.s_rs, Aop.s_cs, us, Aop.r, 1, n, Aop.r, Aop.r,
+                                 1.0, 0.0));
+          g2.push_back(gemm_item(us, Aop.r, 1, Aop.v, Aop.v_rs, Aop.v_cs, c.d, nc, 1, n, nc, Aop.r, 1.0, 0.0));
+        }
+      }
+    }
+    plan_copies<T>(pl, cp);
+    plan_gemms<T>(pl, g1, "qr_gemm");
+    plan_gemms<T>(pl, g2, "qr_gemm");
+    pl.run();
+    ctx.sync();
+  }
+  // 1. C += V^T B
+  {
+    RankSlots<T> rs;
+    std::vector<RawTile<T>*> ts;
+    for (auto& c : Cr) ts.push_back(&c);
+    int32_t* dr = rs.push(ts, s);
+    size_t need = 8192;
+    for (int q = 0; q < nj; ++q)
+      need += typed_gemm_workspace(transpose_op(raw_op(V[q])), raw_op(B[q]), raw_target(Cr[q], dr + q));
+    need += lr_add_bound<T>(nj, n, n, 2 * 64 + 64);
+    ctx.reserve(need);
+    TypedBatch<T> tb;
+    for (int q = 0; q < nj; ++q)
+      typed_gemm(tb, ctx, transpose_op(raw_op(V[q])), raw_op(B[q]), raw_target(Cr[q], dr + q), 1.0, 1.0);
+    run_typed_batch(tb, ctx, lra);
+    rs.pull(s, "tpmqrt");
+  }
+  // 2. C = op(T) C (on C's U, or on dense C)
+  {
+    size_t need = 8192;
+    for (int q = 0; q < nj; ++q)
+      need += sizeof(T) * ((size_t)n * (Cr[q].kind == kLowRank ? std::max(Cr[q].rank, 1) : Cr[q].cols) + 64);
+    ctx.reserve(need);
+    Plan pl(ctx);
+    std::vector<GemmItem> g1, cp;
+    for (int q = 0; q < nj; ++q) {
+      const RawTile<T>& c = Cr[q];
+      const bool lr = c.kind == kLowRank;
+      const int nx = lr ? c.rank : c.cols;
+      if (nx <= 0) continue;
+      T* X = lr ? c.u : c.d;
+      const int64_t xrs = lr ? 1 : c.ld, xcs = lr ? n : 1;
+      T* w = take_ws<T>(ctx, (size_t)n * nx);
+      g1.push_back(trans ? gemm_item(Tm[q], 1, n, X, xrs, xcs, w, nx, 1, n, nx, n, 1.0, 0.0)
+                         : gemm_item(Tm[q], n, 1, X, xrs, xcs, w, nx, 1, n, nx, n, 1.0, 0.0));
+      cp.push_back(gemm_item(w, nx, 1, nullptr, 0, 0, X, xrs, xcs, n, nx, 0, 0.0, 0.0));
+    }
+    plan_gemms<T>(pl, g1, "qr_gemm");
+    plan_copies<T>(pl, cp);
+    pl.run();
+    ctx.sync();
+  }
+  // 3. A -= C ; B -= V C
+  {
+    RankSlots<T> rs;
+    std::vector<RawTile<T>*> ts;
+    for (int q = 0; q < nj; ++q) {
+      ts.push_back(&A[q]);
+      ts.push_back(&B[q]);
+    }
+    int32_t* dr = rs.push(ts, s);
+    const T* I = ctx.identity<T>(n);
+    const int ild = ctx.ident_ld();
+    const Op<T> Iop = dense_op<T>(I, n, n, ild, 1);
+    size_t need = 8192;
+    int mb = n;
+    for (int q = 0; q < nj; ++q) {
+      const Op<T> Cop = raw_op(Cr[q]);
+      need += typed_gemm_workspace(Iop, Cop, raw_target(A[q], dr + 2 * q));
+      need += typed_gemm_workspace(raw_op(V[q]), Cop, raw_target(B[q], dr + 2 * q + 1));
+      mb = std::max(mb, B[q].rows);
+    }
+    need += lr_add_bound<T>(2 * nj, mb, n, 2 * 64 + 64);
+    ctx.reserve(need);
+    TypedBatch<T> tb;
+    for (int q = 0; q < nj; ++q) {
+      const Op<T> Cop = raw_op(Cr[q]);
+      typed_gemm(tb, ctx, Iop, Cop, raw_target(A[q], dr + 2 * q), -1.0, 1.0);
+      typed_gemm(tb, ctx, raw_op(V[q]), Cop, raw_target(B[q], dr + 2 * q + 1), -1.0, 1.0);
+    }
+    run_typed_batch(tb, ctx, lra);
+    rs.pull(s, "tpmqrt");
+  }
+}
+
+// gemm(A, B, C, alpha, beta, TransA, TransB) (gemm.cpp:126-272) for a batch of
+// independent items, every Dense / LowRank combination; a low-rank C is
+// scaled by beta first (C.S *= beta, multiplication.cpp:39-44) and then
+// receives the product through the rounded addition.
+template <typename T>
+void tiles_gemm(const std::vector<RawTile<T>>& A, const std::vector<RawTile<T>>& B, std::vector<RawTile<T>>& C,
+                double alpha, double beta, bool ta, bool tb, int lra, Context& ctx) {
+  const hipStream_t s = ctx.stream();
+  const int nj = (int)A.size();
+  ctx.sync();
+  // low-rank targets: C.S *= beta (beta = 0 empties the tile)
+  if (beta != 1.0) {
+    size_t need = 8192;
+    for (const auto& c : C)
+      if (c.kind == kLowRank) need += sizeof(T) * ((size_t)c.rank * c.rank + 64);
+    ctx.reserve(need);
+    Plan pl(ctx);
+    std::vector<GemmItem> g, cp;
+    const T* I = ctx.identity<T>(64);
+    const int ild = ctx.ident_ld();
+    for (auto& c : C) {
+      if (c.kind != kLowRank || c.rank <= 0) continue;
+      if (beta == 0.0) {
+        c.rank = 0;
+        continue;
+      }
+      T* w = take_ws<T>(ctx, (size_t)c.rank * c.rank);
+      g.push_back(gemm_item(c.s, c.cap, 1, I, ild, 1, w, c.rank, 1, c.rank, c.rank, c.rank, beta, 0.0));
+      cp.push_back(gemm_item(w, c.rank, 1, nullptr, 0, 0, c.s, c.cap, 1, c.rank, c.rank, 0, 0.0, 0.0));
+    }
+    plan_gemms<T>(pl, g, "typed_gemm");
+    plan_copies<T>(pl, cp);
+    pl.run();
+    ctx.sync();
+  }
+  RankSlots<T> rs;
+  std::vector<RawTile<T>*> ts;
+  for (auto& c : C) ts.push_back(&c);
+  int32_t* dr = rs.push(ts, s);
+  size_t need = 8192;
+  int mb = 1, nbx = 1, kmax = 1;
+  std::vector<Op<T>> Ao(nj), Bo(nj);
+  for (int q = 0; q < nj; ++q) {
+    Ao[q] = ta ? transpose_op(raw_op(A[q])) : raw_op(A[q]);
+    Bo[q] = tb ? transpose_op(raw_op(B[q])) : raw_op(B[q]);
+    need += typed_gemm_workspace(Ao[q], Bo[q], raw_target(C[q], dr + q));
+    mb = std::max(mb, C[q].rows);
+    nbx = std::max(nbx, C[q].cols);
+    kmax = std::max({kmax, C[q].kind == kLowRank ? C[q].cap : 0, Ao[q].r + Bo[q].r});
+  }
+  need += lr_add_bound<T>(nj, mb, nbx, 2 * kmax + 64);
+  ctx.reserve(need);
+  TypedBatch<T> tbx;
+  for (int q = 0; q < nj; ++q)
+    typed_gemm(tbx, ctx, Ao[q], Bo[q], raw_target(C[q], dr + q), alpha, C[q].kind == kDense ? beta : 1.0);
+  run_typed_batch(tbx, ctx, lra);
+  rs.pull(s, "gemm");
+}
+
+template void tiles_geqrt<double>(const std::vector<double*>&, int64_t, const std::vector<double*>&, int64_t, int,
+                                  Context&);
+template void tiles_larfb<double>(const std::vector<const double*>&, int64_t, const std::vector<const double*>&,
+                                  int64_t, std::vector<RawTile<double>>&, int, bool, Context&);
+template void tiles_tpqrt<double>(const std::vector<double*>&, int64_t, std::vector<RawTile<double>>&,
+                                  const std::vector<double*>&, int64_t, int, Context&);
+template void tiles_tpmqrt<double>(const std::vector<RawTile<double>>&, const std::vector<const double*>&, int64_t,
+                                   std::vector<RawTile<double>>&, std::vector<RawTile<double>>&, int, bool, int,
+                                   Context&);
+template void tiles_gemm<double>(const std::vector<RawTile<double>>&, const std::vector<RawTile<double>>&,
+                                 std::vector<RawTile<double>>&, double, double, bool, bool, int, Context&);
 
 template struct QrFactors<double>;
 template struct QrFactors<float>;

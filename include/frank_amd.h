@@ -244,6 +244,61 @@ int frk_slr_add_batched(frk_ctx ctx, int lra, int batch, const int32_t* m, const
                         const int32_t* kb);
 
 /* ---------------------------------------------------------------------------
+ * Tile-level operators on caller-owned tiles (FP64, device memory).
+ *
+ * A tile is FRANK's Dense or LowRank (include/FRANK/classes/dense.h,
+ * low_rank.h).  Dense: d, rows x cols row-major with leading dimension ld.
+ * LowRank: slots of capacity cap, U rows x cap, S cap x cap and V cap x cols,
+ * all row-major (as frk_dlr_add_batched takes them), rank <= cap; eps is
+ * the LowRank's accuracy (LowRank::eps: 0 keeps the rank fixed, else the
+ * rounded addition truncates at eps).  Operations that change a low-rank
+ * tile write its new rank back into the descriptor.
+ * ------------------------------------------------------------------------- */
+typedef struct frk_dtile {
+  int32_t kind; /* FRK_TILE_DENSE or FRK_TILE_LOWRANK */
+  int32_t rows, cols;
+  int32_t rank, cap;
+  int32_t reserved;
+  double eps;
+  double* d;
+  int64_t ld;
+  double* u;
+  double* s;
+  double* v;
+} frk_dtile;
+
+/* gemm(A, B, C, alpha, beta, TransA, TransB) (include/FRANK/operations/BLAS.h:49-53,
+ * src/operations/BLAS/gemm.cpp:126-272), every Dense / LowRank combination of
+ * A, B and C, batched over independent items: C = alpha op(A) op(B) + beta C.
+ * A low-rank C is scaled first (C.S *= beta, src/operations/arithmetic/
+ * multiplication.cpp:39-44) and receives the product through the addition
+ * selected by lra (gemm(LowRank, LowRank, LowRank), gemm.cpp:253-272, is the
+ * core product S_A V_A U_B S_B and one addition). */
+int frk_dgemm_tiles_batched(frk_ctx ctx, int batch, const frk_dtile* A, const frk_dtile* B, frk_dtile* C,
+                            double alpha, double beta, int trans_a, int trans_b, int lra);
+/* geqrt(A, T) (include/FRANK/operations/LAPACK.h:345, src/operations/LAPACK/geqrt.cpp:34-43):
+ * n x n tiles (n <= 512); A <- R above / Householder vectors below the
+ * diagonal (LAPACK dgeqrt3 layout), T <- the n x n upper triangular factor. */
+int frk_dgeqrt_batched(frk_ctx ctx, int batch, int n, double* const* A, int64_t lda, double* const* T, int64_t ldt);
+/* larfb(V, T, C, trans) (LAPACK.h:374, src/operations/LAPACK/larfb.cpp:37-56):
+ * C <- (I - Y op(T) Y^T) C with Y the unit-lower part of V (a geqrt output),
+ * op(T) = T^T when trans; a low-rank C is updated through its U. */
+int frk_dlarfb_batched(frk_ctx ctx, int batch, int n, const double* const* V, int64_t ldv, const double* const* T,
+                       int64_t ldt, frk_dtile* C, int trans);
+/* tpqrt(A, B, T) (LAPACK.h:399, src/operations/LAPACK/tpqrt.cpp:35-53): QR of
+ * [R; B] with R the upper triangle of the n x n tile A (updated in place) and
+ * B a dense m x n tile (overwritten by its reflector parts, m <= 512) or a
+ * low-rank tile (V <- reflector parts of S V, S <- I); T <- n x n factor. */
+int frk_dtpqrt_batched(frk_ctx ctx, int batch, int n, double* const* A, int64_t lda, frk_dtile* B,
+                       double* const* T, int64_t ldt);
+/* tpmqrt(V, T, A, B, trans) (LAPACK.h:430, src/operations/LAPACK/tpmqrt.cpp:45-170):
+ * [A; B] <- H^T [A; B] (trans) or H [A; B] with H = I - [I; V] op(T) [I; V]^T:
+ * C = A + V^T B, C = op(T) C, A -= C, B -= V C, every Dense / LowRank
+ * combination of V, A and B (A n x nc, B m x nc, V m x n, T n x n). */
+int frk_dtpmqrt_batched(frk_ctx ctx, int batch, int n, const frk_dtile* V, const double* const* T, int64_t ldt,
+                        frk_dtile* A, frk_dtile* B, int trans, int lra);
+
+/* ---------------------------------------------------------------------------
  * Batched small dense factorizations (operator surface).  Device pointers,
  * row-major with leading dimensions; one entry per problem.
  * ------------------------------------------------------------------------- */

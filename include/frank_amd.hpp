@@ -486,6 +486,8 @@ inline void gemm(const Hierarchical& A, const Dense& B, Dense& C, double alpha =
   for (int64_t i = 0; i < C.dim[0]; ++i)
     for (int64_t j = 0; j < C.dim[1]; ++j) C(i, j) = alpha * y(i, j) + beta * C(i, j);
 }
+// gemm into a LowRank target (defined with the tile-level operators below)
+inline void gemm(const MatrixProxy& A, const MatrixProxy& B, LowRank& C, double alpha, double beta, bool tA, bool tB);
 inline void gemm(const MatrixProxy& A, const MatrixProxy& B, MatrixProxy& C, double alpha = 1, double beta = 1) {
   std::visit(
       [&](const auto& a, const auto& b, auto& c) {
@@ -495,6 +497,9 @@ inline void gemm(const MatrixProxy& A, const MatrixProxy& B, MatrixProxy& C, dou
         if constexpr (std::is_same_v<TC, Dense> && !std::is_same_v<TB, Hierarchical> &&
                       (!std::is_same_v<TA, Hierarchical> || std::is_same_v<TB, Dense>))
           gemm(a, b, c, alpha, beta);
+        else if constexpr (std::is_same_v<TC, LowRank> && !std::is_same_v<TA, Hierarchical> &&
+                           !std::is_same_v<TB, Hierarchical>)
+          gemm(MatrixProxy(a), MatrixProxy(b), c, alpha, beta, false, false);
         else
           undefined("gemm", type_name(a), type_name(b), type_name(c));
       },
@@ -629,6 +634,124 @@ inline std::tuple<Dense, Dense, Dense> rid(const Dense& A, const int64_t p, cons
   Dense QtA = gemm(Q, A, 1.0, true, false);
   auto [Ub, S, V] = id(QtA, k);
   return {gemm(Q, Ub), S, V};
+}
+
+// ---------------------------------------------------------------------------
+// Tile-level QR operators and the typed gemm into low-rank targets
+// (LAPACK.h:345, :374, :399, :430; gemm.cpp:202-272) over the batched C ABI.
+// A LowRank operand travels in slots of capacity max(64, 2 rank) and comes
+// back at its new rank.
+// ---------------------------------------------------------------------------
+namespace detail {
+struct TileArg {
+  frk_dtile t{};
+  std::shared_ptr<double> u, s, v;
+  explicit TileArg(const Dense& D) {
+    t.kind = FRK_TILE_DENSE;
+    t.rows = (int32_t)D.dim[0];
+    t.cols = (int32_t)D.dim[1];
+    t.d = const_cast<double*>(D.device());
+    t.ld = D.dim[1];
+  }
+  explicit TileArg(Dense& D, bool) : TileArg(static_cast<const Dense&>(D)) { t.d = D.device_mut(); }
+  explicit TileArg(const LowRank& A) {
+    const int64_t m = A.dim[0], n = A.dim[1], r = A.rank;
+    const int cap = (int)std::max<int64_t>(64, 2 * r);
+    u = dalloc((size_t)m * cap);
+    s = dalloc((size_t)cap * cap);
+    v = dalloc((size_t)cap * n);
+    if (r > 0) {
+      hip(hipMemcpy2D(u.get(), sizeof(double) * cap, A.U.host(), sizeof(double) * r, sizeof(double) * r, m,
+                      hipMemcpyHostToDevice));
+      hip(hipMemcpy2D(s.get(), sizeof(double) * cap, A.S.host(), sizeof(double) * r, sizeof(double) * r, r,
+                      hipMemcpyHostToDevice));
+      hip(hipMemcpy(v.get(), A.V.host(), sizeof(double) * r * n, hipMemcpyHostToDevice));
+    }
+    t.kind = FRK_TILE_LOWRANK;
+    t.rows = (int32_t)m;
+    t.cols = (int32_t)n;
+    t.rank = (int32_t)r;
+    t.cap = cap;
+    t.eps = A.eps;
+    t.u = u.get();
+    t.s = s.get();
+    t.v = v.get();
+  }
+  // the low-rank tile at its current rank, back on the host
+  LowRank lowrank(double eps) const {
+    const int64_t m = t.rows, n = t.cols, r = t.rank, cap = t.cap;
+    check(frk_ctx_synchronize(ctx()));
+    std::vector<double> hu((size_t)m * r), hs((size_t)r * r), hv((size_t)r * n);
+    if (r > 0) {
+      hip(hipMemcpy2D(hu.data(), sizeof(double) * r, u.get(), sizeof(double) * cap, sizeof(double) * r, m,
+                      hipMemcpyDeviceToHost));
+      hip(hipMemcpy2D(hs.data(), sizeof(double) * r, s.get(), sizeof(double) * cap, sizeof(double) * r, r,
+                      hipMemcpyDeviceToHost));
+      hip(hipMemcpy(hv.data(), v.get(), sizeof(double) * r * n, hipMemcpyDeviceToHost));
+    }
+    Dense U(m, r), S(r, r), V(r, n);
+    for (int64_t i = 0; i < m; ++i)
+      for (int64_t c = 0; c < r; ++c) U(i, c) = hu[(size_t)i * r + c];
+    for (int64_t i = 0; i < r; ++i)
+      for (int64_t c = 0; c < r; ++c) S(i, c) = hs[(size_t)i * r + c];
+    for (int64_t i = 0; i < r; ++i)
+      for (int64_t c = 0; c < n; ++c) V(i, c) = hv[(size_t)i * n + c];
+    return LowRank(U, S, V, eps);
+  }
+};
+inline TileArg tile_in(const MatrixProxy& A) {
+  if (const auto* d = std::get_if<Dense>(&A)) return TileArg(*d);
+  if (const auto* l = std::get_if<LowRank>(&A)) return TileArg(*l);
+  throw Error("tile operand must be Dense or LowRank");
+}
+inline TileArg tile_out(MatrixProxy& A) {
+  if (auto* d = std::get_if<Dense>(&A)) return TileArg(*d, true);
+  if (const auto* l = std::get_if<LowRank>(&A)) return TileArg(*l);
+  throw Error("tile operand must be Dense or LowRank");
+}
+inline void tile_back(MatrixProxy& A, const TileArg& t) {
+  if (auto* l = std::get_if<LowRank>(&A)) *l = t.lowrank(l->eps);
+}
+}  // namespace detail
+
+// geqrt(A, T) (LAPACK.h:345, geqrt.cpp:34-43)
+inline void geqrt(Dense& A, Dense& T) {
+  double* a = A.device_mut();
+  double* t = T.device_mut();
+  detail::check(frk_dgeqrt_batched(detail::ctx(), 1, (int)A.dim[0], &a, A.dim[1], &t, T.dim[1]));
+}
+// larfb(V, T, C, trans) (LAPACK.h:374, larfb.cpp:37-56): Dense C, or LowRank C through its U
+inline void larfb(const Dense& V, const Dense& T, MatrixProxy& C, const bool trans) {
+  detail::TileArg c = detail::tile_out(C);
+  const double* v = V.device();
+  const double* t = T.device();
+  detail::check(frk_dlarfb_batched(detail::ctx(), 1, (int)V.dim[0], &v, V.dim[1], &t, T.dim[1], &c.t, trans));
+  detail::tile_back(C, c);
+}
+// tpqrt(A, B, T) (LAPACK.h:399, tpqrt.cpp:35-53): Dense or LowRank B
+inline void tpqrt(Dense& A, MatrixProxy& B, Dense& T) {
+  detail::TileArg b = detail::tile_out(B);
+  double* a = A.device_mut();
+  double* t = T.device_mut();
+  detail::check(frk_dtpqrt_batched(detail::ctx(), 1, (int)A.dim[0], &a, A.dim[1], &b.t, &t, T.dim[1]));
+  detail::tile_back(B, b);
+}
+// tpmqrt(V, T, A, B, trans) (LAPACK.h:430, tpmqrt.cpp:45-170): every Dense / LowRank combination
+inline void tpmqrt(const MatrixProxy& V, const Dense& T, MatrixProxy& A, MatrixProxy& B, const bool trans) {
+  detail::TileArg v = detail::tile_in(V), a = detail::tile_out(A), b = detail::tile_out(B);
+  const double* t = T.device();
+  detail::check(frk_dtpmqrt_batched(detail::ctx(), 1, (int)T.dim[0], &v.t, &t, T.dim[1], &a.t, &b.t, trans,
+                                    detail::lra() == FRK_LRA_FAST ? FRK_LRA_FAST : FRK_LRA_ROUNDED));
+  detail::tile_back(A, a);
+  detail::tile_back(B, b);
+}
+// gemm into a LowRank target (gemm.cpp:202-272: D D LR, LR D LR, D LR LR, LR LR LR)
+inline void gemm(const MatrixProxy& A, const MatrixProxy& B, LowRank& C, double alpha, double beta, bool tA,
+                 bool tB) {
+  detail::TileArg a = detail::tile_in(A), b = detail::tile_in(B), c(C);
+  detail::check(frk_dgemm_tiles_batched(detail::ctx(), 1, &a.t, &b.t, &c.t, alpha, beta, tA, tB,
+                                        detail::lra() == FRK_LRA_FAST ? FRK_LRA_FAST : FRK_LRA_ROUNDED));
+  C = c.lowrank(C.eps);
 }
 
 // ---------------------------------------------------------------------------

@@ -109,6 +109,36 @@ int main() {
     const double res = l2_error(D, gemm(Qd, R));
     expect(res <= eps, "blr_qr_residual", res);
   }
+  // --- tile-level geqrt / larfb / tpqrt / tpmqrt (LAPACK.h:345-430) ------
+  // QR of [A0; B0] with B0 dense and with B0 low rank; the same reflectors
+  // applied to a copy of [A0; B0] must give [R; 0].
+  for (int lowrank_b = 0; lowrank_b < 2; ++lowrank_b) {
+    const int64_t n = 128;
+    const std::vector<std::vector<double>> x{get_sorted_random_vector(2 * n)};
+    const Dense A0(laplacend, x, n, n, 0, 0);
+    const Dense B0(laplacend, x, n, n, n, 0);
+    Dense A(A0), T0(n, n), T1(n, n);
+    MatrixProxy B = lowrank_b ? MatrixProxy(LowRank(B0, 1e-12)) : MatrixProxy(Dense(B0));
+    Dense Bd0 = lowrank_b ? gemm(gemm(std::get<LowRank>(B).U, std::get<LowRank>(B).S), std::get<LowRank>(B).V)
+                          : Dense(B0);
+    geqrt(A, T0);
+    MatrixProxy Ctop = Dense(A0), Cbot = Dense(Bd0);
+    larfb(A, T0, Ctop, true);
+    tpqrt(A, B, T1);
+    tpmqrt(B, T1, Ctop, Cbot, true);
+    const Dense& ct = std::get<Dense>(Ctop);
+    const Dense& cb = std::get<Dense>(Cbot);
+    double num = 0.0, bot = 0.0, den = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+      for (int64_t j = 0; j < n; ++j) {
+        const double r = (i <= j) ? A(i, j) : 0.0;
+        num += (ct(i, j) - r) * (ct(i, j) - r);
+        den += r * r;
+        bot += cb(i, j) * cb(i, j);
+      }
+    const double e = std::sqrt((num + bot) / den);
+    expect(e <= 1e-12, lowrank_b ? "tile_qr_ops_lowrank_b" : "tile_qr_ops_dense_b", e);
+  }
   std::printf("%s\n", failures ? "FAILED" : "ALL OK");
   return failures ? 1 : 0;
 }

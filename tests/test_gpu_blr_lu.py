@@ -156,3 +156,4 @@ def test_blr_lu_rank_zero_priority_panel(torch_gpu, fa, zero_tile):
     G.solve(bt)
     err = np.linalg.norm(bt.cpu().numpy() - xs) / np.linalg.norm(xs)
     assert err <= 1e-9, err
+

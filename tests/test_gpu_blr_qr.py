@@ -117,3 +117,25 @@ def test_tiled_qr_fixed_accuracy(torch_gpu, fa, n_cols, eps, admis, admis_type):
     Qr = fo.left_multiply_tiled_reflector(H, Tt, Qr, False)
     ref_residual = fo.l2_error(D, Qr.dense() @ fo.blr_r_factor(H))
     assert residual <= 10 * max(ref_residual, 1e-15)
+
+
+def test_clone_keeps_the_rank_cap_policy(torch_gpu, fa):
+    """A clone (and its restore by copy_from) keeps the capped-rank policy of
+    the matrix it copies: the tiled QR of the clone truncates at the cap as
+    the original's does, with the same ranks."""
+    import numpy as np
+    from frank_amd.inputs import morton_sort, uniform_points
+    N, nb, eps, cap = 512, 4, 1e-12, 8
+    print(f"clone cap policy N={N} cap={cap}", flush=True)
+    pts, _ = morton_sort(uniform_points(N, 2), 6)
+    pts = [np.ascontiguousarray(p) for p in pts]
+    ctx = fa.Context()
+    ctx.set_rank_cap(True)
+    A = fa.BlrMatrix.from_kernel_rect(pts, N, N, nb, nb, eps=eps, kcap=cap, ctx=ctx)
+    A.set_lra("rounded_addition")
+    B = A.clone()
+    B.copy_from(A)
+    A.qr()
+    B.qr()
+    assert np.array_equal(A.ranks(), B.ranks())
+    assert A.ranks().max() == cap

@@ -18,4 +18,4 @@ def test_cxx_surface_driver(torch_gpu):
     print(res.stderr)
     assert res.returncode == 0
     lines = [l for l in res.stdout.splitlines() if l.startswith(("ok", "FAIL"))]
-    assert len(lines) == 10 and all(l.startswith("ok") for l in lines)
+    assert len(lines) == 12 and all(l.startswith("ok") for l in lines)
