@@ -31,7 +31,8 @@ flops and bytes per launch come from the library's op log, getrf counted as
 2/3 n^3).  `path_roofline` is the op log's sum over all launches of
 max(flops / peak, bytes / 8 TB/s) over the measured step time.
 `cpu_baseline` times the CPU restatement of the reference algorithm (oracle/)
-on this host, median of three runs, with every host core and with one.
+on this host with one BLAS thread (median of three runs) and with every host
+thread (one run); `value` is the faster of the two.
 """
 from __future__ import annotations
 
@@ -75,15 +76,17 @@ def cpu_model() -> str:
 
 
 def cpu_timed(fn, reps: int = 3):
-    """Median wall time of fn() over reps runs, at every host thread and at one
-    BLAS thread.  Returns (median_all_threads, median_one_thread, threads)."""
+    """Wall time of fn() with one BLAS thread (median of reps runs) and with
+    every host thread (one run: on these small-matrix LAPACK calls the
+    threaded BLAS is many times slower, so repeating it only lengthens the
+    benchmark).  Returns (all_threads, median_one_thread, threads)."""
     from threadpoolctl import threadpool_limits
     nt = host_threads()
     out = []
-    for th in (nt, 1):
+    for th, n in ((nt, 1), (1, reps)):
         ts = []
         with threadpool_limits(th):
-            for _ in range(reps):
+            for _ in range(n):
                 t0 = time.perf_counter()
                 fn()
                 ts.append(time.perf_counter() - t0)
@@ -95,7 +98,8 @@ def cpu_record(t_all: float, t_one: float, nt: int, unit_scale: float, unit: str
     best = min(t_all, t_one)
     return {"value": best * unit_scale, "unit": unit, "cores": nt if t_all <= t_one else 1, "kind": "port",
             "value_all_threads": t_all * unit_scale, "value_1_thread": t_one * unit_scale,
-            "threads_available": nt, "cpu_model": cpu_model(), "timing": "median of 3 runs", "sample": sample}
+            "threads_available": nt, "cpu_model": cpu_model(),
+            "timing": "one BLAS thread: median of 3 runs; all host threads: one run", "sample": sample}
 
 
 def roofline_of(ops: dict, cls: str, steps: int, head_prec: str):

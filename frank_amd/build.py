@@ -20,7 +20,7 @@ BUILD = HERE / "_build"
 LIB = HERE / "libfrank_amd.so"
 ARCH = "gfx950"
 
-SOURCES = ["kernels.hip", "noalloc.hip", "engine.cpp", "qr.cpp", "small.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "noalloc.hip", "solve_df.hip", "engine.cpp", "qr.cpp", "small.cpp", "capi.cpp"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

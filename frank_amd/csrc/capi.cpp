@@ -691,12 +691,12 @@ int frk_dgetrf_nopiv_batched(frk_ctx ctx, double* const* tiles_host, int batch, 
     c.sync();
     std::vector<double*> tiles(tiles_host, tiles_host + batch);
     const int npan = (n + 63) / 64;
-    DevBuf info(sizeof(int32_t) * batch * npan);
-    FRK_HIP_CHECK(hipMemsetAsync(info.as<int32_t>(), 0, sizeof(int32_t) * batch * npan, ctx->stream));
-    getrf_tiles<double>(tiles, n, ld, info.as<int32_t>(), c);
+    if ((size_t)batch * npan > 4096) throw std::invalid_argument("batch too large for the info words (batch * ceil(n / 64) <= 4096)");
+    int32_t* info = c.info();
+    FRK_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int32_t) * batch * npan, ctx->stream));
+    getrf_tiles<double>(tiles, n, ld, info, c);
     std::vector<int32_t> h((size_t)batch * npan);
-    FRK_HIP_CHECK(hipMemcpyAsync(h.data(), info.as<int32_t>(), sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost,
-                                 ctx->stream));
+    FRK_HIP_CHECK(hipMemcpyAsync(h.data(), info, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
     c.sync();
     int status = FRK_OK;
     for (int b = 0; b < batch; ++b) {

@@ -281,6 +281,49 @@ struct SolveUpdItem {
   int32_t m, n, r, pad;
 };
 
+// ---------------------------------------------------------------------------
+// Dataflow block substitution (one launch per sweep, solve_df.hip).  Blocks
+// and tasks are listed in sweep order ("positions").
+// ---------------------------------------------------------------------------
+// Task geometry of the dataflow block substitution: rows of x per D task,
+// rows of bt per A task, updates per Y task (one per wave).
+constexpr int kDfRows = 16;
+constexpr int kDfArows = 64;
+constexpr int kDfYtiles = 4;
+struct DfBlock {
+  const void* tinv;  // n x n, row stride ld; used as op(Tinv) = Tinv^T when trans
+  int64_t ld;
+  const void* b;     // n x nrhs row-major (this sweep's input)
+  void* x;           // n x nrhs row-major (this sweep's output)
+  void* bt;          // n x nrhs scratch: b minus the updates from two or more positions back
+  int32_t n, trans;
+  int32_t nd;        // D tasks of this block (rows of 16)
+  int32_t has_bt;    // D tasks read bt (after A) instead of b
+  int32_t tile0, ntile;  // updates into this block, by source position ascending
+  int32_t last;          // the update from the previous position, or -1
+  int32_t src0, nsrc;    // updates out of this block (by_src), by target position ascending
+  int32_t na;            // A tasks of this block (rows of kDfArows)
+  int32_t near;          // the update from two positions back (applied by the A tasks from x), or -1
+  int32_t pad;
+};
+struct DfTile {  // b_p -= U (S (V x_q)), U n_p x r, S r x r, V r x n_q (strided)
+  const void* U;
+  const void* S;
+  const void* V;
+  int64_t u_rs, u_cs, s_rs, s_cs, v_rs, v_cs;
+  int32_t r, p, q, yoff;  // rank, target / source positions, offset of y (r x NR) in the y buffer
+};
+struct DfSweep {
+  const void* blocks;   // DfBlock[nb]
+  const void* tiles;    // DfTile[ntiles], grouped by target
+  const int32_t* by_src;
+  const void* tasks;    // int4 {type (0 D, 1 Y, 2 A), position, arg0, arg1}
+  void* y;
+  int32_t* flags;       // ticket, error, xcnt[nb], adone[nb], ydone[ntiles]
+  int32_t nb, ntiles, ntasks, nrhs;
+  int64_t spin_limit;
+};
+
 // Diagonal part of one fused substitution step: x (n x nrhs) = op(Tinv) b after
 // the update items[n_upd] (when has_last) is applied to a copy of b.
 struct StepDiag {

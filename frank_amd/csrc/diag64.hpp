@@ -1,7 +1,7 @@
 // Wave-level building blocks for the FP64 tile LU (gfx950, wave64):
 //   * 16 x 16 LU and triangular inverses held in one wave's registers, with
-//     DPP row_newbcast for broadcasts inside a 16-lane row and the
-//     permlane16/32 swaps for broadcasts across rows (no LDS, no barrier);
+//     DPP row_newbcast for broadcasts inside a 16-lane row and ds_bpermute
+//     (the LDS crossbar, no LDS storage) for broadcasts across rows (no barrier);
 //   * 16 x 16 output tiles of FP64 MFMA products (v_mfma_f64_16x16x4f64)
 //     with operands in LDS;
 //   * factor64: the LU of a 64 x 64 block in LDS plus the full inverses of
@@ -22,26 +22,26 @@ namespace d64 {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// (every lane of a DPP row broadcast reads a valid lane, so no old value is
+// needed and each half is one v_mov_b32_dpp)
 template <int CTRL>
 __device__ __forceinline__ double dppd(double v) {
-  return __hiloint2double(__builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false),
-                          __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false));
+  return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true),
+                          __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true));
 }
 // every lane receives lane K of its own 16-lane row (DPP row_newbcast)
 template <int K>
 __device__ __forceinline__ double row_bcast(double v) {
   return dppd<0x150 + K>(v);
 }
-// every lane receives the value at its own position within row S
+// every lane receives the value at its own position within row S: lane
+// (r, q) reads lane (r, S) through the LDS crossbar (ds_bpermute, no LDS
+// storage), one permute per 32-bit half
 template <int S>
 __device__ __forceinline__ double xrow_bcast(double v) {
-  const int hi = __double2hiint(v), lo = __double2loint(v);
-  const auto h16 = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
-  const auto l16 = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
-  const int h1 = h16[S & 1], l1 = l16[S & 1];
-  const auto h32 = __builtin_amdgcn_permlane32_swap(h1, h1, false, false);
-  const auto l32 = __builtin_amdgcn_permlane32_swap(l1, l1, false, false);
-  return __hiloint2double(h32[S >> 1], l32[S >> 1]);
+  const int addr = ((threadIdx.x & 15) + 16 * S) * 4;
+  return __hiloint2double(__builtin_amdgcn_ds_bpermute(addr, __double2hiint(v)),
+                          __builtin_amdgcn_ds_bpermute(addr, __double2loint(v)));
 }
 // 1 / x from the hardware estimate and two Newton steps (the result equals the
 // rounded quotient on the inputs tested; LAPACK's getf2 scales by 1 / pivot too)

@@ -568,11 +568,13 @@ static size_t getrf_workspace(int n, int nt) {
 template <typename T>
 void getrf_tiles(const std::vector<T*>& tiles, int n, int64_t ld, int32_t* info_dev, Context& ctx) {
   const size_t nn = (size_t)n * n;
-  DevBuf inv(sizeof(T) * tiles.size() * nn * 2);
-  FRK_HIP_CHECK(hipMemsetAsync(inv.as<T>(), 0, sizeof(T) * tiles.size() * nn * 2, ctx.stream()));
+  // the inverses of both factors (the kernels exchange blocks through them);
+  // kept in the context between calls
+  T* inv = static_cast<T*>(ctx.persistent(5, sizeof(T) * tiles.size() * nn * 2));
+  FRK_HIP_CHECK(hipMemsetAsync(inv, 0, sizeof(T) * tiles.size() * nn * 2, ctx.stream()));
   std::vector<T*> li(tiles.size()), ui(tiles.size());
   for (size_t t = 0; t < tiles.size(); ++t) {
-    li[t] = inv.as<T>() + t * nn * 2;
+    li[t] = inv + t * nn * 2;
     ui[t] = li[t] + nn;
   }
   ctx.sync();
@@ -2173,12 +2175,204 @@ static void blr_solve_fast(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context&
   pl.run();
 }
 
+// The dataflow block substitution (one persistent launch per sweep) applies
+// when every off-diagonal tile is low rank (rank <= 64) or empty and the
+// diagonal tiles are square, at most 512; FRK_SOLVE_DF=0 selects the
+// launch-per-step path instead.
+template <typename T>
+static bool solve_df_ok(const BlrMatrix<T>& F, int nrhs) {
+  const char* e = std::getenv("FRK_SOLVE_DF");
+  if ((e && e[0] == '0') || nrhs < 1 || nrhs > 16) return false;
+  const int nb = F.nb_r();
+  for (int i = 0; i < nb; ++i)
+    if (F.rows(i) > 512 || F.rows(i) != F.cols(i)) return false;
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) {
+      const int t = F.idx(i, j);
+      if (i == j) continue;
+      if (F.kind[t] == kDense) return false;
+      if (F.kind[t] == kLowRank && F.rank[t] > 64) return false;
+    }
+  return true;
+}
+
+template <typename T>
+static void blr_solve_df(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx, int part) {
+  const int nb = F.nb_r();
+  const hipStream_t s = ctx.stream();
+  const int64_t N = F.roff[nb];
+  const int NR = nrhs > 8 ? 16 : 8;
+  const int rpt = kDfRows;
+  // sizes of one sweep's device scratch
+  int ntiles_max = 0, rsum_max = 0;
+  {
+    int nt = 0, rs = 0;
+    for (size_t t = 0; t < F.rank.size(); ++t)
+      if (F.kind[t] == kLowRank && F.rank[t] > 0) {
+        ++nt;
+        rs += F.rank[t];
+      }
+    ntiles_max = nt;
+    rsum_max = rs;
+  }
+  const size_t nflags = 2 + 2 * (size_t)nb + ntiles_max;
+  ctx.reserve(2 * sizeof(T) * ((size_t)N * nrhs + 64) + 2 * sizeof(T) * ((size_t)rsum_max * NR + 64) +
+              2 * sizeof(int32_t) * (nflags + 64) + 8192);
+  T* y = ctx.take<T>((size_t)N * nrhs + 1);
+  T* bt = ctx.take<T>((size_t)N * nrhs + 1);
+  Plan pl(ctx);
+  std::vector<int32_t*> df_err;
+  auto sweep = [&](bool forward, bool lower_factor, bool transposed, T* src, T* dst) {
+    auto blk_of = [&](int p) { return forward ? p : nb - 1 - p; };
+    std::vector<DfBlock> blocks(nb);
+    std::vector<DfTile> tiles;
+    std::vector<std::vector<int>> out_of(nb);  // tile indices by source position
+    int yoff = 0;
+    for (int p = 0; p < nb; ++p) {
+      const int kb = blk_of(p);
+      DfBlock& B = blocks[p];
+      B = DfBlock{};
+      const int kk = F.idx(kb, kb);
+      B.tinv = lower_factor ? F.Linv[kk] : F.Uinv[kk];
+      B.ld = F.cols(kb);
+      B.b = src + F.roff[kb] * nrhs;
+      B.x = dst + F.roff[kb] * nrhs;
+      B.bt = bt + F.roff[kb] * nrhs;
+      B.n = (int)F.rows(kb);
+      B.trans = transposed ? 1 : 0;
+      B.nd = (B.n + rpt - 1) / rpt;
+      B.na = (B.n + kDfArows - 1) / kDfArows;
+      B.has_bt = p >= 2 ? 1 : 0;
+      B.tile0 = (int)tiles.size();
+      B.last = -1;
+      B.near = -1;
+      for (int q = 0; q < p; ++q) {
+        const int ib = blk_of(q);
+        const int t = transposed ? F.idx(ib, kb) : F.idx(kb, ib);
+        if (F.kind[t] != kLowRank || F.rank[t] <= 0) continue;
+        DfTile d{};
+        d.r = F.rank[t];
+        d.p = p;
+        d.q = q;
+        d.yoff = yoff;
+        yoff += d.r * NR;
+        if (!transposed) {  // b_k -= U (S (V x_i)), tile (k, i)
+          d.U = F.U[t];
+          d.u_rs = 1;
+          d.u_cs = F.rows(kb);
+          d.S = F.S[t];
+          d.s_rs = F.kcap;
+          d.s_cs = 1;
+          d.V = F.V[t];
+          d.v_rs = F.cols(ib);
+          d.v_cs = 1;
+        } else {  // b_k -= V^T (S^T (U^T x_i)), tile (i, k)
+          d.U = F.V[t];
+          d.u_rs = 1;
+          d.u_cs = F.cols(kb);
+          d.S = F.S[t];
+          d.s_rs = 1;
+          d.s_cs = F.kcap;
+          d.V = F.U[t];
+          d.v_rs = F.rows(ib);
+          d.v_cs = 1;
+        }
+        if (q == p - 1) B.last = (int)tiles.size();
+        if (q == p - 2) B.near = (int)tiles.size();
+        out_of[q].push_back((int)tiles.size());
+        tiles.push_back(d);
+      }
+      B.ntile = (int)tiles.size() - B.tile0;
+    }
+    std::vector<int32_t> by_src;
+    for (int q = 0; q < nb; ++q) {
+      blocks[q].src0 = (int)by_src.size();
+      blocks[q].nsrc = (int)out_of[q].size();
+      by_src.insert(by_src.end(), out_of[q].begin(), out_of[q].end());
+    }
+    if (by_src.empty()) by_src.push_back(0);
+    // tasks in dependency order: D(p, *) (after D(p - 1, *)), Y(p, chunks), then A(p + 2, *)
+    std::vector<int4> tasks;
+    for (int p = 0; p < nb; ++p) {
+      for (int g = 0; g < blocks[p].nd; ++g) tasks.push_back(make_int4(0, p, g, 0));
+      // y for the updates three or more blocks ahead (the next block's D tasks
+      // and the A tasks of the block after it apply theirs from x_p directly)
+      const int ns = blocks[p].nsrc;
+      int first = 0;
+      while (first < ns && tiles[by_src[blocks[p].src0 + first]].p <= p + 2) ++first;
+      for (int c = first; c < ns; c += kDfYtiles) tasks.push_back(make_int4(1, p, c, std::min(ns, c + kDfYtiles)));
+      if (p + 2 < nb)
+        for (int h = 0; h < blocks[p + 2].na; ++h) tasks.push_back(make_int4(2, p + 2, h, 0));
+    }
+    DfSweep sw{};
+    const size_t ob = pl.add(blocks);
+    const size_t ot = tiles.empty() ? ob : pl.add(tiles);
+    const size_t os = pl.add(by_src);
+    const size_t ok = pl.add(tasks);
+    sw.y = ctx.take<T>((size_t)std::max(yoff, 1));
+    sw.flags = ctx.take<int32_t>(nflags + 1);
+    sw.nb = nb;
+    sw.ntiles = (int)tiles.size();
+    sw.ntasks = (int)tasks.size();
+    sw.nrhs = nrhs;
+    sw.spin_limit = int64_t(1) << 21;
+    const int nfl = (int)(2 + 2 * nb + tiles.size());
+    double fl = 0, by = 0;
+    for (const auto& B : blocks) {
+      fl += 2.0 * B.n * B.n * nrhs;
+      by += sizeof(T) * ((double)B.n * B.n + 2.0 * B.n * nrhs);
+    }
+    for (const auto& d : tiles) {
+      const double nq = F.rows(blk_of(d.q)), np = F.rows(blk_of(d.p));
+      fl += 2.0 * d.r * (nq + d.r + np) * nrhs;
+      by += sizeof(T) * (double)d.r * (nq + np);
+    }
+    const bool has_t = !tiles.empty();
+    pl.launch(
+        [=](const char* dd) mutable {
+          sw.blocks = dd + ob;
+          sw.tiles = has_t ? dd + ot : nullptr;
+          sw.by_src = reinterpret_cast<const int32_t*>(dd + os);
+          sw.tasks = dd + ok;
+          launch_solve_df<T>(sw, nfl, nrhs, 256, s);
+        },
+        "solve_df", fl, by);
+    df_err.push_back(sw.flags + 1);
+  };
+  const bool do_l = part != 2, do_u = part != 1;
+  const bool first_lower = !trans;
+  const bool do_first = first_lower ? do_l : do_u, do_second = first_lower ? do_u : do_l;
+  T* cur = b;
+  T* other = y;
+  if (do_first) {
+    sweep(true, first_lower, trans, cur, other);
+    std::swap(cur, other);
+  }
+  if (do_second) {
+    sweep(false, !first_lower, trans, cur, other);
+    std::swap(cur, other);
+  }
+  if (cur != b) plan_copies<T>(pl, {gemm_item(cur, nrhs, 1, nullptr, 0, 0, b, nrhs, 1, (int)N, nrhs, 0, 0.0, 0.0)});
+  pl.run();
+  // a wait past its spin limit marks the sweep failed: report it, never return its x
+  for (int32_t* e : df_err) {
+    int32_t h = 0;
+    FRK_HIP_CHECK(hipMemcpyAsync(&h, e, sizeof(h), hipMemcpyDeviceToHost, s));
+    host_wait_stream(s, "dataflow solve");
+    if (h != 0) throw std::runtime_error("dataflow solve: a wait timed out");
+  }
+}
+
 template <typename T>
 void blr_solve(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& ctx, int part) {
   const int nb = F.nb_r();
   ctx.set_default_peak(mfma_peak<T>());
   ctx.sync();
   F.pull_ranks(ctx.stream());
+  if (solve_df_ok(F, nrhs)) {
+    blr_solve_df(F, b, nrhs, trans, ctx, part);
+    return;
+  }
   if (solve_fast_ok(F, nrhs)) {
     blr_solve_fast(F, b, nrhs, trans, ctx, part);
     return;

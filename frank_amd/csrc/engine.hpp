@@ -134,8 +134,10 @@ class Context {
   void account(const char* cls, double flops, double bytes, double peak = 0);
   // peak arithmetic rate of launches that name none (the precision the caller computes in)
   void set_default_peak(double p) { owner_->peak_ = p; }
-  // Device buffers kept between calls (slot < 4), grown on demand: a call that
-  // reuses a size allocates nothing.
+  // Device buffers kept between calls, grown on demand: a call that reuses a
+  // size allocates nothing.  Slots: 0, 1 refinement residual and correction;
+  // 2, 3 tpmqrt's C copies and their ranks; 4 ranks of a typed batch's dense
+  // compressions; 5 inverses of the batched tile LU; 6 tile-level tpqrt T.
   void* persistent(int slot, size_t bytes) {
     DevBuf& b = persist_[slot];
     if (b.bytes() < bytes) {
@@ -179,7 +181,7 @@ class Context {
   hipEvent_t ev_open_ = nullptr;
   Context* owner_ = this;
   double peak_ = kPeakF64;
-  DevBuf persist_[4];
+  DevBuf persist_[8];
   hipStream_t aux_[2] = {nullptr, nullptr};
   void resolve_events();
 };

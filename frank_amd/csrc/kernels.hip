@@ -1755,6 +1755,13 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   const T scale = ldexp(T(1), -sexp), unscale = ldexp(T(1), sexp);
   for (int e = tid; e < K * K; e += NT) G[e / K][e % K] *= scale;
   __syncthreads();
+  if constexpr (VG) {
+    // right vectors too large for LDS: the sweeps rotate only G, and the kept
+    // right vectors are formed afterwards as M^T u_c / sigma_c from this copy
+    // of the (scaled) matrix, column j at vg + j K
+    for (int e = tid; e < K * K; e += NT) vg[(int64_t)(e / K) * K + e % K] = G[e / K][e % K];
+    __syncthreads();
+  }
   for (int j = tid; j < K; j += NT) {
     T c2 = T(0);
     for (int row = 0; row < K; ++row) c2 += G[j][row] * G[j][row];
@@ -1814,9 +1821,11 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
                 const T x = G[i][row], y = G[j][row];
                 G[i][row] = cth * x - sth * y;
                 G[j][row] = sth * x + cth * y;
-                const T vx = VA(i, row), vy = VA(j, row);
-                VA(i, row) = cth * vx - sth * vy;
-                VA(j, row) = sth * vx + cth * vy;
+                if constexpr (!VG) {
+                  const T vx = VA(i, row), vy = VA(j, row);
+                  VA(i, row) = cth * vx - sth * vy;
+                  VA(j, row) = sth * vx + cth * vy;
+                }
               }
               rotated = 1;
             }
@@ -1941,7 +1950,24 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
     const int row = e / r, c = e % r;
     const int src = order[c];
     if (Um != nullptr) Um[row * it.ldu + c] = G[src][row];
-    if (Vm != nullptr) Vm[row * it.ldv + c] = VA(src, row);
+    if constexpr (!VG)
+      if (Vm != nullptr) Vm[row * it.ldv + c] = VA(src, row);
+  }
+  if constexpr (VG) {
+    // V[:, c] = M^T u_c / sigma_c (M scaled as sigma is): a wave per output,
+    // lanes over the reduction index; numerically zero sigma gives a zero column
+    if (Vm != nullptr) {
+      const int lane = tid & 63, wv = tid >> 6;
+      for (int o = wv; o < K * r; o += NT / 64) {
+        const int row = o / r, c = o % r;
+        const int src = order[c];
+        const T sc = sig[src];
+        T acc = T(0);
+        for (int i2 = lane; i2 < K; i2 += 64) acc += vg[(int64_t)row * K + i2] * G[src][i2];
+        acc = wave_sum(acc);
+        if (lane == 0) Vm[row * it.ldv + c] = (sc > sig_floor) ? acc / sc : T(0);
+      }
+    }
   }
 }
 

@@ -126,6 +126,10 @@ void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int
 // One fused block step of a substitution sweep (see StepDiag).
 template <typename T>
 void launch_solve_step(const SolveUpdItem* d_items, int n_upd, const StepDiag& sd, int nrhs, hipStream_t s);
+// Dataflow block substitution: clears the nflags flag words, then one persistent
+// launch of `grid` workgroups over the sweep's tasks (solve_df.hip).
+template <typename T>
+void launch_solve_df(const DfSweep& sw, int nflags, int nrhs, int grid, hipStream_t s);
 // Two strided stacked-row products in one launch (see RowsPair).
 template <typename T>
 void launch_rows_pair(const RowsPair& p, hipStream_t s);

@@ -306,13 +306,14 @@ void run_typed_batch(TypedBatch<T>& b, Context& ctx, int lra) {
     pl.run();
   }
   if (!b.comp.empty()) {
-    DevBuf crank(sizeof(int32_t) * b.comp.size());
-    for (size_t c = 0; c < b.comp.size(); ++c) b.comp[c].d_rank = crank.as<int32_t>() + c;
+    int32_t* crank = static_cast<int32_t*>(ctx.persistent(4, sizeof(int32_t) * b.comp.size()));
+    for (size_t c = 0; c < b.comp.size(); ++c) b.comp[c].d_rank = crank + c;
     Context cside(ctx.stream());
     cside.set_stats_owner(&ctx);
     compress_batched<T>(b.comp, cside);
     std::vector<int32_t> cr(b.comp.size());
-    FRK_HIP_CHECK(hipMemcpy(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost));
+    FRK_HIP_CHECK(hipMemcpyAsync(cr.data(), crank, sizeof(int32_t) * cr.size(), hipMemcpyDeviceToHost, ctx.stream()));
+    host_wait_stream(ctx.stream(), "typed batch compression ranks");
     for (size_t c = 0; c < b.comp.size(); ++c) {
       if (cr[c] < 0) throw std::runtime_error("low-rank capacity exceeded in a dense-product compression");
       b.adds[b.comp_add[c]].kx1 = b.adds[b.comp_add[c]].kx2 = cr[c];
@@ -687,8 +688,8 @@ static void tpmqrt_row(const BlrMatrix<T>& Y, QrFactors<T>& F, int i, int k, Blr
   auto al = [](size_t x) { return (x + 63) & ~size_t(63); };
   for (int j : typed_js) slot = std::max(slot, al((size_t)b * cap) + al((size_t)cap * cap) + al((size_t)cap * Cm.cols(j)) +
                                                    al((size_t)b * Cm.cols(j)));
-  DevBuf cbuf(sizeof(T) * slot * nj);
-  DevBuf crank(sizeof(int32_t) * nj);
+  T* cbuf = static_cast<T*>(ctx.persistent(2, sizeof(T) * slot * nj));
+  int32_t* crank = static_cast<int32_t*>(ctx.persistent(3, sizeof(int32_t) * nj));
   std::vector<Target<T>> Cj(nj);
   {
     ctx.reserve(sizeof(T) * ((size_t)b * cap * nj + 4096) + 8192);
@@ -702,7 +703,7 @@ static void tpmqrt_row(const BlrMatrix<T>& Y, QrFactors<T>& F, int i, int k, Blr
       const Op<T> Aop = tile_op(Cm, k, j);
       const Op<T> Bop = tile_op(Cm, i, j);
       Target<T>& c = Cj[q];
-      T* base = cbuf.as<T>() + slot * q;
+      T* base = cbuf + slot * q;
       c.m = b;
       c.n = n;
       if (Aop.kind == kLowRank && (V.kind == kLowRank || Bop.kind == kLowRank)) {
@@ -712,7 +713,7 @@ static void tpmqrt_row(const BlrMatrix<T>& Y, QrFactors<T>& F, int i, int k, Blr
         c.v = c.s + al((size_t)cap * cap);
         c.cap = cap;
         c.rank = Aop.r;
-        c.d_rank = crank.as<int32_t>() + q;
+        c.d_rank = crank + q;
         c.eps = Cm.eps;
         c.fixed_rank = Cm.fixed_rank;
         c.cap_trunc = Cm.cap_trunc;
@@ -735,7 +736,7 @@ static void tpmqrt_row(const BlrMatrix<T>& Y, QrFactors<T>& F, int i, int k, Blr
         }
       }
     }
-    FRK_HIP_CHECK(hipMemcpyAsync(crank.as<int32_t>(), r0.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice, s));
+    FRK_HIP_CHECK(hipMemcpyAsync(crank, r0.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice, s));
     plan_copies<T>(pl, cp);
     plan_gemms<T>(pl, g1, "qr_gemm");
     plan_gemms<T>(pl, g2, "qr_gemm");
@@ -744,7 +745,8 @@ static void tpmqrt_row(const BlrMatrix<T>& Y, QrFactors<T>& F, int i, int k, Blr
   }
   auto pull_c_ranks = [&]() {
     std::vector<int32_t> rr(nj);
-    FRK_HIP_CHECK(hipMemcpy(rr.data(), crank.as<int32_t>(), sizeof(int32_t) * nj, hipMemcpyDeviceToHost));
+    FRK_HIP_CHECK(hipMemcpyAsync(rr.data(), crank, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, s));
+    host_wait_stream(s, "tpmqrt ranks");
     for (int q = 0; q < nj; ++q)
       if (Cj[q].kind == kLowRank) {
         if (rr[q] < 0) throw std::runtime_error("low-rank capacity exceeded in tpmqrt");
@@ -1062,7 +1064,7 @@ void tiles_tpqrt(const std::vector<T*>& R, int64_t ldr, std::vector<RawTile<T>>&
                  int64_t ldt, int n, Context& ctx) {
   const hipStream_t s = ctx.stream();
   ctx.sync();
-  DevBuf scratch(sizeof(T) * ((size_t)n * n + n + 64));
+  T* scratch = static_cast<T*>(ctx.persistent(6, sizeof(T) * ((size_t)n * n + n + 64)));
   for (size_t q = 0; q < B.size(); ++q) {
     TpqrtB<T> b;
     b.kind = B[q].kind;
@@ -1076,7 +1078,7 @@ void tiles_tpqrt(const std::vector<T*>& R, int64_t ldr, std::vector<RawTile<T>>&
       b.ld = B[q].ld;
       b.m = B[q].rows;
     }
-    T* Tm = scratch.as<T>();
+    T* Tm = scratch;
     T* tau = Tm + (size_t)n * n;
     FRK_HIP_CHECK(hipMemsetAsync(Tm, 0, sizeof(T) * (size_t)n * n, s));
     tpqrt_raw<T>(R[q], ldr, n, b, tau, Tm, ctx);
@@ -1102,7 +1104,7 @@ void tiles_tpmqrt(const std::vector<RawTile<T>>& V, const std::vector<const T*>&
     const size_t cap = std::max(A[q].cap, 1), nc = A[q].cols;
     slot = std::max(slot, al((size_t)n * cap) + al(cap * cap) + al(cap * nc) + al((size_t)n * nc) + al((size_t)n * n));
   }
-  DevBuf cbuf(sizeof(T) * slot * nj);
+  T* cbuf = static_cast<T*>(ctx.persistent(2, sizeof(T) * slot * nj));
   std::vector<RawTile<T>> Cr(nj);
   std::vector<const T*> Tm(nj);
   {
@@ -1114,7 +1116,7 @@ void tiles_tpmqrt(const std::vector<RawTile<T>>& V, const std::vector<const T*>&
     for (int q = 0; q < nj; ++q) {
       const Op<T> Aop = raw_op(A[q]);
       const int nc = A[q].cols;
-      T* base = cbuf.as<T>() + slot * q;
+      T* base = cbuf + slot * q;
       const size_t cap = std::max(A[q].cap, 1);
       T* tcopy = base + al((size_t)n * cap) + al(cap * cap) + al(cap * nc) + al((size_t)n * nc);
       cp.push_back(gemm_item(Tin[q], ldt, 1, nullptr, 0, 0, tcopy, n, 1, n, n, 0, 0.0, 0.0));

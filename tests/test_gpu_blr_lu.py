@@ -157,3 +157,33 @@ def test_blr_lu_rank_zero_priority_panel(torch_gpu, fa, zero_tile):
     err = np.linalg.norm(bt.cpu().numpy() - xs) / np.linalg.norm(xs)
     assert err <= 1e-9, err
 
+
+@pytest.mark.parametrize("prec,trans", [("f64", False), ("f64", True), ("f32", False)])
+def test_dataflow_solve_matches_launch_per_step(torch_gpu, fa, prec, trans, monkeypatch):
+    """The one-launch dataflow block substitution against the launch-per-step
+    sweep on the same factors (both trsm.cpp:40-90 block substitution; weak
+    admissibility, every off-diagonal tile low rank): equal to 1e-12 (FP64) /
+    1e-5 (FP32) relative, plain and transposed."""
+    import numpy as np
+    torch = torch_gpu
+    N, nb, nrhs = 4096, 8, 8
+    print(f"dataflow solve {prec} trans={trans}", flush=True)
+    x = fo.get_sorted_random_vector(N)
+    A = fa.BlrMatrix.from_kernel([x], nb, eps=1e-8, shift=1e-2, kcap=64)
+    A.set_lra("rounded_addition")
+    F = A.convert("f32") if prec == "f32" else A
+    F.getrf()
+    b = np.random.default_rng(5).standard_normal((N, nrhs))
+    outs = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("FRK_SOLVE_DF", mode)
+        v = torch.from_numpy(b).cuda().contiguous()
+        if prec == "f32":  # FP32 factors solve through the refinement's first solve (no correction)
+            xs = torch.zeros_like(v)
+            F.refine(v, xs, tol=1.0, max_iter=0)
+            v = xs
+        else:
+            F.solve(v, trans=trans)
+        outs.append(v.cpu().numpy())
+    tol = 1e-5 if prec == "f32" else 1e-12
+    assert np.abs(outs[0] - outs[1]).max() <= tol * np.abs(outs[0]).max()
