@@ -182,9 +182,9 @@ void Context::sync() {
   resolve_events();
 }
 
-void Context::account(const char* cls, double flops, double bytes, double peak) {
+void Context::account(const char* cls, double flops, double bytes, double peak, int64_t launches) {
   OpStat& st = owner_->stats_[cls];
-  st.launches += 1;
+  st.launches += launches;
   st.flops += flops;
   st.bytes += bytes;
   st.roof_s += std::max(flops / (peak > 0 ? peak : owner_->peak_), bytes / kHbmBytesPerS);
@@ -1380,6 +1380,20 @@ static size_t prio_workspace(const BlrMatrix<T>& A, int64_t bm) {
 // dense tile (k+1, k+1).  With device_ranks (prio_rank_free) the low-rank tiles
 // are processed at capacity and their ranks read on the device; otherwise rl,
 // rb are the ranks of the two panel tiles.
+// Exact op-log cost of the device-rank priority block of step k with panel
+// ranks rl, rb (its launches are counted when issued).
+template <typename T>
+static void account_prio(Context& c, const BlrMatrix<T>& A, int k, int rl, int rb) {
+  const double n_k = (double)A.cols(k), m_k = (double)A.rows(k);
+  const double m = (double)A.rows(k + 1), n = (double)A.cols(k + 1);
+  c.account("trsm_gemm", 2.0 * (rl * n_k * n_k + rb * m_k * m_k),
+            sizeof(T) * (n_k * n_k + m_k * m_k + 2.0 * (rl * n_k + rb * m_k)), 0, 0);
+  c.account("schur_gemm", 2.0 * rl * rb * n_k + 2.0 * rl * (rl + rb) * rb + 2.0 * m * rl * rb + 2.0 * m * n * rb,
+            sizeof(T) * (2.0 * m * n + (rl + rb) * (n_k + m + n)), 0, 0);
+}
+
+// With device ranks and both panels low rank the launches are counted here and
+// their cost is added by account_prio once the panel ranks are known on the host.
 template <typename T>
 static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int rb, bool device_ranks) {
   const hipStream_t s = W.stream();
@@ -1439,12 +1453,11 @@ static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int 
     ui.m = m;
     ui.n = n;
     ui.kcap = A.kcap;
-    pl.launch([=](const char*) { launch_rows_pair<T>(rp, s); }, "trsm_gemm",
-              2.0 * A.kcap * ((double)n_k * n_k + (double)m_k * m_k), sizeof(T) * ((double)n_k * n_k + m_k * m_k));
+    pl.launch([=](const char*) { launch_rows_pair<T>(rp, s); }, "trsm_gemm");
     pl.launch([=](const char*) {
       launch_lr_core<T>(ci, s);
       launch_lr_update<T>(ui, s);
-    }, "schur_gemm", 2.0 * A.kcap * ((double)n_k * A.kcap + (double)m * n), sizeof(T) * (2.0 * m * n));
+    }, "schur_gemm");
     return;
   }
   std::vector<GemmItem> tg, tcp;
@@ -1583,7 +1596,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
   };
   // priority block of step k on the lookahead stream, after the factorization of
   // tile k and (after_rest) the rest of step k-1, then the factorization of tile k+1
+  // priority blocks planned on device ranks whose cost is logged once their
+  // panel ranks are known on the host (at the next rank snapshot)
+  std::vector<char> prio_dev(nb, 0);
   auto enqueue_prio = [&](int k, bool device_ranks, bool after_rest) {
+    prio_dev[k] = device_ranks && A.kind[A.idx(k + 1, k)] == kLowRank && A.kind[A.idx(k, k + 1)] == kLowRank;
     const auto tp = clk::now();
     if (after_rest) FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.c1, 0));
     side.reserve(side_ws);
@@ -1668,6 +1685,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
                                    std::to_string(A.kcap));
       }
     }
+    // the panel ranks of the priority block of step i are final (step i - 1 is done)
+    if (prio_dev[i]) account_prio<T>(ctx, A, i, A.rank[A.idx(i + 1, i)], A.rank[A.idx(i, i + 1)]);
     ctx.host_time("host_sync", ms_since(th));
     th = clk::now();
     if (prio_pending) {

@@ -131,7 +131,9 @@ class Context {
   // 1 = second chunk of a BLR-LU step.
   hipStream_t aux_stream(int k);
   // op log
-  void account(const char* cls, double flops, double bytes, double peak = 0);
+  // (launches: how many launches the entry stands for; 0 adds the cost of
+  // launches already counted when they were issued)
+  void account(const char* cls, double flops, double bytes, double peak = 0, int64_t launches = 1);
   // peak arithmetic rate of launches that name none (the precision the caller computes in)
   void set_default_peak(double p) { owner_->peak_ = p; }
   // Device buffers kept between calls, grown on demand: a call that reuses a

@@ -1654,6 +1654,25 @@ void launch_tri_extract(const T* src, int64_t rs, int64_t cs, T* dst, int64_t ld
 // ===========================================================================
 // One-sided (Hestenes) Jacobi SVD of the K x K core with truncation.
 // ===========================================================================
+// Rotation (c, s) that zeroes the off-diagonal of [[al, ga], [ga, be]]
+// (Rutishauser's formulas: t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
+// zeta = (be - al) / (2 ga), c = 1 / sqrt(1 + t^2), s = c t).  FP32 uses the
+// hardware reciprocal and reciprocal square root (about 1 ulp: the rotation
+// stays orthogonal to rounding level, only its angle moves by an ulp); FP64
+// keeps the correctly rounded operations.
+__device__ __forceinline__ void jacobi_rotation(float al, float be, float ga, float& c, float& s) {
+  const float zeta = (be - al) * __builtin_amdgcn_rcpf(2.0f * ga);
+  const float t = copysignf(1.0f, zeta) * __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_sqrtf(1.0f + zeta * zeta));
+  c = __builtin_amdgcn_rsqf(1.0f + t * t);
+  s = c * t;
+}
+__device__ __forceinline__ void jacobi_rotation(double al, double be, double ga, double& c, double& s) {
+  const double zeta = (be - al) / (2.0 * ga);
+  const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  c = 1.0 / sqrt(1.0 + t * t);
+  s = c * t;
+}
+
 __device__ __forceinline__ int rr_pos(int pos, int r, int k2) {
   return pos == 0 ? 0 : ((pos - 1 + r) % (k2 - 1)) + 1;
 }
@@ -1788,54 +1807,57 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   const int p = tid / gl, sub = tid % gl;
   // rotation threshold: above the rounding noise of a length-K dot product
   const T tol = T(8 * max(K, 8)) * Eps<T>::v;
-  int sweeps = 0;
+  const T itol = T(1) / tol;
+  // The rounds run the round-robin schedule cyclically and stop once K2 - 1
+  // consecutive rounds (every pair once, from any starting round) rotated
+  // nothing: then no column changed while every pair was tested, as after a
+  // rotation-free sweep.
+  int rounds = 0;
   if (K >= 2) {
-    for (int sweep = 0; sweep < 60; ++sweep) {
-      ++sweeps;
+    int quiet = 0;
+    for (int r = 0; quiet < K2 - 1 && rounds < 60 * (K2 - 1); r = (r + 1 == K2 - 1) ? 0 : r + 1) {
+      ++rounds;
       int rotated = 0;
-      for (int r = 0; r < K2 - 1; ++r) {
-        if (p < P) {
-          int i = rr_pos(p, r, K2), j = rr_pos(K2 - 1 - p, r, K2);
-          if (i > j) {
-            const int tmp = i;
-            i = j;
-            j = tmp;
+      if (p < P) {
+        int i = rr_pos(p, r, K2), j = rr_pos(K2 - 1 - p, r, K2);
+        if (i > j) {
+          const int tmp = i;
+          i = j;
+          j = tmp;
+        }
+        if (j < K) {
+          T al = T(0), be = T(0), ga = T(0);
+          for (int row = sub; row < K; row += gl) {
+            const T x = G[i][row], y = G[j][row];
+            al += x * x;
+            be += y * y;
+            ga += x * y;
           }
-          if (j < K) {
-            T al = T(0), be = T(0), ga = T(0);
+          al = group_sum(al, gl);
+          be = group_sum(be, gl);
+          ga = group_sum(ga, gl);
+          const T gt = ga * itol;  // |ga| > tol sqrt(al be), squared with no underflow (al, be > tiny)
+          if (al > tiny && be > tiny && gt * gt > al * be) {
+            T cth, sth;
+            jacobi_rotation(al, be, ga, cth, sth);
             for (int row = sub; row < K; row += gl) {
               const T x = G[i][row], y = G[j][row];
-              al += x * x;
-              be += y * y;
-              ga += x * y;
-            }
-            al = group_sum(al, gl);
-            be = group_sum(be, gl);
-            ga = group_sum(ga, gl);
-            if (al > tiny && be > tiny && fabs(ga) > tol * sqrt(al) * sqrt(be)) {
-              const T zeta = (be - al) / (T(2) * ga);
-              const T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
-              const T cth = T(1) / sqrt(T(1) + t * t);
-              const T sth = cth * t;
-              for (int row = sub; row < K; row += gl) {
-                const T x = G[i][row], y = G[j][row];
-                G[i][row] = cth * x - sth * y;
-                G[j][row] = sth * x + cth * y;
-                if constexpr (!VG) {
-                  const T vx = VA(i, row), vy = VA(j, row);
-                  VA(i, row) = cth * vx - sth * vy;
-                  VA(j, row) = sth * vx + cth * vy;
-                }
+              G[i][row] = cth * x - sth * y;
+              G[j][row] = sth * x + cth * y;
+              if constexpr (!VG) {
+                const T vx = VA(i, row), vy = VA(j, row);
+                VA(i, row) = cth * vx - sth * vy;
+                VA(j, row) = sth * vx + cth * vy;
               }
-              rotated = 1;
             }
+            rotated = 1;
           }
         }
-        __syncthreads();
       }
-      if (!__syncthreads_or(rotated)) break;
+      quiet = __syncthreads_or(rotated) ? 0 : quiet + 1;
     }
   }
+  const int sweeps = (rounds + K2 - 2) / max(1, K2 - 1);
   if (tid == 0 && it.stats != nullptr) {
     atomicAdd(it.stats, 1);
     atomicAdd(it.stats + 1, sweeps);
