@@ -2004,10 +2004,10 @@ static void launch_svd_src(Src d_items, int n_items, int max_k, hipStream_t s) {
     hipLaunchKernelGGL((svd_kernel<T, 64, 256, false, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_k <= 128) {
     if constexpr (sizeof(T) == 4) {
-      hipLaunchKernelGGL((svd_kernel<T, 128, 256, false, Src>), dim3(n_items), dim3(256), 0, s, d_items);
+      hipLaunchKernelGGL((svd_kernel<T, 128, 1024, false, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
     } else {
       // every item must carry a K x K scratch (SvdItem::vwork) for its right vectors
-      hipLaunchKernelGGL((svd_kernel<T, 128, 256, true, Src>), dim3(n_items), dim3(256), 0, s, d_items);
+      hipLaunchKernelGGL((svd_kernel<T, 128, 1024, true, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
     }
   } else {
     throw std::runtime_error("svd: K above 128");
