@@ -1696,6 +1696,17 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   __shared__ T sig[KMAX];
   __shared__ int order[KMAX];
   __shared__ int rank_sh;
+  // QR preconditioning (Drmac-Veselic): M P = Q R by column-pivoted Householder,
+  // then one-sided Jacobi on R^T, whose graded rows converge in few sweeps.  The
+  // reflectors are kept for Q unless the right vectors live in global memory
+  // (VG: the left vectors are then formed as M v / sigma).  The FP32 K = 128
+  // variant has no LDS for them and runs Jacobi on M directly.
+  constexpr bool PRE = VG || KMAX <= 64;
+  __shared__ int perm[KMAX];
+  __shared__ T tauq[KMAX];
+  __shared__ T cnorm[KMAX];
+  __shared__ T Hq[(PRE && !VG) ? KMAX : 1][KMAX + 1];
+  __shared__ int piv_sh;
   const int tid = threadIdx.x;
   if (K <= 0) {
     if (tid == 0 && it.rank_out) *it.rank_out = 0;
@@ -1779,6 +1790,87 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
     // right vectors are formed afterwards as M^T u_c / sigma_c from this copy
     // of the (scaled) matrix, column j at vg + j K
     for (int e = tid; e < K * K; e += NT) vg[(int64_t)(e / K) * K + e % K] = G[e / K][e % K];
+    __syncthreads();
+  }
+  if constexpr (PRE) {
+    for (int c = tid; c < K; c += NT) perm[c] = c;
+    __syncthreads();
+    for (int j = 0; j < K; ++j) {
+      // pivot: the column of largest norm over rows j..K-1
+      for (int c = j + tid; c < K; c += NT) {
+        T q = T(0);
+        for (int row = j; row < K; ++row) q += G[c][row] * G[c][row];
+        cnorm[c] = q;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        int pv = j;
+        for (int c = j + 1; c < K; ++c)
+          if (cnorm[c] > cnorm[pv]) pv = c;
+        piv_sh = pv;
+        if (pv != j) {
+          const int t0 = perm[j];
+          perm[j] = perm[pv];
+          perm[pv] = t0;
+        }
+      }
+      __syncthreads();
+      const int pv = piv_sh;
+      if (pv != j)
+        for (int row = tid; row < K; row += NT) {
+          const T a = G[j][row];
+          G[j][row] = G[pv][row];
+          G[pv][row] = a;
+        }
+      __syncthreads();
+      // Householder reflector of column j below its diagonal (LAPACK dlarfg)
+      if (tid < 64) {
+        T q = T(0);
+        for (int row = j + 1 + tid; row < K; row += 64) q += G[j][row] * G[j][row];
+        q = wave_sum(q);
+        if (tid == 0) {
+          const T alpha = G[j][j];
+          T tau = T(0), beta = alpha, inv = T(0);
+          if (q > T(0)) {
+            beta = (alpha >= T(0) ? T(-1) : T(1)) * sqrt(alpha * alpha + q);
+            tau = (beta - alpha) / beta;
+            inv = T(1) / (alpha - beta);
+          }
+          tauq[j] = tau;
+          cnorm[j] = inv;  // the scaling of v below the diagonal
+          G[j][j] = beta;
+        }
+      }
+      __syncthreads();
+      const T inv = cnorm[j], tau = tauq[j];
+      for (int row = j + 1 + tid; row < K; row += NT) {
+        const T v = G[j][row] * inv;
+        G[j][row] = v;
+        if constexpr (!VG) Hq[j][row] = v;
+      }
+      __syncthreads();
+      // apply H_j to the trailing columns: x -= tau v (v^T x), v = [1; G[j][j+1..]]
+      if (tau != T(0))
+        for (int c = j + 1 + tid; c < K; c += NT) {
+          T w = G[c][j];
+          for (int row = j + 1; row < K; ++row) w += G[j][row] * G[c][row];
+          w *= tau;
+          G[c][j] -= w;
+          for (int row = j + 1; row < K; ++row) G[c][row] -= w * G[j][row];
+        }
+      __syncthreads();
+      for (int row = j + 1 + tid; row < K; row += NT) G[j][row] = T(0);
+      __syncthreads();
+    }
+    // G = R^T (in place): column j of G becomes row j of R
+    for (int e = tid; e < K * K; e += NT) {
+      const int c = e / K, row = e % K;
+      if (row > c) {
+        const T a = G[c][row];
+        G[c][row] = G[row][c];
+        G[row][c] = a;
+      }
+    }
     __syncthreads();
   }
   for (int j = tid; j < K; j += NT) {
@@ -1968,6 +2060,47 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
       const int i = e / r, j = e % r;
       S[i * it.lds + j] = (i == j) ? sig[order[i]] * unscale : T(0);
     }
+  if constexpr (PRE) {
+    // M = (Q V_x) Sigma (P U_x)^T: right vectors from G (rows permuted back),
+    // left vectors Q V_x (reflectors applied in reverse to the rotations), or,
+    // in VG mode, M v / sigma
+    for (int e = tid; e < K * r; e += NT) {
+      const int row = e / r, c = e % r;
+      if (Vm != nullptr) Vm[perm[row] * it.ldv + c] = G[order[c]][row];
+    }
+    if constexpr (!VG) {
+      for (int c = tid; c < r; c += NT) {
+        const int src = order[c];
+        for (int j = K - 1; j >= 0; --j) {
+          const T tau = tauq[j];
+          if (tau == T(0)) continue;
+          T w = VA(src, j);
+          for (int row = j + 1; row < K; ++row) w += Hq[j][row] * VA(src, row);
+          w *= tau;
+          VA(src, j) -= w;
+          for (int row = j + 1; row < K; ++row) VA(src, row) -= w * Hq[j][row];
+        }
+      }
+      __syncthreads();
+      for (int e = tid; e < K * r; e += NT) {
+        const int row = e / r, c = e % r;
+        if (Um != nullptr) Um[row * it.ldu + c] = VA(order[c], row);
+      }
+    } else if (Um != nullptr) {
+      // u_c = M P g_c / sigma_c: column perm[j] of M against entry j of g_c, a wave per output
+      const int lane = tid & 63, wv = tid >> 6;
+      for (int o = wv; o < K * r; o += NT / 64) {
+        const int row = o / r, c = o % r;
+        const int src = order[c];
+        const T sc = sig[src];
+        T acc = T(0);
+        for (int j2 = lane; j2 < K; j2 += 64) acc += vg[(int64_t)perm[j2] * K + row] * G[src][j2];
+        acc = wave_sum(acc);
+        if (lane == 0) Um[row * it.ldu + c] = (sc > sig_floor) ? acc / sc : T(0);
+      }
+    }
+    return;
+  }
   for (int e = tid; e < K * r; e += NT) {
     const int row = e / r, c = e % r;
     const int src = order[c];
