@@ -239,12 +239,18 @@ int frk_ctx_stream(frk_ctx ctx, void** stream) {
 int frk_ctx_counters(frk_ctx ctx, int64_t* out, int n, int reset) {
   return guard([&] {
     if (!ctx || (!out && n > 0) || n < 0) throw std::invalid_argument("null or negative argument");
-    int32_t c[kNumCounters];
+    std::vector<int32_t> c(kCounterWords);
     host_wait_stream(ctx->stream, __func__);
-    FRK_HIP_CHECK(hipMemcpy(c, ctx->ctx->counters(), sizeof(c), hipMemcpyDeviceToHost));
-    for (int i = 0; i < n && i < kNumCounters; ++i) out[i] = c[i];
+    FRK_HIP_CHECK(hipMemcpy(c.data(), ctx->ctx->counters(), sizeof(int32_t) * c.size(), hipMemcpyDeviceToHost));
+    int64_t sum[kNumCounters] = {};
+    for (int k = 0; k < kCounterCopies; ++k)
+      for (int i = 0; i < kNumCounters; ++i) {
+        const int64_t v = c[(size_t)k * kCounterStride + i];
+        sum[i] = (i == kCounterSvdMaxSweeps) ? std::max(sum[i], v) : sum[i] + v;
+      }
+    for (int i = 0; i < n && i < kNumCounters; ++i) out[i] = sum[i];
     if (reset) {
-      FRK_HIP_CHECK(hipMemsetAsync(ctx->ctx->counters(), 0, sizeof(c), ctx->stream));
+      FRK_HIP_CHECK(hipMemsetAsync(ctx->ctx->counters(), 0, sizeof(int32_t) * c.size(), ctx->stream));
       host_wait_stream(ctx->stream, __func__);
     }
     return FRK_OK;

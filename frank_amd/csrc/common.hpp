@@ -25,6 +25,11 @@ namespace frk {
 
 constexpr int kWave = 64;
 
+// The device keeps kCounterCopies copies of the counters, kCounterStride ints
+// apart (one 128-byte line each), so that the workgroups of one launch update
+// different lines (copy blockIdx % kCounterCopies); a read sums (maxes) them.
+constexpr int kCounterCopies = 64, kCounterStride = 32, kCounterWords = kCounterCopies * kCounterStride;
+
 // ---------------------------------------------------------------------------
 // Batched GEMM item: C = alpha * A * B + beta * C, A m x k, B k x n.
 // ---------------------------------------------------------------------------

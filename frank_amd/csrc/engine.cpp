@@ -8,6 +8,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <random>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
 #include <thread>
 
 namespace frk {
@@ -118,8 +121,8 @@ const void* Staging::push_bytes(const void* src, size_t bytes, hipStream_t s) {
 
 Context::Context(hipStream_t s) : s_(s) {
   info_.ensure(sizeof(int32_t) * 4096);
-  counters_.ensure(sizeof(int32_t) * kNumCounters);
-  FRK_HIP_CHECK(hipMemsetAsync(counters_.as<void>(), 0, sizeof(int32_t) * kNumCounters, s_));
+  counters_.ensure(sizeof(int32_t) * kCounterWords);
+  FRK_HIP_CHECK(hipMemsetAsync(counters_.as<void>(), 0, sizeof(int32_t) * kCounterWords, s_));
 }
 
 int32_t* Context::tile_flags() {
@@ -1545,6 +1548,81 @@ static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int 
 // best at N = 32768 (FP32 tiles), one to four chunks tried.
 constexpr int kRestChunks = 2;
 
+// The Schur-update launches of one row chunk of a step, as they are built.
+template <typename T>
+struct ChunkBuild {
+  Context* cc = nullptr;  // stream and workspace of the chunk
+  std::vector<GemmItem> gA, gB, gC, gD;
+  std::vector<LrAddDesc<T>> adds;
+  std::vector<CompressDesc<T>> comp;
+  std::vector<int> comp_add;
+};
+
+// One host thread that runs a posted job while the caller does other work
+// (an exception in the job is rethrown by wait()).
+class HostWorker {
+ public:
+  HostWorker() : th_([this] { loop(); }) {}
+  ~HostWorker() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    th_.join();
+  }
+  void post(std::function<void()> f) {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      job_ = std::move(f);
+      has_ = true;
+      done_ = false;
+    }
+    cv_.notify_all();
+  }
+  void wait() {
+    std::unique_lock<std::mutex> lk(m_);
+    cv_.wait(lk, [&] { return done_; });
+    if (err_) {
+      std::exception_ptr e = err_;
+      err_ = nullptr;
+      std::rethrow_exception(e);
+    }
+  }
+
+ private:
+  void loop() {
+    for (;;) {
+      std::function<void()> j;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return has_ || stop_; });
+        if (!has_) return;
+        j = std::move(job_);
+        has_ = false;
+      }
+      std::exception_ptr e;
+      try {
+        j();
+      } catch (...) {
+        e = std::current_exception();
+      }
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        err_ = e;
+        done_ = true;
+      }
+      cv_.notify_all();
+    }
+  }
+  std::mutex m_;
+  std::condition_variable cv_;
+  std::function<void()> job_;
+  bool has_ = false, done_ = true, stop_ = false;
+  std::exception_ptr err_;
+  std::thread th_;
+};
+
 template <typename T>
 void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
   const hipStream_t s = ctx.stream();
@@ -1580,6 +1658,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
   // does not need them.  The host waits only for the rank snapshot of a finished
   // step, to plan the next rest, and is never on the chain.
   Lookahead la(ctx, A.rank.size());
+  HostWorker worker;
   Context side(la.s);
   side.set_stats_owner(&ctx);
   Context rest2(la.s2);
@@ -1699,9 +1778,6 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     const int ii = A.idx(i, i);
     const int rem = nb - 1 - i;
     // ---- plan the step ---------------------------------------------------
-    std::vector<LrAddDesc<T>> adds;
-    std::vector<CompressDesc<T>> comp;
-    std::vector<int> comp_add;
     const size_t need = step_need(i, max_rank());
     ctx.reserve(need);
     Plan pl(ctx);
@@ -1812,8 +1888,10 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     // the context (stream and workspace) and plan the next targets go to
     Context* cur = &ctx;
     Plan* curp = &pl;
-    // Schur update of one target tile (ic, k): items into gA..gD, additions into adds
-    auto pair_body = [&](int ic, int k) {
+    // Schur update of one target tile (ic, k): items into cb's gA..gD, additions into
+    // cb.adds, temporaries from cb's context (builders of two chunks may run at once:
+    // everything else they touch is read only)
+    auto pair_body = [&](ChunkBuild<T>& cb, int ic, int k) {
       const int tl = A.idx(ic, i), tb = A.idx(i, k), tc = A.idx(ic, k);
       const int kl = A.kind[tl], kb = A.kind[tb], kc = A.kind[tc];
       const int m = (int)A.rows(ic), n = (int)A.cols(k);
@@ -1823,25 +1901,25 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
         if (kl == kLowRank && kb == kLowRank) {
           const int rl = A.rank[tl], rb = A.rank[tb];
           T* Sx = Sxa + (size_t)poff[ic] * RQ + qoff[k];
-          T* z = cur->take<T>((size_t)m * rb + 1);
-          gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, RQ, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
-          gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+          T* z = cb.cc->template take<T>((size_t)m * rb + 1);
+          cb.gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), Sx, RQ, 1, z, 1, m, m, rb, rl, 1.0, 0.0));
+          cb.gD.push_back(gemm_item(z, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
         } else if (kl == kLowRank && kb == kDense) {
           const int rl = A.rank[tl];
-          T* w = cur->take<T>((size_t)rl * n + 1);
-          T* w2 = cur->take<T>((size_t)rl * n + 1);
-          gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
-          gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
-          gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
+          T* w = cb.cc->template take<T>((size_t)rl * n + 1);
+          T* w2 = cb.cc->template take<T>((size_t)rl * n + 1);
+          cb.gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
+          cb.gB.push_back(gemm_item(A.S[tl], A.kcap, 1, w, n, 1, w2, n, 1, rl, n, rl, 1.0, 0.0));
+          cb.gC.push_back(gemm_item(A.U[tl], 1, A.rows(ic), w2, n, 1, C, n, 1, m, n, rl, 1.0, 1.0));
         } else if (kl == kDense && kb == kLowRank) {
           const int rb = A.rank[tb];
-          T* a = cur->take<T>((size_t)m * rb + 1);
-          T* a2 = cur->take<T>((size_t)m * rb + 1);
-          gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
-          gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
-          gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
+          T* a = cb.cc->template take<T>((size_t)m * rb + 1);
+          T* a2 = cb.cc->template take<T>((size_t)m * rb + 1);
+          cb.gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
+          cb.gB.push_back(gemm_item(a, 1, m, A.S[tb], A.kcap, 1, a2, 1, m, m, rb, rb, 1.0, 0.0));
+          cb.gC.push_back(gemm_item(a2, 1, m, A.V[tb], A.cols(k), 1, C, n, 1, m, n, rb, 1.0, 1.0));
         } else {
-          gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, C, n, 1, m, n, n_i, -1.0, 1.0));
+          cb.gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, C, n, 1, m, n, n_i, -1.0, 1.0));
         }
         return;
       }
@@ -1875,8 +1953,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
         ad.xv_cs = 1;
       } else if (kl == kLowRank && kb == kDense) {  // gemm.cpp:219-234
         const int rl = A.rank[tl];
-        T* w = cur->take<T>((size_t)rl * n + 1);
-        gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
+        T* w = cb.cc->template take<T>((size_t)rl * n + 1);
+        cb.gA.push_back(gemm_item(A.V[tl], A.cols(i), 1, A.D[tb], n, 1, w, n, 1, rl, n, n_i, -1.0, 0.0));
         ad.xu = A.U[tl];
         ad.xu_rs = 1;
         ad.xu_cs = A.rows(ic);
@@ -1889,8 +1967,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
         ad.xv_cs = 1;
       } else if (kl == kDense && kb == kLowRank) {  // gemm.cpp:236-251
         const int rb = A.rank[tb];
-        T* a = cur->take<T>((size_t)m * rb + 1);
-        gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
+        T* a = cb.cc->template take<T>((size_t)m * rb + 1);
+        cb.gA.push_back(gemm_item(A.D[tl], n_i, 1, A.U[tb], 1, A.rows(i), a, 1, m, m, rb, n_i, -1.0, 0.0));
         ad.xu = a;
         ad.xu_rs = 1;
         ad.xu_cs = m;
@@ -1902,11 +1980,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
         ad.xv_rs = A.cols(k);
         ad.xv_cs = 1;
       } else {  // D.D -> LR: compress the dense product first (gemm.cpp:202-217)
-        T* prod = cur->take<T>((size_t)m * n + 1);
-        gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, prod, n, 1, m, n, n_i, -1.0, 0.0));
-        T* xu = cur->take<T>((size_t)m * A.kcap + 1);
-        T* xs = cur->take<T>((size_t)A.kcap * A.kcap + 1);
-        T* xv = cur->take<T>((size_t)A.kcap * n + 1);
+        T* prod = cb.cc->template take<T>((size_t)m * n + 1);
+        cb.gA.push_back(gemm_item(A.D[tl], n_i, 1, A.D[tb], n, 1, prod, n, 1, m, n, n_i, -1.0, 0.0));
+        T* xu = cb.cc->template take<T>((size_t)m * A.kcap + 1);
+        T* xs = cb.cc->template take<T>((size_t)A.kcap * A.kcap + 1);
+        T* xv = cb.cc->template take<T>((size_t)A.kcap * n + 1);
         CompressDesc<T> c{};
         c.a = prod;
         c.m = m;
@@ -1924,8 +2002,8 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
         c.lds = A.kcap;
         c.cap = A.kcap;
         c.cap_trunc = A.cap_trunc;
-        comp.push_back(c);
-        comp_add.push_back((int)adds.size());
+        cb.comp.push_back(c);
+        cb.comp_add.push_back((int)cb.adds.size());
         ad.xu = xu;
         ad.xu_rs = 1;
         ad.xu_cs = m;
@@ -1936,59 +2014,66 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
         ad.xv_cs = 1;
         ad.kx1 = ad.kx2 = -1;  // known after compression
       }
-      adds.push_back(ad);
+      cb.adds.push_back(ad);
     };
     // naive additions drain the stream and reset the workspace of the context they run on:
     // give them their own, so Pst, Qst and Sxa survive for the later chunks
-    auto naive_adds = [&]() {
-      if (adds.empty()) return;
-      Context cn(cur->stream());
+    auto naive_adds = [&](ChunkBuild<T>& cb) {
+      if (cb.adds.empty()) return;
+      Context cn(cb.cc->stream());
       cn.set_stats_owner(&ctx);
-      lr_add_naive<T>(adds, cn);
+      lr_add_naive<T>(cb.adds, cn);
+    };
+    // the launches of a chunk's Schur updates into its plan (no D.D -> LR targets)
+    auto plan_chunk = [&](ChunkBuild<T>& cb, Plan& p) {
+      plan_gemms<T>(p, cb.gA, "schur_gemm");
+      plan_gemms<T>(p, cb.gB, "schur_gemm");
+      plan_gemms<T>(p, cb.gC, "schur_gemm");
+      plan_gemms<T>(p, cb.gD, "schur_gemm");
+      plan_lr_add<T>(p, cb.adds, A.lra);
     };
     // plan and issue the Schur updates of a list of targets (the plan may already hold launches)
     auto issue_targets = [&](const std::vector<std::pair<int, int>>& targets, const char* plan_cls) {
-      adds.clear();
-      comp.clear();
-      comp_add.clear();
-      for (auto* g : {&gA, &gB, &gC, &gD}) g->clear();
+      ChunkBuild<T> cb;
+      cb.cc = cur;
       {
         const auto tp0 = clk::now();
-        for (const auto& t : targets) pair_body(t.first, t.second);
+        for (const auto& t : targets) pair_body(cb, t.first, t.second);
         ctx.host_time("host_pairs", ms_since(tp0));
       }
-      plan_gemms<T>(*curp, gA, "schur_gemm");
-      plan_gemms<T>(*curp, gB, "schur_gemm");
-      plan_gemms<T>(*curp, gC, "schur_gemm");
-      plan_gemms<T>(*curp, gD, "schur_gemm");
-      if (!comp.empty()) {
+      plan_gemms<T>(*curp, cb.gA, "schur_gemm");
+      plan_gemms<T>(*curp, cb.gB, "schur_gemm");
+      plan_gemms<T>(*curp, cb.gC, "schur_gemm");
+      plan_gemms<T>(*curp, cb.gD, "schur_gemm");
+      if (!cb.comp.empty()) {
         // D.D -> LR needs the compressed ranks on the host: run what is planned,
         // compress on a side context (own workspace), then plan the additions.
         curp->run();
-        DevBuf crank(sizeof(int32_t) * comp.size());
-        for (size_t c2 = 0; c2 < comp.size(); ++c2) comp[c2].d_rank = crank.as<int32_t>() + c2;
+        DevBuf crank(sizeof(int32_t) * cb.comp.size());
+        for (size_t c2 = 0; c2 < cb.comp.size(); ++c2) cb.comp[c2].d_rank = crank.as<int32_t>() + c2;
         Context cside(cur->stream());
-        compress_batched<T>(comp, cside);
-        std::vector<int32_t> cr(comp.size());
+        compress_batched<T>(cb.comp, cside);
+        std::vector<int32_t> cr(cb.comp.size());
         FRK_HIP_CHECK(hipMemcpyAsync(cr.data(), crank.as<int32_t>(), sizeof(int32_t) * cr.size(),
                                      hipMemcpyDeviceToHost, cur->stream()));
         host_wait_stream(cur->stream(), __func__);
-        for (size_t c2 = 0; c2 < comp.size(); ++c2) adds[comp_add[c2]].kx1 = adds[comp_add[c2]].kx2 = cr[c2];
+        for (size_t c2 = 0; c2 < cb.comp.size(); ++c2)
+          cb.adds[cb.comp_add[c2]].kx1 = cb.adds[cb.comp_add[c2]].kx2 = cr[c2];
         if (A.lra == kLraNaive) {
-          naive_adds();
+          naive_adds(cb);
         } else {
           Plan p2(*cur);
-          plan_lr_add<T>(p2, adds, A.lra);
+          plan_lr_add<T>(p2, cb.adds, A.lra);
           p2.run();
         }
         return;
       }
       if (A.lra == kLraNaive) {
         curp->run();
-        naive_adds();
+        naive_adds(cb);
       } else {
         const auto tl0 = clk::now();
-        plan_lr_add<T>(*curp, adds, A.lra);
+        plan_lr_add<T>(*curp, cb.adds, A.lra);
         ctx.host_time("host_lradd", ms_since(tl0));
         ctx.host_time(plan_cls, ms_since(th));
         th = clk::now();
@@ -2000,26 +2085,72 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     //    on the main stream, the second on the third stream with its own workspace, once
     //    the panels and stacked products above are done.  Each chunk ends in a batched
     //    recompression whose latency does not shrink with the batch, so the two overlap.
-    pl.run();
-    FRK_HIP_CHECK(hipEventRecord(la.sxa, s));
     const int nchunk = std::min(kRestChunks, rem);
-    std::unique_ptr<Plan> pl2;
+    std::vector<std::vector<std::pair<int, int>>> targets(nchunk);
+    bool dd_lr = false;  // a D.D -> LR target needs a host synchronization inside its chunk
     for (int c = 0; c < nchunk; ++c) {
-      if (c > 0) th = clk::now();
       const int ic0 = i + 1 + (rem * c) / nchunk, ic1 = i + 1 + (rem * (c + 1)) / nchunk;
-      std::vector<std::pair<int, int>> targets;
       for (int ic = ic0; ic < ic1; ++ic)
-        for (int k = i + 1; k < nb; ++k)
-          if (!(ic == i + 1 && k == i + 1)) targets.emplace_back(ic, k);
-      if (c == 1) {
-        FRK_HIP_CHECK(hipStreamWaitEvent(la.s2, la.sxa, 0));
-        rest2.reserve(need);
-        pl2 = std::make_unique<Plan>(rest2);
-        cur = &rest2;
-        curp = pl2.get();
+        for (int k = i + 1; k < nb; ++k) {
+          if (ic == i + 1 && k == i + 1) continue;
+          targets[c].emplace_back(ic, k);
+          dd_lr = dd_lr || (A.kind[A.idx(ic, i)] == kDense && A.kind[A.idx(i, k)] == kDense &&
+                            A.kind[A.idx(ic, k)] == kLowRank);
+        }
+    }
+    if (nchunk == 2 && !dd_lr && A.lra != kLraNaive) {
+      // the second chunk's descriptors are built on the worker thread while this
+      // thread issues the panels and the stacked products and builds the first chunk
+      rest2.reserve(need);
+      ChunkBuild<T> cb1;
+      cb1.cc = &rest2;
+      Plan p2(rest2);
+      worker.post([&]() {
+        for (const auto& t : targets[1]) pair_body(cb1, t.first, t.second);
+        plan_chunk(cb1, p2);
+      });
+      // the job uses this frame's cb1, p2 and targets: never leave it (an exception
+      // included) while the job still runs
+      struct JoinOnExit {
+        HostWorker& w;
+        bool armed = true;
+        ~JoinOnExit() {
+          if (armed) try {
+              w.wait();
+            } catch (...) {
+            }
+        }
+      } join_on_exit{worker};
+      pl.run();
+      FRK_HIP_CHECK(hipEventRecord(la.sxa, s));
+      ChunkBuild<T> cb0;
+      cb0.cc = &ctx;
+      for (const auto& t : targets[0]) pair_body(cb0, t.first, t.second);
+      plan_chunk(cb0, pl);
+      ctx.host_time("host_plan", ms_since(th));
+      th = clk::now();
+      pl.run();
+      join_on_exit.armed = false;
+      worker.wait();
+      FRK_HIP_CHECK(hipStreamWaitEvent(la.s2, la.sxa, 0));
+      p2.run();
+      ctx.host_time("host_issue", ms_since(th));
+    } else {
+      pl.run();
+      FRK_HIP_CHECK(hipEventRecord(la.sxa, s));
+      std::unique_ptr<Plan> pl2;
+      for (int c = 0; c < nchunk; ++c) {
+        if (c > 0) th = clk::now();
+        if (c == 1) {
+          FRK_HIP_CHECK(hipStreamWaitEvent(la.s2, la.sxa, 0));
+          rest2.reserve(need);
+          pl2 = std::make_unique<Plan>(rest2);
+          cur = &rest2;
+          curp = pl2.get();
+        }
+        issue_targets(targets[c], c == 0 ? "host_plan" : "host_plan_rest");
+        curp->run();
       }
-      issue_targets(targets, c == 0 ? "host_plan" : "host_plan_rest");
-      curp->run();
     }
     // both chunks done: the rank snapshot (second stream), and the main stream waits for
     // the second chunk before the next priority block

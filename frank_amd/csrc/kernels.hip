@@ -1951,9 +1951,12 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   }
   const int sweeps = (rounds + K2 - 2) / max(1, K2 - 1);
   if (tid == 0 && it.stats != nullptr) {
-    atomicAdd(it.stats, 1);
-    atomicAdd(it.stats + 1, sweeps);
-    atomicMax(it.stats + 2, sweeps);
+    // this workgroup's copy of the counters (a line of its own: no contention
+    // between the items of a launch)
+    int32_t* st = it.stats + (blockIdx.x % kCounterCopies) * kCounterStride;
+    atomicAdd(st, 1);
+    atomicAdd(st + 1, sweeps);
+    atomicMax(st + 2, sweeps);
   }
   for (int j = tid; j < K; j += NT) {
     T s = T(0);
