@@ -295,6 +295,9 @@ struct SolveUpdItem {
 constexpr int kDfRows = 16;
 constexpr int kDfArows = 64;
 constexpr int kDfYtiles = 4;
+// completion flags per block: one per D task (512 / kDfRows) and per A task (512 / kDfArows)
+constexpr int kDfXflags = 512 / kDfRows;
+constexpr int kDfAflags = 512 / kDfArows;
 struct DfBlock {
   const void* tinv;  // n x n, row stride ld; used as op(Tinv) = Tinv^T when trans
   int64_t ld;
@@ -324,7 +327,7 @@ struct DfSweep {
   const int32_t* by_src;
   const void* tasks;    // int4 {type (0 D, 1 Y, 2 A), position, arg0, arg1}
   void* y;
-  int32_t* flags;       // ticket, error, xcnt[nb], adone[nb], ydone[ntiles]
+  int32_t* flags;       // ticket, error, D-task flags [nb][kDfXflags], A-task flags [nb][kDfAflags], ydone[ntiles]
   int32_t nb, ntiles, ntasks, nrhs;
   int64_t spin_limit;
 };

@@ -2365,7 +2365,7 @@ static void blr_solve_df(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& c
     ntiles_max = nt;
     rsum_max = rs;
   }
-  const size_t nflags = 2 + 2 * (size_t)nb + ntiles_max;
+  const size_t nflags = 2 + (size_t)(kDfXflags + kDfAflags) * nb + ntiles_max;
   ctx.reserve(2 * sizeof(T) * ((size_t)N * nrhs + 64) + 2 * sizeof(T) * ((size_t)rsum_max * NR + 64) +
               2 * sizeof(int32_t) * (nflags + 64) + 8192);
   T* y = ctx.take<T>((size_t)N * nrhs + 1);
@@ -2466,7 +2466,7 @@ static void blr_solve_df(BlrMatrix<T>& F, T* b, int nrhs, bool trans, Context& c
     sw.ntasks = (int)tasks.size();
     sw.nrhs = nrhs;
     sw.spin_limit = int64_t(1) << 21;
-    const int nfl = (int)(2 + 2 * nb + tiles.size());
+    const int nfl = (int)(2 + (kDfXflags + kDfAflags) * nb + tiles.size());
     double fl = 0, by = 0;
     for (const auto& B : blocks) {
       fl += 2.0 * B.n * B.n * nrhs;

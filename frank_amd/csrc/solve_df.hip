@@ -80,16 +80,21 @@ __device__ __forceinline__ T sum64(T v) {
   return v;
 }
 
-// Poll until *flag >= target (thread 0), then acquire.  Returns false (in
-// every thread) when the launch has failed or this wait exceeded the limit.
-__device__ __forceinline__ bool df_wait(const int32_t* flag, int32_t target, int32_t* err, int64_t limit,
-                                        int* bail_sh) {
-  if (threadIdx.x == 0) {
+// Poll until the n flags f[0 .. n) are all set (wave 0, a lane per flag, n <= 64),
+// then acquire.  A task's completion is its own flag, never a shared counter:
+// the tasks of a block finish at nearly the same time, and read-modify-writes
+// of one word from many workgroups serialise.
+__device__ __forceinline__ bool df_wait_all(const int32_t* f, int n, int32_t* err, int64_t limit, int* bail_sh) {
+  if (threadIdx.x < 64) {
+    const int lane = threadIdx.x;
+    bool ok = lane >= n;
     int64_t spins = 0;
     int bail = 0;
-    while (__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    for (;;) {
+      if (!ok) ok = __hip_atomic_load(&f[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      if (__all(ok)) break;
       if (++spins > limit) {
-        __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         bail = 1;
         break;
       }
@@ -100,16 +105,16 @@ __device__ __forceinline__ bool df_wait(const int32_t* flag, int32_t target, int
       __builtin_amdgcn_s_sleep(1);
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    *bail_sh = bail;
+    if (lane == 0) *bail_sh = bail;
   }
   __syncthreads();
   return *bail_sh == 0;
 }
 
 // Publish: every thread's stores precede the release (barrier + cumulative release).
-__device__ __forceinline__ void df_add(int32_t* cnt) {
+__device__ __forceinline__ void df_set(int32_t* flag) {
   __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) __hip_atomic_store(flag, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // x (n x nrhs, row-major) into xs[512][NR], zero past n and nrhs: every load of
@@ -141,8 +146,8 @@ __device__ bool df_task_d(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
   const DfBlock* blks = static_cast<const DfBlock*>(sw.blocks);
   const DfTile* tiles = static_cast<const DfTile*>(sw.tiles);
   int32_t* err = sw.flags + 1;
-  int32_t* xcnt = sw.flags + 2;
-  int32_t* adone = xcnt + sw.nb;
+  int32_t* xflag = sw.flags + 2;
+  int32_t* aflag = xflag + kDfXflags * sw.nb;
   const int nrhs = sw.nrhs;
   const int tid = threadIdx.x;
   const DfBlock B = blks[tk.y];
@@ -228,7 +233,7 @@ __device__ bool df_task_d(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
   // pre = op(Tinv)[rows] (b - older updates)
   const T* src = static_cast<const T*>(B.b);
   if (B.has_bt) {
-    if (!df_wait(&adone[tk.y], B.na, err, sw.spin_limit, bail_sh)) return false;
+    if (!df_wait_all(aflag + kDfAflags * tk.y, B.na, err, sw.spin_limit, bail_sh)) return false;
     src = static_cast<const T*>(B.bt);
   }
   df_stage<T, NR>(xs, src, n, nrhs);
@@ -246,7 +251,8 @@ __device__ bool df_task_d(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
   for (int c = 0; c < NR; ++c) pre[c] = sum16(pre[c]);
   if (B.last >= 0) {
     // the previous block complete: x[rows] = pre - W x_{p-1}
-    if (!df_wait(&xcnt[tk.y - 1], blks[tk.y - 1].nd, err, sw.spin_limit, bail_sh)) return false;
+    if (!df_wait_all(xflag + kDfXflags * (tk.y - 1), blks[tk.y - 1].nd, err, sw.spin_limit, bail_sh))
+      return false;
     df_stage<T, NR>(xs, static_cast<const T*>(blks[tk.y - 1].x), nprev, nrhs);
     __syncthreads();
     T d[NR];
@@ -269,7 +275,7 @@ __device__ bool df_task_d(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
       if (c == sub) v = pre[c];
     x[(int64_t)row * nrhs + sub] = v;
   }
-  df_add(&xcnt[tk.y]);
+  df_set(xflag + kDfXflags * tk.y + tk.z);
   return true;
 }
 
@@ -283,8 +289,8 @@ __device__ bool df_task_y(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
   const DfBlock* blks = static_cast<const DfBlock*>(sw.blocks);
   const DfTile* tiles = static_cast<const DfTile*>(sw.tiles);
   int32_t* err = sw.flags + 1;
-  int32_t* xcnt = sw.flags + 2;
-  int32_t* ydone = xcnt + 2 * sw.nb;
+  int32_t* xflag = sw.flags + 2;
+  int32_t* ydone = xflag + (kDfXflags + kDfAflags) * sw.nb;
   T* ybuf = static_cast<T*>(sw.y);
   const int nrhs = sw.nrhs;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -329,7 +335,7 @@ __device__ bool df_task_y(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
       }
     }
   }
-  if (!df_wait(&xcnt[tk.y], B.nd, err, sw.spin_limit, bail_sh)) return false;
+  if (!df_wait_all(xflag + kDfXflags * tk.y, B.nd, err, sw.spin_limit, bail_sh)) return false;
   df_stage<T, NR>(xs, static_cast<const T*>(B.x), n, nrhs);
   __syncthreads();
   // ts = V x (r x NR), four rows of V at a time, lanes over columns
@@ -407,9 +413,9 @@ __device__ bool df_task_a(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
   const DfBlock* blks = static_cast<const DfBlock*>(sw.blocks);
   const DfTile* tiles = static_cast<const DfTile*>(sw.tiles);
   int32_t* err = sw.flags + 1;
-  int32_t* xcnt = sw.flags + 2;
-  int32_t* adone = xcnt + sw.nb;
-  int32_t* ydone = adone + sw.nb;
+  int32_t* xflag = sw.flags + 2;
+  int32_t* aflag = xflag + kDfXflags * sw.nb;
+  int32_t* ydone = aflag + kDfAflags * sw.nb;
   const T* ybuf = static_cast<const T*>(sw.y);
   const int nrhs = sw.nrhs;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -557,7 +563,7 @@ __device__ bool df_task_a(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
         ss[a][b] = (a < r && b < r) ? sv[j] : T(0);
       }
     }
-    if (!df_wait(&xcnt[tk.y - 2], Q.nd, err, sw.spin_limit, bail_sh)) return false;
+    if (!df_wait_all(xflag + kDfXflags * (tk.y - 2), Q.nd, err, sw.spin_limit, bail_sh)) return false;
     df_stage<T, NR>(xs, static_cast<const T*>(Q.x), nq, nrhs);
     __syncthreads();
     // ts = V x: wave w forms rows w, w + 4, ... (lanes over columns)
@@ -622,7 +628,7 @@ __device__ bool df_task_a(const DfSweep& sw, int4 tk, unsigned char* lds, int* b
       const int c = w + 4 * m;
       if (c < nrhs) bt[(int64_t)row * nrhs + c] = b[(int64_t)row * nrhs + c] - acc[m];
     }
-  df_add(&adone[tk.y]);
+  df_set(aflag + kDfAflags * tk.y + tk.z);
   return true;
 }
 
