@@ -2942,16 +2942,19 @@ template void launch_solve_update<float>(const SolveUpdItem*, int, int, hipStrea
 // tile of the block row) as one GEMM over the concatenated rows.  A 16 x 64
 // output tile per workgroup, K staged through LDS in chunks of 64.
 // ===========================================================================
-// The 16-row slab is staged in LDS once; the four waves split K, every lane
-// owns one output column (so the reads of B are contiguous in the row-major
-// case and cache-line reuse serves the transposed one), and the partial sums
-// of the waves meet in LDS.
+// The 16-row slab is staged in LDS once; the kRowsWaves waves split K, every
+// lane owns one output column (so the reads of B are contiguous in the
+// row-major case and cache-line reuse serves the transposed one), and the
+// partial sums of the waves meet in LDS.  The kernel is bound by the chain of
+// B reads of one wave: eight waves halve it against four.
+constexpr int kRowsWaves = 8;
 template <typename T, class RowFn>
 __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_t* __restrict__ d_count,
                                                const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
                                                T* __restrict__ W, int N, int K) {
+  constexpr int NT = 64 * kRowsWaves, KH = (512 + NT - 1) / NT;
   __shared__ T As[16][512 + 1];
-  __shared__ T red[4][16][64];
+  __shared__ T red[kRowsWaves][16][64];
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int r0 = 16 * blockIdx.x, col = 64 * blockIdx.y + lane;
   if (r0 >= R || 64 * (int)blockIdx.y >= N) return;
@@ -2961,12 +2964,12 @@ __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_
     const T* rp[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) rp[r] = row_of(r0 + r < R ? r0 + r : r0);
-    T v[16][2];
+    T v[16][KH];
 #pragma unroll
     for (int r = 0; r < 16; ++r)
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = tid + 256 * h;
+      for (int h = 0; h < KH; ++h) {
+        const int k = tid + NT * h;
         v[r][h] = rp[r][k < K ? k : 0];
       }
     // the device count (rows actually present, <= R) is read only now, so its
@@ -2976,8 +2979,8 @@ __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_
 #pragma unroll
     for (int r = 0; r < 16; ++r)
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int k = tid + 256 * h;
+      for (int h = 0; h < KH; ++h) {
+        const int k = tid + NT * h;
         if (k < K) As[r][k] = (r0 + r < R) ? v[r][h] : T(0);
       }
   }
@@ -2985,7 +2988,7 @@ __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_
   T acc[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = T(0);
-  const int kq = (K + 3) / 4, k0 = wv * kq, k1 = min(K, k0 + kq);
+  const int kq = (K + kRowsWaves - 1) / kRowsWaves, k0 = wv * kq, k1 = min(K, k0 + kq);
   if (col < N) {
     const T* bc = B + (int64_t)col * b_cs;
 #pragma unroll 4
@@ -2998,25 +3001,30 @@ __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_
 #pragma unroll
   for (int r = 0; r < 16; ++r) red[wv][r][lane] = acc[r];
   __syncthreads();
-  for (int e = tid; e < 16 * 64; e += 256) {
+  for (int e = tid; e < 16 * 64; e += NT) {
     const int r = e >> 6, c = e & 63;
     const int oc = 64 * blockIdx.y + c;
-    if (r0 + r < R && oc < N) W[(int64_t)(r0 + r) * N + oc] = red[0][r][c] + red[1][r][c] + red[2][r][c] + red[3][r][c];
+    if (r0 + r < R && oc < N) {
+      T sum = T(0);
+#pragma unroll
+      for (int w2 = 0; w2 < kRowsWaves; ++w2) sum += red[w2][r][c];
+      W[(int64_t)(r0 + r) * N + oc] = sum;
+    }
   }
 }
 
 // d_count (optional): the number of rows actually present, read on the device
 // (the table then holds R >= count entries); workgroups past it exit.
 template <typename T>
-__global__ __launch_bounds__(256) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
-                                                        const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
-                                                        T* __restrict__ W, int N, int K,
-                                                        const int32_t* __restrict__ d_count) {
+__global__ __launch_bounds__(64 * kRowsWaves) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
+                                                                    const T* __restrict__ B, int64_t b_rs,
+                                                                    int64_t b_cs, T* __restrict__ W, int N, int K,
+                                                                    const int32_t* __restrict__ d_count) {
   rows_gemm_body<T>([&](int r) { return static_cast<const T*>(rows[r].a); }, R, d_count, B, b_rs, b_cs, W, N, K);
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void rows_pair_kernel(RowsPair p) {
+__global__ __launch_bounds__(64 * kRowsWaves) void rows_pair_kernel(RowsPair p) {
   const RowsJob j = blockIdx.z ? p.job[1] : p.job[0];
   const T* base = static_cast<const T*>(j.base);
   const int64_t st = j.stride;
@@ -3040,7 +3048,7 @@ void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int
                       hipStream_t s, const int32_t* d_count) {
   if (R <= 0) return;
   if (N <= 0 || K <= 0 || K > 512) throw std::runtime_error("rows gemm: N > 0, 0 < K <= 512");
-  hipLaunchKernelGGL(rows_gemm_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64), dim3(256), 0, s, d_rows, R, B, b_rs,
+  hipLaunchKernelGGL(rows_gemm_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64), dim3(64 * kRowsWaves), 0, s, d_rows, R, B, b_rs,
                      b_cs, W, N, K, d_count);
   FRK_HIP_CHECK(hipGetLastError());
 }
@@ -3060,7 +3068,7 @@ void launch_rows_pair(const RowsPair& p, hipStream_t s) {
     N = std::max(N, (int)j.N);
   }
   if (R <= 0) return;
-  hipLaunchKernelGGL(rows_pair_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64, 2), dim3(256), 0, s, p);
+  hipLaunchKernelGGL(rows_pair_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64, 2), dim3(64 * kRowsWaves), 0, s, p);
   FRK_HIP_CHECK(hipGetLastError());
 }
 template void launch_rows_pair<double>(const RowsPair&, hipStream_t);
