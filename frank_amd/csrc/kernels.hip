@@ -2947,11 +2947,13 @@ template void launch_solve_update<float>(const SolveUpdItem*, int, int, hipStrea
 // row-major case and cache-line reuse serves the transposed one), and the
 // partial sums of the waves meet in LDS.  The kernel is bound by the chain of
 // B reads of one wave: eight waves halve it against four.
-constexpr int kRowsWaves = 8;
+template <typename T>
+constexpr int rows_waves() { return sizeof(T) == 4 ? 16 : 8; }
 template <typename T, class RowFn>
 __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_t* __restrict__ d_count,
                                                const T* __restrict__ B, int64_t b_rs, int64_t b_cs,
                                                T* __restrict__ W, int N, int K) {
+  constexpr int kRowsWaves = rows_waves<T>();
   constexpr int NT = 64 * kRowsWaves, KH = (512 + NT - 1) / NT;
   __shared__ T As[16][512 + 1];
   __shared__ T red[kRowsWaves][16][64];
@@ -3016,7 +3018,7 @@ __device__ __forceinline__ void rows_gemm_body(RowFn row_of, int R, const int32_
 // d_count (optional): the number of rows actually present, read on the device
 // (the table then holds R >= count entries); workgroups past it exit.
 template <typename T>
-__global__ __launch_bounds__(64 * kRowsWaves) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
+__global__ __launch_bounds__(64 * rows_waves<T>()) void rows_gemm_kernel(const RowRef* __restrict__ rows, int R,
                                                                     const T* __restrict__ B, int64_t b_rs,
                                                                     int64_t b_cs, T* __restrict__ W, int N, int K,
                                                                     const int32_t* __restrict__ d_count) {
@@ -3024,7 +3026,7 @@ __global__ __launch_bounds__(64 * kRowsWaves) void rows_gemm_kernel(const RowRef
 }
 
 template <typename T>
-__global__ __launch_bounds__(64 * kRowsWaves) void rows_pair_kernel(RowsPair p) {
+__global__ __launch_bounds__(64 * rows_waves<T>()) void rows_pair_kernel(RowsPair p) {
   const RowsJob j = blockIdx.z ? p.job[1] : p.job[0];
   const T* base = static_cast<const T*>(j.base);
   const int64_t st = j.stride;
@@ -3048,7 +3050,7 @@ void launch_rows_gemm(const RowRef* d_rows, int R, const T* B, int64_t b_rs, int
                       hipStream_t s, const int32_t* d_count) {
   if (R <= 0) return;
   if (N <= 0 || K <= 0 || K > 512) throw std::runtime_error("rows gemm: N > 0, 0 < K <= 512");
-  hipLaunchKernelGGL(rows_gemm_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64), dim3(64 * kRowsWaves), 0, s, d_rows, R, B, b_rs,
+  hipLaunchKernelGGL(rows_gemm_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64), dim3(64 * rows_waves<T>()), 0, s, d_rows, R, B, b_rs,
                      b_cs, W, N, K, d_count);
   FRK_HIP_CHECK(hipGetLastError());
 }
@@ -3068,7 +3070,7 @@ void launch_rows_pair(const RowsPair& p, hipStream_t s) {
     N = std::max(N, (int)j.N);
   }
   if (R <= 0) return;
-  hipLaunchKernelGGL(rows_pair_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64, 2), dim3(64 * kRowsWaves), 0, s, p);
+  hipLaunchKernelGGL(rows_pair_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64, 2), dim3(64 * rows_waves<T>()), 0, s, p);
   FRK_HIP_CHECK(hipGetLastError());
 }
 template void launch_rows_pair<double>(const RowsPair&, hipStream_t);
