@@ -1795,12 +1795,19 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   if constexpr (PRE) {
     for (int c = tid; c < K; c += NT) perm[c] = c;
     __syncthreads();
+    // gq lanes of one wave per column (rows split between them)
+    int gq = 64;
+    while (gq > 1 && K * gq > NT) gq >>= 1;
+    const int qc = tid / gq, qg = tid % gq;
     for (int j = 0; j < K; ++j) {
       // pivot: the column of largest norm over rows j..K-1
-      for (int c = j + tid; c < K; c += NT) {
+      for (int c0 = j; c0 < K; c0 += NT / gq) {
+        const int c = c0 + qc;
         T q = T(0);
-        for (int row = j; row < K; ++row) q += G[c][row] * G[c][row];
-        cnorm[c] = q;
+        if (c < K)
+          for (int row = j + qg; row < K; row += gq) q += G[c][row] * G[c][row];
+        q = group_sum(q, gq);
+        if (c < K && qg == 0) cnorm[c] = q;
       }
       __syncthreads();
       if (tid == 0) {
@@ -1851,12 +1858,18 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
       __syncthreads();
       // apply H_j to the trailing columns: x -= tau v (v^T x), v = [1; G[j][j+1..]]
       if (tau != T(0))
-        for (int c = j + 1 + tid; c < K; c += NT) {
-          T w = G[c][j];
-          for (int row = j + 1; row < K; ++row) w += G[j][row] * G[c][row];
-          w *= tau;
-          G[c][j] -= w;
-          for (int row = j + 1; row < K; ++row) G[c][row] -= w * G[j][row];
+        for (int c0 = j + 1; c0 < K; c0 += NT / gq) {
+          const int c = c0 + qc;
+          T w = T(0);
+          if (c < K) {
+            if (qg == 0) w = G[c][j];
+            for (int row = j + 1 + qg; row < K; row += gq) w += G[j][row] * G[c][row];
+          }
+          w = group_sum(w, gq) * tau;
+          if (c < K) {
+            if (qg == 0) G[c][j] -= w;
+            for (int row = j + 1 + qg; row < K; row += gq) G[c][row] -= w * G[j][row];
+          }
         }
       __syncthreads();
       for (int row = j + 1 + tid; row < K; row += NT) G[j][row] = T(0);
