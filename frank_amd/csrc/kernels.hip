@@ -6,8 +6,10 @@
 //    lane l holds rows l + 64*t.  Column reductions are DPP / swizzle /
 //    permlane butterflies (bitwise identical in every lane); the only workgroup
 //    barrier per Householder step publishes the reflector through LDS.
-//  * Jacobi SVD of the K x K core runs in LDS with a round-robin pairing so
-//    K/2 rotations proceed in parallel, one lane group per pair.
+//  * Jacobi SVD of the K x K core: for K <= 16 one wave keeps the core in
+//    VGPRs (a lane group per column pair, columns moved between pairs by DPP
+//    wave shifts); larger cores run in LDS, QR-preconditioned, with a
+//    round-robin pairing so K/2 rotations proceed in parallel.
 //  * Column-pivoted QR streams the tile through L2 with one thread per column
 //    (row-major tiles => every row sweep is coalesced).
 #include <hip/hip_runtime.h>
@@ -2142,11 +2144,391 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident one-sided Jacobi SVD for K <= KMAX (16 or 32): one wave per
+// core and no LDS traffic inside the rounds.  The round-robin schedule has K2
+// positions (K rounded up to even; a zero column pads odd K) in NS slots of two:
+// the top of slot s is position s, its bottom position K2-1-s.  Lane l serves
+// slot s = l % NS together with the other NL lanes of that slot, holding rows
+// g + NL q (g = l / NS) of the top column in x[], of the bottom column in y[] and
+// of their right vectors in vx[], vy[].  A round rotates every slot's pair at
+// once; the columns then move one place round the circle (top 0 fixed, top k-1
+// -> top k, bottom 0 -> top 1, bottom k+1 -> bottom k, top P-1 -> bottom P-1).
+// Neighbouring slots are neighbouring lanes, so a move is one DPP wave shift per
+// register; the column sums are a DPP / swizzle / permlane butterfly.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T wave_shr1(T v) {  // lane l receives lane l - 1
+  return dpp_mov<0x138>(v);
+}
+template <typename T>
+__device__ __forceinline__ T wave_shl1(T v) {  // lane l receives lane l + 1
+  return dpp_mov<0x130>(v);
+}
+__device__ __forceinline__ float half_max(float v) {
+  const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return fmaxf(__int_as_float(r[0]), __int_as_float(r[1]));
+}
+__device__ __forceinline__ double half_max(double v) {
+  const auto h = __builtin_amdgcn_permlane32_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const auto l = __builtin_amdgcn_permlane32_swap(__double2loint(v), __double2loint(v), false, false);
+  return fmax(__hiloint2double(h[0], l[0]), __hiloint2double(h[1], l[1]));
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+  v = fmax(v, dpp_mov<0xB1>(v));
+  v = fmax(v, dpp_mov<0x4E>(v));
+  v = fmax(v, dpp_mov<0x141>(v));
+  v = fmax(v, dpp_mov<0x140>(v));
+  v = fmax(v, xor16(v));
+  return half_max(v);
+}
+// v[l] + v[l ^ 16] through v_permlane16_swap (gfx950), operands in the same order in every lane
+__device__ __forceinline__ float pair16_sum(float v) {
+  const auto r = __builtin_amdgcn_permlane16_swap(__float_as_int(v), __float_as_int(v), false, false);
+  return __int_as_float(r[0]) + __int_as_float(r[1]);
+}
+__device__ __forceinline__ double pair16_sum(double v) {
+  const auto h = __builtin_amdgcn_permlane16_swap(__double2hiint(v), __double2hiint(v), false, false);
+  const auto l = __builtin_amdgcn_permlane16_swap(__double2loint(v), __double2loint(v), false, false);
+  return __hiloint2double(h[0], l[0]) + __hiloint2double(h[1], l[1]);
+}
+// sum over the NL lanes of a slot (lanes s + NS t), the same bits in each of them
+template <int NL, typename T>
+__device__ __forceinline__ T slot_sum(T v) {
+  static_assert(NL == 4 || NL == 8, "slot of 4 or 8 lanes");
+  if constexpr (NL == 8) v += dpp_mov<0x128>(v);  // row_ror:8 = lane l ^ 8
+  return half_sum(pair16_sum(v));
+}
+
+template <typename T, int KMAX, class Src>
+__global__ __launch_bounds__(64) void svd_reg_kernel(Src items) {
+  constexpr int NS = KMAX / 2, NL = 64 / NS, NR = KMAX / NL;
+  const SvdItem it = items.get(blockIdx.x);
+  const int K = it.K;
+  const int lane = threadIdx.x;
+  const int s = lane % NS, g = lane / NS;
+  // staging (Ru, or M in mode 0; Rv); the left vectors on the completion path
+  __shared__ T A1[KMAX][KMAX + 1];
+  __shared__ T A2[KMAX][KMAX + 1];
+  __shared__ T sig[KMAX];   // by position
+  __shared__ T sigs[KMAX];  // descending
+  __shared__ int order[KMAX];
+  __shared__ int rk_of[KMAX];
+  __shared__ T coef[KMAX];
+  if (K <= 0) {
+    if (lane == 0 && it.rank_out) *it.rank_out = 0;
+    return;
+  }
+  const int K2 = K + (K & 1), P = K2 / 2;
+  const int pt = s, pb = K2 - 1 - s;
+  const bool act = s < P;
+  if (it.mode == 0) {
+    const T* M = static_cast<const T*>(it.M);
+#pragma unroll
+    for (int k = 0; k < KMAX * KMAX / 64; ++k) {
+      const int e = lane + 64 * k;
+      const int i = e / KMAX, j = e % KMAX;
+      A1[i][j] = (i < K && j < K) ? M[i * it.m_rs + j * it.m_cs] : T(0);
+    }
+  } else {
+    const int J = it.kx;
+    const T* Ru = static_cast<const T*>(it.Ru);
+    const T* Rv = static_cast<const T*>(it.Rv);
+#pragma unroll
+    for (int k = 0; k < KMAX * KMAX / 64; ++k) {
+      const int e = lane + 64 * k;
+      const int i = e / KMAX, t = e % KMAX;
+      const bool in = i < K && t < J;
+      A1[i][t] = in ? Ru[i * it.ldr + t] : T(0);
+      A2[i][t] = in ? Rv[i * it.ldr + t] : T(0);
+    }
+  }
+  __syncthreads();
+  T x[NR], y[NR], vx[NR], vy[NR];
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int ri = g + NL * q;
+    x[q] = T(0);
+    y[q] = T(0);
+    vx[q] = (act && ri == pt && ri < K) ? T(1) : T(0);
+    vy[q] = (act && ri == pb && ri < K) ? T(1) : T(0);
+  }
+  if (act) {
+    if (it.mode == 0) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        x[q] = A1[g + NL * q][pt];
+        y[q] = A1[g + NL * q][pb];
+      }
+    } else {
+      // M(i, j) = sum_t Ru(i, t) Rv(j, t) (+ Sa(i, j) inside the leading ka x ka block)
+      const int J = it.kx;
+      for (int t = 0; t < J; ++t) {
+        const T a = A2[pt][t], b = A2[pb][t];
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const T u = A1[g + NL * q][t];
+          x[q] += u * a;
+          y[q] += u * b;
+        }
+      }
+      if (it.ka > 0) {
+        const T* Sa = static_cast<const T*>(it.Sa);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const int ri = g + NL * q;
+          if (ri < it.ka) {
+            if (pt < it.ka) x[q] += Sa[ri * it.lds_in + pt];
+            if (pb < it.ka) y[q] += Sa[ri * it.lds_in + pb];
+          }
+        }
+      }
+    }
+  }
+  // exact power-of-two scaling: max |M_ij| in [1, 2)
+  T mx = T(0);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) mx = fmax(mx, fmax(fabs(x[q]), fabs(y[q])));
+  const T amax = wave_max(mx);
+  int sexp = 0;
+  if (amax > T(0) && isfinite(amax)) sexp = ilogb(amax);
+  const T scale = ldexp(T(1), -sexp), unscale = ldexp(T(1), sexp);
+  T f2 = T(0);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    x[q] *= scale;
+    y[q] *= scale;
+    f2 += x[q] * x[q] + y[q] * y[q];
+  }
+  const T fro2 = wave_sum(f2);
+  // numerically zero columns (below u ||M||_F, and 16x under an eps-rank
+  // threshold) are not rotated
+  T fl = Eps<T>::v;
+  if (it.fixed_rank < 0 && it.eps > 0.0) fl = fmin(fl, static_cast<T>(it.eps / (16.0 * sqrt((double)K))));
+  const T tiny = fl * fl * fro2;
+  const T tol = T(8 * max(K, 8)) * Eps<T>::v;
+  const T itol = T(1) / tol;
+  // cyclic rounds until K2 - 1 consecutive rounds rotate nothing
+  int rounds = 0;
+  if (K >= 2) {
+    int quiet = 0;
+    while (quiet < K2 - 1 && rounds < 60 * (K2 - 1)) {
+      ++rounds;
+      T al = T(0), be = T(0), ga = T(0);
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        al += x[q] * x[q];
+        be += y[q] * y[q];
+        ga += x[q] * y[q];
+      }
+      al = slot_sum<NL>(al);
+      be = slot_sum<NL>(be);
+      ga = slot_sum<NL>(ga);
+      const T gt = ga * itol;
+      const bool rot = act && al > tiny && be > tiny && gt * gt > al * be;
+      if (rot) {
+        T cth, sth;
+        jacobi_rotation(al, be, ga, cth, sth);
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const T a = x[q], b = y[q];
+          x[q] = cth * a - sth * b;
+          y[q] = sth * a + cth * b;
+          const T va = vx[q], vb = vy[q];
+          vx[q] = cth * va - sth * vb;
+          vy[q] = sth * va + cth * vb;
+        }
+      }
+      quiet = __any(rot) ? 0 : quiet + 1;
+      if (P > 1) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q) {
+          const T nt = wave_shr1((s == 0) ? y[q] : x[q]);
+          const T ntv = wave_shr1((s == 0) ? vy[q] : vx[q]);
+          const T nb = wave_shl1(y[q]);
+          const T nbv = wave_shl1(vy[q]);
+          if (s == P - 1) {
+            y[q] = x[q];
+            vy[q] = vx[q];
+          } else {
+            y[q] = nb;
+            vy[q] = nbv;
+          }
+          if (s != 0) {
+            x[q] = nt;
+            vx[q] = ntv;
+          }
+        }
+      }
+    }
+  }
+  const int sweeps = (rounds + K2 - 2) / max(1, K2 - 1);
+  if (lane == 0 && it.stats != nullptr) {
+    int32_t* st = it.stats + (blockIdx.x % kCounterCopies) * kCounterStride;
+    atomicAdd(st, 1);
+    atomicAdd(st + 1, sweeps);
+    atomicMax(st + 2, sweeps);
+  }
+  T nx = T(0), ny = T(0);
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    nx += x[q] * x[q];
+    ny += y[q] * y[q];
+  }
+  nx = sqrt(slot_sum<NL>(nx));
+  ny = sqrt(slot_sum<NL>(ny));
+  // the padding column of odd K started at position K2 - 1 and moved one place
+  // per round along the circle of positions 1 .. K2 - 1
+  const int padpos = (K & 1) ? ((K2 - 2 + rounds) % (K2 - 1)) + 1 : -1;
+  if (g == 0 && act) {
+    sig[pt] = (pt == padpos) ? T(-1) : nx;
+    sig[pb] = (pb == padpos) ? T(-1) : ny;
+  }
+  __syncthreads();
+  if (lane < K2) {
+    const T sj = sig[lane];
+    int rk = 0;
+    for (int i = 0; i < K2; ++i) {
+      const T si = sig[i];
+      rk += (si > sj || (si == sj && i < lane)) ? 1 : 0;
+    }
+    order[rk] = lane;
+    sigs[rk] = sj;
+    rk_of[lane] = rk;
+  }
+  __syncthreads();
+  int r;
+  if (it.fixed_rank >= 0) {
+    r = min(it.fixed_rank, K);
+  } else {
+    // src/operations/misc/misc.cpp:133-146: the smallest r >= 1 with r = md or
+    // sqrt((md - r) sigma_r^2) <= eps ||sigma||, the error summed term by term
+    const int md = (it.kdim > 0) ? min(it.kdim, K) : K;
+    T ss = T(0);
+    for (int i = 0; i < md; ++i) ss += sigs[i] * sigs[i];
+    const T threshold = static_cast<T>(it.eps) * sqrt(ss);
+    bool stop = lane >= 1;
+    if (lane >= 1 && lane < md) {
+      const T sr = sigs[lane];
+      T err = T(0);
+      for (int jj = lane; jj < md; ++jj) err += sr * sr;
+      stop = !(sqrt(err) > threshold);
+    }
+    r = __ffsll((unsigned long long)__ballot(stop)) - 1;
+  }
+  int rout = r;
+  if (r > it.cap) {
+    rout = it.cap_trunc ? it.cap : -r;
+    r = it.cap;
+  }
+  if (lane == 0 && it.rank_out) *it.rank_out = rout;
+  if (nx > T(0)) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) x[q] /= nx;
+  }
+  if (ny > T(0)) {
+#pragma unroll
+    for (int q = 0; q < NR; ++q) y[q] /= ny;
+  }
+  // Orthonormal completion of kept left vectors whose singular value is
+  // numerically zero (Jacobi leaves those columns unrotated): two passes of
+  // classical Gram-Schmidt of e_u against the earlier kept vectors, first u
+  // whose remainder keeps norm^2 > 1/4.  Rare: it runs on an LDS copy.
+  const T sig_floor = sqrt(tiny);
+  bool need = false;
+  for (int c = 0; c < r; ++c) need = need || !(sigs[c] > sig_floor);
+  if (need) {
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const int ri = g + NL * q;
+        A1[pt][ri] = x[q];
+        A1[pb][ri] = y[q];
+      }
+    }
+    __syncthreads();
+    __shared__ int accept_sh;
+    for (int c = 0; c < r; ++c) {
+      const int src = order[c];
+      if (sigs[c] > sig_floor) continue;
+      for (int u = 0; u < K; ++u) {
+        for (int row = lane; row < K; row += 64) A1[src][row] = (row == u) ? T(1) : T(0);
+        __syncthreads();
+        for (int pass = 0; pass < 2; ++pass) {
+          for (int pp = lane; pp < c; pp += 64) {
+            const int ps = order[pp];
+            T d = T(0);
+            for (int row = 0; row < K; ++row) d += A1[ps][row] * A1[src][row];
+            coef[pp] = d;
+          }
+          __syncthreads();
+          for (int row = lane; row < K; row += 64) {
+            T v = A1[src][row];
+            for (int pp = 0; pp < c; ++pp) v -= coef[pp] * A1[order[pp]][row];
+            A1[src][row] = v;
+          }
+          __syncthreads();
+        }
+        if (lane == 0) {
+          T nn = T(0);
+          for (int row = 0; row < K; ++row) nn += A1[src][row] * A1[src][row];
+          accept_sh = nn > T(0.25);
+          coef[0] = (nn > T(0)) ? T(1) / sqrt(nn) : T(0);
+        }
+        __syncthreads();
+        const int ok = accept_sh;
+        const T inv = coef[0];
+        __syncthreads();
+        if (ok) {
+          for (int row = lane; row < K; row += 64) A1[src][row] *= inv;
+          __syncthreads();
+          break;
+        }
+      }
+    }
+    if (act) {
+#pragma unroll
+      for (int q = 0; q < NR; ++q) {
+        const int ri = g + NL * q;
+        x[q] = A1[pt][ri];
+        y[q] = A1[pb][ri];
+      }
+    }
+  }
+  T* S = static_cast<T*>(it.S);
+  T* Um = static_cast<T*>(it.Um);
+  T* Vm = static_cast<T*>(it.Vm);
+  if (S != nullptr) {
+#pragma unroll
+    for (int k = 0; k < KMAX * KMAX / 64; ++k) {
+      const int e = lane + 64 * k;
+      const int i = e / KMAX, j = e % KMAX;
+      if (i < r && j < r) S[i * it.lds + j] = (i == j) ? sigs[i] * unscale : T(0);
+    }
+  }
+  const int ct = act ? rk_of[pt] : KMAX, cb = act ? rk_of[pb] : KMAX;
+#pragma unroll
+  for (int q = 0; q < NR; ++q) {
+    const int ri = g + NL * q;
+    if (ri < K) {
+      if (ct < r) {
+        if (Um != nullptr) Um[ri * it.ldu + ct] = x[q];
+        if (Vm != nullptr) Vm[ri * it.ldv + ct] = vx[q];
+      }
+      if (cb < r) {
+        if (Um != nullptr) Um[ri * it.ldu + cb] = y[q];
+        if (Vm != nullptr) Vm[ri * it.ldv + cb] = vy[q];
+      }
+    }
+  }
+}
+
 template <typename T, class Src>
 static void launch_svd_src(Src d_items, int n_items, int max_k, hipStream_t s) {
   if (n_items <= 0) return;
   if (max_k <= 16) {
-    hipLaunchKernelGGL((svd_kernel<T, 16, 64, false, Src>), dim3(n_items), dim3(64), 0, s, d_items);
+    hipLaunchKernelGGL((svd_reg_kernel<T, 16, Src>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 32) {
     hipLaunchKernelGGL((svd_kernel<T, 32, 64, false, Src>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 64) {
