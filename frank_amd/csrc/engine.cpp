@@ -105,6 +105,7 @@ const void* Staging::push_bytes(const void* src, size_t bytes, hipStream_t s) {
       if (d_) FRK_HIP_CHECK(hipFree(d_));
       size_t c = std::max<size_t>(aligned, size_t(16) << 20);
       FRK_HIP_CHECK(hipHostMalloc((void**)&h_, c));
+      FRK_HIP_CHECK(hipHostGetDevicePointer((void**)&hd_, h_, 0));
       FRK_HIP_CHECK(hipMalloc((void**)&d_, c));
       cap_ = c;
     }
@@ -112,8 +113,9 @@ const void* Staging::push_bytes(const void* src, size_t bytes, hipStream_t s) {
   char* hp = h_ + off_;
   char* dp = d_ + off_;
   if (bytes) {
+    // the slot is `aligned` bytes long on both sides: copy it whole in 16-byte words
     std::memcpy(hp, src, bytes);
-    FRK_HIP_CHECK(hipMemcpyAsync(dp, hp, bytes, hipMemcpyHostToDevice, s));
+    launch_stage_copy(dp, hd_ + off_, aligned / 16, s);
   }
   off_ += aligned;
   return dp;

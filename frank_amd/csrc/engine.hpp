@@ -66,6 +66,7 @@ class Staging {
 
  private:
   char* h_ = nullptr;
+  char* hd_ = nullptr;  // h_ as the device sees it
   char* d_ = nullptr;
   size_t cap_ = 0, off_ = 0;
 };

@@ -1659,12 +1659,12 @@ void launch_tri_extract(const T* src, int64_t rs, int64_t cs, T* dst, int64_t ld
 // Rotation (c, s) that zeroes the off-diagonal of [[al, ga], [ga, be]]
 // (Rutishauser's formulas: t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
 // zeta = (be - al) / (2 ga), c = 1 / sqrt(1 + t^2), s = c t).  FP32 uses the
-// hardware reciprocal and reciprocal square root (about 1 ulp: the rotation
+// hardware reciprocal, square root and reciprocal square root (about 1 ulp: the rotation
 // stays orthogonal to rounding level, only its angle moves by an ulp); FP64
 // keeps the correctly rounded operations.
 __device__ __forceinline__ void jacobi_rotation(float al, float be, float ga, float& c, float& s) {
   const float zeta = (be - al) * __builtin_amdgcn_rcpf(2.0f * ga);
-  const float t = copysignf(1.0f, zeta) * __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_sqrtf(1.0f + zeta * zeta));
+  const float t = copysignf(1.0f, zeta) * __builtin_amdgcn_rcpf(fabsf(zeta) + __builtin_amdgcn_sqrtf(1.0f + zeta * zeta));
   c = __builtin_amdgcn_rsqf(1.0f + t * t);
   s = c * t;
 }
@@ -2157,13 +2157,22 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
 // Neighbouring slots are neighbouring lanes, so a move is one DPP wave shift per
 // register; the column sums are a DPP / swizzle / permlane butterfly.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T wave_shr1(T v) {  // lane l receives lane l - 1
-  return dpp_mov<0x138>(v);
+// (lane 0, resp. 63, receives zero: bound_ctrl writes every lane, so no old value is needed)
+template <int CTRL>
+__device__ __forceinline__ int dpp_bound_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, true);
 }
-template <typename T>
-__device__ __forceinline__ T wave_shl1(T v) {  // lane l receives lane l + 1
-  return dpp_mov<0x130>(v);
+__device__ __forceinline__ float wave_shr1(float v) {  // lane l receives lane l - 1
+  return __int_as_float(dpp_bound_i<0x138>(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_shr1(double v) {
+  return __hiloint2double(dpp_bound_i<0x138>(__double2hiint(v)), dpp_bound_i<0x138>(__double2loint(v)));
+}
+__device__ __forceinline__ float wave_shl1(float v) {  // lane l receives lane l + 1
+  return __int_as_float(dpp_bound_i<0x130>(__float_as_int(v)));
+}
+__device__ __forceinline__ double wave_shl1(double v) {
+  return __hiloint2double(dpp_bound_i<0x130>(__double2hiint(v)), dpp_bound_i<0x130>(__double2loint(v)));
 }
 __device__ __forceinline__ float half_max(float v) {
   const auto r = __builtin_amdgcn_permlane32_swap(__float_as_int(v), __float_as_int(v), false, false);
@@ -3934,6 +3943,33 @@ template void launch_axpy_convert<float>(double*, const float*, int64_t, hipStre
   template void launch_flip_r<T>(const FlipItem*, int, int, hipStream_t);                        \
   template void launch_gen<T>(const KernelSpec&, const GenItem*, int, int, int, hipStream_t);    \
   template void launch_fill_identity<T>(T*, int64_t, int64_t, int64_t, hipStream_t);
+
+// Descriptor upload: an in-stream kernel reading the pinned staging ring (no
+// copy engine, so no cross-engine wait before the launches that use it).
+__global__ __launch_bounds__(256) void stage_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                         size_t n16) {
+  constexpr int U = 4;
+  const size_t base = (size_t)blockIdx.x * (256 * U) + threadIdx.x;
+  uint4 v[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const size_t e = base + (size_t)u * 256;
+    if (e < n16) v[u] = src[e];
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const size_t e = base + (size_t)u * 256;
+    if (e < n16) dst[e] = v[u];
+  }
+}
+
+void launch_stage_copy(void* dst, const void* src_host, size_t n16, hipStream_t s) {
+  if (n16 == 0) return;
+  const size_t g = (n16 + 1023) / 1024;
+  hipLaunchKernelGGL(stage_copy_kernel, dim3((unsigned)g), dim3(256), 0, s, static_cast<uint4*>(dst),
+                     static_cast<const uint4*>(src_host), n16);
+  FRK_HIP_CHECK(hipGetLastError());
+}
 
 FRK_INST(double)
 FRK_INST(float)

@@ -155,5 +155,8 @@ template <typename T>
 void launch_fill_identity(T* p, int64_t rows, int64_t cols, int64_t ld, hipStream_t s);
 template <typename TD, typename TS>
 void launch_convert(TD* dst, const TS* src, int64_t n, hipStream_t s);
+// Copy n16 16-byte words from pinned host memory (a device-visible pointer) to
+// device memory with a kernel on stream s.
+void launch_stage_copy(void* dst, const void* src_host, size_t n16, hipStream_t s);
 
 }  // namespace frk
