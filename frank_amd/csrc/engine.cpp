@@ -1347,6 +1347,7 @@ struct Lookahead {
   hipStream_t s = nullptr, s2 = nullptr;  // lookahead stream, second rest-chunk stream (the context's)
   hipEvent_t prio = nullptr, rest = nullptr, sxa = nullptr, c0 = nullptr, c1 = nullptr;
   int32_t* snap = nullptr;
+  int32_t* snap_dev = nullptr;  // snap as the device sees it
   Lookahead(Context& ctx, size_t nrank) : s(ctx.aux_stream(0)), s2(ctx.aux_stream(1)) {
     // device-side ordering only: no system-scope fence (a cache write-back per
     // record); rest keeps it, since the host reads the snapshot after it
@@ -1355,6 +1356,7 @@ struct Lookahead {
     FRK_HIP_CHECK(hipEventCreateWithFlags(&rest, hipEventDisableTiming));
     // nrank rank slots, then the lookahead context's tile-LU timeout marker
     FRK_HIP_CHECK(hipHostMalloc((void**)&snap, sizeof(int32_t) * (nrank + 1)));
+    FRK_HIP_CHECK(hipHostGetDevicePointer((void**)&snap_dev, snap, 0));
     snap[nrank] = 0;
   }
   ~Lookahead() {
@@ -2160,12 +2162,11 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     FRK_HIP_CHECK(hipStreamWaitEvent(la.s2, la.c0, 0));
     FRK_HIP_CHECK(hipEventRecord(la.c1, la.s2));
     FRK_HIP_CHECK(hipStreamWaitEvent(s, la.c1, 0));
-    FRK_HIP_CHECK(
-        hipMemcpyAsync(la.snap, A.d_rank(0, 0), sizeof(int32_t) * A.rank.size(), hipMemcpyDeviceToHost, la.s2));
-    // the rest of step i ran after the priority block of step i, hence after the
-    // factorization of tile i: its timeout marker is final by now
-    FRK_HIP_CHECK(hipMemcpyAsync(la.snap + A.rank.size(), side.tile_flags() + Context::kTileFlags - 1,
-                                 sizeof(int32_t), hipMemcpyDeviceToHost, la.s2));
+    // the ranks, then the tile LU's timeout marker (the rest of step i ran after the
+    // priority block of step i, hence after the factorization of tile i: the marker
+    // is final by now), written into the pinned snapshot by one kernel
+    launch_snapshot(la.snap_dev, A.d_rank(0, 0), (int)A.rank.size(), side.tile_flags() + Context::kTileFlags - 1,
+                    la.s2);
     FRK_HIP_CHECK(hipEventRecord(la.rest, la.s2));
     cur = &ctx;
     curp = &pl;

@@ -3963,6 +3963,20 @@ __global__ __launch_bounds__(256) void stage_copy_kernel(uint4* __restrict__ dst
   }
 }
 
+__global__ __launch_bounds__(256) void snapshot_kernel(int32_t* __restrict__ dst, const int32_t* __restrict__ src,
+                                                       int n, const int32_t* __restrict__ tail) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e < n)
+    dst[e] = src[e];
+  else if (e == n)
+    dst[e] = *tail;
+}
+
+void launch_snapshot(int32_t* dst_host, const int32_t* src, int n, const int32_t* tail, hipStream_t s) {
+  hipLaunchKernelGGL(snapshot_kernel, dim3((n + 256) / 256), dim3(256), 0, s, dst_host, src, n, tail);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
 void launch_stage_copy(void* dst, const void* src_host, size_t n16, hipStream_t s) {
   if (n16 == 0) return;
   const size_t g = (n16 + 1023) / 1024;

@@ -158,5 +158,8 @@ void launch_convert(TD* dst, const TS* src, int64_t n, hipStream_t s);
 // Copy n16 16-byte words from pinned host memory (a device-visible pointer) to
 // device memory with a kernel on stream s.
 void launch_stage_copy(void* dst, const void* src_host, size_t n16, hipStream_t s);
+// dst_host[0, n) = src[0, n), dst_host[n] = *tail: device words into pinned host
+// memory (a device-visible pointer) with a kernel on stream s.
+void launch_snapshot(int32_t* dst_host, const int32_t* src, int n, const int32_t* tail, hipStream_t s);
 
 }  // namespace frk
