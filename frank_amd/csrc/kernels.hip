@@ -93,6 +93,18 @@ __device__ __forceinline__ T wave_sum(T v) {
   return half_sum(v);
 }
 
+// v of lane l (uniform l) in every lane
+__device__ __forceinline__ float wave_read(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double wave_read(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// 1 / x: the hardware reciprocal in FP32 (about 1 ulp), correctly rounded in FP64
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 // ===========================================================================
@@ -1211,68 +1223,61 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(Src items) {
     }
   }
   const int kp = (it.P != nullptr) ? it.kp : 0;
-  for (int j = 0; j < nref; ++j) {
-    const int qj = j / W, wj = j - qj * W;
-    const int buf = j & 1;
-    if (w == wj) {
-      // column j by predicated selection (keeps a[][] in registers)
-      T col[MR];
+  // Step j = qj W + wj reflects column j, a[qj][] of wave wj: qj is unrolled, so the
+  // column is addressed in registers directly (no selection over the array).
+  static_assert(MR >= 2 && W * CPW <= 128, "row j < 128 lies in a[][0] or a[][1]");
 #pragma unroll
-      for (int t = 0; t < MR; ++t) col[t] = a[0][t];
+  for (int qj = 0; qj < CPW; ++qj) {
+    for (int wj = 0; wj < W; ++wj) {
+      const int j = qj * W + wj;
+      if (j >= nref) break;
+      const int buf = j & 1;
+      if (w == wj) {
+        T x2 = T(0);
 #pragma unroll
-      for (int q = 1; q < CPW; ++q)
+        for (int t = 0; t < MR; ++t) {
+          const int row = lane + 64 * t;
+          if (row > j) x2 += a[qj][t] * a[qj][t];
+        }
+        x2 = wave_sum(x2);
+        const T alpha = (j < 64) ? wave_read(a[qj][0], j) : wave_read(a[qj][1], j - 64);
+        T beta, tau, scal;
+        if (x2 == T(0)) {
+          tau = T(0);
+          beta = alpha;
+          scal = T(0);
+        } else {
+          const T nrm = sqrt(alpha * alpha + x2);
+          beta = (alpha >= T(0)) ? -nrm : nrm;
+          tau = (beta - alpha) * recip(beta);
+          scal = recip(alpha - beta);
+        }
 #pragma unroll
-        for (int t = 0; t < MR; ++t) col[t] = (q == qj) ? a[q][t] : col[t];
-      T x2 = T(0), alpha = T(0);
-#pragma unroll
-      for (int t = 0; t < MR; ++t) {
-        const int row = lane + 64 * t;
-        const T v = col[t];
-        if (row > j && row < m) x2 += v * v;
-        if (row == j) alpha = v;
+        for (int t = 0; t < MR; ++t) {
+          const int row = lane + 64 * t;
+          if (row > j) a[qj][t] *= scal;
+          if (row == j) a[qj][t] = beta;
+          vbuf[buf][lane + 64 * t] = (row == j) ? T(1) : ((row > j && row < m) ? a[qj][t] : T(0));
+        }
+        if (lane == 0) taus[j] = tau;
       }
-      x2 = wave_sum(x2);
-      alpha = wave_sum(alpha);
-      T beta, tau, scal;
-      if (x2 == T(0)) {
-        tau = T(0);
-        beta = alpha;
-        scal = T(0);
-      } else {
-        const T nrm = sqrt(alpha * alpha + x2);
-        beta = (alpha >= T(0)) ? -nrm : nrm;
-        tau = (beta - alpha) / beta;
-        scal = T(1) / (alpha - beta);
-      }
+      __syncthreads();
+      const T tj = taus[j];
+      if (tj != T(0)) {
+        T v[MR];
 #pragma unroll
-      for (int t = 0; t < MR; ++t) {
-        const int row = lane + 64 * t;
-        if (row > j) col[t] *= scal;
-        if (row == j) col[t] = beta;
-        vbuf[buf][lane + 64 * t] = (row == j) ? T(1) : ((row > j && row < m) ? col[t] : T(0));
-      }
+        for (int t = 0; t < MR; ++t) v[t] = vbuf[buf][lane + 64 * t];
 #pragma unroll
-      for (int q = 0; q < CPW; ++q)
+        for (int q = qj; q < CPW; ++q) {
+          const int c = q * W + w;
+          if ((q > qj || w > wj) && c < K) {
+            T d = T(0);
 #pragma unroll
-        for (int t = 0; t < MR; ++t) a[q][t] = (q == qj) ? col[t] : a[q][t];
-      if (lane == 0) taus[j] = tau;
-    }
-    __syncthreads();
-    const T tj = taus[j];
-    if (tj != T(0)) {
-      T v[MR];
+            for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
+            d = wave_sum(d) * tj;
 #pragma unroll
-      for (int t = 0; t < MR; ++t) v[t] = vbuf[buf][lane + 64 * t];
-#pragma unroll
-      for (int q = 0; q < CPW; ++q) {
-        const int c = q * W + w;
-        if (c > j && c < K) {
-          T d = T(0);
-#pragma unroll
-          for (int t = 0; t < MR; ++t) d += v[t] * a[q][t];
-          d = wave_sum(d) * tj;
-#pragma unroll
-          for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
+            for (int t = 0; t < MR; ++t) a[q][t] -= d * v[t];
+          }
         }
       }
     }
