@@ -3585,34 +3585,77 @@ __global__ __launch_bounds__(256) void lr_core_kernel(LrCoreItem it) {
   if (rl == 0 || rb == 0) return;
   // M1 = V_l U_b (rl x rb), 16 x 16 output blocks, K in chunks of KC
   constexpr int NQ = 16 * KC / 256;
-  for (int bi = 0; bi < rl; bi += 16)
-    for (int bj = 0; bj < rb; bj += 16) {
-      T acc = T(0);
-      for (int k0 = 0; k0 < K; k0 += KC) {
-        T va[NQ], ub[NQ];
+  if (rl <= 16 && rb <= 16) {
+    // one output block: FP64 MFMA 16 x 16 x 4 products, each wave a quarter of
+    // every chunk, the four partial blocks summed through T1
+    const int lane = tid & 63, w = tid >> 6, li = lane & 15, lk = lane >> 4;
+    constexpr int KW = KC / 4;
+    d64::d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < K; k0 += KC) {
+      T va[NQ], ub[NQ];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int e = tid + 256 * q;
-          const int r = e / KC, kk = e % KC, kb = e >> 4, c = e & 15;
-          const bool iv = bi + r < rl && k0 + kk < K, iu = bj + c < rb && k0 + kb < K;
-          va[q] = V[iv ? (int64_t)(bi + r) * it.v_rs + (int64_t)(k0 + kk) * it.v_cs : 0];
-          ub[q] = U[iu ? (int64_t)(k0 + kb) * it.u_rs + (int64_t)(bj + c) * it.u_cs : 0];
-          va[q] = iv ? va[q] : T(0);
-          ub[q] = iu ? ub[q] : T(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-          const int e = tid + 256 * q;
-          Va[e / KC][e % KC] = va[q];
-          Ub[e >> 4][e & 15] = ub[q];
-        }
-        __syncthreads();
-#pragma unroll 8
-        for (int kk = 0; kk < KC; ++kk) acc += Va[ti][kk] * Ub[kk][tj];
+      for (int q = 0; q < NQ; ++q) {
+        const int e = tid + 256 * q;
+        const int r = e / KC, kk = e % KC, kb = e >> 4, c = e & 15;
+        const bool iv = r < rl && k0 + kk < K, iu = c < rb && k0 + kb < K;
+        va[q] = V[iv ? (int64_t)r * it.v_rs + (int64_t)(k0 + kk) * it.v_cs : 0];
+        ub[q] = U[iu ? (int64_t)(k0 + kb) * it.u_rs + (int64_t)c * it.u_cs : 0];
+        va[q] = iv ? va[q] : T(0);
+        ub[q] = iu ? ub[q] : T(0);
       }
-      if (bi + ti < rl && bj + tj < rb) M1[bi + ti][bj + tj] = acc;
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int e = tid + 256 * q;
+        Va[e / KC][e % KC] = va[q];
+        Ub[e >> 4][e & 15] = ub[q];
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = w * KW; kk < w * KW + KW; kk += 4) {
+        const double a = static_cast<double>(Va[li][kk + lk]);
+        const double b = static_cast<double>(Ub[kk + lk][li]);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
     }
+    // lane (li, lk) holds rows 4 v + lk, column li of the wave's partial block
+#pragma unroll
+    for (int v = 0; v < 4; ++v) T1[16 * w + 4 * v + lk][li] = static_cast<T>(acc[v]);
+    __syncthreads();
+    {
+      const int i = tid >> 4, j = tid & 15;
+      if (i < rl && j < rb) M1[i][j] = T1[i][j] + T1[16 + i][j] + T1[32 + i][j] + T1[48 + i][j];
+    }
+  } else {
+    for (int bi = 0; bi < rl; bi += 16)
+      for (int bj = 0; bj < rb; bj += 16) {
+        T acc = T(0);
+        for (int k0 = 0; k0 < K; k0 += KC) {
+          T va[NQ], ub[NQ];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e / KC, kk = e % KC, kb = e >> 4, c = e & 15;
+            const bool iv = bi + r < rl && k0 + kk < K, iu = bj + c < rb && k0 + kb < K;
+            va[q] = V[iv ? (int64_t)(bi + r) * it.v_rs + (int64_t)(k0 + kk) * it.v_cs : 0];
+            ub[q] = U[iu ? (int64_t)(k0 + kb) * it.u_rs + (int64_t)(bj + c) * it.u_cs : 0];
+            va[q] = iv ? va[q] : T(0);
+            ub[q] = iu ? ub[q] : T(0);
+          }
+          __syncthreads();
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q;
+            Va[e / KC][e % KC] = va[q];
+            Ub[e >> 4][e & 15] = ub[q];
+          }
+          __syncthreads();
+#pragma unroll 8
+          for (int kk = 0; kk < KC; ++kk) acc += Va[ti][kk] * Ub[kk][tj];
+        }
+        if (bi + ti < rl && bj + tj < rb) M1[bi + ti][bj + tj] = acc;
+      }
+  }
   __syncthreads();
   // T1 = M1 S_b, core = alpha S_l T1 (all operands in LDS)
   for (int e = tid; e < rl * rb; e += 256) {
