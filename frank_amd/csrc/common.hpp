@@ -383,6 +383,21 @@ struct RowsPair {
   RowsJob job[2];
 };
 
+// One low-rank panel tile after its solve: row sigma < r of the solved factor
+// at src + sigma * src_ld (n entries) is written back to dst + sigma * dst_ld
+// (dst null: already in place), and output o < r of the stacked factor,
+// alpha sum_sigma coef(o, sigma) row_sigma with coef(o, sigma) at
+// s[o * s_rs + sigma * s_cs], to pq + o * n.
+struct PanelFin {
+  const void* src;
+  void* dst;
+  const void* s;
+  void* pq;
+  int64_t src_ld, dst_ld, s_rs, s_cs;
+  int32_t r, n;
+  double alpha;
+};
+
 // C (m x n, leading dimension ldc) += U (m x r1) core (r1 x r2) V (r2 x n),
 // r1 / r2 read on the device and clamped to kcap.
 struct LrUpdItem {
@@ -401,6 +416,19 @@ struct LrUpdItem {
 struct RowRef {
   const void* a;
   void* c;
+};
+
+// Two stacked-row products over row lists in one launch (blockIdx.z selects
+// the job): row r < R of job j at rows[r].a.
+struct RowsListJob {
+  const RowRef* rows;
+  const void* b;
+  int64_t b_rs, b_cs;
+  void* w;
+  int32_t R, N, K, pad;
+};
+struct RowsList2 {
+  RowsListJob job[2];
 };
 
 }  // namespace frk

@@ -1792,9 +1792,18 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     //    workspace and a copy back.
     std::vector<GemmItem> tg, tcp;
     std::vector<RowRef> vrows, urows;
+    // first row of each low-rank panel tile in the stacked rows; the largest rank
+    // of the panel tiles (the fused write-back below takes ranks up to 64)
+    std::vector<int> vofs(nb, -1), uofs(nb, -1);
+    int panel_rmax = 0;
+    for (int c = i + 1; c < nb; ++c)
+      for (const int t : {A.idx(c, i), A.idx(i, c)})
+        if (A.kind[t] == kLowRank) panel_rmax = std::max(panel_rmax, A.rank[t]);
+    const bool fused_panels = panel_rmax <= 64;
     for (int ic = i + 2; ic < nb; ++ic) {
       const int t = A.idx(ic, i);
       if (A.kind[t] == kLowRank) {  // V <- V U^-1
+        vofs[ic] = (int)vrows.size();
         for (int rho = 0; rho < A.rank[t]; ++rho) {
           T* row = A.V[t] + (size_t)rho * A.cols(i);
           vrows.push_back(RowRef{row, row});
@@ -1809,6 +1818,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     for (int j = i + 2; j < nb; ++j) {
       const int t = A.idx(i, j);
       if (A.kind[t] == kLowRank) {  // U <- L^-1 U, as U^T <- U^T L^-T on the columns of U
+        uofs[j] = (int)urows.size();
         for (int rho = 0; rho < A.rank[t]; ++rho) {
           T* col = A.U[t] + (size_t)rho * A.rows(i);
           urows.push_back(RowRef{col, col});
@@ -1820,7 +1830,30 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
         tcp.push_back(gemm_item(w, n, 1, nullptr, 0, 0, A.D[t], n, 1, m_i, n, 0, 0.0, 0.0));
       }
     }
-    for (int side2 = 0; side2 < 2; ++side2) {
+    // both panels' stacked-row products in one launch, into workspace; the
+    // write-back to the tiles comes with the stacked factors P and Q below
+    T* Wv = nullptr;
+    T* Wu = nullptr;
+    if (fused_panels && (!vrows.empty() || !urows.empty())) {
+      const int Rv = (int)vrows.size(), Ru = (int)urows.size();
+      Wv = ctx.take<T>((size_t)Rv * n_i + 1);
+      Wu = ctx.take<T>((size_t)Ru * m_i + 1);
+      const size_t ov = pl.add(vrows), ou = pl.add(urows);
+      const T* Ui = A.Uinv[ii];
+      const T* Li = A.Linv[ii];
+      T* wv = Wv;
+      T* wu = Wu;
+      pl.launch(
+          [=](const char* d) {
+            RowsList2 p{};
+            p.job[0] = RowsListJob{reinterpret_cast<const RowRef*>(d + ov), Ui, n_i, 1, wv, Rv, n_i, n_i, 0};
+            p.job[1] = RowsListJob{reinterpret_cast<const RowRef*>(d + ou), Li, 1, m_i, wu, Ru, m_i, m_i, 0};
+            launch_rows_gemm2<T>(p, s);
+          },
+          "trsm_gemm", 2.0 * Rv * n_i * n_i + 2.0 * Ru * m_i * m_i,
+          sizeof(T) * ((double)n_i * n_i + 2.0 * Rv * n_i + (double)m_i * m_i + 2.0 * Ru * m_i));
+    }
+    for (int side2 = 0; side2 < 2 && !fused_panels; ++side2) {
       const std::vector<RowRef>& rows = side2 == 0 ? vrows : urows;
       if (rows.empty()) continue;
       const int R = (int)rows.size();
@@ -1860,23 +1893,83 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     T* Qst = ctx.take<T>((size_t)RQ * m_i + 1);
     T* Sxa = ctx.take<T>((size_t)RP * RQ + 1);
     std::vector<GemmItem> gA, gB, gC, gD;
-    for (int ic = i + 1; ic < nb; ++ic) {
-      const int t = A.idx(ic, i);
-      if (A.kind[t] != kLowRank) continue;
-      const int kl = A.rank[t];
-      P[ic] = Pst + (size_t)poff[ic] * n_i;
-      // P(ic) = -S V: the minus sign of the Schur update lives in P
-      gA.push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, -1.0, 0.0));
+    if (fused_panels) {
+      // one launch: the solved panel rows back into their tiles (those of the
+      // priority block are in place already) and P (row-major, -S V) and Q
+      // (column-major, U S) formed from them
+      std::vector<PanelFin> fin;
+      int fin_n = 0;
+      double fin_fl = 0, fin_by = 0;
+      for (int ic = i + 1; ic < nb; ++ic) {
+        const int t = A.idx(ic, i);
+        if (A.kind[t] != kLowRank || A.rank[t] <= 0) continue;
+        P[ic] = Pst + (size_t)poff[ic] * n_i;
+        const bool solved_here = ic >= i + 2;
+        PanelFin f{};
+        f.src = solved_here ? static_cast<const void*>(Wv + (size_t)vofs[ic] * n_i) : A.V[t];
+        f.dst = solved_here ? A.V[t] : nullptr;
+        f.s = A.S[t];
+        f.pq = P[ic];
+        f.src_ld = n_i;
+        f.dst_ld = A.cols(i);
+        f.s_rs = A.kcap;
+        f.s_cs = 1;
+        f.r = A.rank[t];
+        f.n = n_i;
+        f.alpha = -1.0;
+        fin.push_back(f);
+        fin_n = std::max(fin_n, n_i);
+        fin_fl += 2.0 * f.r * f.r * n_i;
+        fin_by += sizeof(T) * ((solved_here ? 3.0 : 2.0) * f.r * n_i + (double)f.r * f.r);
+      }
+      for (int k = i + 1; k < nb; ++k) {
+        const int t = A.idx(i, k);
+        if (A.kind[t] != kLowRank || A.rank[t] <= 0) continue;
+        Q[k] = Qst + (size_t)qoff[k] * m_i;
+        const bool solved_here = k >= i + 2;
+        PanelFin f{};
+        f.src = solved_here ? static_cast<const void*>(Wu + (size_t)uofs[k] * m_i) : A.U[t];
+        f.dst = solved_here ? A.U[t] : nullptr;
+        f.s = A.S[t];
+        f.pq = Q[k];
+        f.src_ld = m_i;
+        f.dst_ld = A.rows(i);
+        f.s_rs = 1;
+        f.s_cs = A.kcap;
+        f.r = A.rank[t];
+        f.n = m_i;
+        f.alpha = 1.0;
+        fin.push_back(f);
+        fin_n = std::max(fin_n, m_i);
+        fin_fl += 2.0 * f.r * f.r * m_i;
+        fin_by += sizeof(T) * ((solved_here ? 3.0 : 2.0) * f.r * m_i + (double)f.r * f.r);
+      }
+      if (!fin.empty()) {
+        const size_t of = pl.add(fin);
+        const int nf = (int)fin.size();
+        pl.launch(
+            [=](const char* d) { launch_panel_finish<T>(reinterpret_cast<const PanelFin*>(d + of), nf, fin_n, s); },
+            "schur_gemm", fin_fl, fin_by);
+      }
+    } else {
+      for (int ic = i + 1; ic < nb; ++ic) {
+        const int t = A.idx(ic, i);
+        if (A.kind[t] != kLowRank) continue;
+        const int kl = A.rank[t];
+        P[ic] = Pst + (size_t)poff[ic] * n_i;
+        // P(ic) = -S V: the minus sign of the Schur update lives in P
+        gA.push_back(gemm_item(A.S[t], A.kcap, 1, A.V[t], A.cols(i), 1, P[ic], n_i, 1, kl, n_i, kl, -1.0, 0.0));
+      }
+      for (int k = i + 1; k < nb; ++k) {
+        const int t = A.idx(i, k);
+        if (A.kind[t] != kLowRank) continue;
+        const int kb = A.rank[t];
+        Q[k] = Qst + (size_t)qoff[k] * m_i;
+        gA.push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
+      }
+      plan_gemms<T>(pl, gA, "schur_gemm");
+      gA.clear();
     }
-    for (int k = i + 1; k < nb; ++k) {
-      const int t = A.idx(i, k);
-      if (A.kind[t] != kLowRank) continue;
-      const int kb = A.rank[t];
-      Q[k] = Qst + (size_t)qoff[k] * m_i;
-      gA.push_back(gemm_item(A.U[t], 1, A.rows(i), A.S[t], A.kcap, 1, Q[k], 1, m_i, m_i, kb, kb, 1.0, 0.0));
-    }
-    plan_gemms<T>(pl, gA, "schur_gemm");
-    gA.clear();
     if (RP > 0 && RQ > 0) {
       // Sxa = Pst Qst: rows of Pst against the (column-major) stacked Q
       std::vector<RowRef> prow(RP);

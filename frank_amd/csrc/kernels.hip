@@ -3484,6 +3484,73 @@ void launch_rows_pair(const RowsPair& p, hipStream_t s) {
 }
 template void launch_rows_pair<double>(const RowsPair&, hipStream_t);
 template void launch_rows_pair<float>(const RowsPair&, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(64 * rows_waves<T>()) void rows_gemm2_kernel(RowsList2 p) {
+  const RowsListJob j = blockIdx.z ? p.job[1] : p.job[0];
+  const RowRef* rows = j.rows;
+  rows_gemm_body<T>([&](int r) { return static_cast<const T*>(rows[r].a); }, j.R, nullptr,
+                    static_cast<const T*>(j.b), j.b_rs, j.b_cs, static_cast<T*>(j.w), j.N, j.K);
+}
+
+template <typename T>
+void launch_rows_gemm2(const RowsList2& p, hipStream_t s) {
+  int R = 0, N = 0;
+  for (const auto& j : p.job) {
+    if (j.R <= 0) continue;
+    if (j.N <= 0 || j.K <= 0 || j.K > 512) throw std::runtime_error("rows gemm2: N > 0, 0 < K <= 512");
+    R = std::max(R, (int)j.R);
+    N = std::max(N, (int)j.N);
+  }
+  if (R <= 0) return;
+  hipLaunchKernelGGL(rows_gemm2_kernel<T>, dim3((R + 15) / 16, (N + 63) / 64, 2), dim3(64 * rows_waves<T>()), 0, s, p);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_rows_gemm2<double>(const RowsList2&, hipStream_t);
+template void launch_rows_gemm2<float>(const RowsList2&, hipStream_t);
+
+// Panel tiles after their solve, a workgroup per (tile, 64 positions): the
+// r solved rows are staged through LDS (and written back to the tile), then
+// each of the r outputs of the stacked factor is formed from them.
+template <typename T>
+__global__ __launch_bounds__(256) void panel_finish_kernel(const PanelFin* __restrict__ items) {
+  const PanelFin it = items[blockIdx.y];
+  const int x0 = 64 * blockIdx.x;
+  if (x0 >= it.n || it.r <= 0) return;
+  __shared__ T Ws[64][65];
+  __shared__ T Sc[64][65];
+  const int tid = threadIdx.x, lx = tid & 63, g = tid >> 6;
+  const int r = min(it.r, 64), x = x0 + lx;
+  const T* __restrict__ src = static_cast<const T*>(it.src);
+  T* __restrict__ dst = static_cast<T*>(it.dst);
+  const T* __restrict__ S = static_cast<const T*>(it.s);
+  T* __restrict__ pq = static_cast<T*>(it.pq);
+  for (int e = tid; e < r * r; e += 256) {
+    const int o = e / r, sg = e - o * r;
+    Sc[o][sg] = S[o * it.s_rs + sg * it.s_cs];
+  }
+  for (int sg = g; sg < r; sg += 4) {
+    const T v = (x < it.n) ? src[sg * it.src_ld + x] : T(0);
+    Ws[sg][lx] = v;
+    if (dst != nullptr && x < it.n) dst[sg * it.dst_ld + x] = v;
+  }
+  __syncthreads();
+  if (x < it.n)
+    for (int o = g; o < r; o += 4) {
+      T acc = T(0);
+      for (int sg = 0; sg < r; ++sg) acc += Sc[o][sg] * Ws[sg][lx];
+      pq[(int64_t)o * it.n + x] = static_cast<T>(it.alpha) * acc;
+    }
+}
+
+template <typename T>
+void launch_panel_finish(const PanelFin* d_items, int n_items, int max_n, hipStream_t s) {
+  if (n_items <= 0 || max_n <= 0) return;
+  hipLaunchKernelGGL(panel_finish_kernel<T>, dim3((max_n + 63) / 64, n_items), dim3(256), 0, s, d_items);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+template void launch_panel_finish<double>(const PanelFin*, int, int, hipStream_t);
+template void launch_panel_finish<float>(const PanelFin*, int, int, hipStream_t);
 template void launch_rows_gemm<double>(const RowRef*, int, const double*, int64_t, int64_t, double*, int, int,
                                        hipStream_t, const int32_t*);
 template void launch_rows_gemm<float>(const RowRef*, int, const float*, int64_t, int64_t, float*, int, int,

@@ -133,6 +133,10 @@ void launch_solve_df(const DfSweep& sw, int nflags, int nrhs, int grid, hipStrea
 // Two strided stacked-row products in one launch (see RowsPair).
 template <typename T>
 void launch_rows_pair(const RowsPair& p, hipStream_t s);
+template <typename T>
+void launch_rows_gemm2(const RowsList2& p, hipStream_t s);
+template <typename T>
+void launch_panel_finish(const PanelFin* d_items, int n_items, int max_n, hipStream_t s);
 // Low-rank x low-rank Schur update with device ranks (kcap <= 64):
 // core = alpha S_l (V_l U_b) S_b, then C += U_l core V_b on 64 x 64 tiles.
 template <typename T>
