@@ -268,6 +268,17 @@ struct TileLuItem {
   float* a32_out;
   float* linv32;
   float* uinv32;
+  // optional low-rank update applied to the tile before its factorization:
+  // A += U core V with U (n x r1, element (i, a) at pre_u[i + a * pre_u_cs]),
+  // core (r1 x r2, row stride pre_ldk), V (r2 x n, row stride pre_v_rs), the
+  // ranks read on the device and clamped to pre_kcap; FP32 operands when pre_f32
+  const void* pre_u;
+  const void* pre_core;
+  const void* pre_v;
+  const int32_t* pre_r1;
+  const int32_t* pre_r2;
+  int64_t pre_u_cs, pre_v_rs;
+  int32_t pre_ldk, pre_kcap, pre_f32, pre_pad;
 };
 
 // ---------------------------------------------------------------------------

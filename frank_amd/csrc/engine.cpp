@@ -534,9 +534,21 @@ struct TileLu32 {
 };
 
 static void plan_tile_lu(Plan& pl, double* a, double* linv, double* uinv, int n, int64_t ld, int32_t* info,
-                         TileLu32 f32 = {}) {
+                         TileLu32 f32 = {}, const LrUpdItem* pre = nullptr, bool pre_f32 = false) {
   Context& c = pl.ctx();
   TileLuItem it{};
+  if (pre != nullptr) {
+    it.pre_u = pre->u;
+    it.pre_u_cs = pre->u_cs;
+    it.pre_core = pre->core;
+    it.pre_ldk = pre->ldk;
+    it.pre_v = pre->v;
+    it.pre_v_rs = pre->v_rs;
+    it.pre_r1 = pre->r1;
+    it.pre_r2 = pre->r2;
+    it.pre_kcap = pre->kcap;
+    it.pre_f32 = pre_f32 ? 1 : 0;
+  }
   it.a = a;
   it.linv = linv;
   it.uinv = uinv;
@@ -1289,16 +1301,26 @@ std::unique_ptr<BlrMatrix<T>> build_blr_kernel(const KernelDesc& kd, int nb_r, i
 // ============================================================================
 // BLR-LU
 // ============================================================================
-// Factorization of the dense diagonal tile i (always in FP64): plan into p on
-// context c (its stream and workspace).
+// Whether the diagonal tile i is factored by the one-launch tile LU.
 template <typename T>
-static void plan_diag_getrf(Plan& p, Context& c, BlrMatrix<T>& A, int i, int32_t* info) {
+static bool tile_lu_path(const BlrMatrix<T>& A, int i) {
+  return use_tile_lu() && A.rows(i) == A.cols(i) && A.rows(i) <= 512;
+}
+
+// Factorization of the dense diagonal tile i (always in FP64): plan into p on
+// context c (its stream and workspace).  pre (tile path only): a low-rank
+// update of the tile applied by the tile LU kernel before it factors (the
+// priority block's C += U core V).
+template <typename T>
+static void plan_diag_getrf(Plan& p, Context& c, BlrMatrix<T>& A, int i, int32_t* info,
+                            const LrUpdItem* pre = nullptr) {
   const int ii = A.idx(i, i);
   const int m_i = (int)A.rows(i), n_i = (int)A.cols(i);
-  const bool tile = use_tile_lu() && m_i == n_i && m_i <= 512;
+  const bool tile = tile_lu_path(A, i);
+  if (pre != nullptr && !tile) throw std::logic_error("tile LU pre-update without the tile LU");
   if constexpr (sizeof(T) == 8) {
     if (tile)
-      plan_tile_lu(p, A.D[ii], A.Linv[ii], A.Uinv[ii], n_i, n_i, info);
+      plan_tile_lu(p, A.D[ii], A.Linv[ii], A.Uinv[ii], n_i, n_i, info, {}, pre, false);
     else
       plan_getrf<T>(p, {A.D[ii]}, {A.Linv[ii]}, {A.Uinv[ii]}, m_i, n_i, info);
   } else {
@@ -1316,7 +1338,7 @@ static void plan_diag_getrf(Plan& p, Context& c, BlrMatrix<T>& A, int i, int32_t
       f32.a = reinterpret_cast<float*>(dt);
       f32.linv = reinterpret_cast<float*>(lt);
       f32.uinv = reinterpret_cast<float*>(ut);
-      plan_tile_lu(p, d64, l64, u64, n_i, n_i, info, f32);
+      plan_tile_lu(p, d64, l64, u64, n_i, n_i, info, f32, pre, true);
       return;
     }
     p.launch([=](const char*) {
@@ -1402,7 +1424,9 @@ static void account_prio(Context& c, const BlrMatrix<T>& A, int k, int rl, int r
 // With device ranks and both panels low rank the launches are counted here and
 // their cost is added by account_prio once the panel ranks are known on the host.
 template <typename T>
-static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int rb, bool device_ranks) {
+static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int rb, bool device_ranks,
+                      LrUpdItem* defer_update = nullptr, bool* deferred = nullptr) {
+  if (deferred) *deferred = false;
   const hipStream_t s = W.stream();
   const int kk = A.idx(k, k), tl = A.idx(k + 1, k), tb = A.idx(k, k + 1), tc = A.idx(k + 1, k + 1);
   const int m_k = (int)A.rows(k), n_k = (int)A.cols(k);
@@ -1461,10 +1485,17 @@ static void plan_prio(Plan& pl, Context& W, BlrMatrix<T>& A, int k, int rl, int 
     ui.n = n;
     ui.kcap = A.kcap;
     pl.launch([=](const char*) { launch_rows_pair<T>(rp, s); }, "trsm_gemm");
-    pl.launch([=](const char*) {
-      launch_lr_core<T>(ci, s);
-      launch_lr_update<T>(ui, s);
-    }, "schur_gemm");
+    if (defer_update != nullptr) {
+      // the update of C is left to the tile LU of C, which applies it first
+      *defer_update = ui;
+      *deferred = true;
+      pl.launch([=](const char*) { launch_lr_core<T>(ci, s); }, "schur_gemm");
+    } else {
+      pl.launch([=](const char*) {
+        launch_lr_core<T>(ci, s);
+        launch_lr_update<T>(ui, s);
+      }, "schur_gemm");
+    }
     return;
   }
   std::vector<GemmItem> tg, tcp;
@@ -1672,9 +1703,9 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
   const size_t side_ws = diag_getrf_workspace<T>((int)bm) + prio_workspace(A, bm) + 8192;
   // factorization of the diagonal tile k on the lookahead stream (workspace taken
   // after whatever the current job took: the jobs of this stream run in order)
-  auto enqueue_lu = [&](int k) {
+  auto enqueue_lu = [&](int k, const LrUpdItem* pre) {
     Plan pd(side);
-    plan_diag_getrf<T>(pd, side, A, k, info + k * npan_max);
+    plan_diag_getrf<T>(pd, side, A, k, info + k * npan_max, pre);
     pd.run();
   };
   // priority block of step k on the lookahead stream, after the factorization of
@@ -1688,10 +1719,14 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
     if (after_rest) FRK_HIP_CHECK(hipStreamWaitEvent(la.s, la.c1, 0));
     side.reserve(side_ws);
     Plan pp(side);
-    plan_prio<T>(pp, side, A, k, A.rank[A.idx(k + 1, k)], A.rank[A.idx(k, k + 1)], device_ranks);
+    LrUpdItem upd{};
+    bool deferred = false;
+    plan_prio<T>(pp, side, A, k, A.rank[A.idx(k + 1, k)], A.rank[A.idx(k, k + 1)], device_ranks,
+                 tile_lu_path(A, k + 1) ? &upd : nullptr, &deferred);
     pp.run();
+    // the rest of step k needs the solved panels, not the update of tile k+1
     FRK_HIP_CHECK(hipEventRecord(la.prio, la.s));
-    enqueue_lu(k + 1);
+    enqueue_lu(k + 1, deferred ? &upd : nullptr);
     ctx.host_time("host_prio", ms_since(tp));
   };
   // workspace of one step (panel products, per-update temporaries, additions) with
@@ -1745,7 +1780,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
   // allocate (and zero, on the lookahead stream) the lookahead context's tile-LU
   // flags before the first factorization: every step copies their timeout marker
   (void)side.tile_flags();
-  enqueue_lu(0);
+  enqueue_lu(0, nullptr);
   if (nb > 1 && nsteps > 0) enqueue_prio(0, prio_rank_free(A, 0), false);
   bool prio_pending = false;  // the priority block of step i waits for the host ranks
   for (int i = 0; i + 1 < nb && i < nsteps; ++i) {

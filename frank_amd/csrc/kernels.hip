@@ -761,6 +761,44 @@ __global__ __launch_bounds__(256) void tile_lu_kernel(TileLuItem it) {
     blk_load(M, Ablk, ld, mi, nj);
   if (bi == bj)
     for (int r = mi + tid; r < 64; r += 256) M[r * LD + r] = 1.0;  // identity padding of the last diagonal block
+  if (it.pre_core != nullptr) {
+    // 0. the low-rank update of the tile, on this block: M += U_blk (core V_blk)
+    __syncthreads();
+    const int kc = it.pre_kcap;
+    const int r1 = min(kc, max(0, *it.pre_r1)), r2 = min(kc, max(0, *it.pre_r2));
+    if (r1 > 0 && r2 > 0) {
+      auto ld_el = [&](const void* p, int64_t e) -> double {
+        return it.pre_f32 ? (double)static_cast<const float*>(p)[e] : static_cast<const double*>(p)[e];
+      };
+      for (int e = tid; e < 64 * 64; e += 256) {
+        const int a = e >> 6, b = e & 63;
+        Ls[a * LD + b] = (a < r1 && b < r2) ? ld_el(it.pre_core, (int64_t)a * it.pre_ldk + b) : 0.0;
+        Us[a * LD + b] = (a < r2 && b < nj) ? ld_el(it.pre_v, (int64_t)a * it.pre_v_rs + 64 * bj + b) : 0.0;
+      }
+      __syncthreads();
+      double w[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int e = tid + 256 * q, a = e >> 6, c = e & 63;
+        double acc = 0.0;
+        if (a < r1)
+          for (int b = 0; b < r2; ++b) acc += Ls[a * LD + b] * Us[b * LD + c];
+        w[q] = acc;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int e = tid + 256 * q;
+        Us[(e >> 6) * LD + (e & 63)] = w[q];
+      }
+      for (int e = tid; e < 64 * 64; e += 256) {
+        const int row = e >> 6, a = e & 63;
+        Ls[row * LD + a] = (row < mi && a < r1) ? ld_el(it.pre_u, (int64_t)(64 * bi + row) + a * it.pre_u_cs) : 0.0;
+      }
+      __syncthreads();
+      mma64(M, LD, Ls, LD, Us, LD, 64, 1.0, true, false);
+    }
+  }
   __syncthreads();
   // 1. Schur updates from the earlier block columns / rows
   const int pmax = min(bi, bj);
