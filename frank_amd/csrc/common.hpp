@@ -217,12 +217,16 @@ struct LrAddDesc {
 
 // Per-batch workspace of a batched low-rank addition: addition t owns
 // Yu/Yv + t*mmax*K / t*nmax*K (reflectors), tu/tv + t*K, Ru/Rv/Um/Vm + t*K*K
-// (and Vw + t*K*K when set).  lra: 1 rounded, 2 fast (when eps == 0).
+// (and Vw, Mw + t*K*K when set).  lra: 1 rounded, 2 fast (when eps == 0).
+// An addition whose K = kc + kx2 exceeds a tile dimension has a core of
+// Ke = max(min(m, K), min(n, K)) < K: its M = Ru Rv^T (Ke x Ke) is formed into
+// Mw first and the SVD runs on it (Kmax_svd: the largest core of the batch).
 template <typename T>
 struct LrBatch {
-  T *Yu, *Yv, *tu, *tv, *Ru, *Rv, *Um, *Vm, *Vw;
+  T *Yu, *Yv, *tu, *tv, *Ru, *Rv, *Um, *Vm, *Vw, *Mw;
   int32_t* stats;
   int32_t mmax, nmax, K, lra;
+  int32_t Kmax_svd, pad;
 };
 
 // ---------------------------------------------------------------------------

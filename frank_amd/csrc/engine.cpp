@@ -651,7 +651,10 @@ size_t lr_add_workspace(const std::vector<LrAddDesc<T>>& d) {
   }
   const size_t K = kmax;
   const size_t vw = (sizeof(T) == 8 && K > 64) ? K * K : 0;  // Jacobi right-vector scratch
-  return d.size() * sizeof(T) * ((size_t)(mmax + nmax) * K + 2 * K + 4 * K * K + vw) + 8 * 256 + 4096;
+  bool small_core = false;                                   // a core formed before its SVD
+  for (const auto& x : d) small_core = small_core || x.kc + x.kx2 > std::min(x.m, x.n);
+  const size_t mw = small_core ? K * K : 0;
+  return d.size() * sizeof(T) * ((size_t)(mmax + nmax) * K + 2 * K + 4 * K * K + vw + mw) + 10 * 256 + 4096;
 }
 
 template <typename T>
@@ -692,6 +695,17 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
     }
   }
   const int K = kmax;
+  // cores smaller than K (K above a tile dimension, not in fast mode): formed first,
+  // the SVD then runs at the largest core of the batch
+  int kmax_svd = 0;
+  bool core_prod = false;
+  for (const auto& x : d) {
+    const int Kt = x.kc + x.kx2;
+    const bool fast = (lra == kLraFast) && (x.eps == 0.0);
+    const int Ke = fast ? Kt : std::max(std::min(x.m, Kt), std::min(x.n, Kt));
+    core_prod = core_prod || Ke < Kt;
+    kmax_svd = std::max(kmax_svd, Ke);
+  }
   LrBatch<T> b{};
   b.Yu = ctx.take<T>((size_t)nd * mmax * K);
   b.Yv = ctx.take<T>((size_t)nd * nmax * K);
@@ -702,7 +716,9 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   b.Um = ctx.take<T>((size_t)nd * K * K);
   b.Vm = ctx.take<T>((size_t)nd * K * K);
   // FP64 cores above 64 keep their right vectors in global scratch
-  b.Vw = (sizeof(T) == 8 && K > 64) ? ctx.take<T>((size_t)nd * K * K) : nullptr;
+  b.Vw = (sizeof(T) == 8 && kmax_svd > 64) ? ctx.take<T>((size_t)nd * K * K) : nullptr;
+  b.Mw = core_prod ? ctx.take<T>((size_t)nd * K * K) : nullptr;
+  b.Kmax_svd = kmax_svd;
   b.stats = ctx.counters();
   b.mmax = mmax;
   b.nmax = nmax;
@@ -713,6 +729,8 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   const size_t od = pl.add(d);
   pl.launch([=](const char* dd) { launch_lr_add_qr<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, qk, s); },
             "lr_qr", qf, qb);
+  if (core_prod)
+    pl.launch([=](const char* dd) { launch_lr_add_core<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, s); });
   pl.launch([=](const char* dd) { launch_lr_add_svd<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, s); },
             "lr_svd", sf, sb);
   pl.launch(
@@ -1763,7 +1781,7 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
           need += sizeof(T) * ((size_t)bm * kmax + (size_t)kmax * kmax + 512);
         if (kc == kLowRank) {
           const size_t K = (size_t)kmax + ((kl == kDense && kb == kDense) ? (size_t)A.kcap : (size_t)kmax);
-          need += sizeof(T) * (2 * bm * K + 2 * K + 5 * K * K) + 8 * 256;
+          need += sizeof(T) * (2 * bm * K + 2 * K + 6 * K * K) + 10 * 256;
         }
       }
     return need;
@@ -2005,7 +2023,12 @@ void blr_getrf(BlrMatrix<T>& A, Context& ctx, int max_steps) {
       plan_gemms<T>(pl, gA, "schur_gemm");
       gA.clear();
     }
-    if (RP > 0 && RQ > 0) {
+    if ((int64_t)RP * RQ >= (int64_t(1) << 20)) {
+      // Sxa = Pst Qst as one tiled GEMM: with ranks near the tile size (small
+      // tiles, tight eps) the stacked factors have thousands of rows, more than
+      // the stacked-row product's 16-row slabs cover well
+      plan_gemms<T>(pl, {gemm_item(Pst, n_i, 1, Qst, 1, m_i, Sxa, RQ, 1, RP, RQ, n_i, 1.0, 0.0)}, "schur_gemm");
+    } else if (RP > 0 && RQ > 0) {
       // Sxa = Pst Qst: rows of Pst against the (column-major) stacked Q
       std::vector<RowRef> prow(RP);
       for (int r2 = 0; r2 < RP; ++r2) prow[r2] = RowRef{Pst + (size_t)r2 * n_i, nullptr};

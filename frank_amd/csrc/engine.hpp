@@ -449,7 +449,7 @@ void run_typed_batch(TypedBatch<T>& b, Context& ctx, int lra);
 // Workspace bound of plan_lr_add for nd additions of at most m x n with K columns.
 template <typename T>
 size_t lr_add_bound(int nd, int m, int n, int K) {
-  return (size_t)nd * sizeof(T) * ((size_t)(m + n) * K + 2 * (size_t)K + 5 * (size_t)K * K) + 8 * 256 + 4096;
+  return (size_t)nd * sizeof(T) * ((size_t)(m + n) * K + 2 * (size_t)K + 6 * (size_t)K * K) + 10 * 256 + 4096;
 }
 
 // ----------------------------------------------------------------------------

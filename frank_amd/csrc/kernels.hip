@@ -964,6 +964,14 @@ template <typename T>
 __device__ __forceinline__ bool lr_fast(const LrAddDesc<T>& x, const LrBatch<T>& b) {
   return b.lra == 2 && x.eps == 0.0;
 }
+// the core size of a rounded addition: K = kc + kx2, or less when K exceeds a
+// tile dimension (the QR factors then have min(m, K) / min(n, K) nonzero rows)
+template <typename T>
+__device__ __forceinline__ int lr_core_dim(const LrAddDesc<T>& x, const LrBatch<T>& b) {
+  const int Kt = x.kc + x.kx2;
+  if (b.Mw == nullptr || lr_fast(x, b)) return Kt;
+  return max(min(x.m, Kt), min(x.n, Kt));
+}
 
 template <typename T>
 struct LrQrItems {
@@ -1060,7 +1068,15 @@ struct LrSvdItems {
     sv.Rv = b.Rv + (size_t)t * K * K;
     sv.ldr = K;
     sv.K = Kt;
-    if (!fast) {
+    const int Ke = lr_core_dim(x, b);
+    if (Ke < Kt) {
+      // the core M = Ru Rv^T was formed into Mw (lr_core_prod_kernel)
+      sv.mode = 0;
+      sv.M = b.Mw + (size_t)t * K * K;
+      sv.m_rs = K;
+      sv.m_cs = 1;
+      sv.K = Ke;
+    } else if (!fast) {
       sv.kx = Kt;
       sv.ka = 0;
     } else {
@@ -1263,7 +1279,7 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(Src items) {
   const int kp = (it.P != nullptr) ? it.kp : 0;
   // Step j = qj W + wj reflects column j, a[qj][] of wave wj: qj is unrolled, so the
   // column is addressed in registers directly (no selection over the array).
-  static_assert(MR >= 2 && W * CPW <= 128, "row j < 128 lies in a[][0] or a[][1]");
+  static_assert(W * CPW <= 128, "row j < 128 lies in a[][0] or a[][1]");
 #pragma unroll
   for (int qj = 0; qj < CPW; ++qj) {
     for (int wj = 0; wj < W; ++wj) {
@@ -1278,7 +1294,11 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(Src items) {
           if (row > j) x2 += a[qj][t] * a[qj][t];
         }
         x2 = wave_sum(x2);
-        const T alpha = (j < 64) ? wave_read(a[qj][0], j) : wave_read(a[qj][1], j - 64);
+        T alpha;
+        if constexpr (MR == 1)
+          alpha = wave_read(a[qj][0], j);  // nref <= m <= 64
+        else
+          alpha = (j < 64) ? wave_read(a[qj][0], j) : wave_read(a[qj][1], j - 64);
         T beta, tau, scal;
         if (x2 == T(0)) {
           tau = T(0);
@@ -1343,22 +1363,32 @@ __global__ __launch_bounds__(W * 64) void qr_kernel(Src items) {
   }
 }
 
-template <typename T, class Src>
-static void launch_qr_src(Src d_items, int n_items, int max_k, hipStream_t s) {
-  if (n_items <= 0) return;
+template <typename T, int MR, class Src>
+static void launch_qr_mr(Src d_items, int n_items, int max_k, hipStream_t s) {
   if (max_k <= 16) {
-    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 4, Src>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, MR, 4, 4, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_k <= 32) {
-    hipLaunchKernelGGL((qr_kernel<T, 8, 4, 8, Src>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, MR, 4, 8, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_k <= 64) {
     // 16 waves x 4 columns: 32 register values per lane stay within the 1024-thread budget
-    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 4, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, MR, 16, 4, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else if (max_k <= 128) {
-    hipLaunchKernelGGL((qr_kernel<T, 8, 16, 8, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
+    hipLaunchKernelGGL((qr_kernel<T, MR, 16, 8, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else {
     throw std::runtime_error("qr: K exceeds 128");
   }
   FRK_HIP_CHECK(hipGetLastError());
+}
+
+// max_m: the most rows of any item; panels of at most 64 rows hold one row per lane
+template <typename T, class Src>
+static void launch_qr_src(Src d_items, int n_items, int max_k, hipStream_t s, int max_m = 512) {
+  if (n_items <= 0) return;
+  if (max_m > 512) throw std::runtime_error("qr: more than 512 rows");
+  if (max_m <= 64)
+    launch_qr_mr<T, 1>(d_items, n_items, max_k, s);
+  else
+    launch_qr_mr<T, 8>(d_items, n_items, max_k, s);
 }
 
 template <typename T>
@@ -1467,21 +1497,31 @@ __global__ __launch_bounds__(W * 64) void apply_q_kernel(Src items) {
   }
 }
 
-template <typename T, class Src>
-static void launch_apply_q_src(Src d_items, int n_items, int max_ncol, hipStream_t s) {
-  if (n_items <= 0) return;
+template <typename T, int MR, class Src>
+static void launch_apply_q_mr(Src d_items, int n_items, int max_ncol, hipStream_t s) {
   if (max_ncol <= 16) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 4, Src>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, MR, 4, 4, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_ncol <= 32) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 4, 8, Src>), dim3(n_items), dim3(256), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, MR, 4, 8, Src>), dim3(n_items), dim3(256), 0, s, d_items);
   } else if (max_ncol <= 64) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 8, 8, Src>), dim3(n_items), dim3(512), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, MR, 8, 8, Src>), dim3(n_items), dim3(512), 0, s, d_items);
   } else if (max_ncol <= 128) {
-    hipLaunchKernelGGL((apply_q_kernel<T, 8, 16, 8, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
+    hipLaunchKernelGGL((apply_q_kernel<T, MR, 16, 8, Src>), dim3(n_items), dim3(1024), 0, s, d_items);
   } else {
     throw std::runtime_error("apply_q: more than 128 columns");
   }
   FRK_HIP_CHECK(hipGetLastError());
+}
+
+// max_m: the most rows of any item (one row per lane up to 64)
+template <typename T, class Src>
+static void launch_apply_q_src(Src d_items, int n_items, int max_ncol, hipStream_t s, int max_m = 512) {
+  if (n_items <= 0) return;
+  if (max_m > 512) throw std::runtime_error("apply_q: more than 512 rows");
+  if (max_m <= 64)
+    launch_apply_q_mr<T, 1>(d_items, n_items, max_ncol, s);
+  else
+    launch_apply_q_mr<T, 8>(d_items, n_items, max_ncol, s);
 }
 
 template <typename T>
@@ -2605,15 +2645,70 @@ void launch_svd(const SvdItem* d_items, int n_items, int max_k, hipStream_t s) {
 
 template <typename T>
 void launch_lr_add_qr(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_k, hipStream_t s) {
-  launch_qr_src<T>(LrQrItems<T>{d, b}, 2 * nd, max_k, s);
+  launch_qr_src<T>(LrQrItems<T>{d, b}, 2 * nd, max_k, s, std::max(b.mmax, b.nmax));
 }
+// M = Ru Rv^T (Ke x Ke, row stride K) of the additions whose core is smaller
+// than K = kc + kx2 (a workgroup each; the others return at once): Ru and Rv
+// staged through LDS in chunks of 32 of the K columns.
+template <typename T>
+__global__ __launch_bounds__(256) void lr_core_prod_kernel(const LrAddDesc<T>* __restrict__ d, LrBatch<T> b) {
+  const int t = blockIdx.x;
+  const LrAddDesc<T> x = d[t];
+  const int K = b.K, Kt = x.kc + x.kx2, Ke = lr_core_dim(x, b);
+  if (Ke >= Kt) return;
+  constexpr int QC = 32, NO = 128 * 128 / 256;
+  __shared__ T Us[128][QC + 1];
+  __shared__ T Vs[128][QC + 1];
+  const T* Ru = b.Ru + (size_t)t * K * K;
+  const T* Rv = b.Rv + (size_t)t * K * K;
+  T* M = b.Mw + (size_t)t * K * K;
+  const int tid = threadIdx.x;
+  T acc[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) acc[k] = T(0);
+  for (int q0 = 0; q0 < Kt; q0 += QC) {
+    __syncthreads();
+    for (int e = tid; e < Ke * QC; e += 256) {
+      const int i = e / QC, q = e % QC;
+      const bool in = q0 + q < Kt;
+      Us[i][q] = in ? Ru[(int64_t)i * K + q0 + q] : T(0);
+      Vs[i][q] = in ? Rv[(int64_t)i * K + q0 + q] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+      const int e = tid + 256 * k;
+      if (e < Ke * Ke) {
+        const int i = e / Ke, j = e - (e / Ke) * Ke;
+        T a = acc[k];
+#pragma unroll 8
+        for (int q = 0; q < QC; ++q) a += Us[i][q] * Vs[j][q];
+        acc[k] = a;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    const int e = tid + 256 * k;
+    if (e < Ke * Ke) M[(int64_t)(e / Ke) * K + e % Ke] = acc[k];
+  }
+}
+
+template <typename T>
+void launch_lr_add_core(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s) {
+  if (nd <= 0 || b.Mw == nullptr) return;
+  if (b.K > 128) throw std::runtime_error("lr core product: K above 128");
+  hipLaunchKernelGGL(lr_core_prod_kernel<T>, dim3(nd), dim3(256), 0, s, d, b);
+  FRK_HIP_CHECK(hipGetLastError());
+}
+
 template <typename T>
 void launch_lr_add_svd(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s) {
-  launch_svd_src<T>(LrSvdItems<T>{d, b}, nd, b.K, s);
+  launch_svd_src<T>(LrSvdItems<T>{d, b}, nd, b.Kmax_svd > 0 ? b.Kmax_svd : b.K, s);
 }
 template <typename T>
 void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_ncol, hipStream_t s) {
-  launch_apply_q_src<T>(LrApplyItems<T>{d, b}, 2 * nd, max_ncol, s);
+  launch_apply_q_src<T>(LrApplyItems<T>{d, b}, 2 * nd, max_ncol, s, std::max(b.mmax, b.nmax));
 }
 
 // ===========================================================================
@@ -4090,6 +4185,7 @@ template void launch_axpy_convert<float>(double*, const float*, int64_t, hipStre
   template void launch_svd<T>(const SvdItem*, int, int, hipStream_t);                            \
   template void launch_lr_add_qr<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, int, hipStream_t);  \
   template void launch_lr_add_svd<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, hipStream_t);      \
+  template void launch_lr_add_core<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, hipStream_t);     \
   template void launch_lr_add_apply<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, int, hipStream_t); \
   template void launch_geqp3<T>(const Geqp3Item*, int, hipStream_t);                             \
   template void launch_permute_r<T>(const PermuteItem*, int, int, int, hipStream_t);             \

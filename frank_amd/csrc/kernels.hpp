@@ -120,6 +120,8 @@ void launch_rows_scatter(const RowRef* d_rows, int R, const T* W, int N, hipStre
 template <typename T>
 void launch_lr_add_qr(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_k, hipStream_t s);
 template <typename T>
+void launch_lr_add_core(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s);
+template <typename T>
 void launch_lr_add_svd(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s);
 template <typename T>
 void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_ncol, hipStream_t s);
