@@ -1380,13 +1380,15 @@ static void launch_qr_mr(Src d_items, int n_items, int max_k, hipStream_t s) {
   FRK_HIP_CHECK(hipGetLastError());
 }
 
-// max_m: the most rows of any item; panels of at most 64 rows hold one row per lane
+// max_m: the most rows of any item; a lane holds ceil(max_m / 64) rows (1, 4 or 8)
 template <typename T, class Src>
 static void launch_qr_src(Src d_items, int n_items, int max_k, hipStream_t s, int max_m = 512) {
   if (n_items <= 0) return;
   if (max_m > 512) throw std::runtime_error("qr: more than 512 rows");
   if (max_m <= 64)
     launch_qr_mr<T, 1>(d_items, n_items, max_k, s);
+  else if (max_m <= 256)
+    launch_qr_mr<T, 4>(d_items, n_items, max_k, s);
   else
     launch_qr_mr<T, 8>(d_items, n_items, max_k, s);
 }
@@ -1513,13 +1515,15 @@ static void launch_apply_q_mr(Src d_items, int n_items, int max_ncol, hipStream_
   FRK_HIP_CHECK(hipGetLastError());
 }
 
-// max_m: the most rows of any item (one row per lane up to 64)
+// max_m: the most rows of any item; a lane holds ceil(max_m / 64) rows (1, 4 or 8)
 template <typename T, class Src>
 static void launch_apply_q_src(Src d_items, int n_items, int max_ncol, hipStream_t s, int max_m = 512) {
   if (n_items <= 0) return;
   if (max_m > 512) throw std::runtime_error("apply_q: more than 512 rows");
   if (max_m <= 64)
     launch_apply_q_mr<T, 1>(d_items, n_items, max_ncol, s);
+  else if (max_m <= 256)
+    launch_apply_q_mr<T, 4>(d_items, n_items, max_ncol, s);
   else
     launch_apply_q_mr<T, 8>(d_items, n_items, max_ncol, s);
 }
