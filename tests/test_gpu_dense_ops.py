@@ -59,6 +59,28 @@ def test_svd_reconstructs_with_lapack_singular_values(torch_gpu, m, n):
     np.testing.assert_allclose(V @ V.T, np.eye(K), atol=1e-12)
 
 
+@pytest.mark.parametrize("m,n,rank", [(1, 1, 1), (2, 2, 2), (3, 5, 3), (9, 9, 9), (16, 16, 16), (16, 7, 7),
+                                       (7, 16, 7), (12, 12, 4), (13, 13, 5), (15, 15, 0)])
+def test_svd_small_cores(torch_gpu, m, n, rank):
+    """K = min(m, n) <= 16: the one-wave Jacobi with the core in registers (odd K pads
+    the round-robin with a zero column).  Rank-deficient inputs keep singular values
+    that are zero, whose left vectors come from the orthonormal completion."""
+    torch = torch_gpu
+    from frank_amd import dense
+    rng = np.random.default_rng(100 * m + n + rank)
+    A = rng.standard_normal((m, rank)) @ rng.standard_normal((rank, n)) if rank else np.zeros((m, n))
+    U, S, V = dense.svd(torch.from_numpy(A).cuda())
+    U, S, V = U.cpu().numpy(), S.cpu().numpy(), V.cpu().numpy()
+    s_ref = np.linalg.svd(A, compute_uv=False)
+    scale = max(s_ref[0], 1.0)
+    np.testing.assert_allclose(np.diag(S), s_ref, rtol=0, atol=1e-13 * scale)
+    assert np.all(np.diff(np.diag(S)) <= 0)
+    np.testing.assert_allclose(U @ S @ V, A, rtol=0, atol=1e-13 * scale * max(m, n))
+    K = min(m, n)
+    np.testing.assert_allclose(U.T @ U, np.eye(K), atol=1e-12)
+    np.testing.assert_allclose(V @ V.T, np.eye(K), atol=1e-12)
+
+
 @pytest.mark.parametrize("eps", [1e-6, 1e-10])
 @pytest.mark.parametrize("m,n", [(128, 128), (256, 192)])
 def test_truncated_geqp3_rank_and_reconstruction(torch_gpu, eps, m, n):
