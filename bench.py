@@ -371,7 +371,11 @@ def run_c0(args, ctx):
         F.solve(sol)
         return "ok"
 
+    ctx.counters(reset=True)
     dt, ops, _ = timed_steps(ctx, step, max(args.steps, 5), 1, "C0")
+    cnt = ctx.counters(reset=True)
+    log(f"C0: {cnt['svd_problems']} small SVDs, {cnt['svd_sweeps'] / max(cnt['svd_problems'], 1):.2f} Jacobi "
+        f"sweeps each (max {cnt['svd_max_sweeps']})")
     F = mats.last
     err = float(torch.linalg.norm(sol - xs) / torch.linalg.norm(xs))
     r = F.ranks()

@@ -2117,12 +2117,23 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   // Orthonormal completion of kept left vectors whose singular value is
   // numerically zero (below eps * ||M||_F: Jacobi leaves those columns
   // unrotated): two passes of classical Gram-Schmidt of e_u against the
-  // earlier kept vectors, first u whose remainder keeps norm^2 > 1/4.
+  // earlier kept vectors, accepted when the remainder keeps norm^2 > thr.
+  // Candidates are scanned from u = K - 1 down (the pivoted leading coordinates
+  // carry the kept vectors) and the scan resumes where the previous column's
+  // stopped: a rejected remainder only shrinks as the span grows, and an
+  // accepted e_u lies in it.  The remainders' squared norms sum to K - c, so a
+  // second scan with thr = 1/(4K) always accepts one.
   const T sig_floor = sqrt(tiny);
+  int u = K - 1;
+  T thr = T(0.25);
   for (int c = 0; c < r; ++c) {
     const int src = order[c];
     if (sig[src] > sig_floor) continue;
-    for (int u = 0; u < K; ++u) {
+    for (int tries = 0; tries < 2 * K; ++tries, --u) {
+      if (u < 0) {
+        u = K - 1;
+        thr = T(0.25) / T(K);
+      }
       for (int row = tid; row < K; row += NT) G[src][row] = (row == u) ? T(1) : T(0);
       __syncthreads();
       for (int pass = 0; pass < 2; ++pass) {
@@ -2143,7 +2154,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
       if (tid == 0) {
         T nn = T(0);
         for (int row = 0; row < K; ++row) nn += G[src][row] * G[src][row];
-        accept_sh = nn > T(0.25);
+        accept_sh = nn > thr;
         coef[0] = (nn > T(0)) ? T(1) / sqrt(nn) : T(0);
       }
       __syncthreads();
@@ -2153,6 +2164,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
       if (ok) {
         for (int row = tid; row < K; row += NT) G[src][row] *= inv;
         __syncthreads();
+        --u;
         break;
       }
     }
@@ -2529,8 +2541,10 @@ __global__ __launch_bounds__(64) void svd_reg_kernel(Src items) {
   }
   // Orthonormal completion of kept left vectors whose singular value is
   // numerically zero (Jacobi leaves those columns unrotated): two passes of
-  // classical Gram-Schmidt of e_u against the earlier kept vectors, first u
-  // whose remainder keeps norm^2 > 1/4.  Rare: it runs on an LDS copy.
+  // classical Gram-Schmidt of e_u against the earlier kept vectors, accepted
+  // when the remainder keeps norm^2 > thr, with the resumed descending scan of
+  // svd_kernel (a rejected remainder only shrinks as the span grows; a second
+  // scan with thr = 1/(4K) always accepts one).  Rare: it runs on an LDS copy.
   const T sig_floor = sqrt(tiny);
   bool need = false;
   for (int c = 0; c < r; ++c) need = need || !(sigs[c] > sig_floor);
@@ -2545,10 +2559,16 @@ __global__ __launch_bounds__(64) void svd_reg_kernel(Src items) {
     }
     __syncthreads();
     __shared__ int accept_sh;
+    int u = K - 1;
+    T thr = T(0.25);
     for (int c = 0; c < r; ++c) {
       const int src = order[c];
       if (sigs[c] > sig_floor) continue;
-      for (int u = 0; u < K; ++u) {
+      for (int tries = 0; tries < 2 * K; ++tries, --u) {
+        if (u < 0) {
+          u = K - 1;
+          thr = T(0.25) / T(K);
+        }
         for (int row = lane; row < K; row += 64) A1[src][row] = (row == u) ? T(1) : T(0);
         __syncthreads();
         for (int pass = 0; pass < 2; ++pass) {
@@ -2569,7 +2589,7 @@ __global__ __launch_bounds__(64) void svd_reg_kernel(Src items) {
         if (lane == 0) {
           T nn = T(0);
           for (int row = 0; row < K; ++row) nn += A1[src][row] * A1[src][row];
-          accept_sh = nn > T(0.25);
+          accept_sh = nn > thr;
           coef[0] = (nn > T(0)) ? T(1) / sqrt(nn) : T(0);
         }
         __syncthreads();
@@ -2579,6 +2599,7 @@ __global__ __launch_bounds__(64) void svd_reg_kernel(Src items) {
         if (ok) {
           for (int row = lane; row < K; row += 64) A1[src][row] *= inv;
           __syncthreads();
+          --u;
           break;
         }
       }

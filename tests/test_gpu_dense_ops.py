@@ -60,11 +60,13 @@ def test_svd_reconstructs_with_lapack_singular_values(torch_gpu, m, n):
 
 
 @pytest.mark.parametrize("m,n,rank", [(1, 1, 1), (2, 2, 2), (3, 5, 3), (9, 9, 9), (16, 16, 16), (16, 7, 7),
-                                       (7, 16, 7), (12, 12, 4), (13, 13, 5), (15, 15, 0)])
+                                       (7, 16, 7), (12, 12, 4), (13, 13, 5), (15, 15, 0),
+                                       (32, 32, 4), (24, 30, 11), (40, 40, 7), (64, 64, 0), (100, 48, 20)])
 def test_svd_small_cores(torch_gpu, m, n, rank):
     """K = min(m, n) <= 16: the one-wave Jacobi with the core in registers (odd K pads
-    the round-robin with a zero column).  Rank-deficient inputs keep singular values
-    that are zero, whose left vectors come from the orthonormal completion."""
+    the round-robin with a zero column); K up to 64: the LDS Jacobi after QR
+    preconditioning.  Rank-deficient inputs keep singular values that are zero,
+    whose left vectors come from the orthonormal completion (K - rank of them)."""
     torch = torch_gpu
     from frank_amd import dense
     rng = np.random.default_rng(100 * m + n + rank)
