@@ -2646,6 +2646,8 @@ static void launch_svd_src(Src d_items, int n_items, int max_k, hipStream_t s) {
   if (n_items <= 0) return;
   if (max_k <= 16) {
     hipLaunchKernelGGL((svd_reg_kernel<T, 16, Src>), dim3(n_items), dim3(64), 0, s, d_items);
+  } else if (max_k <= 32 && sizeof(T) == 8) {
+    hipLaunchKernelGGL((svd_reg_kernel<T, 32, Src>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 32) {
     hipLaunchKernelGGL((svd_kernel<T, 32, 64, false, Src>), dim3(n_items), dim3(64), 0, s, d_items);
   } else if (max_k <= 64) {
