@@ -428,7 +428,11 @@ def run_c1(args, ctx_unused):
         holder[:] = [mats.next().qr()]
         return "ok"
 
+    ctx.counters(reset=True)
     dt, ops, _ = timed_steps(ctx, step, steps, 1, "C1")
+    cnt = ctx.counters(reset=True)
+    log(f"C1: {cnt['svd_problems']} small SVDs, {cnt['svd_sweeps'] / max(cnt['svd_problems'], 1):.2f} Jacobi "
+        f"sweeps each (max {cnt['svd_max_sweeps']})")
     F = mats.last
     r = F.ranks()
     H = oracle_hier(A, eps)

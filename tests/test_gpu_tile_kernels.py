@@ -159,6 +159,39 @@ def test_rounded_addition_eps(torch_gpu, fa, eps):
     assert abs(err - fo.l2_error(d + d2, ref.dense())) <= 10 * eps
 
 
+@pytest.mark.parametrize("b,r0,c0,r1,c1", [(128, 0, 256, 0, 384), (256, 0, 768, 256, 768)])
+def test_rounded_addition_large_cores(torch_gpu, fa, b, r0, c0, r1, c1):
+    """FP64 cores of 77 and 92 columns (2-D Laplace blocks, the second term
+    overlapping the first): the LDS Jacobi with a global K x K scratch, whose
+    left vectors are formed as M v / sigma.  Rank, product and singular values
+    against the oracle's rounded addition; the factors are orthonormal to the
+    accuracy that formula gives, u ||M|| / sigma_min (u the unit roundoff)."""
+    torch = torch_gpu
+    from frank_amd.inputs import morton_sort, uniform_points
+    pts, _ = morton_sort(uniform_points(1024, 2), 6)
+    pts = [np.ascontiguousarray(p) for p in pts]
+    eps, cap = 1e-8, 96
+    d = fo.laplacend(pts, r0, b, c0, b)
+    d2 = 0.5 * d + fo.laplacend(pts, r1, b, c1, b)
+    A = fo.LowRank.from_eps(d, eps)
+    B = fo.LowRank.from_eps(d2, eps)
+    assert 64 < A.rank + B.rank <= 128
+    ref = fo.rounded_addition(A, B)
+    sa = _lr_slot(torch, A, cap)
+    ranks = fa.lr_add_batched([sa], [(dev(torch, B.U), dev(torch, B.S), dev(torch, B.V))], eps=eps,
+                              lra="rounded_addition", cap=cap)
+    assert ranks[0] == ref.rank
+    k = ranks[0]
+    u, s, v = (sa[0][:, :k].cpu().numpy(), sa[1][:k, :k].cpu().numpy(), sa[2][:k].cpu().numpy())
+    scale = np.abs(d + d2).max()
+    np.testing.assert_allclose(u @ s @ v, ref.dense(), rtol=0, atol=1e-12 * scale + 2 * eps * scale)
+    assert abs(fo.l2_error(d + d2, u @ s @ v) - fo.l2_error(d + d2, ref.dense())) <= 10 * eps
+    np.testing.assert_allclose(np.diag(s), np.diag(ref.S), rtol=0, atol=1e-13 * ref.S[0, 0])
+    sv = np.diag(ref.S)
+    np.testing.assert_allclose(u.T @ u, np.eye(k), atol=64 * np.finfo(float).eps * sv[0] / sv[k - 1])
+    np.testing.assert_allclose(v @ v.T, np.eye(k), atol=1e-12)
+
+
 @pytest.mark.parametrize("lra", ["fast_rounded_addition", "rounded_addition"])
 @pytest.mark.parametrize("rank", [1, 2, 4, 8])
 def test_fixed_rank_addition_keeps_rank(torch_gpu, fa, lra, rank):
