@@ -668,7 +668,9 @@ int frk_blr_kernel_residual(frk_blr a, const double* x_dev, const double* b_dev,
     if (!a || !x_dev || !b_dev || !r_dev || nrhs <= 0) throw std::invalid_argument("bad argument");
     if (!a->has_kernel) throw std::invalid_argument("matrix was not built from a kernel");
     const int64_t n = a->a64 ? a->a64->roff.back() : a->a32->roff.back();
-    launch_kernel_residual(a->ks, x_dev, b_dev, r_dev, n, nrhs, a->owner->stream);
+    double* part = static_cast<double*>(
+        a->owner->ctx->persistent(7, sizeof(double) * kernel_residual_scratch(n, nrhs)));
+    launch_kernel_residual(a->ks, x_dev, b_dev, r_dev, n, nrhs, part, a->owner->stream);
     host_wait_stream(a->owner->stream, __func__);
     return FRK_OK;
   });
@@ -683,7 +685,8 @@ int frk_kernel_apply(frk_ctx ctx, int kernel_kind, int dim, int64_t n, const dou
     KernelDesc kd{kernel_kind, dim, n, pts, ell, shift};
     DevBuf dpts;
     const KernelSpec ks = upload_kernel(kd, dpts, ctx->stream);
-    launch_kernel_residual(ks, x_dev, nullptr, y_dev, n, nrhs, ctx->stream);
+    double* part = static_cast<double*>(ctx->ctx->persistent(7, sizeof(double) * kernel_residual_scratch(n, nrhs)));
+    launch_kernel_residual(ks, x_dev, nullptr, y_dev, n, nrhs, part, ctx->stream);
     host_wait_stream(ctx->stream, __func__);
     return FRK_OK;
   });

@@ -2825,7 +2825,8 @@ double kernel_norm_inf(const KernelSpec& ks, int64_t n, Context& ctx) {
   double* r = ctx.take<double>(n);
   launch_fill_value(ones, n, 1.0, ctx.stream());
   launch_fill_value(zero, n, 0.0, ctx.stream());
-  launch_kernel_residual(ks, ones, zero, r, n, 1, ctx.stream());  // r = -A 1 (non-negative entries)
+  double* part = static_cast<double*>(ctx.persistent(7, sizeof(double) * kernel_residual_scratch(n, 1)));
+  launch_kernel_residual(ks, ones, zero, r, n, 1, part, ctx.stream());  // r = -A 1 (non-negative entries)
   return colmax(r, n, 1, ctx)[0];
 }
 
@@ -2838,6 +2839,7 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
   // residual and correction vectors persist in the context between calls
   double* r = static_cast<double*>(ctx.persistent(0, sizeof(double) * n * nrhs));
   T* d = static_cast<T*>(ctx.persistent(1, sizeof(T) * n * nrhs));
+  double* part = static_cast<double*>(ctx.persistent(7, sizeof(double) * kernel_residual_scratch(n, nrhs)));
   ctx.sync();
   ctx.reserve(sizeof(double) * kColmaxBlocks * nrhs * 4 + 4096);
   const std::vector<double> bn = colmax(b, n, nrhs, ctx);
@@ -2847,7 +2849,7 @@ RefineResult blr_refine(BlrMatrix<T>& F, const KernelSpec& ks, double anorm, con
   launch_convert<double, T>(x, d, n * nrhs, s);
   RefineResult res{0, 0.0};
   for (int it = 0;; ++it) {
-    launch_kernel_residual(ks, x, b, r, n, nrhs, s);
+    launch_kernel_residual(ks, x, b, r, n, nrhs, part, s);
     // op log: the product with the exact matrix (2 n^2 nrhs) plus the kernel
     // evaluation (difference, absolute value, shift, quotient: 4 n^2), FP64; the
     // matrix is generated on the fly, so only x, b and r cross HBM

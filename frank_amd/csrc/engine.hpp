@@ -140,7 +140,8 @@ class Context {
   // Device buffers kept between calls, grown on demand: a call that reuses a
   // size allocates nothing.  Slots: 0, 1 refinement residual and correction;
   // 2, 3 tpmqrt's C copies and their ranks; 4 ranks of a typed batch's dense
-  // compressions; 5 inverses of the batched tile LU; 6 tile-level tpqrt T.
+  // compressions; 5 inverses of the batched tile LU; 6 tile-level tpqrt T;
+  // 7 partial sums of the kernel residual.
   void* persistent(int slot, size_t bytes) {
     DevBuf& b = persist_[slot];
     if (b.bytes() < bytes) {

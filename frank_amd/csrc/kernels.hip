@@ -4118,17 +4118,12 @@ __global__ void residual_reduce_kernel(const double* __restrict__ part, const do
   R[e] = (B != nullptr) ? B[e] - s : s;
 }
 
+size_t kernel_residual_scratch(int64_t n, int nrhs) { return (size_t)kResSplits * n * nrhs; }
+
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R, int64_t n,
-                            int nrhs, hipStream_t s) {
+                            int nrhs, double* part, hipStream_t s) {
   if (nrhs > kResMaxRhs) throw std::runtime_error("residual: at most 16 right-hand sides");
-  static thread_local double* part = nullptr;
-  static thread_local int64_t part_cap = 0;
-  const int64_t need = (int64_t)kResSplits * n * nrhs;
-  if (need > part_cap) {
-    if (part) FRK_HIP_CHECK(hipFree(part));
-    FRK_HIP_CHECK(hipMalloc(&part, need * sizeof(double)));
-    part_cap = need;
-  }
+  if (part == nullptr) throw std::invalid_argument("residual: missing partial-sum scratch");
   dim3 grid((unsigned)((n + 255) / 256), kResSplits);
   if (ks.kind == 0 && ks.dim == 1)
     launch_residual_laplace<1>(grid, s, ks, X, part, n, nrhs);

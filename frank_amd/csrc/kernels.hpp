@@ -145,8 +145,12 @@ template <typename T>
 void launch_lr_core(const LrCoreItem& it, hipStream_t s);
 template <typename T>
 void launch_lr_update(const LrUpdItem& it, hipStream_t s);
+// R = B - A X (B null: R = A X) with the kernel matrix generated on the fly; part
+// holds kernel_residual_scratch(n, nrhs) doubles of partial sums, stream-ordered
+// with s (the caller's context owns it)
+size_t kernel_residual_scratch(int64_t n, int nrhs);
 void launch_kernel_residual(const KernelSpec& ks, const double* X, const double* B, double* R,
-                            int64_t n, int nrhs, hipStream_t s);
+                            int64_t n, int nrhs, double* part, hipStream_t s);
 
 // Per-column max |a| over row blocks: out[b * nrhs + c] for b < kColmaxBlocks.
 constexpr int kColmaxBlocks = 64;
