@@ -274,6 +274,25 @@ class BlrMatrix:
             r0 += rows[i]
         return out
 
+    # -- structure and rank report --------------------------------------------
+    def to_json(self) -> dict:
+        """Structure and rank report in the layout of the reference's write_JSON
+        (src/util/print.cpp:54-128): the root is the Hierarchical grid (type,
+        dim, abs_pos [0, 0], level 0, children as rows of blocks); each block
+        carries type, dim, abs_pos [i, j], level 1 and its singular values
+        (a LowRank block those of its dense expansion, plus its rank).  Like the
+        reference's, this is a host-side report: tiles are copied from the
+        device and their singular values computed by host LAPACK.  An Empty
+        block raises, as the reference has no to_json method for one."""
+        return structure_json(self.shape, self.tile_info, self.tile)
+
+    def write_json(self, filename: str) -> None:
+        """write_JSON(A, filename) (src/util/print.cpp:124-128): the report of
+        to_json as one compact JSON document with sorted keys."""
+        import json
+        with open(filename, "w") as f:
+            json.dump(self.to_json(), f, sort_keys=True, separators=(",", ":"))
+
     def clone(self) -> "BlrMatrix":
         h = C.c_void_p()
         L.check(L.lib().frk_blr_clone(self.h, C.byref(h)))
@@ -375,6 +394,36 @@ def gemm(A, B, C_, alpha: float = 1.0, beta: float = 1.0, trans_a: bool = False,
     L.check(L.lib().frk_dgemm(ctx.h, int(trans_a), int(trans_b), m, n, k, alpha, _dptr(A), A.stride(0),
                               _dptr(B), B.stride(0), beta, _dptr(C_), C_.stride(0)))
     return C_
+
+
+def structure_json(shape: Tuple[int, int], tile_info, tile) -> dict:
+    """The write_JSON document of a one-level grid (src/util/print.cpp:54-122),
+    from accessors: shape = (tile rows, tile cols); tile_info(i, j) -> (kind,
+    rank, rows, cols); tile(i, j) -> ndarray (Dense) or (U, S, V) (LowRank).
+    Singular values are all min(m, n) of them, descending (get_singular_values,
+    src/operations/LAPACK/svd.cpp:46-61)."""
+    nr, nc = shape
+    rows = [tile_info(i, 0)[2] for i in range(nr)]
+    cols = [tile_info(0, j)[3] for j in range(nc)]
+    children = []
+    for i in range(nr):
+        row = []
+        for j in range(nc):
+            kind, rank, m, n = tile_info(i, j)
+            if kind == L.TILE_EMPTY:
+                raise TypeError(f"to_json: no method for the Empty block ({i}, {j})")
+            t = tile(i, j)
+            node = {"type": "LowRank" if kind == L.TILE_LOWRANK else "Dense", "dim": [int(m), int(n)],
+                    "abs_pos": [i, j], "level": 1}
+            dense = (t[0] @ t[1]) @ t[2] if kind == L.TILE_LOWRANK else t
+            node["svalues"] = [float(x) for x in np.linalg.svd(np.asarray(dense, dtype=np.float64),
+                                                               compute_uv=False)]
+            if kind == L.TILE_LOWRANK:
+                node["rank"] = int(rank)
+            row.append(node)
+        children.append(row)
+    return {"type": "Hierarchical", "dim": [int(sum(rows)), int(sum(cols))], "abs_pos": [0, 0], "level": 0,
+            "children": children}
 
 
 def lr_add_batched(As: list, Bs: list, *, eps: float = 0.0, lra: str = "fast_rounded_addition",

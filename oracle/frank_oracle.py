@@ -676,6 +676,38 @@ class Hier:
         return r
 
 
+def get_singular_values(a: np.ndarray) -> List[float]:
+    """src/operations/LAPACK/svd.cpp:46-61: dgesvd 'N','N', all min(m, n) values, descending."""
+    return [float(x) for x in sla.svd(np.asarray(a, dtype=np.float64), compute_uv=False, lapack_driver="gesvd")]
+
+
+def to_json(H: Hier) -> dict:
+    """src/util/print.cpp:54-122 (to_json over a one-level Hierarchical): every block
+    carries type, dim, abs_pos and level; Hierarchical adds children (rows of
+    blocks, abs_pos i_abs * dim0 + i), Dense adds svalues, LowRank adds svalues
+    of its dense expansion and rank.  An Empty block has no to_json method in
+    the reference (print.cpp:114-120 reports an error), so it raises here."""
+    n_r = sum(r[1] for r in H.row_ranges)
+    n_c = sum(c[1] for c in H.col_ranges)
+    out = {"type": "Hierarchical", "dim": [n_r, n_c], "abs_pos": [0, 0], "level": 0, "children": []}
+    for i, (_, rn) in enumerate(H.row_ranges):
+        row = []
+        for j, (_, cn) in enumerate(H.col_ranges):
+            b = H[i, j]
+            if b is None:
+                raise TypeError("to_json: no method for an Empty block")
+            node = {"type": "LowRank" if isinstance(b, LowRank) else "Dense", "dim": [rn, cn],
+                    "abs_pos": [i, j], "level": 1}
+            if isinstance(b, LowRank):
+                node["svalues"] = get_singular_values(b.dense())
+                node["rank"] = b.rank
+            else:
+                node["svalues"] = get_singular_values(b)
+            row.append(node)
+        out["children"].append(row)
+    return out
+
+
 def build_blr(kernel, n_rows: int, n_cols: int, nb_r: int, nb_c: int, *, eps: float = 0.0,
               rank: int = 0, admis: float = 0.0, admis_type: str = "position",
               points=None, cap: Optional[int] = None) -> Hier:
