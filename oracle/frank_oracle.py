@@ -29,6 +29,8 @@ How each function is pinned (tests/test_oracle_*.py):
   same rule, orthogonality of the factors, rank preservation.
 * BLR getrf / solve: dense LAPACK factor/solve of the same matrix, the
   reference's ``l2_error <= eps`` contract.
+* to_json / get_singular_values: a document written out by hand for a grid
+  whose blocks have known singular values (tests/test_structure_json.py).
 """
 from __future__ import annotations
 
