@@ -1786,15 +1786,16 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   __shared__ int order[KMAX];
   __shared__ int rank_sh;
   // QR preconditioning (Drmac-Veselic): M P = Q R by column-pivoted Householder,
-  // then one-sided Jacobi on R^T, whose graded rows converge in few sweeps.  The
-  // reflectors are kept for Q unless the right vectors live in global memory
-  // (VG: the left vectors are then formed as M v / sigma).  The FP32 K = 128
-  // variant has no LDS for them and runs Jacobi on M directly.
+  // then one-sided Jacobi on R^T, whose graded rows converge in few sweeps.  Q is
+  // formed in the right-vector array before the rotations, which then turn it
+  // into the left vectors, unless the right vectors live in global memory (VG:
+  // the left vectors are then formed as M v / sigma).  The FP32 K = 128 variant
+  // has no LDS for the right vectors' preconditioned form and runs Jacobi on M
+  // directly.
   constexpr bool PRE = VG || KMAX <= 64;
   __shared__ int perm[KMAX];
   __shared__ T tauq[KMAX];
   __shared__ T cnorm[KMAX];
-  __shared__ T Hq[(PRE && !VG) ? KMAX : 1][KMAX + 1];
   __shared__ int piv_sh;
   const int tid = threadIdx.x;
   if (K <= 0) {
@@ -1939,11 +1940,7 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
       }
       __syncthreads();
       const T inv = cnorm[j], tau = tauq[j];
-      for (int row = j + 1 + tid; row < K; row += NT) {
-        const T v = G[j][row] * inv;
-        G[j][row] = v;
-        if constexpr (!VG) Hq[j][row] = v;
-      }
+      for (int row = j + 1 + tid; row < K; row += NT) G[j][row] *= inv;
       __syncthreads();
       // apply H_j to the trailing columns: x -= tau v (v^T x), v = [1; G[j][j+1..]]
       if (tau != T(0))
@@ -1961,16 +1958,38 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
           }
         }
       __syncthreads();
-      for (int row = j + 1 + tid; row < K; row += NT) G[j][row] = T(0);
-      __syncthreads();
     }
-    // G = R^T (in place): column j of G becomes row j of R
+    if constexpr (!VG) {
+      // Q = H_0 H_1 ... H_{K-1} formed in the right-vector array (identity so
+      // far), from the reflectors kept below G's diagonal, last one first: the
+      // rotations then carry Q along, so the array ends as Q V_x, the left
+      // vectors, with no back-transform (and no LDS for the reflectors)
+      for (int j = K - 1; j >= 0; --j) {
+        const T tau = tauq[j];
+        if (tau != T(0))
+          for (int c0 = j; c0 < K; c0 += NT / gq) {
+            const int c = c0 + qc;
+            T w = T(0);
+            if (c < K) {
+              if (qg == 0) w = VA(c, j);
+              for (int row = j + 1 + qg; row < K; row += gq) w += G[j][row] * VA(c, row);
+            }
+            w = group_sum(w, gq) * tau;
+            if (c < K) {
+              if (qg == 0) VA(c, j) -= w;
+              for (int row = j + 1 + qg; row < K; row += gq) VA(c, row) -= w * G[j][row];
+            }
+          }
+        __syncthreads();
+      }
+    }
+    // G = R^T (in place): column j of G becomes row j of R, and the reflectors
+    // below R's diagonal give way to zeros
     for (int e = tid; e < K * K; e += NT) {
       const int c = e / K, row = e % K;
       if (row > c) {
-        const T a = G[c][row];
         G[c][row] = G[row][c];
-        G[row][c] = a;
+        G[row][c] = T(0);
       }
     }
     __syncthreads();
@@ -2179,26 +2198,12 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
     }
   if constexpr (PRE) {
     // M = (Q V_x) Sigma (P U_x)^T: right vectors from G (rows permuted back),
-    // left vectors Q V_x (reflectors applied in reverse to the rotations), or,
-    // in VG mode, M v / sigma
+    // left vectors Q V_x (the rotated Q), or, in VG mode, M v / sigma
     for (int e = tid; e < K * r; e += NT) {
       const int row = e / r, c = e % r;
       if (Vm != nullptr) Vm[perm[row] * it.ldv + c] = G[order[c]][row];
     }
     if constexpr (!VG) {
-      for (int c = tid; c < r; c += NT) {
-        const int src = order[c];
-        for (int j = K - 1; j >= 0; --j) {
-          const T tau = tauq[j];
-          if (tau == T(0)) continue;
-          T w = VA(src, j);
-          for (int row = j + 1; row < K; ++row) w += Hq[j][row] * VA(src, row);
-          w *= tau;
-          VA(src, j) -= w;
-          for (int row = j + 1; row < K; ++row) VA(src, row) -= w * Hq[j][row];
-        }
-      }
-      __syncthreads();
       for (int e = tid; e < K * r; e += NT) {
         const int row = e / r, c = e % r;
         if (Um != nullptr) Um[row * it.ldu + c] = VA(order[c], row);
