@@ -82,7 +82,8 @@ class BlrSolveBatch(torch.autograd.Function):
     uneven N).  The systems are factorized and solved concurrently, one library
     context (its own streams) per worker thread; every system keeps its factors
     for the backward pass, which runs the transposed sweeps and the
-    matrix-free dA/d ell contractions concurrently in the same way."""
+    matrix-free dA/d ell contractions with one worker per context, the systems
+    of one context in turn."""
 
     @staticmethod
     def forward(fctx, ells: torch.Tensor, systems: list, workers: int, *bs: torch.Tensor):
@@ -114,7 +115,6 @@ class BlrSolveBatch(torch.autograd.Function):
         fctx.factors = [r[0] for r in res]
         fctx.pts = [r[1] for r in res]
         fctx.ells = ell_v
-        fctx.workers = ctxs.qsize()
         us = [r[2] for r in res]
         fctx.save_for_backward(*us)
         return tuple(us)
@@ -132,8 +132,21 @@ class BlrSolveBatch(torch.autograd.Function):
             dA_u = kernel_apply(fctx.pts[s], us[s], "gaussian_dell", ell=fctx.ells[s], ctx=A.ctx)
             return lam, float(-(lam * dA_u).sum())
 
-        with cf.ThreadPoolExecutor(max_workers=fctx.workers) as ex:
-            res = list(ex.map(one, range(len(us))))
+        # A context is used by one thread at a time (its host-side state and
+        # streams are not shared safely): the systems factored on one context
+        # run one after another on one worker, the contexts concurrently.
+        groups = {}
+        for s, A in enumerate(fctx.factors):
+            groups.setdefault(id(A.ctx), []).append(s)
+
+        def run(group):
+            return [(s, one(s)) for s in group]
+
+        res = [None] * len(us)
+        with cf.ThreadPoolExecutor(max_workers=len(groups)) as ex:
+            for out in ex.map(run, list(groups.values())):
+                for s, r in out:
+                    res[s] = r
         grad_ells = torch.tensor([r[1] for r in res], dtype=torch.float64)
         return (grad_ells, None, None) + tuple(r[0] for r in res)
 

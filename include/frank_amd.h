@@ -11,6 +11,8 @@
  * Layouts follow FRANK's Dense (include/FRANK/classes/dense.h:43-66): row-major
  * with a leading dimension.  Device pointers are HIP device memory; host
  * pointers are plain host memory.  All work is ordered on the context's stream.
+ * A context (with the matrices and factors it owns) is used by one thread at a
+ * time; separate contexts may be driven from separate threads concurrently.
  */
 #ifndef FRANK_AMD_H
 #define FRANK_AMD_H
