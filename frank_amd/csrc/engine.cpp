@@ -658,11 +658,30 @@ size_t lr_add_workspace(const std::vector<LrAddDesc<T>>& d) {
 }
 
 template <typename T>
-void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
-  if (d.empty()) return;
+void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d_in, int lra) {
+  if (d_in.empty()) return;
   Context& ctx = pl.ctx();
   const hipStream_t s = ctx.stream();
-  const int nd = (int)d.size();
+  const int nd = (int)d_in.size();
+  // The SVD kernel is chosen by the batch's largest core.  In a large batch
+  // (throughput-bound: many waves of workgroups) the cores above 64 columns are
+  // ordered last and launched apart, so that the smaller ones do not run in the
+  // 128-column kernel (one workgroup per CU).  Small batches keep one launch:
+  // there the launches' latencies would add up.  Every kernel of the batch
+  // indexes its workspace by the item, so the order is the same for all.
+  auto core_dim = [lra](const LrAddDesc<T>& x) {
+    const int Kt = x.kc + x.kx2;
+    const bool fast = (lra == kLraFast) && (x.eps == 0.0);
+    return fast ? Kt : std::max(std::min(x.m, Kt), std::min(x.n, Kt));
+  };
+  constexpr int kSplitBatch = 1024;
+  const bool split = nd >= kSplitBatch;
+  auto svd_class = [split](int k) { return (split && k > 64) ? 1 : 0; };
+  std::vector<LrAddDesc<T>> d(d_in);
+  if (split)
+    std::stable_sort(d.begin(), d.end(), [&](const LrAddDesc<T>& a, const LrAddDesc<T>& c) {
+      return svd_class(core_dim(a)) < svd_class(core_dim(c));
+    });
   int kmax = 0, mmax = 0, nmax = 0, qk = 0, capmax = 0;
   double qf = 0, qb = 0, sf = 0, sb = 0, af = 0, ab = 0;
   for (const auto& x : d) {
@@ -699,12 +718,16 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
   // the SVD then runs at the largest core of the batch
   int kmax_svd = 0;
   bool core_prod = false;
-  for (const auto& x : d) {
+  int cls_end[2] = {0, 0}, cls_kmax[2] = {0, 0};
+  for (int t = 0; t < nd; ++t) {
+    const auto& x = d[t];
     const int Kt = x.kc + x.kx2;
-    const bool fast = (lra == kLraFast) && (x.eps == 0.0);
-    const int Ke = fast ? Kt : std::max(std::min(x.m, Kt), std::min(x.n, Kt));
+    const int Ke = core_dim(x);
     core_prod = core_prod || Ke < Kt;
     kmax_svd = std::max(kmax_svd, Ke);
+    const int c = svd_class(Ke);
+    cls_end[c] = t + 1;
+    cls_kmax[c] = std::max(cls_kmax[c], Ke);
   }
   LrBatch<T> b{};
   b.Yu = ctx.take<T>((size_t)nd * mmax * K);
@@ -731,8 +754,18 @@ void plan_lr_add(Plan& pl, const std::vector<LrAddDesc<T>>& d, int lra) {
             "lr_qr", qf, qb);
   if (core_prod)
     pl.launch([=](const char* dd) { launch_lr_add_core<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, s); });
-  pl.launch([=](const char* dd) { launch_lr_add_svd<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, s); },
-            "lr_svd", sf, sb);
+  int t0 = 0;
+  bool first = true;
+  for (int c = 0; c < 2; ++c) {
+    if (cls_end[c] <= t0) continue;
+    const int n = cls_end[c] - t0, kc_max = cls_kmax[c];
+    // the batch's SVD flops and bytes are logged with its first launch
+    pl.launch([=](const char* dd) {
+      launch_lr_add_svd<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), t0, n, kc_max, b, s);
+    }, "lr_svd", first ? sf : 0.0, first ? sb : 0.0);
+    first = false;
+    t0 = cls_end[c];
+  }
   pl.launch(
       [=](const char* dd) { launch_lr_add_apply<T>(reinterpret_cast<const LrAddDesc<T>*>(dd + od), nd, b, capmax, s); },
       "lr_apply_q", af, ab);

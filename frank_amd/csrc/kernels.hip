@@ -1057,7 +1057,9 @@ template <typename T>
 struct LrSvdItems {
   const LrAddDesc<T>* d;
   LrBatch<T> b;
-  __device__ SvdItem get(int t) const {
+  int t0;  // first item of this launch (workspace slots are indexed by the item)
+  __device__ SvdItem get(int tl) const {
+    const int t = t0 + tl;
     const LrAddDesc<T> x = d[t];
     const int K = b.K;
     const int Kt = x.kc + x.kx2;
@@ -2735,8 +2737,8 @@ void launch_lr_add_core(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipS
 }
 
 template <typename T>
-void launch_lr_add_svd(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s) {
-  launch_svd_src<T>(LrSvdItems<T>{d, b}, nd, b.Kmax_svd > 0 ? b.Kmax_svd : b.K, s);
+void launch_lr_add_svd(const LrAddDesc<T>* d, int t0, int n, int max_k, const LrBatch<T>& b, hipStream_t s) {
+  launch_svd_src<T>(LrSvdItems<T>{d, b, t0}, n, max_k, s);
 }
 template <typename T>
 void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_ncol, hipStream_t s) {
@@ -4211,7 +4213,7 @@ template void launch_axpy_convert<float>(double*, const float*, int64_t, hipStre
   template void launch_apply_q<T>(const ApplyQItem*, int, int, hipStream_t);                     \
   template void launch_svd<T>(const SvdItem*, int, int, hipStream_t);                            \
   template void launch_lr_add_qr<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, int, hipStream_t);  \
-  template void launch_lr_add_svd<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, hipStream_t);      \
+  template void launch_lr_add_svd<T>(const LrAddDesc<T>*, int, int, int, const LrBatch<T>&, hipStream_t); \
   template void launch_lr_add_core<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, hipStream_t);     \
   template void launch_lr_add_apply<T>(const LrAddDesc<T>*, int, const LrBatch<T>&, int, hipStream_t); \
   template void launch_geqp3<T>(const Geqp3Item*, int, hipStream_t);                             \

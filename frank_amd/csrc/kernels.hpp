@@ -122,7 +122,8 @@ void launch_lr_add_qr(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int ma
 template <typename T>
 void launch_lr_add_core(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s);
 template <typename T>
-void launch_lr_add_svd(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, hipStream_t s);
+// SVDs of items [t0, t0 + n) of a batch, in the kernel for cores of up to max_k
+void launch_lr_add_svd(const LrAddDesc<T>* d, int t0, int n, int max_k, const LrBatch<T>& b, hipStream_t s);
 template <typename T>
 void launch_lr_add_apply(const LrAddDesc<T>* d, int nd, const LrBatch<T>& b, int max_ncol, hipStream_t s);
 // One fused block step of a substitution sweep (see StepDiag).
