@@ -1791,10 +1791,9 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   // then one-sided Jacobi on R^T, whose graded rows converge in few sweeps.  Q is
   // formed in the right-vector array before the rotations, which then turn it
   // into the left vectors, unless the right vectors live in global memory (VG:
-  // the left vectors are then formed as M v / sigma).  The FP32 K = 128 variant
-  // has no LDS for the right vectors' preconditioned form and runs Jacobi on M
-  // directly.
-  constexpr bool PRE = VG || KMAX <= 64;
+  // the left vectors are then formed as M v / sigma).  Every variant
+  // preconditions (the flag marks the preconditioning code).
+  constexpr bool PRE = true;
   __shared__ int perm[KMAX];
   __shared__ T tauq[KMAX];
   __shared__ T cnorm[KMAX];
@@ -1878,9 +1877,9 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
   for (int e = tid; e < K * K; e += NT) G[e / K][e % K] *= scale;
   __syncthreads();
   if constexpr (VG) {
-    // right vectors too large for LDS: the sweeps rotate only G, and the kept
-    // right vectors are formed afterwards as M^T u_c / sigma_c from this copy
-    // of the (scaled) matrix, column j at vg + j K
+    // the rotations' vectors do not fit in LDS: the sweeps rotate only G, and
+    // the kept left vectors are formed afterwards as M P g_c / sigma_c from this
+    // copy of the (scaled) matrix, column j at vg + j K
     for (int e = tid; e < K * K; e += NT) vg[(int64_t)(e / K) * K + e % K] = G[e / K][e % K];
     __syncthreads();
   }
@@ -2221,30 +2220,6 @@ __global__ __launch_bounds__(NT) void svd_kernel(Src items) {
         for (int j2 = lane; j2 < K; j2 += 64) acc += vg[(int64_t)perm[j2] * K + row] * G[src][j2];
         acc = wave_sum(acc);
         if (lane == 0) Um[row * it.ldu + c] = (sc > sig_floor) ? acc / sc : T(0);
-      }
-    }
-    return;
-  }
-  for (int e = tid; e < K * r; e += NT) {
-    const int row = e / r, c = e % r;
-    const int src = order[c];
-    if (Um != nullptr) Um[row * it.ldu + c] = G[src][row];
-    if constexpr (!VG)
-      if (Vm != nullptr) Vm[row * it.ldv + c] = VA(src, row);
-  }
-  if constexpr (VG) {
-    // V[:, c] = M^T u_c / sigma_c (M scaled as sigma is): a wave per output,
-    // lanes over the reduction index; numerically zero sigma gives a zero column
-    if (Vm != nullptr) {
-      const int lane = tid & 63, wv = tid >> 6;
-      for (int o = wv; o < K * r; o += NT / 64) {
-        const int row = o / r, c = o % r;
-        const int src = order[c];
-        const T sc = sig[src];
-        T acc = T(0);
-        for (int i2 = lane; i2 < K; i2 += 64) acc += vg[(int64_t)row * K + i2] * G[src][i2];
-        acc = wave_sum(acc);
-        if (lane == 0) Vm[row * it.ldv + c] = (sc > sig_floor) ? acc / sc : T(0);
       }
     }
   }
